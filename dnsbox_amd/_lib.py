@@ -1,0 +1,90 @@
+"""ctypes binding of libdnsbox_hip.so (include/dnsbox.h).
+
+The library is the product: when it is missing or fails to load, every entry
+point raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libdnsbox_hip.so")
+
+(D_EKIN, D_DISP, D_PROD, D_COURANT, D_DIVMAX, D_EZERO, D_EONE, D_ESQRT2, D_EBAND, D_TIME,
+ D_DT, NDIAG) = range(12)
+DIAG_NAMES = ("Ekin", "Disp", "Prod", "Courant", "divMax", "EZero", "EOne", "Esqrt2", "Eband",
+              "time", "dt")
+
+
+class Params(C.Structure):
+    """dnsbox_params (include/dnsbox.h)."""
+    _fields_ = [
+        ("nu", C.c_double), ("Q", C.c_double), ("Pin", C.c_double), ("kcut", C.c_double),
+        ("kolmo_amp", C.c_double), ("deltak", C.c_double), ("cfl_min", C.c_double),
+        ("cfl_max", C.c_double), ("dt_max", C.c_double), ("dt", C.c_double),
+        ("n", C.c_int), ("precision", C.c_int), ("kolmo_k", C.c_int), ("bandlim", C.c_int),
+        ("spherical", C.c_int), ("parity_mode", C.c_int), ("adaptive_dt", C.c_int),
+        ("reserved", C.c_int),
+    ]
+
+
+class DnsboxError(RuntimeError):
+    pass
+
+
+# exported symbols and their signatures: (restype, argtypes)
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+SIGNATURES = {
+    "dnsbox_last_error": (C.c_char_p, []),
+    "dnsbox_version": (C.c_char_p, []),
+    "dnsbox_params_default": (C.c_int, [C.POINTER(Params)]),
+    "dnsbox_create": (C.c_int, [C.POINTER(Params), C.POINTER(_vp)]),
+    "dnsbox_destroy": (C.c_int, [_vp]),
+    "dnsbox_set_stream": (C.c_int, [_vp, _vp]),
+    "dnsbox_sync": (C.c_int, [_vp]),
+    "dnsbox_nspec": (C.c_int, [_vp]),
+    "dnsbox_set_state": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "dnsbox_get_state": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "dnsbox_set_state_physical": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "dnsbox_get_state_physical": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "dnsbox_init_random": (C.c_int, [_vp, C.c_uint64, C.c_int, C.c_double, C.c_double]),
+    "dnsbox_project": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "dnsbox_int_fact": (C.c_int, [_vp, C.c_double]),
+    "dnsbox_rhs_nonlinear": (C.c_int, [_vp]),
+    "dnsbox_get_nonlin": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "dnsbox_step": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "dnsbox_diagnostics": (C.c_int, [_vp, _dp, _dp]),
+    "dnsbox_get_time": (C.c_int, [_vp, _dp, _dp]),
+    "dnsbox_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "dnsbox_state_ptr": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "dnsbox_rhs_dev": (C.c_int, [_vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DnsboxError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = load().dnsbox_last_error().decode(errors="replace")
+        raise DnsboxError(f"dnsbox error {rc}: {msg}")
+    return rc

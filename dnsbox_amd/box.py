@@ -1,0 +1,179 @@
+"""Python front-end of one MI355X box, mirroring the reference module API.
+
+The reference exposes argument-less Fortran module procedures over module
+globals (rhs.f90, state.f90, dnsboxheun.f90).  ``Box`` keeps those names and
+their meaning on an explicit object:
+
+=====================  ==============================================  =========================
+Box method             reference routine                                C-ABI entry point
+=====================  ==============================================  =========================
+``rhs_int_fact(dt)``   ``rhs_int_fact``            rhs.f90:8-16        ``dnsbox_int_fact``
+``rhs_nonlinear()``    ``rhs_nonlinear``           rhs.f90:18-135      ``dnsbox_rhs_nonlinear``
+``step(n)``            loop body                   dnsboxheun.f90:81-157  ``dnsbox_step``
+``state_project()``    ``state_project``           state.f90:120-179   ``dnsbox_project``
+``state_dealias()``    ``state_dealias``           state.f90:181-209   ``dnsbox_project``
+``stats()``            ``io_save_stats``           io.f90:628-661      ``dnsbox_diagnostics``
+``spectrum()``         ``state_spectrum``          state.f90:350-411   ``dnsbox_diagnostics``
+``set_state_physical`` ``state_u2uhat``            state.f90:75-81     ``dnsbox_set_state_physical``
+``get_state_physical`` ``state_uhat2u``            state.f90:83-96     ``dnsbox_get_state_physical``
+``init_random``        ``state_init_rand``         state.f90:211-285   ``dnsbox_init_random``
+=====================  ==============================================  =========================
+
+Host arrays use the reference layout: spectral numpy arrays of shape
+(Nx/2+1, Ny, Nz) complex128 (C order == Fortran uhat(Nz, Ny, Nx/2+1)), physical
+arrays of shape (Nz, Ny, Nx) float64 (== Fortran u(Nx, Ny, Nz)).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Params, check, load
+
+
+def _dp(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    check(load().dnsbox_params_default(C.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+class Box:
+    """One periodic box [0, 2 pi)^3 on the GPU."""
+
+    def __init__(self, n: int = 32, precision: int = 64, **params):
+        self.lib = load()
+        self.params = default_params(n=n, precision=precision, **params)
+        h = C.c_void_p()
+        check(self.lib.dnsbox_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.n = n
+        self.precision = precision
+
+    # -- lifetime -------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            check(self.lib.dnsbox_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def spec_shape(self):
+        return (self.n // 2 + 1, self.n, self.n)
+
+    @property
+    def phys_shape(self):
+        return (self.n, self.n, self.n)
+
+    def nspec(self) -> int:
+        return int(self.lib.dnsbox_nspec(self._h))
+
+    def sync(self):
+        check(self.lib.dnsbox_sync(self._h))
+
+    def set_stream(self, stream_ptr: int):
+        check(self.lib.dnsbox_set_stream(self._h, C.c_void_p(stream_ptr)))
+
+    # -- state ----------------------------------------------------------------------
+    def set_state(self, uhat, vhat, what):
+        arrs = [np.ascontiguousarray(a, np.complex128) for a in (uhat, vhat, what)]
+        for a in arrs:
+            if a.shape != self.spec_shape:
+                raise ValueError(f"expected {self.spec_shape}, got {a.shape}")
+        check(self.lib.dnsbox_set_state(self._h, *[_dp(a) for a in arrs]))
+
+    def get_state(self):
+        out = [np.zeros(self.spec_shape, np.complex128) for _ in range(3)]
+        check(self.lib.dnsbox_get_state(self._h, *[_dp(a) for a in out]))
+        return out
+
+    def set_state_physical(self, u, v, w):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (u, v, w)]
+        for a in arrs:
+            if a.shape != self.phys_shape:
+                raise ValueError(f"expected {self.phys_shape}, got {a.shape}")
+        check(self.lib.dnsbox_set_state_physical(self._h, *[_dp(a) for a in arrs]))
+
+    def get_state_physical(self):
+        out = [np.zeros(self.phys_shape, np.float64) for _ in range(3)]
+        check(self.lib.dnsbox_get_state_physical(self._h, *[_dp(a) for a in out]))
+        return out
+
+    def init_random(self, seed: int, box: int = 0, k0: float = 4.0, energy: float = 0.5):
+        check(self.lib.dnsbox_init_random(self._h, seed, box, k0, energy))
+
+    def state_project(self):
+        check(self.lib.dnsbox_project(self._h, 0, 1))
+
+    def state_dealias(self):
+        check(self.lib.dnsbox_project(self._h, 1, 0))
+
+    # -- hot path -------------------------------------------------------------------
+    def rhs_int_fact(self, dt: float):
+        check(self.lib.dnsbox_int_fact(self._h, dt))
+
+    def rhs_nonlinear(self):
+        """N(state) -> returns (nonlinuhat, nonlinvhat, nonlinwhat) in the reference layout."""
+        check(self.lib.dnsbox_rhs_nonlinear(self._h))
+        out = [np.zeros(self.spec_shape, np.complex128) for _ in range(3)]
+        check(self.lib.dnsbox_get_nonlin(self._h, *[_dp(a) for a in out]))
+        return out
+
+    def step(self, nsteps: int = 1, stepper: int = 1):
+        check(self.lib.dnsbox_step(self._h, nsteps, stepper))
+
+    def time(self):
+        t = C.c_double()
+        dt = C.c_double()
+        check(self.lib.dnsbox_get_time(self._h, C.byref(t), C.byref(dt)))
+        return t.value, dt.value
+
+    def set_time(self, t: float, dt: float):
+        check(self.lib.dnsbox_set_time(self._h, t, dt))
+
+    # -- diagnostics ------------------------------------------------------------------
+    def diagnostics(self):
+        diag = np.zeros(_lib.NDIAG)
+        espec = np.zeros(self.nspec())
+        check(self.lib.dnsbox_diagnostics(self._h, _dp(diag), _dp(espec)))
+        return diag, espec
+
+    def stats(self) -> dict:
+        diag, _ = self.diagnostics()
+        return dict(zip(_lib.DIAG_NAMES, diag.tolist()))
+
+    def spectrum(self):
+        return self.diagnostics()[1]
+
+    def state_ptr(self) -> int:
+        p = C.c_void_p()
+        check(self.lib.dnsbox_state_ptr(self._h, C.byref(p)))
+        return p.value
+
+    def rhs_dev(self, in_ptr: int, out_ptr: int):
+        check(self.lib.dnsbox_rhs_dev(self._h, C.c_void_p(in_ptr), C.c_void_p(out_ptr)))

@@ -1,0 +1,924 @@
+// Kernels of one periodic box on gfx950.
+//
+// Data layout in HBM (per box, per component):
+//   spectral  complex (Nz, Ny, Nx/2+1), kx fastest   (numpy/torch rfftn layout of u[z][y][x])
+//   physical  real    (Nz, Ny, Nx),     x  fastest   (== Fortran u(Nx,Ny,Nz) of the reference)
+// The reference keeps its spectral arrays z-fastest (p3dfft STRIDE1, rhs.f90:68);
+// the host boundary converts (dnsbox_capi.hip).
+//
+// One RHS (rhs.f90:18-135) is five launches, each one pass over HBM:
+//   zinv  : z-inverse FFT of (u, v, w) with the spectral curl fused in the
+//           prologue -> six half-spectra (u, v, w, wx, wy, wz), scaled by 1/N^3
+//   ycol  : y-inverse FFT of the six fields, in place
+//   xpass : per x-line: six c2r -> (omega x u) in registers -> three r2c; the
+//           physical fields never reach HBM; max|u_i| for the CFL number is
+//           reduced on the way
+//   ycol  : y-forward FFT of the three fields, in place
+//   zfwd  : z-forward FFT + mask + projection + forcing + the stepper update
+//           (IF-Heun predictor / corrector, IF-AB2) in the epilogue
+#pragma once
+
+#include "fft_core.hpp"
+
+namespace dnsbox {
+
+// ---------------------------------------------------------------------------
+// device-resident per-box scalars (FP64)
+// ---------------------------------------------------------------------------
+struct BoxScalars {
+    double dt;          // step size of the current step
+    double dt_prev;     // step size of the previous step (CFL controller input)
+    double t;           // simulated time
+    double eband;       // band energy of the RHS input (rhs.f90:137-168)
+    double courant;     // Courant number of the last controller evaluation
+    double pad0;
+    unsigned long long maxu[3];   // max |u_i| bits (double, >= 0) from the cross-product pass
+    unsigned long long divmax;    // ordered bits of the signed max of div u
+    long long nstep;
+    double pad1[5];
+};
+
+struct BoxPhys {
+    double nu, Q, Pin, kcut, kolmo_amp;
+    double mask_thr;    // (N/2)(2/3): cubic 2/3 rule threshold  (rhs.f90:80-85)
+    double sph_thr2;    // ((N/2) 0.5)^2: spherical threshold     (rhs.f90:75-78)
+    double cmin, cmax, dtmax;
+    int kolmo_k, bandlim, spherical, parity, adaptive, pad;
+};
+
+__device__ __forceinline__ int kval(int i, int n) { return i < n / 2 ? i : (i == n / 2 ? 0 : i - n); }
+
+// rhs.f90:74-88: cubic ".ge." 2/3 rule on the table wavenumbers (Nyquist k = 0),
+// spherical option, k = 0; parity = 0 also removes the Nyquist-index modes.
+__device__ __forceinline__ bool masked(int ix, int iy, int iz, int kx, int ky, int kz, int n,
+                                       const BoxPhys& ph)
+{
+    const double k2 = double(kx * kx + ky * ky + kz * kz);
+    if (ph.spherical && k2 >= ph.sph_thr2) return true;
+    if (double(abs(kx)) >= ph.mask_thr || double(abs(ky)) >= ph.mask_thr ||
+        double(abs(kz)) >= ph.mask_thr)
+        return true;
+    if (kx == 0 && ky == 0 && kz == 0) return true;
+    if (!ph.parity && (ix == n / 2 || iy == n / 2 || iz == n / 2)) return true;
+    return false;
+}
+
+__device__ __forceinline__ unsigned long long dbl_bits(double v)
+{
+    return (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ unsigned long long ordered_bits(double v)
+{
+    unsigned long long b = dbl_bits(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// per-size configuration
+//   CP : radices of the length-N column transforms (z and y)
+//   W  : columns per tile (coalescing width)
+//   RP : radices of the length-N/2 complex row transforms (x; real data packed in pairs)
+//   RB : x-lines per block in the x pass, NTX its threads
+// ---------------------------------------------------------------------------
+template <int W_, int RB_, int NTX_, class CP_, class RP_>
+struct CfgBase {
+    using CP = CP_;
+    using RP = RP_;
+    static constexpr int W = W_;
+    static constexpr int RB = RB_;
+    static constexpr int NTX = NTX_;
+};
+template <int N, class T>
+struct Cfg;
+// FP32: radix <= 16
+template <> struct Cfg<16, float>  : CfgBase<64, 64, 256, Plan<4, 4>,      Plan<8>>     {};
+template <> struct Cfg<32, float>  : CfgBase<64, 32, 256, Plan<8, 4>,      Plan<16>>    {};
+template <> struct Cfg<48, float>  : CfgBase<64, 16, 256, Plan<12, 4>,     Plan<8, 3>>  {};
+template <> struct Cfg<64, float>  : CfgBase<32, 16, 256, Plan<8, 8>,      Plan<8, 4>>  {};
+template <> struct Cfg<128, float> : CfgBase<32, 8, 256,  Plan<16, 8>,     Plan<8, 8>>  {};
+template <> struct Cfg<256, float> : CfgBase<16, 4, 256,  Plan<16, 16>,    Plan<16, 8>> {};
+template <> struct Cfg<512, float> : CfgBase<16, 2, 256,  Plan<16, 16, 2>, Plan<16, 16>>{};
+// FP64: radix <= 8 (register budget of the fused prologue/epilogue)
+template <> struct Cfg<16, double>  : CfgBase<64, 32, 256, Plan<4, 4>,    Plan<8>>       {};
+template <> struct Cfg<32, double>  : CfgBase<64, 16, 256, Plan<8, 4>,    Plan<4, 4>>    {};
+template <> struct Cfg<48, double>  : CfgBase<32, 8, 256,  Plan<12, 4>,   Plan<8, 3>>    {};
+template <> struct Cfg<64, double>  : CfgBase<32, 8, 256,  Plan<8, 8>,    Plan<8, 4>>    {};
+template <> struct Cfg<128, double> : CfgBase<16, 4, 256,  Plan<8, 4, 4>, Plan<8, 8>>    {};
+template <> struct Cfg<256, double> : CfgBase<8, 2, 256,   Plan<8, 8, 4>, Plan<8, 4, 4>> {};
+template <> struct Cfg<512, double> : CfgBase<8, 1, 256,   Plan<8, 8, 8>, Plan<8, 8, 4>> {};
+
+template <int N, class T>
+struct Geo {
+    using C = Cfg<N, T>;
+    using CF_I = ColFFT<typename C::CP, C::W, C::W, +1, T>;
+    using CF_F = ColFFT<typename C::CP, C::W, C::W, -1, T>;
+    static constexpr int H = N / 2 + 1;                 // kx modes
+    static constexpr long long FS = (long long)N * N * H; // elements per spectral field
+    static constexpr int NT = CF_I::NT;                 // threads of a column pass
+    static constexpr int RM = CF_I::RM;
+    static constexpr int ZCOLS = N * H;                 // z-pass columns (ky, kx)
+    static constexpr int YCOLS = N * H;                 // y-pass columns (kz, kx)
+    static constexpr int ZTILES = (ZCOLS + C::W - 1) / C::W;
+    static constexpr int YTILES = (YCOLS + C::W - 1) / C::W;
+    static constexpr int XROWS = N * N;
+    static constexpr int XBLOCKS = XROWS / C::RB;
+    static_assert(XROWS % C::RB == 0, "x-lines per block must divide N^2");
+    static_assert(C::CP::L == N && C::RP::L == N / 2, "plan lengths");
+};
+
+// ===========================================================================
+// z-inverse pass with a fused spectral prologue
+// ===========================================================================
+// Prologue functors: FI input fields -> FO output fields, mode by mode.
+template <class T>
+struct ProCurl {  // (u, v, w) -> s (u, v, w, i k x u)      state_derivatives + state_vorticity
+    static constexpr int FI = 3, FO = 6;
+    T s;
+    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, int kx, int ky, int kz) const
+    {
+        const cpx<T> u = in[0], v = in[1], w = in[2];
+        switch (o) {
+            case 0: return s * u;
+            case 1: return s * v;
+            case 2: return s * w;
+            case 3: return s * mul_i<1>(T(ky) * w - T(kz) * v);  // wy - vz
+            case 4: return s * mul_i<1>(T(kz) * u - T(kx) * w);  // uz - wx
+            default: return s * mul_i<1>(T(kx) * v - T(ky) * u); // vx - uy
+        }
+    }
+};
+template <class T>
+struct ProVelDiv {  // (u, v, w) -> s (u, v, w, i k . u)       io_Courant + state_divergence
+    static constexpr int FI = 3, FO = 4;
+    T s;
+    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, int kx, int ky, int kz) const
+    {
+        if (o < 3) return s * in[o];
+        return s * mul_i<1>(T(kx) * in[0] + T(ky) * in[1] + T(kz) * in[2]);
+    }
+};
+template <class T>
+struct ProScale {  // (u, v, w) -> s (u, v, w)               state_uhat2u
+    static constexpr int FI = 3, FO = 3;
+    T s;
+    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, int, int, int) const
+    {
+        return s * in[o];
+    }
+};
+
+template <int N, class T, class Pro>
+__device__ __forceinline__ void zinv_tile(const cpx<T>* const* __restrict__ in,
+                                          cpx<T>* const* __restrict__ out, int tile,
+                                          cpx<T>* lds, const cpx<T>* __restrict__ tw, const Pro& pro)
+{
+    using G = Geo<N, T>;
+    using CF = typename G::CF_I;
+    constexpr int W = G::C::W, RM = G::RM, H = G::H;
+    constexpr long long ZS = (long long)N * H;  // stride between kz planes
+    const int tid = threadIdx.x;
+    const int c = tid % W, g = tid / W;
+    const int col = tile * W + c;
+    const bool valid = col < G::ZCOLS;
+    const int ix = col % H, iy = col / H;
+    const int kx = kval(ix, N), ky = kval(iy, N);
+    cpx<T> U[Pro::FI][RM];
+#pragma unroll
+    for (int f = 0; f < Pro::FI; ++f)
+#pragma unroll
+        for (int q = 0; q < RM; ++q)
+            U[f][q] = valid ? in[f][CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
+#pragma unroll 1
+    for (int o = 0; o < Pro::FO; ++o) {
+        cpx<T> a[RM];
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+            cpx<T> v[Pro::FI];
+#pragma unroll
+            for (int f = 0; f < Pro::FI; ++f) v[f] = U[f][q];
+            a[q] = pro(o, v, kx, ky, kval(CF::in_pos(g, q), N));
+        }
+        CF::run(a, lds, c, g, tw, N);
+        if (valid) {
+#pragma unroll
+            for (int q = 0; q < RM; ++q) out[o][CF::out_pos(g, q) * ZS + col] = a[q];
+        }
+    }
+}
+
+template <int N, class T, class Pro>
+__global__ void __launch_bounds__((Geo<N, T>::NT))
+    k_zinv(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
+           long long out_fs, const cpx<T>* __restrict__ tw, Pro pro)
+{
+    using G = Geo<N, T>;
+    __shared__ cpx<T> lds[N * G::C::W];
+    const cpx<T>* in[Pro::FI];
+    cpx<T>* out[Pro::FO];
+#pragma unroll
+    for (int f = 0; f < Pro::FI; ++f) in[f] = in0 + f * in_fs;
+#pragma unroll
+    for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
+    zinv_tile<N, T, Pro>(in, out, blockIdx.x, lds, tw, pro);
+}
+
+// ===========================================================================
+// y pass (plain, in place), one field per blockIdx.y
+// ===========================================================================
+template <int N, class T, int DIR>
+__device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, cpx<T>* lds,
+                                          const cpx<T>* __restrict__ tw)
+{
+    using G = Geo<N, T>;
+    using CF = ColFFT<typename G::C::CP, G::C::W, G::C::W, DIR, T>;
+    constexpr int W = G::C::W, RM = G::RM, H = G::H;
+    const int tid = threadIdx.x;
+    const int c = tid % W, g = tid / W;
+    const int col = tile * W + c;
+    const bool valid = col < G::YCOLS;
+    const int ix = col % H, iz = col / H;
+    cpx<T>* base = data + (long long)iz * N * H + ix;
+    cpx<T> a[RM];
+#pragma unroll
+    for (int q = 0; q < RM; ++q) a[q] = valid ? base[CF::in_pos(g, q) * H] : cpx<T>{T(0), T(0)};
+    CF::run(a, lds, c, g, tw, N);
+    if (valid) {
+#pragma unroll
+        for (int q = 0; q < RM; ++q) base[CF::out_pos(g, q) * H] = a[q];
+    }
+}
+
+template <int N, class T, int DIR>
+__global__ void __launch_bounds__((Geo<N, T>::NT))
+    k_ycol(cpx<T>* __restrict__ data0, long long fs, const cpx<T>* __restrict__ tw)
+{
+    using G = Geo<N, T>;
+    __shared__ cpx<T> lds[N * G::C::W];
+    ycol_tile<N, T, DIR>(data0 + blockIdx.y * fs, blockIdx.x, lds, tw);
+}
+
+// ===========================================================================
+// x pass: FI inputs (half-spectrum lines -> c2r, or real lines) -> pointwise op
+// in physical space -> FO outputs (r2c half-spectrum lines, or real lines)
+// ===========================================================================
+struct XRed {  // reductions carried by the x pass
+    double m[4];
+};
+
+template <class T>
+struct OpCross {  // rhs.f90:39-65: (omega x u); io_Courant's max |u_i| on the way
+    static constexpr int FI = 6, FO = 3, NRED = 3;
+    static constexpr bool IN_SPEC = true, OUT_SPEC = true;
+    __device__ __forceinline__ static void apply(const T* in, T* out, XRed& r)
+    {
+        const T u = in[0], v = in[1], w = in[2], ox = in[3], oy = in[4], oz = in[5];
+        out[0] = oy * w - oz * v;
+        out[1] = oz * u - ox * w;
+        out[2] = ox * v - oy * u;
+        r.m[0] = fmax(r.m[0], (double)fabs(u));
+        r.m[1] = fmax(r.m[1], (double)fabs(v));
+        r.m[2] = fmax(r.m[2], (double)fabs(w));
+    }
+};
+template <class T>
+struct OpVelDivMax {  // max|u|, max|v|, max|w|, signed max(div u); no output
+    static constexpr int FI = 4, FO = 0, NRED = 4;
+    static constexpr bool IN_SPEC = true, OUT_SPEC = true;
+    __device__ __forceinline__ static void apply(const T* in, T*, XRed& r)
+    {
+        r.m[0] = fmax(r.m[0], (double)fabs(in[0]));
+        r.m[1] = fmax(r.m[1], (double)fabs(in[1]));
+        r.m[2] = fmax(r.m[2], (double)fabs(in[2]));
+        r.m[3] = fmax(r.m[3], (double)in[3]);
+    }
+};
+template <class T, int F, bool INS, bool OUTS>
+struct OpCopy {
+    static constexpr int FI = F, FO = F, NRED = 0;
+    static constexpr bool IN_SPEC = INS, OUT_SPEC = OUTS;
+    __device__ __forceinline__ static void apply(const T* in, T* out, XRed&)
+    {
+#pragma unroll
+        for (int f = 0; f < F; ++f) out[f] = in[f];
+    }
+};
+
+template <int N, class T, class Op>
+struct XGeo {
+    static constexpr int M = N / 2, H = M + 1, HP = H;
+    static constexpr int RB = Cfg<N, T>::RB;
+    static constexpr int NTX = Cfg<N, T>::NTX;
+    static constexpr int FMAX = Op::FI > Op::FO ? Op::FI : Op::FO;
+    static constexpr int LDS = FMAX * RB * HP;
+};
+
+template <int N, class T, class Op>
+__device__ __forceinline__ void xpass_tile(const void* const* __restrict__ in,
+                                           void* const* __restrict__ out, int blk, cpx<T>* lds,
+                                           const cpx<T>* __restrict__ tw, XRed& red)
+{
+    using X = XGeo<N, T, Op>;
+    constexpr int M = X::M, H = X::H, HP = X::HP, RB = X::RB, NTX = X::NTX;
+    constexpr int FI = Op::FI, FO = Op::FO;
+    const int tid = threadIdx.x;
+    const long long row0 = (long long)blk * RB;
+
+    // ---- load (+ c2r pre-processing) ----
+    if constexpr (Op::IN_SPEC) {
+        for (int e = tid; e < FI * RB * H; e += NTX) {
+            const int f = e / (RB * H), rk = e % (RB * H);
+            lds[f * RB * HP + (rk / H) * HP + rk % H] =
+                static_cast<const cpx<T>*>(in[f])[row0 * H + rk];
+        }
+        __syncthreads();
+        // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),  k < M;
+        // imaginary parts of X[0], X[M] are ignored (c2r convention).
+        constexpr int NI = FI * RB * M;
+        constexpr int Q = (NI + NTX - 1) / NTX;
+        cpx<T> z[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int e = tid + q * NTX;
+            if (NI % NTX == 0 || e < NI) {
+                const int fr = e / M, k = e % M;
+                const cpx<T>* row = lds + fr * HP;
+                cpx<T> A = row[k], B = conj(row[M - k]);
+                if (k == 0) {
+                    A.y = T(0);
+                    B.y = T(0);
+                }
+                cpx<T> w = tw[k];
+                w.y = -w.y;
+                cpx<T> fe = A + B, fo = (A - B) * w;
+                z[q] = fe + mul_i<1>(fo);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int e = tid + q * NTX;
+            if (NI % NTX == 0 || e < NI) lds[(e / M) * HP + e % M] = z[q];
+        }
+        __syncthreads();
+        RowFFT<typename Cfg<N, T>::RP, FI * RB, HP, NTX, +1, T>::run(lds, tid, tw, N);
+    } else {
+        for (int e = tid; e < FI * RB * M; e += NTX) {
+            const int f = e / (RB * M), rm = e % (RB * M);
+            lds[f * RB * HP + (rm / M) * HP + rm % M] =
+                static_cast<const cpx<T>*>(in[f])[row0 * M + rm];
+        }
+        __syncthreads();
+    }
+
+    // ---- pointwise op on the physical samples x[2m] = Re z[m], x[2m+1] = Im z[m] ----
+    for (int e = tid; e < RB * M; e += NTX) {
+        const int r = e / M, m = e % M;
+        T vr[FI], vi[FI], orr[FO > 0 ? FO : 1], oi[FO > 0 ? FO : 1];
+#pragma unroll
+        for (int f = 0; f < FI; ++f) {
+            cpx<T> v = lds[f * RB * HP + r * HP + m];
+            vr[f] = v.x;
+            vi[f] = v.y;
+        }
+        Op::apply(vr, orr, red);
+        Op::apply(vi, oi, red);
+#pragma unroll
+        for (int f = 0; f < FO; ++f) lds[f * RB * HP + r * HP + m] = cpx<T>{orr[f], oi[f]};
+    }
+    if constexpr (FO == 0) return;
+    __syncthreads();
+
+    // ---- r2c or real store ----
+    if constexpr (FO == 0) {
+    } else if constexpr (Op::OUT_SPEC) {
+        RowFFT<typename Cfg<N, T>::RP, FO * RB, HP, NTX, -1, T>::run(lds, tid, tw, N);
+        // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]), k <= M
+        for (int e = tid; e < FO * RB * H; e += NTX) {
+            const int f = e / (RB * H), rk = e % (RB * H);
+            const int r = rk / H, k = rk % H;
+            const cpx<T>* row = lds + f * RB * HP + r * HP;
+            const cpx<T> zk = row[k == M ? 0 : k];
+            const cpx<T> zm = conj(row[k == 0 ? 0 : M - k]);
+            const cpx<T> a = T(0.5) * (zk + zm);
+            const cpx<T> b = (T(0.5) * (zk - zm)) * tw[k];
+            static_cast<cpx<T>*>(out[f])[row0 * H + rk] = a + mul_i<-1>(b);
+        }
+    } else {
+        for (int e = tid; e < FO * RB * M; e += NTX) {
+            const int f = e / (RB * M), rm = e % (RB * M);
+            static_cast<cpx<T>*>(out[f])[row0 * M + rm] = lds[f * RB * HP + (rm / M) * HP + rm % M];
+        }
+    }
+}
+
+template <int NRED>
+__device__ __forceinline__ void block_max_atomic(XRed& red, unsigned long long* dst_bits,
+                                                 bool signed_last, double* wsm)
+{
+    // dst_bits[j]: max of |.| (plain bits) or, for the last entry when signed_last, ordered bits
+    const int lane = threadIdx.x % 64, wid = threadIdx.x / 64, nw = blockDim.x / 64;
+#pragma unroll
+    for (int j = 0; j < NRED; ++j) {
+        double v = wave_max(red.m[j]);
+        if (lane == 0) wsm[j * 16 + wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NRED) {
+        const int j = threadIdx.x;
+        double v = wsm[j * 16];
+        for (int i = 1; i < nw; ++i) v = fmax(v, wsm[j * 16 + i]);
+        if (signed_last && j == NRED - 1) atomicMax(dst_bits + j, ordered_bits(v));
+        else atomicMax(dst_bits + j, dbl_bits(v));
+    }
+}
+
+template <int N, class T, class Op>
+__global__ void __launch_bounds__((Cfg<N, T>::NTX))
+    k_xpass(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
+            long long out_fs, const cpx<T>* __restrict__ tw, unsigned long long* red_bits,
+            int signed_last)
+{
+    using X = XGeo<N, T, Op>;
+    __shared__ cpx<T> lds[X::LDS];
+    __shared__ double wsm[4 * 16];
+    const void* in[Op::FI];
+    void* out[Op::FO > 0 ? Op::FO : 1];
+#pragma unroll
+    for (int f = 0; f < Op::FI; ++f) in[f] = in0 + f * in_fs;
+#pragma unroll
+    for (int f = 0; f < Op::FO; ++f) out[f] = out0 + f * out_fs;
+    XRed red;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) red.m[j] = Op::NRED == 4 && j == 3 ? -1e300 : 0.0;
+    xpass_tile<N, T, Op>(in, out, blockIdx.x, lds, tw, red);
+    if constexpr (Op::NRED > 0) {
+        if (red_bits) block_max_atomic<Op::NRED>(red, red_bits, signed_last != 0, wsm);
+    }
+}
+
+// ===========================================================================
+// z-forward pass with the stepper epilogue
+// ===========================================================================
+enum EpiMode { EPI_RHS = 0, EPI_PRED = 1, EPI_CORR = 2, EPI_AB2 = 3, EPI_STORE = 4 };
+
+template <class T>
+struct EpiArgs {
+    cpx<T>* out;          // EPI_RHS: N(u);  EPI_PRED: predicted state;  EPI_STORE: destination
+    cpx<T>* state;        // EPI_PRED/CORR/AB2: the state (updated in place); band forcing input
+    cpx<T>* nold;         // EPI_AB2: N_{n-1} (read, then overwritten with N_n); EPI_PRED: N_0 if set
+    const cpx<T>* rhs_in; // RHS input u (band forcing); may alias state
+    long long fs;         // field stride of out/state/nold/rhs_in
+    BoxScalars* sc;
+    BoxPhys ph;
+};
+
+// dnsboxheun.f90:227-246 set_time_step, triggered as at :153-157
+__device__ __forceinline__ double cfl_dt(double c, double dt, const BoxPhys& ph)
+{
+    if (c > ph.cmax || (c < ph.cmin && dt < ph.dtmax)) {
+        dt = ((ph.cmax + ph.cmin) / (2.0 * c)) * dt;
+        if (dt > ph.dtmax) dt = ph.dtmax;
+    }
+    return dt;
+}
+
+template <int N>
+__device__ __forceinline__ double step_dt(BoxScalars* sc, const BoxPhys& ph)
+{
+    double dt = sc->dt_prev;
+    if (ph.adaptive) {
+        const double dx = 6.283185307179586476925286766559 / double(N);
+        const double c = (__longlong_as_double((long long)sc->maxu[0]) +
+                          __longlong_as_double((long long)sc->maxu[1]) +
+                          __longlong_as_double((long long)sc->maxu[2])) * dt / dx;
+        dt = cfl_dt(c, dt, ph);
+    }
+    return dt;
+}
+
+template <int N, class T, int MODE>
+__device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long long in_fs, int tile,
+                                          cpx<T>* lds, const cpx<T>* __restrict__ tw,
+                                          const EpiArgs<T>& ea)
+{
+    using G = Geo<N, T>;
+    using CF = typename G::CF_F;
+    constexpr int W = G::C::W, RM = G::RM, H = G::H;
+    constexpr long long ZS = (long long)N * H;
+    const int tid = threadIdx.x;
+    const int c = tid % W, g = tid / W;
+    const int col = tile * W + c;
+    const bool valid = col < G::ZCOLS;
+    const int ix = col % H, iy = col / H;
+    const int kx = kval(ix, N), ky = kval(iy, N);
+
+    cpx<T> O[3][RM];
+#pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+        cpx<T> a[RM];
+        const cpx<T>* src = in0 + f * in_fs;
+#pragma unroll
+        for (int q = 0; q < RM; ++q) a[q] = valid ? src[CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
+        CF::run(a, lds, c, g, tw, N);
+#pragma unroll
+        for (int q = 0; q < RM; ++q) O[f][q] = a[q];
+    }
+    if (!valid) return;
+
+    const BoxPhys& ph = ea.ph;
+    double dt = 0.0;
+    if constexpr (MODE == EPI_PRED) dt = step_dt<N>(ea.sc, ph);
+    if constexpr (MODE == EPI_CORR || MODE == EPI_AB2) dt = ea.sc->dt;
+    const double n3 = double(N) * double(N) * double(N);
+    double fband = 0.0;
+    if (ph.bandlim && MODE != EPI_STORE) fband = ph.Pin / (2.0 * ea.sc->eband);
+
+#pragma unroll
+    for (int q = 0; q < RM; ++q) {
+        const int iz = CF::out_pos(g, q);
+        const int kz = kval(iz, N);
+        const long long idx = (long long)iz * ZS + col;
+        cpx<T> nl[3];
+        if constexpr (MODE == EPI_STORE) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) ea.out[f * ea.fs + idx] = O[f][q];
+            continue;
+        } else {
+            if (masked(ix, iy, iz, kx, ky, kz, N, ph)) {
+#pragma unroll
+                for (int f = 0; f < 3; ++f) nl[f] = cpx<T>{T(0), T(0)};
+            } else {
+                // phat = i (k . n) / |k|^2 ;  N = -n - (i k) phat = -n + k (k . n)/|k|^2
+                const int k2 = kx * kx + ky * ky + kz * kz;
+                const T inv = T(1) / T(k2);
+                const cpx<T> kn = T(kx) * O[0][q] + T(ky) * O[1][q] + T(kz) * O[2][q];
+                const cpx<T> pk = inv * kn;
+                nl[0] = T(kx) * pk - O[0][q];
+                nl[1] = T(ky) * pk - O[1][q];
+                nl[2] = T(kz) * pk - O[2][q];
+                if (ph.bandlim && sqrt(double(k2)) <= ph.kcut) {  // rhs.f90:109-118
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) nl[f] = nl[f] + T(fband) * ea.rhs_in[f * ea.fs + idx];
+                }
+                if (ph.kolmo_k > 0 && ix == 0 && iz == 0) {  // f = amp sin(kf y) e_x
+                    if (iy == ph.kolmo_k) nl[0].y -= T(0.5 * ph.kolmo_amp * n3);
+                    if (iy == N - ph.kolmo_k) nl[0].y += T(0.5 * ph.kolmo_amp * n3);
+                }
+            }
+        }
+        if constexpr (MODE == EPI_RHS) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) ea.out[f * ea.fs + idx] = nl[f];
+        } else {
+            // rhs.f90:8-16 integrating factor, exponent in FP64
+            const double k2d = double(kx * kx + ky * ky + kz * kz);
+            const double e64 = exp((-ph.nu * k2d + ph.Q) * dt);
+            const T e = T(e64), tdt = T(dt);
+            if constexpr (MODE == EPI_PRED) {  // dnsboxheun.f90:88-114
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<T> u = ea.state[f * ea.fs + idx];
+                    ea.out[f * ea.fs + idx] = e * (u + tdt * nl[f]);
+                    ea.state[f * ea.fs + idx] = e * (u + (tdt * T(0.5)) * nl[f]);
+                    if (ea.nold) ea.nold[f * ea.fs + idx] = nl[f];
+                }
+            } else if constexpr (MODE == EPI_CORR) {  // dnsboxheun.f90:120-126
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<T> u = ea.state[f * ea.fs + idx];
+                    ea.state[f * ea.fs + idx] = u + (tdt * T(0.5)) * nl[f];
+                }
+            } else if constexpr (MODE == EPI_AB2) {
+                const T e2 = T(e64 * e64);
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<T> u = ea.state[f * ea.fs + idx];
+                    const cpx<T> no = ea.nold[f * ea.fs + idx];
+                    ea.state[f * ea.fs + idx] =
+                        e * u + tdt * (T(1.5) * e * nl[f] - T(0.5) * e2 * no);
+                    ea.nold[f * ea.fs + idx] = nl[f];
+                }
+            }
+        }
+    }
+}
+
+template <int N, class T, int MODE>
+__global__ void __launch_bounds__((Geo<N, T>::NT))
+    k_zfwd(const cpx<T>* __restrict__ in0, long long in_fs, const cpx<T>* __restrict__ tw,
+           EpiArgs<T> ea)
+{
+    using G = Geo<N, T>;
+    __shared__ cpx<T> lds[N * G::C::W];
+    zfwd_tile<N, T, MODE>(in0, in_fs, blockIdx.x, lds, tw, ea);
+    if constexpr (MODE == EPI_PRED) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            const double dt = step_dt<N>(ea.sc, ea.ph);
+            if (ea.ph.adaptive) {
+                const double dx = 6.283185307179586476925286766559 / double(N);
+                ea.sc->courant = (__longlong_as_double((long long)ea.sc->maxu[0]) +
+                                  __longlong_as_double((long long)ea.sc->maxu[1]) +
+                                  __longlong_as_double((long long)ea.sc->maxu[2])) * ea.sc->dt_prev / dx;
+            }
+            ea.sc->dt = dt;
+        }
+    }
+}
+
+// end of step bookkeeping: t += dt, dt_prev = dt, clear the CFL maxima
+template <int D = 0>
+__global__ void k_step_end(BoxScalars* sc)
+{
+    if (threadIdx.x == 0) {
+        sc->t += sc->dt;
+        sc->dt_prev = sc->dt;
+        sc->maxu[0] = sc->maxu[1] = sc->maxu[2] = 0ull;
+        sc->nstep += 1;
+    }
+}
+
+// ===========================================================================
+// spectral reductions: Ekin, dissipation, Eband, EZero/EOne/Esqrt2, shell spectrum
+// (state.f90:350-454, rhs.f90:137-168; dissipation by Parseval from the spectrum)
+// ===========================================================================
+enum { DG_EKIN = 0, DG_DISP, DG_EZERO, DG_EONE, DG_ESQRT2, DG_EBAND, DG_NSCAL };
+
+template <int N, class T>
+__global__ void __launch_bounds__(256)
+    k_spec_reduce(const cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, double deltak,
+                  int nspec, double* __restrict__ partial /* [gridDim.x][DG_NSCAL + nspec] */)
+{
+    constexpr int H = N / 2 + 1;
+    constexpr long long TOT = (long long)N * N * H;
+    extern __shared__ double bins[];  // nspec
+    __shared__ double wsm[DG_NSCAL][4];
+    for (int i = threadIdx.x; i < nspec; i += blockDim.x) bins[i] = 0.0;
+    __syncthreads();
+    // per-thread compensated sums
+    double s[DG_NSCAL], cmp[DG_NSCAL];
+#pragma unroll
+    for (int j = 0; j < DG_NSCAL; ++j) s[j] = cmp[j] = 0.0;
+    const double sqrt2 = 1.4142135623730951;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < TOT;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int ix = int(idx % H);
+        const int iy = int((idx / H) % N);
+        const int iz = int(idx / ((long long)H * N));
+        const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
+        double e = 0.0;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const cpx<T> a = u0[f * fs + idx];
+            e += double(a.x) * double(a.x) + double(a.y) * double(a.y);
+        }
+        const double k2 = double(kx * kx + ky * ky + kz * kz);
+        const double absk = sqrt(k2);
+        const double wx = (kx == 0) ? 0.5 : 1.0;
+        double v[DG_NSCAL];
+        v[DG_EKIN] = wx * e;
+        v[DG_DISP] = 2.0 * wx * k2 * e;
+        v[DG_EZERO] = absk == 0.0 ? e : 0.0;
+        v[DG_EONE] = absk == 1.0 ? e : 0.0;
+        v[DG_ESQRT2] = (absk - sqrt2 < 1.0e-9) ? e : 0.0;  // state.f90:383 as written
+        v[DG_EBAND] = (absk <= ph.kcut) ? wx * e : 0.0;
+#pragma unroll
+        for (int j = 0; j < DG_NSCAL; ++j) {  // Kahan
+            const double y = v[j] - cmp[j];
+            const double t = s[j] + y;
+            cmp[j] = (t - s[j]) - y;
+            s[j] = t;
+        }
+        const int iq = int(absk / deltak);
+        const int nk = (absk < double(float(iq)) + 0.5 * deltak) ? iq : iq + 1;
+        if (nk < nspec && nk != 0) atomicAdd(&bins[nk - 1], e);
+    }
+    const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+#pragma unroll
+    for (int j = 0; j < DG_NSCAL; ++j) {
+        const double v = wave_sum(s[j]);
+        if (lane == 0) wsm[j][wid] = v;
+    }
+    __syncthreads();
+    double* out = partial + (long long)blockIdx.x * (DG_NSCAL + nspec);
+    if (threadIdx.x < DG_NSCAL) {
+        const int j = threadIdx.x;
+        out[j] = ((wsm[j][0] + wsm[j][1]) + wsm[j][2]) + wsm[j][3];
+    }
+    for (int i = threadIdx.x; i < nspec; i += blockDim.x) out[DG_NSCAL + i] = bins[i];
+}
+
+// sums the per-block partials in a fixed order (Kahan), scales, writes [DG_NSCAL + nspec]
+template <int D = 0>
+__global__ void k_spec_finish(const double* __restrict__ partial, int nblk, int nslot,
+                              double sm2, double nu, double deltak, double* __restrict__ out,
+                              BoxScalars* sc)
+{
+    for (int j = threadIdx.x; j < nslot; j += blockDim.x) {
+        double s = 0.0, c = 0.0;
+        for (int b = 0; b < nblk; ++b) {
+            const double y = partial[(long long)b * nslot + j] - c;
+            const double t = s + y;
+            c = (t - s) - y;
+            s = t;
+        }
+        double r = s;
+        if (j == DG_EKIN || j == DG_EBAND) r = s * sm2;
+        else if (j == DG_DISP) r = s * sm2 * nu;
+        else if (j >= DG_NSCAL) r = s * sm2 / deltak;
+        out[j] = r;
+        if (sc && j == DG_EBAND) sc->eband = r;
+    }
+}
+
+// band energy of a state, one block over the low modes (rhs.f90:137-168).  The
+// table wavenumber decides membership, so with the reference k-table the
+// Nyquist index (k = 0) counts as low: per axis the candidate indices are
+// |k| <= K plus the Nyquist index.
+template <int N, class T>
+__global__ void __launch_bounds__(256)
+    k_eband(const cpx<T>* __restrict__ u0, long long fs, double kcut, double sm2, BoxScalars* sc)
+{
+    constexpr int H = N / 2 + 1;
+    int K = int(kcut);
+    if (K > N / 2 - 1) K = N / 2 - 1;
+    const int nxc = K + 2;          // kx indices 0..K, N/2
+    const int nyc = 2 * K + 2;      // ky indices 0..K, N-K..N-1, N/2
+    const int tot = nxc * nyc * nyc;
+    double s = 0.0;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int a = e / (nyc * nyc), b = (e / nyc) % nyc, c = e % nyc;
+        const int ix = a <= K ? a : N / 2;
+        const int iy = b <= K ? b : (b < 2 * K + 1 ? N - (b - K) : N / 2);
+        const int iz = c <= K ? c : (c < 2 * K + 1 ? N - (c - K) : N / 2);
+        const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
+        const double absk = sqrt(double(kx * kx + ky * ky + kz * kz));
+        if (absk > kcut) continue;
+        const long long idx = ((long long)iz * N + iy) * H + ix;
+        double en = 0.0;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const cpx<T> v = u0[f * fs + idx];
+            en += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+        }
+        s += (kx == 0 ? 0.5 : 1.0) * en;
+    }
+    __shared__ double wsm[16];
+    const double v = wave_sum(s);
+    if (threadIdx.x % 64 == 0) wsm[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < int(blockDim.x / 64); ++i) t += wsm[i];
+        sc->eband = t * sm2;
+    }
+}
+
+// ===========================================================================
+// seeded solenoidal initial field (same generator as the CPU oracle)
+// ===========================================================================
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ unsigned long long hash3(unsigned long long seed, unsigned long long box,
+                                                    unsigned long long idx)
+{
+    return splitmix64(seed ^ splitmix64(box ^ splitmix64(idx)));
+}
+
+template <int N, class T>
+__global__ void __launch_bounds__(256)
+    k_init_solenoidal(cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, unsigned long long seed,
+                      int box, double k0)
+{
+    constexpr int H = N / 2 + 1;
+    constexpr long long TOT = (long long)N * N * H;
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= TOT) return;
+    const int ix = int(idx % H);
+    const int iy = int((idx / H) % N);
+    const int iz = int(idx / ((long long)H * N));
+    cpx<T> outv[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
+    bool keep = !(ix == N / 2 || iy == N / 2 || iz == N / 2);
+    int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
+    if (keep && masked(ix, iy, iz, kx, ky, kz, N, ph)) keep = false;
+    if (keep) {
+        bool conjugate = false;
+        int cy = iy, cz = iz;
+        if (ix == 0 && !(ky > 0 || (ky == 0 && kz > 0))) {
+            conjugate = true;
+            cy = (N - iy) % N;
+            cz = (N - iz) % N;
+            ky = -ky;
+            kz = -kz;
+        }
+        // canonical index in the oracle's [kx][ky][kz] ordering
+        const unsigned long long cidx = ((unsigned long long)ix * N + cy) * N + cz;
+        double ar[3], ai[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const unsigned long long h = hash3(seed, (unsigned long long)box, 3 * cidx + c);
+            const double phs = 2.0 * 3.14159265358979323846 * (double(h >> 11) * (1.0 / 9007199254740992.0));
+            ar[c] = cos(phs);
+            ai[c] = sin(phs);
+        }
+        const double k2 = double(kx) * kx + double(ky) * ky + double(kz) * kz;
+        const double kdr = (kx * ar[0] + ky * ar[1] + kz * ar[2]) / k2;
+        const double kdi = (kx * ai[0] + ky * ai[1] + kz * ai[2]) / k2;
+        const double kk[3] = {double(kx), double(ky), double(kz)};
+        double na2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ar[c] -= kk[c] * kdr;
+            ai[c] -= kk[c] * kdi;
+            na2 += ar[c] * ar[c] + ai[c] * ai[c];
+        }
+        const double na = sqrt(na2);
+        if (na >= 1e-300) {
+            const double kn = sqrt(k2);
+            const double ek = kn * kn * kn * kn * exp(-2.0 * (kn / k0) * (kn / k0));
+            const double amp = sqrt(ek / (4.0 * 3.14159265358979323846 * k2)) / na;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                outv[c] = cpx<T>{T(ar[c] * amp), T((conjugate ? -ai[c] : ai[c]) * amp)};
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u0[c * fs + idx] = outv[c];
+}
+
+// u *= sqrt(target / Ekin), Ekin read from device memory (no host round trip)
+template <class T>
+__global__ void k_scale_to_energy(cpx<T>* __restrict__ u0, long long n, const double* ekin,
+                                  double target)
+{
+    const double e = *ekin;
+    const T s = T(e > 0.0 ? sqrt(target / e) : 0.0);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        cpx<T> a = u0[i];
+        u0[i] = cpx<T>{s * a.x, s * a.y};
+    }
+}
+
+// state.f90:120-179 state_project / state.f90:181-209 state_dealias, in place
+template <int N, class T>
+__global__ void __launch_bounds__(256)
+    k_project_dealias(cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, int do_dealias,
+                      int do_project)
+{
+    constexpr int H = N / 2 + 1;
+    constexpr long long TOT = (long long)N * N * H;
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= TOT) return;
+    const int ix = int(idx % H);
+    const int iy = int((idx / H) % N);
+    const int iz = int(idx / ((long long)H * N));
+    const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
+    cpx<T> u = u0[idx], v = u0[fs + idx], w = u0[2 * fs + idx];
+    if (do_project && !(kx == 0 && ky == 0 && kz == 0)) {
+        const T inv = T(1) / T(kx * kx + ky * ky + kz * kz);
+        const cpx<T> kd = inv * (T(kx) * u + T(ky) * v + T(kz) * w);
+        u = u - T(kx) * kd;
+        v = v - T(ky) * kd;
+        w = w - T(kz) * kd;
+    }
+    if (do_dealias) {
+        // state.f90:184-197: spherical uses (N/3)^2 here
+        const double thr = ph.mask_thr;
+        const double k2 = double(kx * kx + ky * ky + kz * kz);
+        bool m = (ph.spherical && k2 >= thr * thr) || double(abs(kx)) >= thr ||
+                 double(abs(ky)) >= thr || double(abs(kz)) >= thr || (kx == 0 && ky == 0 && kz == 0) ||
+                 (!ph.parity && (ix == N / 2 || iy == N / 2 || iz == N / 2));
+        if (m) u = v = w = cpx<T>{T(0), T(0)};
+    }
+    u0[idx] = u;
+    u0[fs + idx] = v;
+    u0[2 * fs + idx] = w;
+}
+
+template <class TO, class TI>
+__global__ void k_convert(TO* __restrict__ dst, const TI* __restrict__ src, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = TO(src[i]);
+}
+
+}  // namespace dnsbox

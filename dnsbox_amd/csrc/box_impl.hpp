@@ -1,0 +1,350 @@
+// Host-side launchers of one box: owns the device buffers of a box and
+// sequences the kernel passes of box.hpp.  Instantiated per (N, precision) in
+// box_f32.hip / box_f64.hip; the C-ABI (dnsbox_capi.hip) sees only BoxBase.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "box.hpp"
+
+namespace dnsbox {
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define DNSBOX_HIP(call)                                                                 \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::dnsbox::HipError(std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define DNSBOX_LAUNCH_CHECK() DNSBOX_HIP(hipGetLastError())
+
+struct BoxConfig {
+    int n = 32;
+    int precision = 64;
+    BoxPhys ph{};
+    double deltak = 1.0;
+    double dt = 1e-3;
+};
+
+// Per-box device buffers and the operations the C-ABI exposes.
+class BoxBase {
+   public:
+    virtual ~BoxBase() = default;
+
+    BoxConfig cfg;
+    hipStream_t stream = nullptr;
+    int nspec = 0;
+    long long fs = 0;        // complex elements per spectral field
+    size_t cbytes = 0;       // bytes per complex element
+    size_t rbytes = 0;       // bytes per real element
+
+    // device buffers
+    void* state = nullptr;   // 3 fields
+    void* pred = nullptr;    // 3 fields (IF-Heun predicted state)
+    void* scratch = nullptr; // 6 fields
+    void* nonlin = nullptr;  // 3 fields
+    void* nold = nullptr;    // 3 fields (AB2 history), lazily
+    void* tw = nullptr;      // twiddle table
+    BoxScalars* sc = nullptr;
+    double* red_partial = nullptr;
+    double* red_out = nullptr;
+    unsigned long long* red_bits = nullptr;  // 4 slots
+    int red_blocks = 0;
+    bool ab2_started = false;
+
+    virtual void rhs(const void* in, void* out) = 0;
+    virtual void heun_step() = 0;
+    virtual void ab2_step() = 0;
+    virtual void spec_reduce(const void* u) = 0;  // -> red_out[DG_NSCAL + nspec]
+    virtual void phys_reduce(const void* u) = 0;  // -> red_bits[0..3]
+    virtual void init_random(unsigned long long seed, int box, double k0, double energy) = 0;
+    virtual void project(int dealias, int proj) = 0;
+    virtual void to_physical(const void* spec, void* phys3) = 0;   // device real (3, N^3)
+    virtual void from_physical(const void* phys3, void* spec) = 0; // device real (3, N^3)
+    virtual void convert_in(void* dst, const double* src, long long n) = 0;   // host-staged
+    virtual void convert_out(double* dst, const void* src, long long n) = 0;
+};
+
+BoxBase* make_box_f32(const BoxConfig& c);
+BoxBase* make_box_f64(const BoxConfig& c);
+
+template <int N, class T>
+class BoxT final : public BoxBase {
+    using G = Geo<N, T>;
+    using C = cpx<T>;
+    static constexpr int NT = G::NT;
+
+   public:
+    explicit BoxT(const BoxConfig& c)
+    {
+        cfg = c;
+        fs = G::FS;
+        cbytes = sizeof(C);
+        rbytes = sizeof(T);
+        const double ax = (double(N) / 2.0) * (2.0 * 1.0 / 3.0);
+        if (cfg.ph.spherical) nspec = int(ax / cfg.deltak) - 1;
+        else nspec = int(std::sqrt(3.0 * ax * ax) / cfg.deltak) - 1;
+        if (nspec < 1) nspec = 1;
+        const size_t fb = size_t(fs) * sizeof(C);
+        try {
+            DNSBOX_HIP(hipMalloc(&state, 3 * fb));
+            DNSBOX_HIP(hipMalloc(&pred, 3 * fb));
+            DNSBOX_HIP(hipMalloc(&scratch, 6 * fb));
+            DNSBOX_HIP(hipMalloc(&nonlin, 3 * fb));
+            DNSBOX_HIP(hipMalloc(&tw, N * sizeof(C)));
+            DNSBOX_HIP(hipMalloc(&sc, sizeof(BoxScalars)));
+            red_blocks = 512;
+            DNSBOX_HIP(hipMalloc(&red_partial, size_t(red_blocks) * (DG_NSCAL + nspec) * sizeof(double)));
+            DNSBOX_HIP(hipMalloc(&red_out, (DG_NSCAL + nspec) * sizeof(double)));
+            DNSBOX_HIP(hipMalloc(&red_bits, 4 * sizeof(unsigned long long)));
+            DNSBOX_HIP(hipMemset(state, 0, 3 * fb));
+            DNSBOX_HIP(hipMemset(pred, 0, 3 * fb));
+            DNSBOX_HIP(hipMemset(nonlin, 0, 3 * fb));
+            std::vector<C> h(N);
+            for (int j = 0; j < N; ++j) {
+                long double a = 6.283185307179586476925286766559005768L * (long double)j / (long double)N;
+                h[j].x = T(std::cos(a));
+                h[j].y = T(-std::sin(a));
+            }
+            DNSBOX_HIP(hipMemcpy(tw, h.data(), N * sizeof(C), hipMemcpyHostToDevice));
+            BoxScalars s{};
+            s.dt = s.dt_prev = cfg.dt;
+            DNSBOX_HIP(hipMemcpy(sc, &s, sizeof(s), hipMemcpyHostToDevice));
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~BoxT() override { release(); }
+
+    void release()
+    {
+        void* ps[] = {state, pred, scratch, nonlin, nold, tw, sc, red_partial, red_out, red_bits};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        state = pred = scratch = nonlin = nold = tw = nullptr;
+        sc = nullptr;
+        red_partial = red_out = nullptr;
+        red_bits = nullptr;
+    }
+
+    const C* twc() const { return static_cast<const C*>(tw); }
+    C* X() const { return static_cast<C*>(scratch); }
+
+    // five passes of one RHS; epilogue selected by MODE
+    template <int MODE>
+    void rhs_passes(const C* in, const EpiArgs<T>& ea, bool track_cfl)
+    {
+        const T s = T(1.0 / (double(N) * N * N));
+        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(NT), 0, stream, in, fs,
+                           X(), fs, twc(), ProCurl<T>{s});
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(NT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
+                           stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(NT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_zfwd<N, T, MODE>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs, twc(),
+                           ea);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    EpiArgs<T> epi() const
+    {
+        EpiArgs<T> ea{};
+        ea.fs = fs;
+        ea.sc = sc;
+        ea.ph = cfg.ph;
+        return ea;
+    }
+
+    void eband(const void* u)
+    {
+        if (!cfg.ph.bandlim) return;
+        const double sm = 1.0 / (double(N) * N * N);
+        hipLaunchKernelGGL((k_eband<N, T>), dim3(1), dim3(256), 0, stream, static_cast<const C*>(u), fs,
+                           cfg.ph.kcut, sm * sm, sc);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void rhs(const void* in, void* out) override
+    {
+        eband(in);
+        EpiArgs<T> ea = epi();
+        ea.out = static_cast<C*>(out);
+        ea.rhs_in = static_cast<const C*>(in);
+        rhs_passes<EPI_RHS>(static_cast<const C*>(in), ea, false);
+    }
+
+    void heun_step() override
+    {
+        C* S = static_cast<C*>(state);
+        C* P = static_cast<C*>(pred);
+        eband(S);
+        EpiArgs<T> ea = epi();
+        ea.out = P;
+        ea.state = S;
+        ea.rhs_in = S;
+        ea.nold = nullptr;
+        rhs_passes<EPI_PRED>(S, ea, true);
+        eband(P);
+        EpiArgs<T> eb = epi();
+        eb.state = S;
+        eb.rhs_in = P;
+        rhs_passes<EPI_CORR>(P, eb, false);
+        hipLaunchKernelGGL(k_step_end<0>, dim3(1), dim3(64), 0, stream, sc);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void ab2_step() override
+    {
+        C* S = static_cast<C*>(state);
+        if (!nold) DNSBOX_HIP(hipMalloc(&nold, 3 * size_t(fs) * sizeof(C)));
+        if (!ab2_started) {
+            // first step: IF-Heun, keeping N(u_0) as the AB2 history (dnsbox.f90:87-141)
+            C* P = static_cast<C*>(pred);
+            eband(S);
+            EpiArgs<T> ea = epi();
+            ea.out = P;
+            ea.state = S;
+            ea.rhs_in = S;
+            ea.nold = static_cast<C*>(nold);
+            rhs_passes<EPI_PRED>(S, ea, true);
+            eband(P);
+            EpiArgs<T> eb = epi();
+            eb.state = S;
+            eb.rhs_in = P;
+            rhs_passes<EPI_CORR>(P, eb, false);
+            ab2_started = true;
+        } else {
+            eband(S);
+            EpiArgs<T> ea = epi();
+            ea.state = S;
+            ea.rhs_in = S;
+            ea.nold = static_cast<C*>(nold);
+            rhs_passes<EPI_AB2>(S, ea, true);
+        }
+        hipLaunchKernelGGL(k_step_end<0>, dim3(1), dim3(64), 0, stream, sc);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void spec_reduce(const void* u) override
+    {
+        const double sm = 1.0 / (double(N) * N * N);
+        const int nslot = DG_NSCAL + nspec;
+        hipLaunchKernelGGL((k_spec_reduce<N, T>), dim3(red_blocks), dim3(256), nspec * sizeof(double),
+                           stream, static_cast<const C*>(u), fs, cfg.ph, cfg.deltak, nspec, red_partial);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_spec_finish<0>, dim3(1), dim3(256), 0, stream, red_partial, red_blocks, nslot,
+                           sm * sm, cfg.ph.nu, cfg.deltak, red_out, (BoxScalars*)nullptr);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void phys_reduce(const void* u) override
+    {
+        DNSBOX_HIP(hipMemsetAsync(red_bits, 0, 4 * sizeof(unsigned long long), stream));
+        const T s = T(1.0 / (double(N) * N * N));
+        hipLaunchKernelGGL((k_zinv<N, T, ProVelDiv<T>>), dim3(G::ZTILES), dim3(NT), 0, stream,
+                           static_cast<const C*>(u), fs, X(), fs, twc(), ProVelDiv<T>{s});
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 4), dim3(NT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_xpass<N, T, OpVelDivMax<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
+                           stream, X(), fs, X(), fs, twc(), red_bits, 1);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void init_random(unsigned long long seed, int box, double k0, double energy) override
+    {
+        const long long tot = fs;
+        hipLaunchKernelGGL((k_init_solenoidal<N, T>), dim3((tot + 255) / 256), dim3(256), 0, stream,
+                           static_cast<C*>(state), fs, cfg.ph, seed, box, k0);
+        DNSBOX_LAUNCH_CHECK();
+        spec_reduce(state);
+        hipLaunchKernelGGL((k_scale_to_energy<T>), dim3(1024), dim3(256), 0, stream,
+                           static_cast<C*>(state), 3 * fs, (const double*)(red_out + DG_EKIN), energy);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void project(int dealias, int proj) override
+    {
+        hipLaunchKernelGGL((k_project_dealias<N, T>), dim3((fs + 255) / 256), dim3(256), 0, stream,
+                           static_cast<C*>(state), fs, cfg.ph, dealias, proj);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void to_physical(const void* spec, void* phys3) override
+    {
+        const T s = T(1.0 / (double(N) * N * N));
+        hipLaunchKernelGGL((k_zinv<N, T, ProScale<T>>), dim3(G::ZTILES), dim3(NT), 0, stream,
+                           static_cast<const C*>(spec), fs, X(), fs, twc(), ProScale<T>{s});
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 3), dim3(NT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        // real output field f at phys3 + f * N^3 reals
+        const long long pf = (long long)N * N * N / 2;  // complex pairs per real field
+        hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, true, false>>), dim3(G::XBLOCKS),
+                           dim3(Cfg<N, T>::NTX), 0, stream, X(), fs, static_cast<C*>(phys3), pf, twc(),
+                           (unsigned long long*)nullptr, 0);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void from_physical(const void* phys3, void* spec) override
+    {
+        const long long pf = (long long)N * N * N / 2;
+        hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, false, true>>), dim3(G::XBLOCKS),
+                           dim3(Cfg<N, T>::NTX), 0, stream, static_cast<const C*>(phys3), pf, X(), fs,
+                           twc(), (unsigned long long*)nullptr, 0);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(NT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        EpiArgs<T> ea = epi();
+        ea.out = static_cast<C*>(spec);
+        hipLaunchKernelGGL((k_zfwd<N, T, EPI_STORE>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs,
+                           twc(), ea);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void convert_in(void* dst, const double* src, long long n) override
+    {
+        // src: device FP64 array of n reals
+        hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, stream, static_cast<T*>(dst),
+                           src, n);
+        DNSBOX_LAUNCH_CHECK();
+    }
+    void convert_out(double* dst, const void* src, long long n) override
+    {
+        hipLaunchKernelGGL((k_convert<double, T>), dim3(1024), dim3(256), 0, stream, dst,
+                           static_cast<const T*>(src), n);
+        DNSBOX_LAUNCH_CHECK();
+    }
+};
+
+template <class T>
+BoxBase* make_box_t(const BoxConfig& c)
+{
+    switch (c.n) {
+        case 16: return new BoxT<16, T>(c);
+        case 32: return new BoxT<32, T>(c);
+        case 48: return new BoxT<48, T>(c);
+        case 64: return new BoxT<64, T>(c);
+        case 128: return new BoxT<128, T>(c);
+        case 256: return new BoxT<256, T>(c);
+        case 512: return new BoxT<512, T>(c);
+        default: return nullptr;
+    }
+}
+
+}  // namespace dnsbox

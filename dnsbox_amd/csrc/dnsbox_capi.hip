@@ -1,0 +1,467 @@
+// C-ABI of include/dnsbox.h.  Converts between the reference's host layout
+// (Fortran uhat(Nz,Ny,Nx/2+1), z fastest) and the device layout (kx fastest),
+// maps exceptions to error codes, and never aborts the host program.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/dnsbox.h"
+#include "box_impl.hpp"
+
+using dnsbox::BoxBase;
+using dnsbox::BoxConfig;
+
+struct dnsbox_ctx {
+    dnsbox_params p;
+    std::unique_ptr<BoxBase> box;
+    int device = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const dnsbox::HipError& e) {
+        return fail(DNSBOX_E_HIP, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(DNSBOX_E_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(DNSBOX_E_ARG, e.what());
+    }
+}
+
+bool supported_n(int n)
+{
+    return n == 16 || n == 32 || n == 48 || n == 64 || n == 128 || n == 256 || n == 512;
+}
+
+// device staging buffer of n doubles, reused per context call
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) { DNSBOX_HIP(hipMalloc(&p, bytes)); }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// reference layout [ix][iy][iz] (complex) -> device layout [iz][iy][ix]
+void ref_to_dev_layout(const double* src, std::vector<double>& dst, int n)
+{
+    const int h = n / 2 + 1;
+    dst.resize(size_t(2) * n * n * h);
+    for (int ix = 0; ix < h; ++ix)
+        for (int iy = 0; iy < n; ++iy)
+            for (int iz = 0; iz < n; ++iz) {
+                const size_t s = (size_t(ix) * n + iy) * n + iz;
+                const size_t d = (size_t(iz) * n + iy) * h + ix;
+                dst[2 * d] = src[2 * s];
+                dst[2 * d + 1] = src[2 * s + 1];
+            }
+}
+
+void dev_to_ref_layout(const std::vector<double>& src, double* dst, int n)
+{
+    const int h = n / 2 + 1;
+    for (int ix = 0; ix < h; ++ix)
+        for (int iy = 0; iy < n; ++iy)
+            for (int iz = 0; iz < n; ++iz) {
+                const size_t d = (size_t(ix) * n + iy) * n + iz;
+                const size_t s = (size_t(iz) * n + iy) * h + ix;
+                dst[2 * d] = src[2 * s];
+                dst[2 * d + 1] = src[2 * s + 1];
+            }
+}
+
+// copy 3 host FP64 arrays (already in device layout, nper reals each) into dev (context precision)
+void upload3(dnsbox_ctx* c, void* dev, const std::vector<double>* h, long long nper, long long dstride_bytes)
+{
+    BoxBase* b = c->box.get();
+    DevBuf tmp(size_t(nper) * sizeof(double));
+    for (int f = 0; f < 3; ++f) {
+        DNSBOX_HIP(hipMemcpyAsync(tmp.p, h[f].data(), size_t(nper) * sizeof(double), hipMemcpyHostToDevice,
+                                  b->stream));
+        b->convert_in(static_cast<char*>(dev) + f * dstride_bytes, static_cast<const double*>(tmp.p), nper);
+    }
+    DNSBOX_HIP(hipStreamSynchronize(b->stream));
+}
+
+void download3(dnsbox_ctx* c, const void* dev, std::vector<double>* h, long long nper,
+               long long sstride_bytes)
+{
+    BoxBase* b = c->box.get();
+    DevBuf tmp(size_t(nper) * sizeof(double));
+    for (int f = 0; f < 3; ++f) {
+        h[f].resize(nper);
+        b->convert_out(static_cast<double*>(tmp.p), static_cast<const char*>(dev) + f * sstride_bytes, nper);
+        DNSBOX_HIP(hipMemcpyAsync(h[f].data(), tmp.p, size_t(nper) * sizeof(double), hipMemcpyDeviceToHost,
+                                  b->stream));
+        DNSBOX_HIP(hipStreamSynchronize(b->stream));
+    }
+}
+
+int check_ctx(dnsbox_ctx* c)
+{
+    if (!c || !c->box) return fail(DNSBOX_E_ARG, "null context");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dnsbox_last_error(void) { return g_err.c_str(); }
+const char* dnsbox_version(void) { return "dnsbox-mi355x 0.1 (gfx950)"; }
+
+int dnsbox_params_default(dnsbox_params* p)
+{
+    if (!p) return fail(DNSBOX_E_ARG, "null params");
+    std::memset(p, 0, sizeof(*p));
+    p->nu = 0.005;       // parameters.f90:20
+    p->Q = 0.0;          // :21
+    p->Pin = 0.1;        // :23
+    p->kcut = 2.0;       // :22
+    p->deltak = 1.0;     // :26
+    p->cfl_min = 0.15;   // :33
+    p->cfl_max = 0.2;    // :34
+    p->dt_max = 1.0e-2;  // :35
+    p->dt = 1.0e-3;      // variables.f90:9
+    p->n = 32;
+    p->precision = 64;
+    p->bandlim = 1;      // :52
+    p->spherical = 0;    // :50
+    p->parity_mode = 1;
+    p->adaptive_dt = 1;  // tStepFix = .false. (:37)
+    return 0;
+}
+
+int dnsbox_create(const dnsbox_params* p, dnsbox_ctx** out)
+{
+    if (!p || !out) return fail(DNSBOX_E_ARG, "null argument");
+    *out = nullptr;
+    if (!supported_n(p->n))
+        return fail(DNSBOX_E_ARG, "unsupported n=" + std::to_string(p->n) +
+                                      " (supported: 16, 32, 48, 64, 128, 256, 512)");
+    if (p->precision != 32 && p->precision != 64)
+        return fail(DNSBOX_E_ARG, "precision must be 32 or 64");
+    if (p->kolmo_k < 0 || p->kolmo_k >= p->n / 2) return fail(DNSBOX_E_ARG, "kolmo_k out of range");
+    if (p->bandlim && !(p->kcut > 0.0)) return fail(DNSBOX_E_ARG, "bandlim needs kcut > 0");
+    if (!(p->dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+    if (p->adaptive_dt && !(p->cfl_max >= p->cfl_min && p->cfl_min > 0.0 && p->dt_max > 0.0))
+        return fail(DNSBOX_E_ARG, "adaptive dt needs 0 < cfl_min <= cfl_max and dt_max > 0");
+    return guarded([&]() -> int {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            return fail(DNSBOX_E_HIP, "no HIP device available");
+        BoxConfig bc;
+        bc.n = p->n;
+        bc.precision = p->precision;
+        bc.deltak = p->deltak > 0 ? p->deltak : 1.0;
+        bc.dt = p->dt;
+        dnsbox::BoxPhys& ph = bc.ph;
+        ph.nu = p->nu;
+        ph.Q = p->Q;
+        ph.Pin = p->Pin;
+        ph.kcut = p->kcut;
+        ph.kolmo_amp = p->kolmo_amp;
+        ph.mask_thr = (double(p->n) / 2.0) * (2.0 * 1.0 / 3.0);
+        const double ts = (double(p->n) / 2.0) * 0.5 * 1.0;
+        ph.sph_thr2 = ts * ts;
+        ph.cmin = p->cfl_min;
+        ph.cmax = p->cfl_max;
+        ph.dtmax = p->dt_max;
+        ph.kolmo_k = p->kolmo_amp != 0.0 ? p->kolmo_k : 0;
+        ph.bandlim = p->bandlim;
+        ph.spherical = p->spherical;
+        ph.parity = p->parity_mode;
+        ph.adaptive = p->adaptive_dt;
+        auto c = std::make_unique<dnsbox_ctx>();
+        c->p = *p;
+        (void)hipGetDevice(&c->device);
+        c->box.reset(p->precision == 64 ? dnsbox::make_box_f64(bc) : dnsbox::make_box_f32(bc));
+        if (!c->box) return fail(DNSBOX_E_ARG, "unsupported configuration");
+        *out = c.release();
+        return 0;
+    });
+}
+
+int dnsbox_destroy(dnsbox_ctx* c)
+{
+    if (!c) return 0;
+    return guarded([&]() -> int {
+        if (c->box && c->box->stream) (void)hipStreamSynchronize(c->box->stream);
+        else (void)hipDeviceSynchronize();
+        delete c;
+        return 0;
+    });
+}
+
+int dnsbox_set_stream(dnsbox_ctx* c, void* s)
+{
+    if (int e = check_ctx(c)) return e;
+    c->box->stream = static_cast<hipStream_t>(s);
+    return 0;
+}
+
+int dnsbox_sync(dnsbox_ctx* c)
+{
+    if (int e = check_ctx(c)) return e;
+    return guarded([&]() -> int {
+        DNSBOX_HIP(hipStreamSynchronize(c->box->stream));
+        return 0;
+    });
+}
+
+int dnsbox_nspec(dnsbox_ctx* c)
+{
+    if (int e = check_ctx(c)) return e;
+    return c->box->nspec;
+}
+
+int dnsbox_set_state(dnsbox_ctx* c, const double* u, const double* v, const double* w)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null state array");
+    return guarded([&]() -> int {
+        const int n = c->p.n;
+        std::vector<double> h[3];
+        const double* src[3] = {u, v, w};
+        for (int f = 0; f < 3; ++f) ref_to_dev_layout(src[f], h[f], n);
+        BoxBase* b = c->box.get();
+        upload3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->cbytes);
+        b->ab2_started = false;
+        return 0;
+    });
+}
+
+int dnsbox_get_state(dnsbox_ctx* c, double* u, double* v, double* w)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null state array");
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        std::vector<double> h[3];
+        download3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->cbytes);
+        double* dst[3] = {u, v, w};
+        for (int f = 0; f < 3; ++f) dev_to_ref_layout(h[f], dst[f], c->p.n);
+        return 0;
+    });
+}
+
+int dnsbox_get_nonlin(dnsbox_ctx* c, double* u, double* v, double* w)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        std::vector<double> h[3];
+        download3(c, b->nonlin, h, 2 * b->fs, b->fs * (long long)b->cbytes);
+        double* dst[3] = {u, v, w};
+        for (int f = 0; f < 3; ++f) dev_to_ref_layout(h[f], dst[f], c->p.n);
+        return 0;
+    });
+}
+
+int dnsbox_set_state_physical(dnsbox_ctx* c, const double* u, const double* v, const double* w)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        const long long n3 = (long long)c->p.n * c->p.n * c->p.n;
+        std::vector<double> h[3];
+        const double* src[3] = {u, v, w};
+        for (int f = 0; f < 3; ++f) h[f].assign(src[f], src[f] + n3);
+        DevBuf phys(size_t(3 * n3) * b->rbytes);
+        upload3(c, phys.p, h, n3, n3 * (long long)b->rbytes);
+        b->from_physical(phys.p, b->state);
+        DNSBOX_HIP(hipStreamSynchronize(b->stream));
+        b->ab2_started = false;
+        return 0;
+    });
+}
+
+int dnsbox_get_state_physical(dnsbox_ctx* c, double* u, double* v, double* w)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        const long long n3 = (long long)c->p.n * c->p.n * c->p.n;
+        DevBuf phys(size_t(3 * n3) * b->rbytes);
+        b->to_physical(b->state, phys.p);
+        std::vector<double> h[3];
+        download3(c, phys.p, h, n3, n3 * (long long)b->rbytes);
+        double* dst[3] = {u, v, w};
+        for (int f = 0; f < 3; ++f) std::memcpy(dst[f], h[f].data(), size_t(n3) * sizeof(double));
+        return 0;
+    });
+}
+
+int dnsbox_init_random(dnsbox_ctx* c, uint64_t seed, int box, double k0, double energy)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!(k0 > 0.0) || !(energy >= 0.0)) return fail(DNSBOX_E_ARG, "k0 must be > 0, energy >= 0");
+    return guarded([&]() -> int {
+        c->box->init_random(seed, box, k0, energy);
+        c->box->ab2_started = false;
+        return 0;
+    });
+}
+
+int dnsbox_project(dnsbox_ctx* c, int dealias, int project)
+{
+    if (int e = check_ctx(c)) return e;
+    return guarded([&]() -> int {
+        c->box->project(dealias, project);
+        return 0;
+    });
+}
+
+int dnsbox_int_fact(dnsbox_ctx* c, double dt)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        DNSBOX_HIP(hipStreamSynchronize(b->stream));
+        dnsbox::BoxScalars s{};
+        DNSBOX_HIP(hipMemcpy(&s, b->sc, sizeof(s), hipMemcpyDeviceToHost));
+        s.dt = s.dt_prev = dt;
+        DNSBOX_HIP(hipMemcpy(b->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+        c->p.dt = dt;
+        return 0;
+    });
+}
+
+int dnsbox_rhs_nonlinear(dnsbox_ctx* c)
+{
+    if (int e = check_ctx(c)) return e;
+    return guarded([&]() -> int {
+        c->box->rhs(c->box->state, c->box->nonlin);
+        return 0;
+    });
+}
+
+int dnsbox_step(dnsbox_ctx* c, int nsteps, int stepper)
+{
+    if (int e = check_ctx(c)) return e;
+    if (nsteps < 0) return fail(DNSBOX_E_ARG, "nsteps < 0");
+    if (stepper != 1 && stepper != 2) return fail(DNSBOX_E_ARG, "stepper must be 1 (IF-Heun) or 2 (IF-AB2)");
+    if (stepper == 2 && c->p.adaptive_dt)
+        return fail(DNSBOX_E_STATE, "IF-AB2 runs at constant dt; create the box with adaptive_dt = 0");
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        for (int i = 0; i < nsteps; ++i) {
+            if (stepper == 1) b->heun_step();
+            else b->ab2_step();
+        }
+        return 0;
+    });
+}
+
+int dnsbox_get_time(dnsbox_ctx* c, double* t, double* dt)
+{
+    if (int e = check_ctx(c)) return e;
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        DNSBOX_HIP(hipStreamSynchronize(b->stream));
+        dnsbox::BoxScalars s{};
+        DNSBOX_HIP(hipMemcpy(&s, b->sc, sizeof(s), hipMemcpyDeviceToHost));
+        if (t) *t = s.t;
+        if (dt) *dt = s.dt_prev;
+        return 0;
+    });
+}
+
+int dnsbox_set_time(dnsbox_ctx* c, double t, double dt)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        DNSBOX_HIP(hipStreamSynchronize(b->stream));
+        dnsbox::BoxScalars s{};
+        DNSBOX_HIP(hipMemcpy(&s, b->sc, sizeof(s), hipMemcpyDeviceToHost));
+        s.t = t;
+        s.dt = s.dt_prev = dt;
+        DNSBOX_HIP(hipMemcpy(b->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+        return 0;
+    });
+}
+
+int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!diag) return fail(DNSBOX_E_ARG, "null diag");
+    return guarded([&]() -> int {
+        BoxBase* b = c->box.get();
+        b->spec_reduce(b->state);
+        b->phys_reduce(b->state);
+        std::vector<double> red(dnsbox::DG_NSCAL + b->nspec);
+        unsigned long long bits[4];
+        dnsbox::BoxScalars s{};
+        DNSBOX_HIP(hipMemcpyAsync(red.data(), b->red_out, red.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  b->stream));
+        DNSBOX_HIP(hipMemcpyAsync(bits, b->red_bits, sizeof(bits), hipMemcpyDeviceToHost, b->stream));
+        DNSBOX_HIP(hipMemcpyAsync(&s, b->sc, sizeof(s), hipMemcpyDeviceToHost, b->stream));
+        DNSBOX_HIP(hipStreamSynchronize(b->stream));
+        double mx[3];
+        for (int i = 0; i < 3; ++i) std::memcpy(&mx[i], &bits[i], sizeof(double));
+        unsigned long long ob = bits[3];
+        ob = (ob >> 63) ? (ob & 0x7fffffffffffffffull) : ~ob;
+        double divmax;
+        std::memcpy(&divmax, &ob, sizeof(double));
+        const double dx = 6.283185307179586476925286766559 / double(c->p.n);
+        const double dt = s.dt_prev;
+        diag[DNSBOX_D_EKIN] = red[dnsbox::DG_EKIN];
+        diag[DNSBOX_D_DISP] = red[dnsbox::DG_DISP];
+        diag[DNSBOX_D_PROD] = 2.0 * c->p.Q * red[dnsbox::DG_EKIN];
+        diag[DNSBOX_D_COURANT] = (mx[0] + mx[1] + mx[2]) * dt / dx;
+        diag[DNSBOX_D_DIVMAX] = divmax;
+        diag[DNSBOX_D_EZERO] = red[dnsbox::DG_EZERO];
+        diag[DNSBOX_D_EONE] = red[dnsbox::DG_EONE];
+        diag[DNSBOX_D_ESQRT2] = red[dnsbox::DG_ESQRT2];
+        diag[DNSBOX_D_EBAND] = red[dnsbox::DG_EBAND];
+        diag[DNSBOX_D_TIME] = s.t;
+        diag[DNSBOX_D_DT] = dt;
+        if (espec)
+            for (int i = 0; i < b->nspec; ++i) espec[i] = red[dnsbox::DG_NSCAL + i];
+        return 0;
+    });
+}
+
+int dnsbox_state_ptr(dnsbox_ctx* c, void** p)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!p) return fail(DNSBOX_E_ARG, "null pointer");
+    *p = c->box->state;
+    return 0;
+}
+
+int dnsbox_rhs_dev(dnsbox_ctx* c, const void* in, void* out)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!in || !out) return fail(DNSBOX_E_ARG, "null device pointer");
+    return guarded([&]() -> int {
+        c->box->rhs(in, out);
+        return 0;
+    });
+}
+
+}  // extern "C"

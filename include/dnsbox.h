@@ -1,0 +1,133 @@
+/*
+ * dnsbox.h — C-ABI of the MI355X (gfx950) pseudospectral Navier-Stokes box.
+ *
+ * Drop-in for the reference's hot path: the rhs / rhsAdj / stepper module
+ * procedures of dnsbox (Fortran 90), which operate on module globals.  Each
+ * entry point below names the reference routine it replaces (file:line in the
+ * reference tree).  Plain pointers, sizes and a POD parameter struct only; no
+ * torch or HIP types cross this boundary (streams are passed as void*).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative DNSBOX_E* code on
+ *     failure; dnsbox_last_error() gives the message (thread-local).  Nothing
+ *     calls stop/abort (the reference stops on allocation failure,
+ *     variables.f90:165-168).
+ *   - A context owns its device memory (state, scratch, FFT tables).  Host
+ *     pointers are borrowed for the duration of the call.
+ *   - Work is stream-ordered on the context's stream; calls that return host
+ *     data (get_*, diagnostics) synchronise that stream.
+ *   - One context per host thread at a time; no MPI (one GPU holds the box).
+ *   - Host spectral arrays use the reference layout: Fortran
+ *     uhat(Nz, Ny, Nx/2+1) complex(8) (p3dfft STRIDE1, z fastest), one array per
+ *     velocity component.  Host physical arrays: Fortran u(Nx, Ny, Nz) real(8).
+ *   - Device-pointer entry points (suffix _dev) use the internal layout:
+ *     (3, Nz, Ny, Nx/2+1) complex in the context precision, kx fastest.
+ */
+#ifndef DNSBOX_H
+#define DNSBOX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNSBOX_E_ARG     (-1)  /* invalid argument / unsupported size       */
+#define DNSBOX_E_HIP     (-2)  /* HIP runtime error                         */
+#define DNSBOX_E_NOMEM   (-3)  /* device allocation failed                  */
+#define DNSBOX_E_STATE   (-4)  /* call not valid in the current state       */
+
+/* diagnostics vector (io.f90:651-658 Ekin.dat columns, plus state_spectrum prints) */
+enum {
+    DNSBOX_D_EKIN = 0,   /* state_kinetic            state.f90:429-454           */
+    DNSBOX_D_DISP,       /* state_dissipation        state.f90:517-544           */
+    DNSBOX_D_PROD,       /* 2 Q Ekin                 io.f90:644,654              */
+    DNSBOX_D_COURANT,    /* io_Courant               io.f90:664-676              */
+    DNSBOX_D_DIVMAX,     /* state_divergence         state.f90:413-427           */
+    DNSBOX_D_EZERO,      /* state_spectrum           state.f90:370-374           */
+    DNSBOX_D_EONE,       /*                          state.f90:376-381           */
+    DNSBOX_D_ESQRT2,     /*                          state.f90:383-388           */
+    DNSBOX_D_EBAND,      /* rhs_Eband                rhs.f90:137-168             */
+    DNSBOX_D_TIME,       /* time(n+1)                dnsboxheun.f90:128          */
+    DNSBOX_D_DT,         /* dt                                                   */
+    DNSBOX_NDIAG
+};
+
+/* Runtime replacement of the compile-time parameters.f90 constants. */
+typedef struct dnsbox_params {
+    double nu;          /* kinematic viscosity          parameters.f90:20 */
+    double Q;           /* linear forcing multiplier    parameters.f90:21 */
+    double Pin;         /* band power input             parameters.f90:23 */
+    double kcut;        /* band forcing cut-off         parameters.f90:22 */
+    double kolmo_amp;   /* Kolmogorov forcing f = amp sin(kolmo_k y) e_x (0 = off) */
+    double deltak;      /* shell width                  parameters.f90:26 */
+    double cfl_min;     /* CourantMin                   parameters.f90:33 */
+    double cfl_max;     /* CourantMax                   parameters.f90:34 */
+    double dt_max;      /* tStepMax                     parameters.f90:35 */
+    double dt;          /* initial time step            variables.f90:9   */
+    int n;              /* Nx = Ny = Nz: 16, 32, 48, 64, 128, 256, 512   parameters.f90:12-14 */
+    int precision;      /* 64: all FP64; 32: FFTs/nonlinear term FP32, IF/CFL/diagnostics FP64 */
+    int kolmo_k;        /* Kolmogorov forcing wavenumber (0 = off) */
+    int bandlim;        /* band-limited forcing         parameters.f90:52 */
+    int spherical;      /* spherical truncation         parameters.f90:50 */
+    int parity_mode;    /* 1: reference k-table (Nyquist wavenumber 0, Nyquist modes kept by
+                           the mask, variables.f90:229,245,259); 0: Nyquist modes removed */
+    int adaptive_dt;    /* 1: CFL controller (tStepFix = .false., parameters.f90:37) */
+    int reserved;
+} dnsbox_params;
+
+typedef struct dnsbox_ctx dnsbox_ctx;
+
+/* library */
+const char* dnsbox_last_error(void);
+const char* dnsbox_version(void);
+int dnsbox_params_default(dnsbox_params* p);               /* parameters.f90 values, n = 32 */
+
+/* context: var_init (variables.f90:99-280) / deallocate (dnsboxheun.f90:197-204) */
+int dnsbox_create(const dnsbox_params* p, dnsbox_ctx** out);
+int dnsbox_destroy(dnsbox_ctx* ctx);
+int dnsbox_set_stream(dnsbox_ctx* ctx, void* hip_stream);
+int dnsbox_sync(dnsbox_ctx* ctx);
+int dnsbox_nspec(dnsbox_ctx* ctx);                          /* Nspec, variables.f90:112-123 */
+
+/* state transfer (host, reference layout) */
+int dnsbox_set_state(dnsbox_ctx* ctx, const double* uhat, const double* vhat, const double* what);
+int dnsbox_get_state(dnsbox_ctx* ctx, double* uhat, double* vhat, double* what);
+/* state_u2uhat (state.f90:75-81): physical u(Nx,Ny,Nz) -> state (unnormalised r2c) */
+int dnsbox_set_state_physical(dnsbox_ctx* ctx, const double* u, const double* v, const double* w);
+/* state_uhat2u (state.f90:83-96): state -> physical velocity (scaled by 1/N^3) */
+int dnsbox_get_state_physical(dnsbox_ctx* ctx, double* u, double* v, double* w);
+
+/* initial fields and field operators */
+/* seeded solenoidal field, E(k) ~ k^4 exp(-2 (k/k0)^2), state_kinetic energy = energy
+   (replaces state_init_rand, state.f90:211-285) */
+int dnsbox_init_random(dnsbox_ctx* ctx, uint64_t seed, int box, double k0, double energy);
+/* state_dealias (state.f90:181-209) and/or state_project (state.f90:120-179) */
+int dnsbox_project(dnsbox_ctx* ctx, int dealias, int project);
+
+/* rhs_int_fact (rhs.f90:8-16): sets the step size of the integrating factor; the factor
+   exp((-nu|k|^2+Q) dt) is evaluated in FP64 on the fly in the stepper epilogue */
+int dnsbox_int_fact(dnsbox_ctx* ctx, double dt);
+/* rhs_nonlinear (rhs.f90:18-135): nonlin <- N(state) (rotational form, 2/3 mask,
+   projection, band / Kolmogorov forcing) */
+int dnsbox_rhs_nonlinear(dnsbox_ctx* ctx);
+int dnsbox_get_nonlin(dnsbox_ctx* ctx, double* nuhat, double* nvhat, double* nwhat);
+/* the time loop body of dnsboxheun.f90:81-157: stepper 1 = IF-Heun predictor-corrector
+   (dnsboxheun.f90:84-126), 2 = IF-AB2 (dnsbox.f90:143-193, consistent weights), with the
+   CFL controller set_time_step (dnsboxheun.f90:227-246) when adaptive_dt */
+int dnsbox_step(dnsbox_ctx* ctx, int nsteps, int stepper);
+/* io_save_stats (io.f90:628-661) + state_spectrum (state.f90:350-411): diag[DNSBOX_NDIAG],
+   espec[Nspec] (may be NULL) */
+int dnsbox_diagnostics(dnsbox_ctx* ctx, double* diag, double* espec);
+int dnsbox_get_time(dnsbox_ctx* ctx, double* t, double* dt);
+int dnsbox_set_time(dnsbox_ctx* ctx, double t, double dt);
+
+/* device-pointer entry points (internal layout, context precision) */
+int dnsbox_state_ptr(dnsbox_ctx* ctx, void** dev_state);   /* the context's state buffer */
+/* rhs_nonlinear on device buffers: out <- N(in) */
+int dnsbox_rhs_dev(dnsbox_ctx* ctx, const void* in, void* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DNSBOX_H */

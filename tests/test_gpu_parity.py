@@ -1,0 +1,143 @@
+"""GPU FP64 path vs the CPU oracle (oracle/dnsbox_oracle.c), through the C-ABI.
+
+Tolerances (FP64): RHS <= 1e-12 relative L2; state after 100 IF-Heun steps
+<= 1e-9 relative L2; spectrum shell by shell <= 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    num = sum(np.linalg.norm((x - y).ravel()) ** 2 for x, y in zip(a, b))
+    den = sum(np.linalg.norm(y.ravel()) ** 2 for y in b)
+    return float(np.sqrt(num / den))
+
+
+def make(n, **kw):
+    from dnsbox_amd import Box
+    return Box(n=n, precision=kw.pop("precision", 64), **kw)
+
+
+@pytest.mark.parametrize("n", [16, 32, 48])
+def test_init_random_matches_oracle(gpu_lib, n):
+    o = Oracle(n)
+    ref = o.init_solenoidal(7, 0, 4.0, 0.5)
+    with make(n, adaptive_dt=0, bandlim=0) as b:
+        b.init_random(7, 0, 4.0, 0.5)
+        got = b.get_state()
+    print(f"init n={n} rel_l2={rel_l2(got, ref):.3e}", flush=True)
+    assert rel_l2(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("n", [16, 32, 48, 64])
+def test_rhs_matches_oracle(gpu_lib, n):
+    kw = dict(nu=0.05, kolmo_amp=1.0, kolmo_k=4 if n >= 16 else 1, bandlim=0)
+    o = Oracle(n, nu=0.05, kolmo_amp=1.0, kolmo_k=kw["kolmo_k"])
+    f = o.init_solenoidal(3, 0, 4.0, 0.5)
+    ref = o.rhs(f)
+    with make(n, adaptive_dt=0, **kw) as b:
+        b.set_state(*f)
+        got = b.rhs_nonlinear()
+    err = rel_l2(got, ref)
+    print(f"rhs n={n} rel_l2={err:.3e}", flush=True)
+    assert err <= 1e-12
+
+
+def test_rhs_bandlimited_matches_oracle(gpu_lib):
+    n = 32
+    o = Oracle(n, nu=0.005, Pin=0.1, kcut=2.0, bandlim=True)
+    f = o.init_solenoidal(5, 0, 3.0, 0.5)
+    ref = o.rhs(f)
+    with make(n, nu=0.005, Pin=0.1, kcut=2.0, bandlim=1, adaptive_dt=0) as b:
+        b.set_state(*f)
+        got = b.rhs_nonlinear()
+    err = rel_l2(got, ref)
+    print(f"rhs band rel_l2={err:.3e}", flush=True)
+    assert err <= 1e-12
+
+
+def test_heun_100_steps_matches_oracle(gpu_lib):
+    """The 32^3 FP64 workload: nu = 0.05, Kolmogorov sin 4y, fixed dt = 0.005, 100 steps."""
+    n = 32
+    o = Oracle(n, nu=0.05, kolmo_amp=1.0, kolmo_k=4)
+    f = o.init_solenoidal(0, 0, 4.0, 0.5)
+    ref = o.heun(f, 100, 0.005)
+    with make(n, nu=0.05, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=0.005) as b:
+        b.set_state(*f)
+        b.rhs_int_fact(0.005)
+        b.step(100)
+        got = b.get_state()
+        t, dt = b.time()
+        gd, gs = b.diagnostics()
+    err = rel_l2(got, ref)
+    print(f"heun100 rel_l2={err:.3e} t={t}", flush=True)
+    assert err <= 1e-9
+    assert abs(t - 0.5) < 1e-12
+    od, os_ = o.diagnostics(ref, 0.005)
+    nz = os_ > 1e-30
+    assert np.max(np.abs(gs[nz] - os_[nz]) / os_[nz]) < 1e-9
+    for i in (0, 1, 2, 3, 5, 6, 7):
+        assert abs(gd[i] - od[i]) <= 1e-9 * max(abs(od[i]), 1e-300) + 1e-13, (i, gd[i], od[i])
+
+
+def test_diagnostics_match_oracle(gpu_lib):
+    n = 32
+    o = Oracle(n, nu=0.01, Q=0.1, Pin=0.1, kcut=2.5)
+    f = o.init_solenoidal(11, 0, 4.0, 0.5)
+    od, os_ = o.diagnostics(f, 0.004)
+    with make(n, nu=0.01, Q=0.1, kcut=2.5, bandlim=0, adaptive_dt=0, dt=0.004) as b:
+        b.set_state(*f)
+        gd, gs = b.diagnostics()
+    # Ekin, Disp, Prod, Courant, EZero, EOne, Esqrt2, Eband
+    for i in (0, 1, 2, 3, 5, 6, 7, 8):
+        assert abs(gd[i] - od[i]) <= 1e-11 * max(abs(od[i]), 1e-300) + 1e-14, (i, gd[i], od[i])
+    assert abs(gd[4] - od[4]) < 1e-12  # divMax (rounding-level for a solenoidal field)
+    np.testing.assert_allclose(gs, os_, rtol=1e-11, atol=1e-300)
+
+
+def test_physical_roundtrip(gpu_lib):
+    n = 32
+    o = Oracle(n, nu=0.05)
+    u = o.init_shapiro(0.0)
+    with make(n, adaptive_dt=0, bandlim=0) as b:
+        b.set_state_physical(*u)
+        spec = b.get_state()
+        back = b.get_state_physical()
+    ref = [o.r2c(a) for a in u]
+    assert rel_l2(spec, ref) < 1e-13
+    assert max(np.abs(a - x).max() for a, x in zip(back, u)) < 1e-13
+
+
+def test_shapiro_kat(gpu_lib):
+    """state.f90:287-348: IF-Heun on the Shapiro/Beltrami field reproduces exp(-3 nu t)."""
+    n = 32
+    nu = 0.05
+    o = Oracle(n, nu=nu)
+    u = o.init_shapiro(0.0)
+    with make(n, nu=nu, adaptive_dt=0, bandlim=0, dt=0.01) as b:
+        b.set_state_physical(*u)
+        b.state_dealias()
+        b.state_project()
+        b.rhs_int_fact(0.01)
+        b.step(100)
+        phys = b.get_state_physical()
+    err = o.shapiro_error(1.0, phys)
+    print(f"shapiro err={err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+def test_fp32_rhs_close_to_fp64(gpu_lib):
+    n = 64
+    o = Oracle(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4)
+    f = o.init_solenoidal(2, 0, 4.0, 0.5)
+    ref = o.rhs(f)
+    with make(n, precision=32, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0) as b:
+        b.set_state(*f)
+        got = b.rhs_nonlinear()
+    err = rel_l2(got, ref)
+    print(f"fp32 rhs rel_l2={err:.3e}", flush=True)
+    assert err < 1e-5
