@@ -13,6 +13,9 @@ LIB_PATH = os.path.join(PKG, "lib", "libdnsbox_hip.so")
 
 (D_EKIN, D_DISP, D_PROD, D_COURANT, D_DIVMAX, D_EZERO, D_EONE, D_ESQRT2, D_EBAND, D_TIME,
  D_DT, NDIAG) = range(12)
+PASS_NAMES = ("zinv_curl", "yinv6", "xcross", "yfwd3", "zfwd_pred", "zfwd_corr")
+# spectral fields (read, written) per launch of each pass
+PASS_FIELDS = ((3, 6), (6, 6), (6, 3), (3, 3), (6, 6), (6, 3))
 DIAG_NAMES = ("Ekin", "Disp", "Prod", "Courant", "divMax", "EZero", "EOne", "Esqrt2", "Eband",
               "time", "dt")
 
@@ -58,6 +61,7 @@ SIGNATURES = {
     "dnsbox_diagnostics": (C.c_int, [_vp, _dp, _dp]),
     "dnsbox_get_time": (C.c_int, [_vp, _dp, _dp]),
     "dnsbox_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "dnsbox_time_passes": (C.c_int, [_vp, C.c_int, _dp]),
     "dnsbox_state_ptr": (C.c_int, [_vp, C.POINTER(_vp)]),
     "dnsbox_rhs_dev": (C.c_int, [_vp, _vp, _vp]),
 }

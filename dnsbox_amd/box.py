@@ -170,6 +170,12 @@ class Box:
     def spectrum(self):
         return self.diagnostics()[1]
 
+    def time_passes(self, reps: int = 20):
+        """Average duration (ms) of each kernel pass of the step, by HIP events."""
+        ms = np.zeros(len(_lib.PASS_NAMES))
+        check(self.lib.dnsbox_time_passes(self._h, reps, _dp(ms)))
+        return dict(zip(_lib.PASS_NAMES, ms.tolist()))
+
     def state_ptr(self) -> int:
         p = C.c_void_p()
         check(self.lib.dnsbox_state_ptr(self._h, C.byref(p)))

@@ -71,6 +71,9 @@ class BoxBase {
     virtual void project(int dealias, int proj) = 0;
     virtual void to_physical(const void* spec, void* phys3) = 0;   // device real (3, N^3)
     virtual void from_physical(const void* phys3, void* spec) = 0; // device real (3, N^3)
+    // average device time of each RHS pass over `reps` back-to-back launches (ms),
+    // measured with HIP events on the box stream; order of PASS_* below
+    virtual void time_passes(int reps, double* ms) = 0;
     virtual void convert_in(void* dst, const double* src, long long n) = 0;   // host-staged
     virtual void convert_out(double* dst, const void* src, long long n) = 0;
 };
@@ -315,6 +318,56 @@ class BoxT final : public BoxBase {
         hipLaunchKernelGGL((k_zfwd<N, T, EPI_STORE>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs,
                            twc(), ea);
         DNSBOX_LAUNCH_CHECK();
+    }
+
+    void time_passes(int reps, double* ms) override
+    {
+        const T s = T(1.0 / (double(N) * N * N));
+        C* P = static_cast<C*>(pred);
+        C* NL = static_cast<C*>(nonlin);
+        const C* S = static_cast<const C*>(state);
+        hipEvent_t e0, e1;
+        DNSBOX_HIP(hipEventCreate(&e0));
+        DNSBOX_HIP(hipEventCreate(&e1));
+        auto timed = [&](int which, auto&& launch) {
+            launch();  // warm
+            DNSBOX_HIP(hipEventRecord(e0, stream));
+            for (int r = 0; r < reps; ++r) launch();
+            DNSBOX_HIP(hipEventRecord(e1, stream));
+            DNSBOX_HIP(hipEventSynchronize(e1));
+            float t = 0.f;
+            DNSBOX_HIP(hipEventElapsedTime(&t, e0, e1));
+            ms[which] = double(t) / reps;
+        };
+        timed(0, [&] {
+            hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(NT), 0, stream, S, fs, X(),
+                               fs, twc(), ProCurl<T>{s});
+        });
+        timed(1, [&] {
+            hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(NT), 0, stream, X(), fs, twc());
+        });
+        timed(2, [&] {
+            hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream,
+                               X(), fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+        });
+        timed(3, [&] {
+            hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(NT), 0, stream, X(), fs, twc());
+        });
+        EpiArgs<T> ea = epi();
+        ea.out = NL;
+        ea.state = P;
+        ea.rhs_in = P;
+        timed(4, [&] {
+            hipLaunchKernelGGL((k_zfwd<N, T, EPI_PRED>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs, twc(),
+                               ea);
+        });
+        timed(5, [&] {
+            hipLaunchKernelGGL((k_zfwd<N, T, EPI_CORR>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs, twc(),
+                               ea);
+        });
+        DNSBOX_LAUNCH_CHECK();
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
 
     void convert_in(void* dst, const double* src, long long n) override

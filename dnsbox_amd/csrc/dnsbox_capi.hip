@@ -446,6 +446,16 @@ int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
     });
 }
 
+int dnsbox_time_passes(dnsbox_ctx* c, int reps, double* ms)
+{
+    if (int e = check_ctx(c)) return e;
+    if (reps < 1 || !ms) return fail(DNSBOX_E_ARG, "reps >= 1 and an output array are required");
+    return guarded([&]() -> int {
+        c->box->time_passes(reps, ms);
+        return 0;
+    });
+}
+
 int dnsbox_state_ptr(dnsbox_ctx* c, void** p)
 {
     if (int e = check_ctx(c)) return e;

@@ -122,6 +122,20 @@ int dnsbox_diagnostics(dnsbox_ctx* ctx, double* diag, double* espec);
 int dnsbox_get_time(dnsbox_ctx* ctx, double* t, double* dt);
 int dnsbox_set_time(dnsbox_ctx* ctx, double t, double dt);
 
+/* measurement: average duration (ms, HIP events on the context stream) of each kernel pass
+   of the step over `reps` back-to-back launches, in the order of DNSBOX_PASS_*.  No reference
+   counterpart (the reference has no timers). */
+enum {
+    DNSBOX_PASS_ZINV = 0,  /* z-inverse FFT + spectral curl: reads 3 fields, writes 6     */
+    DNSBOX_PASS_YINV,      /* y-inverse FFT of 6 fields in place: reads 6, writes 6       */
+    DNSBOX_PASS_XCROSS,    /* c2r + (omega x u) + r2c along x: reads 6, writes 3          */
+    DNSBOX_PASS_YFWD,      /* y-forward FFT of 3 fields in place: reads 3, writes 3       */
+    DNSBOX_PASS_ZPRED,     /* z-forward FFT + projection + IF-Heun predictor: reads 6, writes 6 */
+    DNSBOX_PASS_ZCORR,     /* z-forward FFT + projection + IF-Heun corrector: reads 6, writes 3 */
+    DNSBOX_NPASS
+};
+int dnsbox_time_passes(dnsbox_ctx* ctx, int reps, double* ms);
+
 /* device-pointer entry points (internal layout, context precision) */
 int dnsbox_state_ptr(dnsbox_ctx* ctx, void** dev_state);   /* the context's state buffer */
 /* rhs_nonlinear on device buffers: out <- N(in) */
