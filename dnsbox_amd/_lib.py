@@ -13,9 +13,9 @@ LIB_PATH = os.path.join(PKG, "lib", "libdnsbox_hip.so")
 
 (D_EKIN, D_DISP, D_PROD, D_COURANT, D_DIVMAX, D_EZERO, D_EONE, D_ESQRT2, D_EBAND, D_TIME,
  D_DT, NDIAG) = range(12)
-PASS_NAMES = ("zinv_curl", "yinv6", "xcross", "yfwd3", "zfwd_pred", "zfwd_corr")
+PASS_NAMES = ("zinv_curl", "yinv6", "xcross", "yfwd3", "zpred_zinv", "zcorr_zinv")
 # spectral fields (read, written) per launch of each pass
-PASS_FIELDS = ((3, 6), (6, 6), (6, 3), (3, 3), (6, 6), (6, 3))
+PASS_FIELDS = ((3, 6), (6, 6), (6, 3), (3, 3), (6, 9), (6, 9))
 DIAG_NAMES = ("Ekin", "Disp", "Prod", "Courant", "divMax", "EZero", "EOne", "Esqrt2", "Eband",
               "time", "dt")
 

@@ -88,55 +88,61 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // ---------------------------------------------------------------------------
 // per-size configuration
-//   CP : radices of the length-N column transforms (z and y)
-//   W  : columns per tile (coalescing width)
-//   RP : radices of the length-N/2 complex row transforms (x; real data packed in pairs)
-//   RB : x-lines per block in the x pass, NTX its threads
+//   ZP, ZW : radices and tile width (columns) of the z passes.  ZP is symmetric
+//            (first radix == last radix) so that a thread's output positions of the
+//            forward transform are its input positions of the inverse one: the
+//            stepper epilogue hands its result to the next z-inverse in registers.
+//   YP, YW : the same for the y passes
+//   RP     : radices of the length-N/2 complex row transforms (x; real pairs packed)
+//   RB     : x-lines per block in the x pass, NTX its threads
 // ---------------------------------------------------------------------------
-template <int W_, int RB_, int NTX_, class CP_, class RP_>
+template <int ZW_, int YW_, int RB_, int NTX_, class ZP_, class YP_, class RP_>
 struct CfgBase {
-    using CP = CP_;
+    using ZP = ZP_;
+    using YP = YP_;
     using RP = RP_;
-    static constexpr int W = W_;
+    static constexpr int ZW = ZW_;
+    static constexpr int YW = YW_;
     static constexpr int RB = RB_;
     static constexpr int NTX = NTX_;
 };
 template <int N, class T>
 struct Cfg;
-// FP32: radix <= 16
-template <> struct Cfg<16, float>  : CfgBase<64, 64, 256, Plan<4, 4>,      Plan<8>>     {};
-template <> struct Cfg<32, float>  : CfgBase<64, 32, 256, Plan<8, 4>,      Plan<16>>    {};
-template <> struct Cfg<48, float>  : CfgBase<64, 16, 256, Plan<12, 4>,     Plan<8, 3>>  {};
-template <> struct Cfg<64, float>  : CfgBase<32, 16, 256, Plan<8, 8>,      Plan<8, 4>>  {};
-template <> struct Cfg<128, float> : CfgBase<32, 8, 256,  Plan<16, 8>,     Plan<8, 8>>  {};
-template <> struct Cfg<256, float> : CfgBase<16, 4, 256,  Plan<16, 16>,    Plan<16, 8>> {};
-template <> struct Cfg<512, float> : CfgBase<16, 2, 256,  Plan<16, 16, 2>, Plan<16, 16>>{};
-// FP64: radix <= 8 (register budget of the fused prologue/epilogue)
-template <> struct Cfg<16, double>  : CfgBase<64, 32, 256, Plan<4, 4>,    Plan<8>>       {};
-template <> struct Cfg<32, double>  : CfgBase<64, 16, 256, Plan<8, 4>,    Plan<4, 4>>    {};
-template <> struct Cfg<48, double>  : CfgBase<32, 8, 256,  Plan<12, 4>,   Plan<8, 3>>    {};
-template <> struct Cfg<64, double>  : CfgBase<32, 8, 256,  Plan<8, 8>,    Plan<8, 4>>    {};
-template <> struct Cfg<128, double> : CfgBase<16, 4, 256,  Plan<8, 4, 4>, Plan<8, 8>>    {};
-template <> struct Cfg<256, double> : CfgBase<8, 2, 256,   Plan<8, 8, 4>, Plan<8, 4, 4>> {};
-template <> struct Cfg<512, double> : CfgBase<8, 1, 256,   Plan<8, 8, 8>, Plan<8, 8, 4>> {};
+//                                        ZW  YW  RB  NTX  z plan              y plan            x (row) plan
+template <> struct Cfg<16, float>  : CfgBase<64, 64, 64, 256, Plan<4, 4>,       Plan<4, 4>,       Plan<8>>      {};
+template <> struct Cfg<32, float>  : CfgBase<32, 64, 32, 256, Plan<4, 2, 4>,    Plan<8, 4>,       Plan<16>>     {};
+template <> struct Cfg<48, float>  : CfgBase<64, 64, 16, 256, Plan<2, 12, 2>,   Plan<12, 4>,      Plan<8, 3>>   {};
+template <> struct Cfg<64, float>  : CfgBase<32, 32, 16, 256, Plan<8, 8>,       Plan<8, 8>,       Plan<8, 4>>   {};
+template <> struct Cfg<128, float> : CfgBase<16, 32, 8, 256,  Plan<4, 8, 4>,    Plan<16, 8>,      Plan<8, 8>>   {};
+template <> struct Cfg<256, float> : CfgBase<16, 16, 4, 256,  Plan<8, 4, 8>,    Plan<16, 16>,     Plan<16, 8>>  {};
+template <> struct Cfg<512, float> : CfgBase<8, 16, 2, 256,   Plan<8, 8, 8>,    Plan<16, 16, 2>,  Plan<16, 16>> {};
+template <> struct Cfg<16, double>  : CfgBase<64, 64, 32, 256, Plan<4, 4>,      Plan<4, 4>,       Plan<8>>       {};
+template <> struct Cfg<32, double>  : CfgBase<32, 64, 16, 256, Plan<4, 2, 4>,   Plan<8, 4>,       Plan<4, 4>>    {};
+template <> struct Cfg<48, double>  : CfgBase<64, 32, 8, 256,  Plan<2, 12, 2>,  Plan<12, 4>,      Plan<8, 3>>    {};
+template <> struct Cfg<64, double>  : CfgBase<16, 32, 8, 256,  Plan<4, 4, 4>,   Plan<8, 8>,       Plan<8, 4>>    {};
+template <> struct Cfg<128, double> : CfgBase<16, 16, 4, 256,  Plan<4, 8, 4>,   Plan<8, 4, 4>,    Plan<8, 8>>    {};
+template <> struct Cfg<256, double> : CfgBase<8, 8, 2, 256,    Plan<8, 4, 8>,   Plan<8, 8, 4>,    Plan<8, 4, 4>> {};
+template <> struct Cfg<512, double> : CfgBase<8, 8, 1, 256,    Plan<8, 8, 8>,   Plan<8, 8, 8>,    Plan<8, 8, 4>> {};
 
 template <int N, class T>
 struct Geo {
     using C = Cfg<N, T>;
-    using CF_I = ColFFT<typename C::CP, C::W, C::W, +1, T>;
-    using CF_F = ColFFT<typename C::CP, C::W, C::W, -1, T>;
+    using CFZ_I = ColFFT<typename C::ZP, C::ZW, C::ZW, +1, T>;
+    using CFZ_F = ColFFT<typename C::ZP, C::ZW, C::ZW, -1, T>;
     static constexpr int H = N / 2 + 1;                 // kx modes
     static constexpr long long FS = (long long)N * N * H; // elements per spectral field
-    static constexpr int NT = CF_I::NT;                 // threads of a column pass
-    static constexpr int RM = CF_I::RM;
+    static constexpr int ZNT = CFZ_I::NT;               // threads of a z pass
+    static constexpr int ZRM = CFZ_I::RM;
+    static constexpr int YNT = ColFFT<typename C::YP, C::YW, C::YW, +1, T>::NT;
     static constexpr int ZCOLS = N * H;                 // z-pass columns (ky, kx)
     static constexpr int YCOLS = N * H;                 // y-pass columns (kz, kx)
-    static constexpr int ZTILES = (ZCOLS + C::W - 1) / C::W;
-    static constexpr int YTILES = (YCOLS + C::W - 1) / C::W;
+    static constexpr int ZTILES = (ZCOLS + C::ZW - 1) / C::ZW;
+    static constexpr int YTILES = (YCOLS + C::YW - 1) / C::YW;
     static constexpr int XROWS = N * N;
     static constexpr int XBLOCKS = XROWS / C::RB;
     static_assert(XROWS % C::RB == 0, "x-lines per block must divide N^2");
-    static_assert(C::CP::L == N && C::RP::L == N / 2, "plan lengths");
+    static_assert(C::ZP::L == N && C::YP::L == N && C::RP::L == N / 2, "plan lengths");
+    static_assert(C::ZP::radix(0) == C::ZP::radix(C::ZP::S - 1), "z plans must be symmetric");
 };
 
 // ===========================================================================
@@ -180,28 +186,22 @@ struct ProScale {  // (u, v, w) -> s (u, v, w)               state_uhat2u
     }
 };
 
+// z-inverse of the FO prologue outputs, starting from the FI inputs held in
+// registers at the stage-0 positions of the z plan: U[f][q] = in_f(kz = in_pos(g, q)).
 template <int N, class T, class Pro>
-__device__ __forceinline__ void zinv_tile(const cpx<T>* const* __restrict__ in,
-                                          cpx<T>* const* __restrict__ out, int tile,
-                                          cpx<T>* lds, const cpx<T>* __restrict__ tw, const Pro& pro)
+__device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<N, T>::ZRM],
+                                               cpx<T>* const* __restrict__ out, int col, bool valid,
+                                               cpx<T>* lds, const cpx<T>* __restrict__ tw, const Pro& pro)
 {
     using G = Geo<N, T>;
-    using CF = typename G::CF_I;
-    constexpr int W = G::C::W, RM = G::RM, H = G::H;
+    using CF = typename G::CFZ_I;
+    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
     constexpr long long ZS = (long long)N * H;  // stride between kz planes
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
-    const int col = tile * W + c;
-    const bool valid = col < G::ZCOLS;
     const int ix = col % H, iy = col / H;
     const int kx = kval(ix, N), ky = kval(iy, N);
-    cpx<T> U[Pro::FI][RM];
 #pragma unroll
-    for (int f = 0; f < Pro::FI; ++f)
-#pragma unroll
-        for (int q = 0; q < RM; ++q)
-            U[f][q] = valid ? in[f][CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
-#pragma unroll 1
     for (int o = 0; o < Pro::FO; ++o) {
         cpx<T> a[RM];
 #pragma unroll
@@ -220,12 +220,34 @@ __device__ __forceinline__ void zinv_tile(const cpx<T>* const* __restrict__ in,
 }
 
 template <int N, class T, class Pro>
-__global__ void __launch_bounds__((Geo<N, T>::NT))
+__device__ __forceinline__ void zinv_tile(const cpx<T>* const* __restrict__ in,
+                                          cpx<T>* const* __restrict__ out, int tile,
+                                          cpx<T>* lds, const cpx<T>* __restrict__ tw, const Pro& pro)
+{
+    using G = Geo<N, T>;
+    using CF = typename G::CFZ_I;
+    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)N * H;
+    const int tid = threadIdx.x;
+    const int c = tid % W, g = tid / W;
+    const int col = tile * W + c;
+    const bool valid = col < G::ZCOLS;
+    cpx<T> U[Pro::FI][RM];
+#pragma unroll
+    for (int f = 0; f < Pro::FI; ++f)
+#pragma unroll
+        for (int q = 0; q < RM; ++q)
+            U[f][q] = valid ? in[f][CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
+    zinv_from_regs<N, T, Pro>(U, out, col, valid, lds, tw, pro);
+}
+
+template <int N, class T, class Pro>
+__global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zinv(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
            long long out_fs, const cpx<T>* __restrict__ tw, Pro pro)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::W];
+    __shared__ cpx<T> lds[N * G::C::ZW];
     const cpx<T>* in[Pro::FI];
     cpx<T>* out[Pro::FO];
 #pragma unroll
@@ -243,8 +265,8 @@ __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, c
                                           const cpx<T>* __restrict__ tw)
 {
     using G = Geo<N, T>;
-    using CF = ColFFT<typename G::C::CP, G::C::W, G::C::W, DIR, T>;
-    constexpr int W = G::C::W, RM = G::RM, H = G::H;
+    using CF = ColFFT<typename G::C::YP, G::C::YW, G::C::YW, DIR, T>;
+    constexpr int W = G::C::YW, RM = CF::RM, H = G::H;
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
     const int col = tile * W + c;
@@ -262,11 +284,11 @@ __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, c
 }
 
 template <int N, class T, int DIR>
-__global__ void __launch_bounds__((Geo<N, T>::NT))
+__global__ void __launch_bounds__((Geo<N, T>::YNT))
     k_ycol(cpx<T>* __restrict__ data0, long long fs, const cpx<T>* __restrict__ tw)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::W];
+    __shared__ cpx<T> lds[N * G::C::YW];
     ycol_tile<N, T, DIR>(data0 + blockIdx.y * fs, blockIdx.x, lds, tw);
 }
 
@@ -509,24 +531,19 @@ __device__ __forceinline__ double step_dt(BoxScalars* sc, const BoxPhys& ph)
     return dt;
 }
 
-template <int N, class T, int MODE>
-__device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long long in_fs, int tile,
-                                          cpx<T>* lds, const cpx<T>* __restrict__ tw,
-                                          const EpiArgs<T>& ea)
+// forward z FFT of three fields of a column tile; O[f][q] holds mode kz = out_pos(g, q)
+template <int N, class T>
+__device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long long in_fs, int col,
+                                          bool valid, cpx<T>* lds, const cpx<T>* __restrict__ tw,
+                                          cpx<T> (&O)[3][Geo<N, T>::ZRM])
 {
     using G = Geo<N, T>;
-    using CF = typename G::CF_F;
-    constexpr int W = G::C::W, RM = G::RM, H = G::H;
+    using CF = typename G::CFZ_F;
+    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
     constexpr long long ZS = (long long)N * H;
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
-    const int col = tile * W + c;
-    const bool valid = col < G::ZCOLS;
-    const int ix = col % H, iy = col / H;
-    const int kx = kval(ix, N), ky = kval(iy, N);
-
-    cpx<T> O[3][RM];
-#pragma unroll 1
+#pragma unroll
     for (int f = 0; f < 3; ++f) {
         cpx<T> a[RM];
         const cpx<T>* src = in0 + f * in_fs;
@@ -536,8 +553,23 @@ __device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long l
 #pragma unroll
         for (int q = 0; q < RM; ++q) O[f][q] = a[q];
     }
-    if (!valid) return;
+}
 
+// Stepper epilogue on the forward-transformed n_hat = FFT((omega x u)/N^3) of a column
+// tile: mask, pressure projection, band / Kolmogorov forcing (rhs.f90:67-133), then
+// the update of MODE.  When NEXT is set, U receives the input of the next RHS at the
+// same positions (the predicted state for PRED, the new state for CORR).
+template <int N, class T, int MODE, bool NEXT>
+__device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZRM], int col, bool valid,
+                                              const EpiArgs<T>& ea, cpx<T> (&U)[3][Geo<N, T>::ZRM])
+{
+    using G = Geo<N, T>;
+    using CF = typename G::CFZ_F;
+    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)N * H;
+    const int g = threadIdx.x / W;
+    const int ix = col % H, iy = col / H;
+    const int kx = kval(ix, N), ky = kval(iy, N);
     const BoxPhys& ph = ea.ph;
     double dt = 0.0;
     if constexpr (MODE == EPI_PRED) dt = step_dt<N>(ea.sc, ph);
@@ -553,11 +585,13 @@ __device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long l
         const long long idx = (long long)iz * ZS + col;
         cpx<T> nl[3];
         if constexpr (MODE == EPI_STORE) {
+            if (valid) {
 #pragma unroll
-            for (int f = 0; f < 3; ++f) ea.out[f * ea.fs + idx] = O[f][q];
+                for (int f = 0; f < 3; ++f) ea.out[f * ea.fs + idx] = O[f][q];
+            }
             continue;
         } else {
-            if (masked(ix, iy, iz, kx, ky, kz, N, ph)) {
+            if (!valid || masked(ix, iy, iz, kx, ky, kz, N, ph)) {
 #pragma unroll
                 for (int f = 0; f < 3; ++f) nl[f] = cpx<T>{T(0), T(0)};
             } else {
@@ -580,8 +614,10 @@ __device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long l
             }
         }
         if constexpr (MODE == EPI_RHS) {
+            if (valid) {
 #pragma unroll
-            for (int f = 0; f < 3; ++f) ea.out[f * ea.fs + idx] = nl[f];
+                for (int f = 0; f < 3; ++f) ea.out[f * ea.fs + idx] = nl[f];
+            }
         } else {
             // rhs.f90:8-16 integrating factor, exponent in FP64
             const double k2d = double(kx * kx + ky * ky + kz * kz);
@@ -590,21 +626,28 @@ __device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long l
             if constexpr (MODE == EPI_PRED) {  // dnsboxheun.f90:88-114
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    const cpx<T> u = ea.state[f * ea.fs + idx];
-                    ea.out[f * ea.fs + idx] = e * (u + tdt * nl[f]);
-                    ea.state[f * ea.fs + idx] = e * (u + (tdt * T(0.5)) * nl[f]);
-                    if (ea.nold) ea.nold[f * ea.fs + idx] = nl[f];
+                    const cpx<T> u = valid ? ea.state[f * ea.fs + idx] : cpx<T>{T(0), T(0)};
+                    const cpx<T> up = e * (u + tdt * nl[f]);
+                    if (NEXT) U[f][q] = up;
+                    if (valid) {
+                        if (ea.out) ea.out[f * ea.fs + idx] = up;
+                        ea.state[f * ea.fs + idx] = e * (u + (tdt * T(0.5)) * nl[f]);
+                        if (ea.nold) ea.nold[f * ea.fs + idx] = nl[f];
+                    }
                 }
             } else if constexpr (MODE == EPI_CORR) {  // dnsboxheun.f90:120-126
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    const cpx<T> u = ea.state[f * ea.fs + idx];
-                    ea.state[f * ea.fs + idx] = u + (tdt * T(0.5)) * nl[f];
+                    const cpx<T> u = valid ? ea.state[f * ea.fs + idx] : cpx<T>{T(0), T(0)};
+                    const cpx<T> un = u + (tdt * T(0.5)) * nl[f];
+                    if (NEXT) U[f][q] = un;
+                    if (valid) ea.state[f * ea.fs + idx] = un;
                 }
             } else if constexpr (MODE == EPI_AB2) {
                 const T e2 = T(e64 * e64);
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
+                    if (!valid) continue;
                     const cpx<T> u = ea.state[f * ea.fs + idx];
                     const cpx<T> no = ea.nold[f * ea.fs + idx];
                     ea.state[f * ea.fs + idx] =
@@ -616,25 +659,58 @@ __device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long l
     }
 }
 
+template <int N>
+__device__ __forceinline__ void pred_scalars(const BoxScalars* /*unused*/, BoxScalars* sc, const BoxPhys& ph)
+{
+    const double dt = step_dt<N>(sc, ph);
+    if (ph.adaptive) {
+        const double dx = 6.283185307179586476925286766559 / double(N);
+        sc->courant = (__longlong_as_double((long long)sc->maxu[0]) +
+                       __longlong_as_double((long long)sc->maxu[1]) +
+                       __longlong_as_double((long long)sc->maxu[2])) * sc->dt_prev / dx;
+    }
+    sc->dt = dt;
+}
+
 template <int N, class T, int MODE>
-__global__ void __launch_bounds__((Geo<N, T>::NT))
+__global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zfwd(const cpx<T>* __restrict__ in0, long long in_fs, const cpx<T>* __restrict__ tw,
            EpiArgs<T> ea)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::W];
-    zfwd_tile<N, T, MODE>(in0, in_fs, blockIdx.x, lds, tw, ea);
+    __shared__ cpx<T> lds[N * G::C::ZW];
+    const int col = blockIdx.x * G::C::ZW + threadIdx.x % G::C::ZW;
+    const bool valid = col < G::ZCOLS;
+    cpx<T> O[3][G::ZRM];
+    zfwd_regs<N, T>(in0, in_fs, col, valid, lds, tw, O);
+    zfwd_epilogue<N, T, MODE, false>(O, col, valid, ea, O);
     if constexpr (MODE == EPI_PRED) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            const double dt = step_dt<N>(ea.sc, ea.ph);
-            if (ea.ph.adaptive) {
-                const double dx = 6.283185307179586476925286766559 / double(N);
-                ea.sc->courant = (__longlong_as_double((long long)ea.sc->maxu[0]) +
-                                  __longlong_as_double((long long)ea.sc->maxu[1]) +
-                                  __longlong_as_double((long long)ea.sc->maxu[2])) * ea.sc->dt_prev / dx;
-            }
-            ea.sc->dt = dt;
-        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
+    }
+}
+
+// Forward z pass + stepper epilogue (PRED or CORR) fused with the z-inverse + curl
+// of the next RHS: the next RHS input never makes a round trip through HBM.  The
+// input tile (fields 0..2 of X) and the output tile (fields 0..5 of X) cover the
+// same columns, so the block reads its whole tile before it writes.
+template <int N, class T, int MODE>
+__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+    k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw, EpiArgs<T> ea,
+                ProCurl<T> pro)
+{
+    using G = Geo<N, T>;
+    __shared__ cpx<T> lds[N * G::C::ZW];
+    const int col = blockIdx.x * G::C::ZW + threadIdx.x % G::C::ZW;
+    const bool valid = col < G::ZCOLS;
+    cpx<T> O[3][G::ZRM];
+    zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
+    zfwd_epilogue<N, T, MODE, true>(O, col, valid, ea, O);
+    cpx<T>* out[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) out[f] = X + f * fs;
+    zinv_from_regs<N, T, ProCurl<T>>(O, out, col, valid, lds, tw, pro);
+    if constexpr (MODE == EPI_PRED) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
     }
 }
 

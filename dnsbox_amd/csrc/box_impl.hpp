@@ -61,6 +61,7 @@ class BoxBase {
     unsigned long long* red_bits = nullptr;  // 4 slots
     int red_blocks = 0;
     bool ab2_started = false;
+    bool x6_valid = false;   // scratch X holds the z-inverse + curl of the current state
 
     virtual void rhs(const void* in, void* out) = 0;
     virtual void heun_step() = 0;
@@ -85,7 +86,8 @@ template <int N, class T>
 class BoxT final : public BoxBase {
     using G = Geo<N, T>;
     using C = cpx<T>;
-    static constexpr int NT = G::NT;
+    static constexpr int ZNT = G::ZNT;
+    static constexpr int YNT = G::YNT;
 
    public:
     explicit BoxT(const BoxConfig& c)
@@ -149,17 +151,17 @@ class BoxT final : public BoxBase {
     void rhs_passes(const C* in, const EpiArgs<T>& ea, bool track_cfl)
     {
         const T s = T(1.0 / (double(N) * N * N));
-        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(NT), 0, stream, in, fs,
+        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in, fs,
                            X(), fs, twc(), ProCurl<T>{s});
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(NT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
                            stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(NT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_zfwd<N, T, MODE>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs, twc(),
+        hipLaunchKernelGGL((k_zfwd<N, T, MODE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
                            ea);
         DNSBOX_LAUNCH_CHECK();
     }
@@ -191,28 +193,56 @@ class BoxT final : public BoxBase {
         rhs_passes<EPI_RHS>(static_cast<const C*>(in), ea, false);
     }
 
+    // middle passes of an RHS: y-inverse (6), x cross product, y-forward (3)
+    void rhs_middle(bool track_cfl)
+    {
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
+                           stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    // IF-Heun step (dnsboxheun.f90:84-126) as a pipeline: the forward z pass of each
+    // RHS carries the stepper epilogue and the z-inverse + curl of the next RHS.
     void heun_step() override
     {
         C* S = static_cast<C*>(state);
         C* P = static_cast<C*>(pred);
-        eband(S);
+        const T s = T(1.0 / (double(N) * N * N));
+        if (!x6_valid) {
+            eband(S);
+            hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, S, fs,
+                               X(), fs, twc(), ProCurl<T>{s});
+            DNSBOX_LAUNCH_CHECK();
+        }
+        rhs_middle(true);
         EpiArgs<T> ea = epi();
-        ea.out = P;
+        ea.out = cfg.ph.bandlim ? P : nullptr;  // the predicted state is kept only for band forcing
         ea.state = S;
         ea.rhs_in = S;
-        ea.nold = nullptr;
-        rhs_passes<EPI_PRED>(S, ea, true);
+        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
+                           twc(), ea, ProCurl<T>{s});
+        DNSBOX_LAUNCH_CHECK();
         eband(P);
+        rhs_middle(false);
         EpiArgs<T> eb = epi();
         eb.state = S;
         eb.rhs_in = P;
-        rhs_passes<EPI_CORR>(P, eb, false);
+        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
+                           twc(), eb, ProCurl<T>{s});
+        DNSBOX_LAUNCH_CHECK();
+        eband(S);
         hipLaunchKernelGGL(k_step_end<0>, dim3(1), dim3(64), 0, stream, sc);
         DNSBOX_LAUNCH_CHECK();
+        x6_valid = true;
     }
 
     void ab2_step() override
     {
+        x6_valid = false;
         C* S = static_cast<C*>(state);
         if (!nold) DNSBOX_HIP(hipMalloc(&nold, 3 * size_t(fs) * sizeof(C)));
         if (!ab2_started) {
@@ -259,10 +289,10 @@ class BoxT final : public BoxBase {
     {
         DNSBOX_HIP(hipMemsetAsync(red_bits, 0, 4 * sizeof(unsigned long long), stream));
         const T s = T(1.0 / (double(N) * N * N));
-        hipLaunchKernelGGL((k_zinv<N, T, ProVelDiv<T>>), dim3(G::ZTILES), dim3(NT), 0, stream,
+        hipLaunchKernelGGL((k_zinv<N, T, ProVelDiv<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream,
                            static_cast<const C*>(u), fs, X(), fs, twc(), ProVelDiv<T>{s});
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 4), dim3(NT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 4), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_xpass<N, T, OpVelDivMax<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
                            stream, X(), fs, X(), fs, twc(), red_bits, 1);
@@ -291,10 +321,10 @@ class BoxT final : public BoxBase {
     void to_physical(const void* spec, void* phys3) override
     {
         const T s = T(1.0 / (double(N) * N * N));
-        hipLaunchKernelGGL((k_zinv<N, T, ProScale<T>>), dim3(G::ZTILES), dim3(NT), 0, stream,
+        hipLaunchKernelGGL((k_zinv<N, T, ProScale<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream,
                            static_cast<const C*>(spec), fs, X(), fs, twc(), ProScale<T>{s});
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 3), dim3(NT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
         // real output field f at phys3 + f * N^3 reals
         const long long pf = (long long)N * N * N / 2;  // complex pairs per real field
@@ -311,11 +341,11 @@ class BoxT final : public BoxBase {
                            dim3(Cfg<N, T>::NTX), 0, stream, static_cast<const C*>(phys3), pf, X(), fs,
                            twc(), (unsigned long long*)nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(NT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
         EpiArgs<T> ea = epi();
         ea.out = static_cast<C*>(spec);
-        hipLaunchKernelGGL((k_zfwd<N, T, EPI_STORE>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs,
+        hipLaunchKernelGGL((k_zfwd<N, T, EPI_STORE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
                            twc(), ea);
         DNSBOX_LAUNCH_CHECK();
     }
@@ -324,7 +354,6 @@ class BoxT final : public BoxBase {
     {
         const T s = T(1.0 / (double(N) * N * N));
         C* P = static_cast<C*>(pred);
-        C* NL = static_cast<C*>(nonlin);
         const C* S = static_cast<const C*>(state);
         hipEvent_t e0, e1;
         DNSBOX_HIP(hipEventCreate(&e0));
@@ -340,30 +369,30 @@ class BoxT final : public BoxBase {
             ms[which] = double(t) / reps;
         };
         timed(0, [&] {
-            hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(NT), 0, stream, S, fs, X(),
+            hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, S, fs, X(),
                                fs, twc(), ProCurl<T>{s});
         });
         timed(1, [&] {
-            hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(NT), 0, stream, X(), fs, twc());
+            hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
         });
         timed(2, [&] {
             hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream,
                                X(), fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
         });
         timed(3, [&] {
-            hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(NT), 0, stream, X(), fs, twc());
+            hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
         });
         EpiArgs<T> ea = epi();
-        ea.out = NL;
+        ea.out = nullptr;
         ea.state = P;
         ea.rhs_in = P;
         timed(4, [&] {
-            hipLaunchKernelGGL((k_zfwd<N, T, EPI_PRED>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs, twc(),
-                               ea);
+            hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
+                               twc(), ea, ProCurl<T>{s});
         });
         timed(5, [&] {
-            hipLaunchKernelGGL((k_zfwd<N, T, EPI_CORR>), dim3(G::ZTILES), dim3(NT), 0, stream, X(), fs, twc(),
-                               ea);
+            hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
+                               twc(), ea, ProCurl<T>{s});
         });
         DNSBOX_LAUNCH_CHECK();
         (void)hipEventDestroy(e0);

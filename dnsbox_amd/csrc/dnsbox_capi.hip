@@ -236,6 +236,7 @@ int dnsbox_nspec(dnsbox_ctx* c)
 int dnsbox_set_state(dnsbox_ctx* c, const double* u, const double* v, const double* w)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null state array");
     return guarded([&]() -> int {
         const int n = c->p.n;
@@ -280,6 +281,7 @@ int dnsbox_get_nonlin(dnsbox_ctx* c, double* u, double* v, double* w)
 int dnsbox_set_state_physical(dnsbox_ctx* c, const double* u, const double* v, const double* w)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
     return guarded([&]() -> int {
         BoxBase* b = c->box.get();
@@ -299,6 +301,7 @@ int dnsbox_set_state_physical(dnsbox_ctx* c, const double* u, const double* v, c
 int dnsbox_get_state_physical(dnsbox_ctx* c, double* u, double* v, double* w)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
     return guarded([&]() -> int {
         BoxBase* b = c->box.get();
@@ -316,6 +319,7 @@ int dnsbox_get_state_physical(dnsbox_ctx* c, double* u, double* v, double* w)
 int dnsbox_init_random(dnsbox_ctx* c, uint64_t seed, int box, double k0, double energy)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     if (!(k0 > 0.0) || !(energy >= 0.0)) return fail(DNSBOX_E_ARG, "k0 must be > 0, energy >= 0");
     return guarded([&]() -> int {
         c->box->init_random(seed, box, k0, energy);
@@ -327,6 +331,7 @@ int dnsbox_init_random(dnsbox_ctx* c, uint64_t seed, int box, double k0, double 
 int dnsbox_project(dnsbox_ctx* c, int dealias, int project)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     return guarded([&]() -> int {
         c->box->project(dealias, project);
         return 0;
@@ -352,6 +357,7 @@ int dnsbox_int_fact(dnsbox_ctx* c, double dt)
 int dnsbox_rhs_nonlinear(dnsbox_ctx* c)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     return guarded([&]() -> int {
         c->box->rhs(c->box->state, c->box->nonlin);
         return 0;
@@ -408,6 +414,7 @@ int dnsbox_set_time(dnsbox_ctx* c, double t, double dt)
 int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     if (!diag) return fail(DNSBOX_E_ARG, "null diag");
     return guarded([&]() -> int {
         BoxBase* b = c->box.get();
@@ -449,6 +456,7 @@ int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
 int dnsbox_time_passes(dnsbox_ctx* c, int reps, double* ms)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     if (reps < 1 || !ms) return fail(DNSBOX_E_ARG, "reps >= 1 and an output array are required");
     return guarded([&]() -> int {
         c->box->time_passes(reps, ms);
@@ -467,6 +475,7 @@ int dnsbox_state_ptr(dnsbox_ctx* c, void** p)
 int dnsbox_rhs_dev(dnsbox_ctx* c, const void* in, void* out)
 {
     if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
     if (!in || !out) return fail(DNSBOX_E_ARG, "null device pointer");
     return guarded([&]() -> int {
         c->box->rhs(in, out);

@@ -130,8 +130,10 @@ enum {
     DNSBOX_PASS_YINV,      /* y-inverse FFT of 6 fields in place: reads 6, writes 6       */
     DNSBOX_PASS_XCROSS,    /* c2r + (omega x u) + r2c along x: reads 6, writes 3          */
     DNSBOX_PASS_YFWD,      /* y-forward FFT of 3 fields in place: reads 3, writes 3       */
-    DNSBOX_PASS_ZPRED,     /* z-forward FFT + projection + IF-Heun predictor: reads 6, writes 6 */
-    DNSBOX_PASS_ZCORR,     /* z-forward FFT + projection + IF-Heun corrector: reads 6, writes 3 */
+    DNSBOX_PASS_ZPRED,     /* z-forward FFT + projection + IF-Heun predictor + z-inverse/curl of
+                              the predicted state: reads 6, writes 9                        */
+    DNSBOX_PASS_ZCORR,     /* z-forward FFT + projection + IF-Heun corrector + z-inverse/curl of
+                              the new state: reads 6, writes 9                              */
     DNSBOX_NPASS
 };
 int dnsbox_time_passes(dnsbox_ctx* ctx, int reps, double* ms);

@@ -141,3 +141,83 @@ def test_fp32_rhs_close_to_fp64(gpu_lib):
     err = rel_l2(got, ref)
     print(f"fp32 rhs rel_l2={err:.3e}", flush=True)
     assert err < 1e-5
+
+
+def test_adaptive_heun_matches_oracle(gpu_lib):
+    """CFL controller (dnsboxheun.f90:153-157, 227-246) evaluated on the current field."""
+    n = 32
+    o = Oracle(n, nu=0.01, kolmo_amp=1.0, kolmo_k=4)
+    f = o.init_solenoidal(1, 0, 4.0, 0.5)
+    ref, dt_ref, t_ref = o.heun_adaptive(f, 40, 0.003, 0.5, 0.5, 0.01)
+    with make(n, nu=0.01, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=1, cfl_min=0.5,
+              cfl_max=0.5, dt_max=0.01, dt=0.003) as b:
+        b.set_state(*f)
+        b.step(40)
+        got = b.get_state()
+        t, dt = b.time()
+    print(f"adaptive: t={t} ({t_ref}) dt={dt} ({dt_ref}) err={rel_l2(got, ref):.3e}", flush=True)
+    assert abs(t - t_ref) < 1e-12 and abs(dt - dt_ref) < 1e-13
+    assert rel_l2(got, ref) < 1e-10
+
+
+def test_bandlimited_heun_matches_oracle(gpu_lib):
+    n = 32
+    o = Oracle(n, nu=0.005, Pin=0.1, kcut=2.0, bandlim=True)
+    f = o.init_solenoidal(6, 0, 3.0, 0.5)
+    ref = o.heun(f, 50, 0.004)
+    with make(n, nu=0.005, Pin=0.1, kcut=2.0, bandlim=1, adaptive_dt=0, dt=0.004) as b:
+        b.set_state(*f)
+        b.step(50)
+        got = b.get_state()
+    err = rel_l2(got, ref)
+    print(f"band heun rel_l2={err:.3e}", flush=True)
+    assert err < 1e-10
+
+
+def test_ab2_matches_oracle(gpu_lib):
+    n = 32
+    o = Oracle(n, nu=0.05, kolmo_amp=1.0, kolmo_k=4)
+    f = o.init_solenoidal(4, 0, 4.0, 0.5)
+    ref = o.ab2(f, 60, 0.005)
+    with make(n, nu=0.05, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=0.005) as b:
+        b.set_state(*f)
+        b.step(60, stepper=2)
+        got = b.get_state()
+    err = rel_l2(got, ref)
+    print(f"ab2 rel_l2={err:.3e}", flush=True)
+    assert err < 1e-10
+
+
+def test_step_pipeline_is_call_invariant(gpu_lib):
+    """The fused z pass keeps the next RHS's z-inverse in scratch across dnsbox_step calls;
+    splitting a run into calls, or interleaving diagnostics, changes nothing."""
+    n = 32
+    kw = dict(nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=1, cfl_min=0.5,
+              cfl_max=0.5, dt=0.003)
+    with make(n, **kw) as a, make(n, **kw) as b:
+        a.init_random(9, 0, 4.0, 0.5)
+        b.init_random(9, 0, 4.0, 0.5)
+        a.step(50)
+        b.step(20)
+        b.diagnostics()
+        b.step(13)
+        b.rhs_nonlinear()
+        b.step(17)
+        sa, sb = a.get_state(), b.get_state()
+        assert all(np.array_equal(x, y) for x, y in zip(sa, sb))
+        assert a.time() == b.time()
+
+
+def test_fp32_heun_close_to_fp64(gpu_lib):
+    n = 64
+    kw = dict(nu=0.01, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=0.004)
+    with make(n, precision=64, **kw) as a, make(n, precision=32, **kw) as b:
+        for x in (a, b):
+            x.init_random(2, 0, 4.0, 0.5)
+            x.step(100)
+        (d64, s64), (d32, s32) = a.diagnostics(), b.diagnostics()
+    rel = np.abs(s32 - s64)[s64 > 0] / s64[s64 > 0]
+    print(f"fp32 vs fp64 after 100 steps: energy {abs(d32[0]-d64[0])/d64[0]:.2e} "
+          f"spectrum max {rel.max():.2e}", flush=True)
+    assert abs(d32[0] - d64[0]) / d64[0] < 1e-5
+    assert rel[:20].max() < 1e-4
