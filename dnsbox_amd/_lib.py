@@ -64,6 +64,10 @@ SIGNATURES = {
     "dnsbox_time_passes": (C.c_int, [_vp, C.c_int, _dp]),
     "dnsbox_state_ptr": (C.c_int, [_vp, C.POINTER(_vp)]),
     "dnsbox_rhs_dev": (C.c_int, [_vp, _vp, _vp]),
+    "dnsbox_step_dev": (C.c_int, [_vp, _vp, C.c_int]),
+    "dnsbox_rhs_vjp_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "dnsbox_step_vjp_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
+    "dnsbox_rhs_adj_dev": (C.c_int, [_vp, _vp, _vp, C.c_int]),
 }
 
 _lib = None

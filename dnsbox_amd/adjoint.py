@@ -1,0 +1,144 @@
+"""Differentiable T-step map and adjoint descent on periodic-orbit residuals.
+
+``step_map(box, u0, T, dt, k)`` returns u_T = Phi_T(u0) for T fixed-dt IF-Heun steps
+(dnsboxheun.f90:84-126) as a ``torch.autograd.Function``.  The forward keeps a
+checkpoint every k steps; the backward recomputes each segment from its checkpoint
+and runs the hand-written step-adjoint kernels (``dnsbox_step_vjp_dev``) backwards
+through it.  Gradients follow torch's convention for complex tensors
+(dL/dRe + i dL/dIm), i.e. the Euclidean adjoint in the stored coordinates.
+
+``residual`` is J(u) = 1/2 <|Phi_T(u) - u|^2> (the mean-square physical residual, in
+FP64), ``adjoint_descent`` minimises it with an FP64 backtracking line search along
+the projected L2-metric gradient.  This is the T-step-map counterpart of the
+reference's equilibrium adjoint descent (dnsboxAdj.f90:76-126 with rhsAdj.f90).
+"""
+from __future__ import annotations
+
+import torch
+
+
+def _k_tables(n, device):
+    k = torch.arange(n, device=device, dtype=torch.float64)
+    k = torch.where(k < n // 2, k, torch.where(k == n // 2, torch.zeros_like(k), k - n))
+    kz = k.view(n, 1, 1)
+    ky = k.view(1, n, 1)
+    kx = k[: n // 2 + 1].view(1, 1, n // 2 + 1)
+    return kx, ky, kz
+
+
+def hermitian_weight(n, device):
+    """1 on the kx = 0 and kx = N/2 planes, 2 elsewhere (c2r counts those twice)."""
+    w = torch.full((1, 1, n // 2 + 1), 2.0, dtype=torch.float64, device=device)
+    w[..., 0] = 1.0
+    w[..., n // 2] = 1.0
+    return w
+
+
+class _StepMap(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0, box, T, dt, k):
+        box.rhs_int_fact(dt)
+        u = u0.detach().clone().contiguous()
+        ckpts = []
+        for t in range(T):
+            if t % k == 0:
+                ckpts.append(u.clone())
+            box.step_dev(u, 1)
+        ctx.box, ctx.T, ctx.dt, ctx.k = box, T, dt, k
+        ctx.ckpts = ckpts
+        return u
+
+    @staticmethod
+    def backward(ctx, g):
+        box, T, dt, k = ctx.box, ctx.T, ctx.dt, ctx.k
+        box.rhs_int_fact(dt)
+        lam = g.detach().clone().contiguous()
+        out = torch.empty_like(lam)
+        for seg in reversed(range(len(ctx.ckpts))):
+            start = seg * k
+            stop = min(start + k, T)
+            states = [ctx.ckpts[seg]]
+            for _ in range(start, stop - 1):
+                u = states[-1].clone()
+                box.step_dev(u, 1)
+                states.append(u)
+            for j in reversed(range(stop - start)):
+                box.step_vjp_dev(states[j], lam, out, dt)
+                lam, out = out, lam
+        return lam, None, None, None, None
+
+
+def step_map(box, u0, T: int, dt: float, k: int = 20):
+    """Phi_T(u0): T fixed-dt IF-Heun steps, differentiable, checkpointed every k steps."""
+    return _StepMap.apply(u0, box, T, dt, k)
+
+
+def mean_square(r, n):
+    """<|r_phys|^2> of a spectral field (3, N, N, N/2+1) in FP64 (Parseval, Hermitian weights)."""
+    w = hermitian_weight(n, r.device)
+    rr = r.to(torch.complex128)
+    return (w * (rr.real ** 2 + rr.imag ** 2)).sum() / float(n) ** 6
+
+
+def residual(box, u0, T, dt, k=20):
+    """J(u0) = 1/2 <|Phi_T(u0) - u0|^2>."""
+    uT = step_map(box, u0, T, dt, k)
+    return 0.5 * mean_square(uT - u0, box.n)
+
+
+def project_masked(g, n, parity=True):
+    """Leray projection and 2/3 mask (state_project + state_dealias, state.f90:120-209)."""
+    kx, ky, kz = _k_tables(n, g.device)
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    thr = (n / 2.0) * (2.0 / 3.0)
+    m = (kx.abs() >= thr) | (ky.abs() >= thr) | (kz.abs() >= thr) | (k2 == 0)
+    if not parity:
+        idx = torch.arange(n, device=g.device)
+        nyq = idx == n // 2
+        m = m | nyq.view(n, 1, 1) | nyq.view(1, n, 1) | nyq[: n // 2 + 1].view(1, 1, n // 2 + 1)
+    k2s = torch.where(k2 == 0, torch.ones_like(k2), k2)
+    gg = g.to(torch.complex128)
+    kd = (kx * gg[0] + ky * gg[1] + kz * gg[2]) / k2s
+    out = torch.stack([gg[0] - kx * kd, gg[1] - ky * kd, gg[2] - kz * kd])
+    return torch.where(m.unsqueeze(0), torch.zeros_like(out), out).to(g.dtype)
+
+
+def adjoint_descent(box, u0, T, dt, k=20, iters=50, step0=1.0, c1=1e-4, shrink=0.5,
+                    max_backtrack=20, callback=None):
+    """Minimise J(u) = 1/2 <|Phi_T(u) - u|^2> from u0 by gradient descent in the L2 metric
+    (W^-1 times the Euclidean gradient, projected onto retained solenoidal fields) with an
+    Armijo backtracking line search in FP64.  Returns (u, history of J)."""
+    n = box.n
+    w = hermitian_weight(n, u0.device)
+    u = u0.detach().clone()
+    hist = []
+    alpha = step0
+    for it in range(iters):
+        x = u.clone().requires_grad_(True)
+        J = residual(box, x, T, dt, k)
+        (g,) = torch.autograd.grad(J, x)
+        # J = 1/2 sum W |r|^2 / N^6 in the metric <a, b>_M = sum W Re(conj a b) / N^6, whose
+        # gradient is N^6 W^-1 times the Euclidean one; descend along its projection
+        gm = g.to(torch.complex128) / w * float(n) ** 6
+        d = -project_masked(gm, n).to(u.dtype)
+        slope = float((g.to(torch.complex128).conj() * d.to(torch.complex128)).real.sum())
+        J0 = float(J)
+        hist.append(J0)
+        if callback:
+            callback(it, J0)
+        accepted = False
+        for _ in range(max_backtrack):
+            trial = u + alpha * d
+            with torch.no_grad():
+                Jt = float(residual(box, trial, T, dt, k))
+            if Jt <= J0 + c1 * alpha * slope:
+                u = trial
+                accepted = True
+                alpha *= 2.0
+                break
+            alpha *= shrink
+        if not accepted:
+            break
+    with torch.no_grad():
+        hist.append(float(residual(box, u, T, dt, k)))
+    return u, hist

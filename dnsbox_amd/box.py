@@ -181,5 +181,49 @@ class Box:
         check(self.lib.dnsbox_state_ptr(self._h, C.byref(p)))
         return p.value
 
-    def rhs_dev(self, in_ptr: int, out_ptr: int):
-        check(self.lib.dnsbox_rhs_dev(self._h, C.c_void_p(in_ptr), C.c_void_p(out_ptr)))
+    # -- device tensors (torch, internal layout (3, N, N, N/2+1), context precision) ----
+    def _check_tensor(self, t):
+        import torch
+        want = torch.complex128 if self.precision == 64 else torch.complex64
+        shape = (3, self.n, self.n, self.n // 2 + 1)
+        if not (t.is_cuda and t.dtype == want and tuple(t.shape) == shape and t.is_contiguous()):
+            raise ValueError(f"expected a contiguous cuda {want} tensor of shape {shape}, got "
+                             f"{t.dtype} {tuple(t.shape)} on {t.device}")
+        return C.c_void_p(t.data_ptr())
+
+    def empty_state(self):
+        import torch
+        dt = torch.complex128 if self.precision == 64 else torch.complex64
+        return torch.zeros((3, self.n, self.n, self.n // 2 + 1), dtype=dt, device="cuda")
+
+    def _on_torch_stream(self):
+        import torch
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def rhs_dev(self, u, out):
+        """out <- N(u) (rhs_nonlinear on device tensors)."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_rhs_dev(self._h, self._check_tensor(u), self._check_tensor(out)))
+
+    def step_dev(self, state, nsteps: int = 1):
+        """IF-Heun steps in place on a device tensor (dt from rhs_int_fact or the controller)."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_step_dev(self._h, self._check_tensor(state), nsteps))
+
+    def rhs_vjp_dev(self, u, lam, out):
+        """out <- J_N(u)^T lam (Euclidean inner product of the stored coordinates)."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_rhs_vjp_dev(self._h, self._check_tensor(u), self._check_tensor(lam),
+                                          self._check_tensor(out)))
+
+    def step_vjp_dev(self, u, lam, out, dt: float):
+        """out <- (d u_{n+1}/d u_n)^T lam for one fixed-dt IF-Heun step from u."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_step_vjp_dev(self._h, self._check_tensor(u), self._check_tensor(lam),
+                                           self._check_tensor(out), dt))
+
+    def rhs_adj_dev(self, u, out, lin_sign: int = 1):
+        """rhsAdj (rhsAdj.f90:356-381, phatadj read as phat); lin_sign=-1: descent direction."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_rhs_adj_dev(self._h, self._check_tensor(u), self._check_tensor(out),
+                                          lin_sign))

@@ -148,13 +148,19 @@ struct Geo {
 // ===========================================================================
 // z-inverse pass with a fused spectral prologue
 // ===========================================================================
+// table wavenumbers and indices of one mode
+struct ModeK {
+    int kx, ky, kz, ix, iy, iz;
+};
+
 // Prologue functors: FI input fields -> FO output fields, mode by mode.
 template <class T>
 struct ProCurl {  // (u, v, w) -> s (u, v, w, i k x u)      state_derivatives + state_vorticity
     static constexpr int FI = 3, FO = 6;
     T s;
-    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, int kx, int ky, int kz) const
+    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
     {
+        const int kx = m.kx, ky = m.ky, kz = m.kz;
         const cpx<T> u = in[0], v = in[1], w = in[2];
         switch (o) {
             case 0: return s * u;
@@ -170,8 +176,9 @@ template <class T>
 struct ProVelDiv {  // (u, v, w) -> s (u, v, w, i k . u)       io_Courant + state_divergence
     static constexpr int FI = 3, FO = 4;
     T s;
-    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, int kx, int ky, int kz) const
+    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
     {
+        const int kx = m.kx, ky = m.ky, kz = m.kz;
         if (o < 3) return s * in[o];
         return s * mul_i<1>(T(kx) * in[0] + T(ky) * in[1] + T(kz) * in[2]);
     }
@@ -180,7 +187,7 @@ template <class T>
 struct ProScale {  // (u, v, w) -> s (u, v, w)               state_uhat2u
     static constexpr int FI = 3, FO = 3;
     T s;
-    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, int, int, int) const
+    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK&) const
     {
         return s * in[o];
     }
@@ -209,7 +216,8 @@ __device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<N,
             cpx<T> v[Pro::FI];
 #pragma unroll
             for (int f = 0; f < Pro::FI; ++f) v[f] = U[f][q];
-            a[q] = pro(o, v, kx, ky, kval(CF::in_pos(g, q), N));
+            const int iz = CF::in_pos(g, q);
+            a[q] = pro(o, v, ModeK{kx, ky, kval(iz, N), ix, iy, iz});
         }
         CF::run(a, lds, c, g, tw, N);
         if (valid) {
@@ -494,7 +502,7 @@ __global__ void __launch_bounds__((Cfg<N, T>::NTX))
 // ===========================================================================
 // z-forward pass with the stepper epilogue
 // ===========================================================================
-enum EpiMode { EPI_RHS = 0, EPI_PRED = 1, EPI_CORR = 2, EPI_AB2 = 3, EPI_STORE = 4 };
+enum EpiMode { EPI_RHS = 0, EPI_PRED = 1, EPI_CORR = 2, EPI_AB2 = 3, EPI_STORE = 4, EPI_PREDONLY = 5 };
 
 template <class T>
 struct EpiArgs {
@@ -503,6 +511,7 @@ struct EpiArgs {
     cpx<T>* nold;         // EPI_AB2: N_{n-1} (read, then overwritten with N_n); EPI_PRED: N_0 if set
     const cpx<T>* rhs_in; // RHS input u (band forcing); may alias state
     long long fs;         // field stride of out/state/nold/rhs_in
+    double dt_fixed;      // EPI_PREDONLY: step size
     BoxScalars* sc;
     BoxPhys ph;
 };
@@ -531,11 +540,11 @@ __device__ __forceinline__ double step_dt(BoxScalars* sc, const BoxPhys& ph)
     return dt;
 }
 
-// forward z FFT of three fields of a column tile; O[f][q] holds mode kz = out_pos(g, q)
-template <int N, class T>
+// forward z FFT of F fields of a column tile; O[f][q] holds mode kz = out_pos(g, q)
+template <int N, class T, int F>
 __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long long in_fs, int col,
                                           bool valid, cpx<T>* lds, const cpx<T>* __restrict__ tw,
-                                          cpx<T> (&O)[3][Geo<N, T>::ZRM])
+                                          cpx<T> (&O)[F][Geo<N, T>::ZRM])
 {
     using G = Geo<N, T>;
     using CF = typename G::CFZ_F;
@@ -544,7 +553,7 @@ __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long l
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < F; ++f) {
         cpx<T> a[RM];
         const cpx<T>* src = in0 + f * in_fs;
 #pragma unroll
@@ -574,6 +583,7 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
     double dt = 0.0;
     if constexpr (MODE == EPI_PRED) dt = step_dt<N>(ea.sc, ph);
     if constexpr (MODE == EPI_CORR || MODE == EPI_AB2) dt = ea.sc->dt;
+    if constexpr (MODE == EPI_PREDONLY) dt = ea.dt_fixed;
     const double n3 = double(N) * double(N) * double(N);
     double fband = 0.0;
     if (ph.bandlim && MODE != EPI_STORE) fband = ph.Pin / (2.0 * ea.sc->eband);
@@ -634,6 +644,13 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
                         ea.state[f * ea.fs + idx] = e * (u + (tdt * T(0.5)) * nl[f]);
                         if (ea.nold) ea.nold[f * ea.fs + idx] = nl[f];
                     }
+                }
+            } else if constexpr (MODE == EPI_PREDONLY) {  // u* of dnsboxheun.f90:90-91, state untouched
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    if (!valid) continue;
+                    const cpx<T> u = ea.state[f * ea.fs + idx];
+                    ea.out[f * ea.fs + idx] = e * (u + tdt * nl[f]);
                 }
             } else if constexpr (MODE == EPI_CORR) {  // dnsboxheun.f90:120-126
 #pragma unroll

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "box.hpp"
 
 namespace dnsbox {
@@ -62,6 +63,9 @@ class BoxBase {
     int red_blocks = 0;
     bool ab2_started = false;
     bool x6_valid = false;   // scratch X holds the z-inverse + curl of the current state
+    int scratch_fields = 6;  // capacity of scratch in spectral fields
+    void* adjbuf = nullptr;  // 9 fields: A, Mu, u~ (adjoint passes), lazily
+    double* band2 = nullptr; // band-forcing scalars of the VJP
 
     virtual void rhs(const void* in, void* out) = 0;
     virtual void heun_step() = 0;
@@ -75,6 +79,11 @@ class BoxBase {
     // average device time of each RHS pass over `reps` back-to-back launches (ms),
     // measured with HIP events on the box stream; order of PASS_* below
     virtual void time_passes(int reps, double* ms) = 0;
+    // adjoint and caller-buffer entry points (device pointers, internal layout)
+    virtual void rhs_vjp(const void* u, const void* lam, void* out) = 0;
+    virtual void step_vjp(const void* u, const void* lam, void* lam_out, double dt) = 0;
+    virtual void rhs_adj(const void* u, void* out, double lin_sign) = 0;
+    virtual void step_ext(void* st, int nsteps) = 0;
     virtual void convert_in(void* dst, const double* src, long long n) = 0;   // host-staged
     virtual void convert_out(double* dst, const void* src, long long n) = 0;
 };
@@ -134,10 +143,11 @@ class BoxT final : public BoxBase {
 
     void release()
     {
-        void* ps[] = {state, pred, scratch, nonlin, nold, tw, sc, red_partial, red_out, red_bits};
+        void* ps[] = {state, pred, scratch, nonlin, nold, tw, sc, red_partial, red_out, red_bits, adjbuf, band2};
         for (void* p : ps)
             if (p) (void)hipFree(p);
-        state = pred = scratch = nonlin = nold = tw = nullptr;
+        state = pred = scratch = nonlin = nold = tw = adjbuf = nullptr;
+        band2 = nullptr;
         sc = nullptr;
         red_partial = red_out = nullptr;
         red_bits = nullptr;
@@ -348,6 +358,168 @@ class BoxT final : public BoxBase {
         hipLaunchKernelGGL((k_zfwd<N, T, EPI_STORE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
                            twc(), ea);
         DNSBOX_LAUNCH_CHECK();
+    }
+
+    void ensure_scratch(int nf)
+    {
+        if (scratch_fields >= nf) return;
+        DNSBOX_HIP(hipStreamSynchronize(stream));
+        DNSBOX_HIP(hipFree(scratch));
+        scratch = nullptr;
+        DNSBOX_HIP(hipMalloc(&scratch, size_t(nf) * size_t(fs) * sizeof(C)));
+        scratch_fields = nf;
+        x6_valid = false;
+    }
+    void ensure_adj()
+    {
+        if (adjbuf) return;
+        DNSBOX_HIP(hipMalloc(&adjbuf, 9 * size_t(fs) * sizeof(C)));
+        DNSBOX_HIP(hipMalloc(&band2, 2 * sizeof(double)));
+    }
+    C* adjA() const { return static_cast<C*>(adjbuf); }
+    C* adjMu() const { return static_cast<C*>(adjbuf) + 3 * fs; }
+    C* adjUt() const { return static_cast<C*>(adjbuf) + 6 * fs; }
+
+    // VJP of N at u with cotangent lam_scale * lam; epilogue MODE of k_zfwd_vjp
+    template <int MODE>
+    void vjp_passes(const C* u, const C* lam, double lam_scale, VjpArgs<T> va)
+    {
+        const double sm = 1.0 / (double(N) * N * N);
+        if (cfg.ph.bandlim) {
+            hipLaunchKernelGGL((k_band_vjp_scalars<N, T>), dim3(1), dim3(256), 0, stream, u, lam, fs, cfg.ph,
+                               lam_scale, sm * sm, band2);
+            DNSBOX_LAUNCH_CHECK();
+        }
+        Ptrs<T, 6> in{{lam, lam + fs, lam + 2 * fs, u, u + fs, u + 2 * fs}};
+        ProVjpIn<T> pro{T(sm), T(lam_scale), N, cfg.ph};
+        hipLaunchKernelGGL((k_zinv_p<N, T, ProVjpIn<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in, X(), fs,
+                           twc(), pro);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 9), dim3(YNT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_xpass<N, T, OpVjp<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream, X(),
+                           fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        va.u = u;
+        va.lamN = lam;
+        va.lam_scale = lam_scale;
+        va.band = band2;
+        va.fs = fs;
+        va.ph = cfg.ph;
+        hipLaunchKernelGGL((k_zfwd_vjp<N, T, MODE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), va);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void rhs_vjp(const void* u, const void* lam, void* out) override
+    {
+        x6_valid = false;
+        ensure_scratch(9);
+        ensure_adj();
+        VjpArgs<T> va{};
+        va.out = static_cast<C*>(out);
+        vjp_passes<VJP_RHS>(static_cast<const C*>(u), static_cast<const C*>(lam), 1.0, va);
+    }
+
+    // lam_out = (d u_{n+1} / d u_n)^T lam for one fixed-dt IF-Heun step from u
+    void step_vjp(const void* u_, const void* lam_, void* lam_out, double dt) override
+    {
+        x6_valid = false;
+        ensure_scratch(9);
+        ensure_adj();
+        const C* u = static_cast<const C*>(u_);
+        const C* lam = static_cast<const C*>(lam_);
+        C* P = static_cast<C*>(pred);
+        const T s = T(1.0 / (double(N) * N * N));
+        // u* = E (u + dt N(u))
+        eband(u);
+        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, u, fs, X(), fs,
+                           twc(), ProCurl<T>{s});
+        DNSBOX_LAUNCH_CHECK();
+        rhs_middle(false);
+        EpiArgs<T> ea = epi();
+        ea.out = P;
+        ea.state = const_cast<C*>(u);
+        ea.rhs_in = u;
+        ea.dt_fixed = dt;
+        hipLaunchKernelGGL((k_zfwd<N, T, EPI_PREDONLY>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
+                           ea);
+        DNSBOX_LAUNCH_CHECK();
+        // lam_* = J(u*)^T (dt/2 lam);  A = E (lam + lam_*),  Mu = E (dt/2 lam + dt lam_*)
+        VjpArgs<T> v1{};
+        v1.lam = lam;
+        v1.A = adjA();
+        v1.Mu = adjMu();
+        v1.dt = dt;
+        vjp_passes<VJP_S1>(P, lam, 0.5 * dt, v1);
+        // lam_n = A + J(u)^T Mu
+        VjpArgs<T> v2{};
+        v2.out = static_cast<C*>(lam_out);
+        v2.A = adjA();
+        v2.dt = dt;
+        vjp_passes<VJP_S2>(u, adjMu(), 1.0, v2);
+    }
+
+    // rhsAdj.f90:64-381 (phatadj read as phat)
+    void rhs_adj(const void* u_, void* out, double lin_sign) override
+    {
+        x6_valid = false;
+        ensure_scratch(12);
+        ensure_adj();
+        const C* u = static_cast<const C*>(u_);
+        const T s = T(1.0 / (double(N) * N * N));
+        AdjArgs<T> aa{};
+        aa.out = static_cast<C*>(out);
+        aa.u = u;
+        aa.ut = adjUt();
+        aa.fs = fs;
+        aa.lin_sign = lin_sign;
+        aa.ph = cfg.ph;
+        Ptrs<T, 3> in1{{u, u + fs, u + 2 * fs}};
+        hipLaunchKernelGGL((k_zinv_p<N, T, ProGrad12<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in1, X(), fs,
+                           twc(), ProGrad12<T>{s});
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 12), dim3(YNT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_xpass<N, T, OpConv<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream, X(),
+                           fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_zfwd_adj<N, T, 1>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), aa);
+        DNSBOX_LAUNCH_CHECK();
+        const C* ut = adjUt();
+        Ptrs<T, 6> in2{{ut, ut + fs, ut + 2 * fs, u, u + fs, u + 2 * fs}};
+        hipLaunchKernelGGL((k_zinv_p<N, T, ProAdj12<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in2, X(), fs,
+                           twc(), ProAdj12<T>{s});
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 12), dim3(YNT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_xpass<N, T, OpAdjN<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream, X(),
+                           fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_zfwd_adj<N, T, 2>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), aa);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    // IF-Heun steps on a caller-owned state buffer
+    void step_ext(void* st, int nsteps) override
+    {
+        void* own = state;
+        x6_valid = false;
+        state = st;
+        try {
+            for (int i = 0; i < nsteps; ++i) heun_step();
+        } catch (...) {
+            state = own;
+            x6_valid = false;
+            throw;
+        }
+        state = own;
+        x6_valid = false;
     }
 
     void time_passes(int reps, double* ms) override

@@ -483,4 +483,50 @@ int dnsbox_rhs_dev(dnsbox_ctx* c, const void* in, void* out)
     });
 }
 
+int dnsbox_step_dev(dnsbox_ctx* c, void* state, int nsteps)
+{
+    if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
+    if (!state || nsteps < 0) return fail(DNSBOX_E_ARG, "null state or nsteps < 0");
+    return guarded([&]() -> int {
+        c->box->step_ext(state, nsteps);
+        return 0;
+    });
+}
+
+int dnsbox_rhs_vjp_dev(dnsbox_ctx* c, const void* u, const void* lam, void* out)
+{
+    if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
+    if (!u || !lam || !out) return fail(DNSBOX_E_ARG, "null device pointer");
+    return guarded([&]() -> int {
+        c->box->rhs_vjp(u, lam, out);
+        return 0;
+    });
+}
+
+int dnsbox_step_vjp_dev(dnsbox_ctx* c, const void* u, const void* lam, void* lam_out, double dt)
+{
+    if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
+    if (!u || !lam || !lam_out) return fail(DNSBOX_E_ARG, "null device pointer");
+    if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+    return guarded([&]() -> int {
+        c->box->step_vjp(u, lam, lam_out, dt);
+        return 0;
+    });
+}
+
+int dnsbox_rhs_adj_dev(dnsbox_ctx* c, const void* u, void* out, int lin_sign)
+{
+    if (int e = check_ctx(c)) return e;
+    c->box->x6_valid = false;
+    if (!u || !out) return fail(DNSBOX_E_ARG, "null device pointer");
+    if (lin_sign != 1 && lin_sign != -1) return fail(DNSBOX_E_ARG, "lin_sign must be +1 or -1");
+    return guarded([&]() -> int {
+        c->box->rhs_adj(u, out, double(lin_sign));
+        return 0;
+    });
+}
+
 }  // extern "C"

@@ -142,6 +142,20 @@ int dnsbox_time_passes(dnsbox_ctx* ctx, int reps, double* ms);
 int dnsbox_state_ptr(dnsbox_ctx* ctx, void** dev_state);   /* the context's state buffer */
 /* rhs_nonlinear on device buffers: out <- N(in) */
 int dnsbox_rhs_dev(dnsbox_ctx* ctx, const void* in, void* out);
+/* the time loop of dnsboxheun.f90:81-126 on a caller-owned device state (in place) */
+int dnsbox_step_dev(dnsbox_ctx* ctx, void* state, int nsteps);
+/* exact discrete adjoint of rhs_nonlinear (rhs.f90:18-135): out <- J_N(u)^T lam in the
+   Euclidean inner product of the stored coordinates, sum Re(conj(a) b).  No reference
+   counterpart: the reference's adjoint (rhsAdj.f90) is the continuous one below. */
+int dnsbox_rhs_vjp_dev(dnsbox_ctx* ctx, const void* u, const void* lam, void* out);
+/* adjoint of one fixed-dt IF-Heun step (dnsboxheun.f90:84-126) taken from state u:
+   lam_out <- (d u_{n+1} / d u_n)^T lam.  Backward step of the T-step map Phi_T. */
+int dnsbox_step_vjp_dev(dnsbox_ctx* ctx, const void* u, const void* lam, void* lam_out, double dt);
+/* rhsAdj (rhsAdj.f90:356-381 via rhsNonlinearAdj :261-354, rhsupptilde :177-213, convective
+   rhsNonlinear :64-175), with the never-assigned phatadj (:343-348) read as phat (:335-340).
+   lin_sign = +1 reproduces the reference's L u~ - P n_adj; lin_sign = -1 gives the descent
+   direction -DF^+(u~) = -L u~ - P n_adj of J = 1/2 ||F(u)||^2_{H^-1}. */
+int dnsbox_rhs_adj_dev(dnsbox_ctx* ctx, const void* u, void* out, int lin_sign);
 
 #ifdef __cplusplus
 }

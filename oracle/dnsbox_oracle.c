@@ -723,3 +723,261 @@ double orc_shapiro_error(const orc_params* p, double t, const double* u, const d
     free(ex);
     return mu + mv + mw;
 }
+
+/* ------------------------------------------------------------------------- */
+/* adjoints                                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* Hermitian weight of a stored half-spectrum entry: the c2r sums every kx plane
+ * except kx = 0 and kx = Nx/2 twice. */
+static inline double wc(const orc_params* p, int ix) { return (ix == 0 || ix == p->nx / 2) ? 1.0 : 2.0; }
+
+/* Exact discrete vector-Jacobian product of rhs_nonlinear (rotational form, rhs.f90:18-135
+ * plus the Kolmogorov body force, which is constant) in the Euclidean inner product of the
+ * stored half-spectrum coordinates, <a, b> = sum Re(conj(a) b):
+ *     g = J_N(u)^T lambda.
+ * Chain (forward: u_t = c2r(u), w = c2r(s i k x u), y = s (w x u_t), n = r2c(y),
+ *        N = -P M n [+ Pin/(2 Eband(u)) u on the band]):
+ *     nbar = -P M lambda;  ybar = c2r(nbar / W);  wbar = s (u_t x ybar);  utbar = s (ybar x w)
+ *     g = W r2c(utbar) + s (i k x)(W r2c(wbar)) [+ band terms],  W = Hermitian weight. */
+void orc_rhs_vjp(const orc_params* p, const double* uh_, const double* vh_, const double* wh_,
+                 const double* lu_, const double* lv_, const double* lw_, double* gu_, double* gv_,
+                 double* gw_)
+{
+    const cplx* u[3] = {(const cplx*)uh_, (const cplx*)vh_, (const cplx*)wh_};
+    const cplx* lam[3] = {(const cplx*)lu_, (const cplx*)lv_, (const cplx*)lw_};
+    cplx* g[3] = {(cplx*)gu_, (cplx*)gv_, (cplx*)gw_};
+    const size_t ms = nspec_elems(p), mp = nphys_elems(p);
+    const double sm = 1.0 / ((double)p->nx * p->ny * p->nz);
+    ktabs t = ktabs_make(p);
+
+    /* forward intermediates: u_t (N^3 scale) and the vorticity */
+    double* ut = (double*)malloc(sizeof(double) * mp * 3);
+    double* om = (double*)malloc(sizeof(double) * mp * 3);
+    cplx* tmp = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    for (int c = 0; c < 3; ++c) orc_c2r(p, (const double*)u[c], ut + c * mp);
+    for (int ix = 0; ix < nxh(p); ++ix)
+        for (int iy = 0; iy < p->ny; ++iy)
+            for (int iz = 0; iz < p->nz; ++iz) {
+                size_t m = sidx(p, ix, iy, iz);
+                const double kx = t.kx[ix], ky = t.ky[iy], kz = t.kz[iz];
+                tmp[m] = I * (ky * u[2][m] - kz * u[1][m]) * sm;
+                tmp[ms + m] = I * (kz * u[0][m] - kx * u[2][m]) * sm;
+                tmp[2 * ms + m] = I * (kx * u[1][m] - ky * u[0][m]) * sm;
+            }
+    for (int c = 0; c < 3; ++c) orc_c2r(p, (const double*)(tmp + c * ms), om + c * mp);
+
+    /* nbar = -P M lambda, divided by the Hermitian weight */
+    for (int ix = 0; ix < nxh(p); ++ix)
+        for (int iy = 0; iy < p->ny; ++iy)
+            for (int iz = 0; iz < p->nz; ++iz) {
+                size_t m = sidx(p, ix, iy, iz);
+                if (orc_masked(p, ix, iy, iz)) {
+                    tmp[m] = tmp[ms + m] = tmp[2 * ms + m] = 0;
+                    continue;
+                }
+                const double k[3] = {t.kx[ix], t.ky[iy], t.kz[iz]};
+                const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                const cplx kl = (k[0] * lam[0][m] + k[1] * lam[1][m] + k[2] * lam[2][m]) / k2;
+                for (int c = 0; c < 3; ++c) tmp[c * ms + m] = -(lam[c][m] - k[c] * kl) / wc(p, ix);
+            }
+    double* yb = (double*)malloc(sizeof(double) * mp * 3);
+    for (int c = 0; c < 3; ++c) orc_c2r(p, (const double*)(tmp + c * ms), yb + c * mp);
+
+    /* through y = s (w x u_t) */
+    double* utb = (double*)malloc(sizeof(double) * mp * 3);
+    double* omb = (double*)malloc(sizeof(double) * mp * 3);
+    for (size_t i = 0; i < mp; ++i) {
+        const double a[3] = {ut[i], ut[mp + i], ut[2 * mp + i]};
+        const double w[3] = {om[i], om[mp + i], om[2 * mp + i]};
+        const double y[3] = {yb[i], yb[mp + i], yb[2 * mp + i]};
+        /* wbar = s (u_t x ybar) */
+        omb[i] = sm * (a[1] * y[2] - a[2] * y[1]);
+        omb[mp + i] = sm * (a[2] * y[0] - a[0] * y[2]);
+        omb[2 * mp + i] = sm * (a[0] * y[1] - a[1] * y[0]);
+        /* utbar = s (ybar x w) */
+        utb[i] = sm * (y[1] * w[2] - y[2] * w[1]);
+        utb[mp + i] = sm * (y[2] * w[0] - y[0] * w[2]);
+        utb[2 * mp + i] = sm * (y[0] * w[1] - y[1] * w[0]);
+    }
+    cplx* Ub = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    cplx* Wb = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    for (int c = 0; c < 3; ++c) {
+        orc_r2c(p, utb + c * mp, (double*)(Ub + c * ms));
+        orc_r2c(p, omb + c * mp, (double*)(Wb + c * ms));
+    }
+    for (int ix = 0; ix < nxh(p); ++ix)
+        for (int iy = 0; iy < p->ny; ++iy)
+            for (int iz = 0; iz < p->nz; ++iz) {
+                size_t m = sidx(p, ix, iy, iz);
+                const double kx = t.kx[ix], ky = t.ky[iy], kz = t.kz[iz];
+                const double w = wc(p, ix);
+                const cplx a0 = w * Wb[m], a1 = w * Wb[ms + m], a2 = w * Wb[2 * ms + m];
+                g[0][m] = w * Ub[m] + sm * I * (ky * a2 - kz * a1);
+                g[1][m] = w * Ub[ms + m] + sm * I * (kz * a0 - kx * a2);
+                g[2][m] = w * Ub[2 * ms + m] + sm * I * (kx * a1 - ky * a0);
+            }
+    /* band forcing: N += c(u) u on unmasked |k| <= kcut, c = Pin / (2 Eband(u)) */
+    if (p->bandlim) {
+        const double eb = orc_eband(p, uh_, vh_, wh_);
+        const double cf = p->Pin / (2.0 * eb);
+        double lu = 0.0;
+        for (int ix = 0; ix < nxh(p); ++ix)
+            for (int iy = 0; iy < p->ny; ++iy)
+                for (int iz = 0; iz < p->nz; ++iz) {
+                    const double absk = sqrt(t.kx[ix] * t.kx[ix] + t.ky[iy] * t.ky[iy] + t.kz[iz] * t.kz[iz]);
+                    if (absk > p->kcut || orc_masked(p, ix, iy, iz)) continue;
+                    size_t m = sidx(p, ix, iy, iz);
+                    for (int c = 0; c < 3; ++c) {
+                        g[c][m] += cf * lam[c][m];
+                        lu += creal(conj(lam[c][m]) * u[c][m]);
+                    }
+                }
+        /* d Eband / d u = 2 s^2 w_e u on |k| <= kcut (w_e = 1/2 where kx = 0) */
+        const double f = -p->Pin / (2.0 * eb * eb) * lu;
+        for (int ix = 0; ix < nxh(p); ++ix)
+            for (int iy = 0; iy < p->ny; ++iy)
+                for (int iz = 0; iz < p->nz; ++iz) {
+                    const double absk = sqrt(t.kx[ix] * t.kx[ix] + t.ky[iy] * t.ky[iy] + t.kz[iz] * t.kz[iz]);
+                    if (absk > p->kcut) continue;
+                    size_t m = sidx(p, ix, iy, iz);
+                    const double we = (t.kx[ix] == 0.0) ? 0.5 : 1.0;
+                    for (int c = 0; c < 3; ++c) g[c][m] += f * 2.0 * sm * sm * we * u[c][m];
+                }
+    }
+    free(Ub); free(Wb); free(utb); free(omb); free(yb); free(tmp); free(om); free(ut);
+    ktabs_free(&t);
+}
+
+/* Vector-Jacobian product of one fixed-dt IF-Heun step (dnsboxheun.f90:84-126):
+ *   u* = E(u + dt N(u)),  u_{n+1} = E(u + dt/2 N(u)) + dt/2 N(u*)
+ *   lam_* = J_N(u*)^T (dt/2 lam);  lam_n = E(lam + lam_*) + J_N(u)^T E(dt/2 lam + dt lam_*).
+ * lam (3 components) is overwritten with lam_n. */
+void orc_heun_vjp(const orc_params* p, const double* uh, const double* vh, const double* wh,
+                  double dt, double* lu_, double* lv_, double* lw_)
+{
+    const size_t ms = nspec_elems(p);
+    cplx* lam[3] = {(cplx*)lu_, (cplx*)lv_, (cplx*)lw_};
+    const cplx* u[3] = {(const cplx*)uh, (const cplx*)vh, (const cplx*)wh};
+    double* itf = (double*)malloc(sizeof(double) * ms);
+    orc_int_fact(p, dt, itf);
+    cplx* buf = (cplx*)malloc(sizeof(cplx) * ms * 12);
+    cplx *nl = buf, *us = buf + 3 * ms, *a = buf + 6 * ms, *ls = buf + 9 * ms;
+    orc_rhs_nonlinear(p, uh, vh, wh, (double*)nl, (double*)(nl + ms), (double*)(nl + 2 * ms));
+    for (int c = 0; c < 3; ++c)
+        for (size_t m = 0; m < ms; ++m) {
+            us[c * ms + m] = itf[m] * (u[c][m] + dt * nl[c * ms + m]);
+            a[c * ms + m] = 0.5 * dt * lam[c][m];
+        }
+    orc_rhs_vjp(p, (double*)us, (double*)(us + ms), (double*)(us + 2 * ms), (double*)a,
+                (double*)(a + ms), (double*)(a + 2 * ms), (double*)ls, (double*)(ls + ms),
+                (double*)(ls + 2 * ms));
+    for (int c = 0; c < 3; ++c)
+        for (size_t m = 0; m < ms; ++m) a[c * ms + m] = itf[m] * (0.5 * dt * lam[c][m] + dt * ls[c * ms + m]);
+    orc_rhs_vjp(p, uh, vh, wh, (double*)a, (double*)(a + ms), (double*)(a + 2 * ms), (double*)nl,
+                (double*)(nl + ms), (double*)(nl + 2 * ms));
+    for (int c = 0; c < 3; ++c)
+        for (size_t m = 0; m < ms; ++m)
+            lam[c][m] = itf[m] * (lam[c][m] + ls[c * ms + m]) + nl[c * ms + m];
+    free(buf);
+    free(itf);
+}
+
+/* rhsAdj.f90:64-381 as written, with one correction: the never-assigned phatadj
+ * (rhsAdj.f90:343-348) is read as the phat computed just above it (:335-340).
+ *   forward (convective form, ".gt." mask, eps at k = 0)   rhsNonlinear :64-175
+ *   u~ = H (L u + N(u)),  H = 1/(1+|k|^2), L = -nu|k|^2 + Q  rhsupptilde  :177-213
+ *   n_adj_i = sum_j (d_i u~_j + d_j u~_i) u_j               rhsNonlinearAdj :261-309
+ *   rhsadj = L u~ - n_adj - k phat(n_adj)                   :311-381 */
+static void grads_of(const orc_params* p, const ktabs* t, const cplx* fh, cplx* tmp, double* d)
+{
+    derivs(p, t, fh, tmp, d, d + nphys_elems(p), d + 2 * nphys_elems(p));
+}
+
+static int masked_gt(const orc_params* p, const ktabs* t, int ix, int iy, int iz)
+{
+    const double tx = ((double)p->nx / 2.0) * (2.0 * 1.0 / 3.0);
+    const double ty = ((double)p->ny / 2.0) * (2.0 * 1.0 / 3.0);
+    const double tz = ((double)p->nz / 2.0) * (2.0 * 1.0 / 3.0);
+    if (!p->parity_mode && (ix == p->nx / 2 || iy == p->ny / 2 || iz == p->nz / 2)) return 1;
+    return fabs(t->kx[ix]) > tx || fabs(t->ky[iy]) > ty || fabs(t->kz[iz]) > tz;
+}
+
+static void project_gt(const orc_params* p, const ktabs* t, cplx* n[3])
+{
+    for (int ix = 0; ix < nxh(p); ++ix)
+        for (int iy = 0; iy < p->ny; ++iy)
+            for (int iz = 0; iz < p->nz; ++iz) {
+                size_t m = sidx(p, ix, iy, iz);
+                if (masked_gt(p, t, ix, iy, iz)) {
+                    n[0][m] = n[1][m] = n[2][m] = 0;
+                    continue;
+                }
+                const cplx kx = t->kx[ix] * I, ky = t->ky[iy] * I, kz = t->kz[iz] * I;
+                const double eps = (t->kx[ix] == 0.0 && t->ky[iy] == 0.0 && t->kz[iz] == 0.0) ? 1e-13 : 0.0;
+                cplx phat = -1.0 * (kx * n[0][m] + ky * n[1][m] + kz * n[2][m]) / (kx * kx + ky * ky + kz * kz + eps);
+                n[0][m] = -n[0][m] - kx * phat;
+                n[1][m] = -n[1][m] - ky * phat;
+                n[2][m] = -n[2][m] - kz * phat;
+            }
+}
+
+void orc_rhs_adj(const orc_params* p, const double* uh_, const double* vh_, const double* wh_,
+                 double* au_, double* av_, double* aw_)
+{
+    const cplx* u[3] = {(const cplx*)uh_, (const cplx*)vh_, (const cplx*)wh_};
+    cplx* out[3] = {(cplx*)au_, (cplx*)av_, (cplx*)aw_};
+    const size_t ms = nspec_elems(p), mp = nphys_elems(p);
+    const double sm = 1.0 / ((double)p->nx * p->ny * p->nz);
+    ktabs t = ktabs_make(p);
+    cplx* tmp = (cplx*)malloc(sizeof(cplx) * ms);
+    double* d = (double*)malloc(sizeof(double) * mp * 9);  /* d[3 i + j] = d_j u_i */
+    double* ut = (double*)malloc(sizeof(double) * mp * 3);
+    double* tr = (double*)malloc(sizeof(double) * mp);
+    cplx* nl = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    cplx* uti = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    cplx* n[3] = {nl, nl + ms, nl + 2 * ms};
+    /* rhsNonlinear (:64-175) */
+    for (int c = 0; c < 3; ++c) grads_of(p, &t, u[c], tmp, d + 3 * c * mp);
+    for (int c = 0; c < 3; ++c) orc_c2r(p, (const double*)u[c], ut + c * mp);
+    for (int c = 0; c < 3; ++c) {
+        for (size_t i = 0; i < mp; ++i)
+            tr[i] = (ut[i] * d[(3 * c + 0) * mp + i] + ut[mp + i] * d[(3 * c + 1) * mp + i] +
+                     ut[2 * mp + i] * d[(3 * c + 2) * mp + i]) * sm;
+        orc_r2c(p, tr, (double*)n[c]);
+    }
+    project_gt(p, &t, n);
+    /* rhsupptilde (:177-213), hminone = .true. */
+    for (int ix = 0; ix < nxh(p); ++ix)
+        for (int iy = 0; iy < p->ny; ++iy)
+            for (int iz = 0; iz < p->nz; ++iz) {
+                size_t m = sidx(p, ix, iy, iz);
+                const cplx kx = t.kx[ix] * I, ky = t.ky[iy] * I, kz = t.kz[iz] * I;
+                const cplx kk = kx * kx + ky * ky + kz * kz;
+                const double factor = 1.0 / (1.0 - creal(kk));
+                for (int c = 0; c < 3; ++c) uti[c * ms + m] = factor * ((p->nu * kk + p->Q) * u[c][m] + n[c][m]);
+            }
+    /* rhsDerivativespp (:215-259) and rhsNonlinearAdj (:261-309) */
+    for (int c = 0; c < 3; ++c) grads_of(p, &t, uti + c * ms, tmp, d + 3 * c * mp);
+    for (int i = 0; i < 3; ++i) {
+        for (size_t q = 0; q < mp; ++q) {
+            double acc = 0.0;
+            for (int j = 0; j < 3; ++j)
+                acc += d[(3 * j + i) * mp + q] * ut[j * mp + q] + d[(3 * i + j) * mp + q] * ut[j * mp + q];
+            tr[q] = acc * sm;
+        }
+        orc_r2c(p, tr, (double*)n[i]);
+    }
+    project_gt(p, &t, n);
+    /* rhsAdj (:356-381) */
+    for (int ix = 0; ix < nxh(p); ++ix)
+        for (int iy = 0; iy < p->ny; ++iy)
+            for (int iz = 0; iz < p->nz; ++iz) {
+                size_t m = sidx(p, ix, iy, iz);
+                const cplx kx = t.kx[ix] * I, ky = t.ky[iy] * I, kz = t.kz[iz] * I;
+                const cplx L = p->nu * (kx * kx + ky * ky + kz * kz) + p->Q;
+                for (int c = 0; c < 3; ++c) out[c][m] = L * uti[c * ms + m] + n[c][m];
+            }
+    free(uti); free(nl); free(tr); free(ut); free(d); free(tmp);
+    ktabs_free(&t);
+}

@@ -78,7 +78,6 @@ void   orc_rhs_vjp(const orc_params* p, const double* uh, const double* vh, cons
                    double* gu, double* gv, double* gw);
 void   orc_heun_vjp(const orc_params* p, const double* uh, const double* vh, const double* wh,
                     double dt, double* lu, double* lv, double* lw);
-double orc_inner(const orc_params* p, const double* a, const double* b);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,9 @@ def lib():
             "orc_init_shapiro": (None, [P, C.c_double, d, d, d]),
             "orc_shapiro_error": (C.c_double, [P, C.c_double, d, d, d]),
             "orc_hash": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
+            "orc_rhs_vjp": (None, [P, d, d, d, d, d, d, d, d, d]),
+            "orc_heun_vjp": (None, [P, d, d, d, C.c_double, d, d, d]),
+            "orc_rhs_adj": (None, [P, d, d, d, d, d, d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -211,6 +214,32 @@ class Oracle:
     def shapiro_error(self, t, u):
         u = [np.ascontiguousarray(a, np.float64) for a in u]
         return self.L.orc_shapiro_error(C.byref(self.p), t, *[_p(a) for a in u])
+
+    def rhs_vjp(self, f, lam):
+        f = [np.ascontiguousarray(a, np.complex128) for a in f]
+        lam = [np.ascontiguousarray(a, np.complex128) for a in lam]
+        out = self.zeros_spec()
+        self.L.orc_rhs_vjp(C.byref(self.p), *[_p(a) for a in f], *[_p(a) for a in lam],
+                           *[_p(a) for a in out])
+        return out
+
+    def heun_vjp(self, f, lam, dt):
+        f = [np.ascontiguousarray(a, np.complex128) for a in f]
+        lam = [np.array(a, np.complex128, order="C") for a in lam]
+        self.L.orc_heun_vjp(C.byref(self.p), *[_p(a) for a in f], dt, *[_p(a) for a in lam])
+        return lam
+
+    def rhs_adj(self, f):
+        f = [np.ascontiguousarray(a, np.complex128) for a in f]
+        out = self.zeros_spec()
+        self.L.orc_rhs_adj(C.byref(self.p), *[_p(a) for a in f], *[_p(a) for a in out])
+        return out
+
+    def hermitian_weights(self):
+        w = np.full(self.spec_shape, 2.0)
+        w[0] = 1.0
+        w[self.nx // 2] = 1.0
+        return w
 
     def hash(self, seed, box, idx):
         return self.L.orc_hash(seed, box, idx)

@@ -1,0 +1,153 @@
+"""GPU adjoint kernels vs the CPU oracle, and the differentiable T-step map vs central
+finite differences (through the C-ABI, FP64)."""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle, from_torch_layout, to_torch_layout
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(f):
+    import torch
+    return torch.from_numpy(np.stack([to_torch_layout(a) for a in f])).cuda().contiguous()
+
+
+def host(t):
+    a = t.detach().cpu().numpy()
+    return [from_torch_layout(a[i]) for i in range(3)]
+
+
+def rel_l2(a, b):
+    num = sum(np.linalg.norm((x - y).ravel()) ** 2 for x, y in zip(a, b))
+    return float(np.sqrt(num / sum(np.linalg.norm(y.ravel()) ** 2 for y in b)))
+
+
+def make(n, **kw):
+    from dnsbox_amd import Box
+    return Box(n=n, precision=kw.pop("precision", 64), **kw)
+
+
+def rand_spec(o, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal(o.spec_shape) + 1j * rng.standard_normal(o.spec_shape) for _ in range(3)]
+
+
+@pytest.mark.parametrize("n", [32, 48])
+def test_rhs_vjp_matches_oracle(gpu_lib, n):
+    o = Oracle(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4)
+    u = o.init_solenoidal(1, 0, 4.0, 0.5)
+    lam = rand_spec(o, 2)
+    ref = o.rhs_vjp(u, lam)
+    with make(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0) as b:
+        out = b.empty_state()
+        b.rhs_vjp_dev(dev(u), dev(lam), out)
+        got = host(out)
+    err = rel_l2(got, ref)
+    print(f"rhs_vjp n={n} rel_l2={err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+def test_rhs_vjp_band_matches_oracle(gpu_lib):
+    n = 32
+    o = Oracle(n, nu=0.01, Pin=0.1, kcut=2.5, bandlim=True)
+    u = o.init_solenoidal(3, 0, 2.0, 0.5)
+    lam = rand_spec(o, 4)
+    ref = o.rhs_vjp(u, lam)
+    with make(n, nu=0.01, Pin=0.1, kcut=2.5, bandlim=1, adaptive_dt=0) as b:
+        out = b.empty_state()
+        b.rhs_vjp_dev(dev(u), dev(lam), out)
+        got = host(out)
+    err = rel_l2(got, ref)
+    print(f"rhs_vjp band rel_l2={err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+def test_step_vjp_matches_oracle(gpu_lib):
+    n = 32
+    dt = 0.005
+    o = Oracle(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4)
+    u = o.init_solenoidal(5, 0, 4.0, 0.5)
+    lam = rand_spec(o, 6)
+    ref = o.heun_vjp(u, lam, dt)
+    with make(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        out = b.empty_state()
+        b.step_vjp_dev(dev(u), dev(lam), out, dt)
+        got = host(out)
+    err = rel_l2(got, ref)
+    print(f"step_vjp rel_l2={err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("n,sign", [(32, 1), (32, -1), (48, 1)])
+def test_rhs_adj_matches_oracle(gpu_lib, n, sign):
+    o = Oracle(n, nu=0.02)
+    u = o.init_solenoidal(7, 0, 4.0, 0.5)
+    ref = o.rhs_adj(u)
+    if sign < 0:
+        k = np.meshgrid(o.ktable(n)[: n // 2 + 1], o.ktable(n), o.ktable(n), indexing="ij")
+        k2 = k[0] ** 2 + k[1] ** 2 + k[2] ** 2
+        L, H = -0.02 * k2, 1.0 / (1.0 + k2)
+        ut = [H * (L * a + b) for a, b in zip(u, Oracle(n, nu=0.02).rhs(u))]
+        ref = [r - 2 * L * a for r, a in zip(ref, ut)]
+    with make(n, nu=0.02, bandlim=0, adaptive_dt=0) as b:
+        out = b.empty_state()
+        b.rhs_adj_dev(dev(u), out, sign)
+        got = host(out)
+    m = np.ones(ref[0].shape, bool)
+    if sign < 0:  # u~ above uses the rotational RHS: compare off the Nyquist planes and k = 0
+        m[n // 2] = False
+        m[:, n // 2] = False
+        m[:, :, n // 2] = False
+        m[0, 0, 0] = False
+    num = sum(np.linalg.norm((g - r)[m]) ** 2 for g, r in zip(got, ref))
+    den = sum(np.linalg.norm(r[m]) ** 2 for r in ref)
+    err = float(np.sqrt(num / den))
+    print(f"rhs_adj n={n} sign={sign} rel_l2={err:.3e}", flush=True)
+    assert err < 1e-11
+
+
+def test_step_dev_matches_internal_state(gpu_lib):
+    n = 32
+    kw = dict(nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=0.005)
+    with make(n, **kw) as a:
+        a.init_random(3, 0, 4.0, 0.5)
+        f = a.get_state()
+        t = dev(f)
+        a.rhs_int_fact(0.005)
+        a.step_dev(t, 10)
+        a.set_state(*f)
+        a.set_time(0.0, 0.005)
+        a.step(10)
+        ref = a.get_state()
+    assert rel_l2(host(t), ref) < 1e-14
+
+
+@pytest.mark.parametrize("n,T,k", [(32, 20, 5), (64, 200, 20)])
+def test_tstep_gradient_vs_finite_differences(gpu_lib, n, T, k):
+    """d/de J(u + e v) by central differences (e = 1e-6) along seeded solenoidal directions
+    vs <grad J, v> from the checkpointed adjoint; tolerance 1e-6 relative."""
+    import torch
+    from dnsbox_amd.adjoint import residual
+    dt = 0.005
+    nu = 0.02
+    o = Oracle(n, nu=nu, kolmo_amp=1.0, kolmo_k=4)
+    ndir = 8 if n == 32 else 2
+    with make(n, nu=nu, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        u0 = dev(o.init_solenoidal(11, 0, 4.0, 0.5))
+        x = u0.clone().requires_grad_(True)
+        J = residual(b, x, T, dt, k)
+        (g,) = torch.autograd.grad(J, x)
+        worst = 0.0
+        for d in range(ndir):
+            v = dev(o.init_solenoidal(100 + d, 0, 4.0, 0.5)) * 1.0
+            eps = 1e-6
+            with torch.no_grad():
+                jp = float(residual(b, u0 + eps * v, T, dt, k))
+                jm = float(residual(b, u0 - eps * v, T, dt, k))
+            fd = (jp - jm) / (2 * eps)
+            an = float((g.conj() * v).real.sum())
+            rel = abs(fd - an) / abs(fd)
+            worst = max(worst, rel)
+            print(f"T-step grad n={n} T={T} dir={d}: fd={fd:.10e} adj={an:.10e} rel={rel:.2e}", flush=True)
+        assert worst < 1e-6
