@@ -78,6 +78,13 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7.  Importing torch
+    # first makes the library bind to that already-loaded runtime (same soname), so torch
+    # tensors and the library share one runtime whichever is used first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise DnsboxError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
