@@ -102,10 +102,12 @@ struct ProAdj12 {  // (u~, u) -> (s u_i, s i k_j u~_i)
 template <int N, class T, class Pro>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zinv_p(Ptrs<T, Pro::FI> in, cpx<T>* __restrict__ out0, long long out_fs,
-             const cpx<T>* __restrict__ tw, Pro pro)
+             const cpx<T>* __restrict__ tw_g, Pro pro)
 {
     using G = Geo<N, T>;
     __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     cpx<T>* out[Pro::FO];
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
@@ -183,13 +185,15 @@ struct VjpArgs {
 
 template <int N, class T, int MODE>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
-    k_zfwd_vjp(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw, VjpArgs<T> va)
+    k_zfwd_vjp(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, VjpArgs<T> va)
 {
     using G = Geo<N, T>;
     using CF = typename G::CFZ_F;
     constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
     constexpr long long ZS = (long long)N * H;
     __shared__ cpx<T> lds[N * W];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     const int col = blockIdx.x * W + threadIdx.x % W;
     const int g = threadIdx.x / W;
     const bool valid = col < G::ZCOLS;
@@ -314,13 +318,15 @@ struct AdjArgs {
 
 template <int N, class T, int STAGE>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
-    k_zfwd_adj(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw, AdjArgs<T> aa)
+    k_zfwd_adj(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, AdjArgs<T> aa)
 {
     using G = Geo<N, T>;
     using CF = typename G::CFZ_F;
     constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
     constexpr long long ZS = (long long)N * H;
     __shared__ cpx<T> lds[N * W];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     const int col = blockIdx.x * W + threadIdx.x % W;
     const int g = threadIdx.x / W;
     const bool valid = col < G::ZCOLS;

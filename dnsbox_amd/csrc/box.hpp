@@ -20,6 +20,10 @@
 
 #include "fft_core.hpp"
 
+#ifndef DNSBOX_Z_PREFETCH
+#define DNSBOX_Z_PREFETCH 1
+#endif
+
 namespace dnsbox {
 
 // ---------------------------------------------------------------------------
@@ -96,7 +100,7 @@ __device__ __forceinline__ double wave_sum(double v)
 //   RP     : radices of the length-N/2 complex row transforms (x; real pairs packed)
 //   RB     : x-lines per block in the x pass, NTX its threads
 // ---------------------------------------------------------------------------
-template <int ZW_, int YW_, int RB_, int NTX_, class ZP_, class YP_, class RP_>
+template <int ZW_, int YW_, int RB_, int NTX_, class ZP_, class YP_, class RP_, int XPAD_ = 0>
 struct CfgBase {
     using ZP = ZP_;
     using YP = YP_;
@@ -105,6 +109,7 @@ struct CfgBase {
     static constexpr int YW = YW_;
     static constexpr int RB = RB_;
     static constexpr int NTX = NTX_;
+    static constexpr int XPAD = XPAD_;  // LDS row padding (complex elements) of the x pass
 };
 template <int N, class T>
 struct Cfg;
@@ -114,14 +119,22 @@ template <> struct Cfg<32, float>  : CfgBase<32, 64, 32, 256, Plan<4, 2, 4>,    
 template <> struct Cfg<48, float>  : CfgBase<64, 64, 16, 256, Plan<2, 12, 2>,   Plan<12, 4>,      Plan<8, 3>>   {};
 template <> struct Cfg<64, float>  : CfgBase<32, 32, 16, 256, Plan<8, 8>,       Plan<8, 8>,       Plan<8, 4>>   {};
 template <> struct Cfg<128, float> : CfgBase<16, 32, 8, 256,  Plan<4, 8, 4>,    Plan<16, 8>,      Plan<8, 8>>   {};
-template <> struct Cfg<256, float> : CfgBase<16, 16, 4, 256,  Plan<8, 4, 8>,    Plan<16, 16>,     Plan<16, 8>>  {};
+#ifdef DNSBOX_CFG256F
+template <> struct Cfg<256, float> : DNSBOX_CFG256F {};
+#else
+template <> struct Cfg<256, float> : CfgBase<16, 64, 4, 256,  Plan<8, 4, 8>,    Plan<16, 16>,     Plan<16, 8>>  {};
+#endif
 template <> struct Cfg<512, float> : CfgBase<8, 16, 2, 256,   Plan<8, 8, 8>,    Plan<16, 16, 2>,  Plan<16, 16>> {};
 template <> struct Cfg<16, double>  : CfgBase<64, 64, 32, 256, Plan<4, 4>,      Plan<4, 4>,       Plan<8>>       {};
 template <> struct Cfg<32, double>  : CfgBase<32, 64, 16, 256, Plan<4, 2, 4>,   Plan<8, 4>,       Plan<4, 4>>    {};
 template <> struct Cfg<48, double>  : CfgBase<64, 32, 8, 256,  Plan<2, 12, 2>,  Plan<12, 4>,      Plan<8, 3>>    {};
 template <> struct Cfg<64, double>  : CfgBase<16, 32, 8, 256,  Plan<4, 4, 4>,   Plan<8, 8>,       Plan<8, 4>>    {};
 template <> struct Cfg<128, double> : CfgBase<16, 16, 4, 256,  Plan<4, 8, 4>,   Plan<8, 4, 4>,    Plan<8, 8>>    {};
-template <> struct Cfg<256, double> : CfgBase<8, 8, 2, 256,    Plan<8, 4, 8>,   Plan<8, 8, 4>,    Plan<8, 4, 4>> {};
+#ifdef DNSBOX_CFG256D
+template <> struct Cfg<256, double> : DNSBOX_CFG256D {};
+#else
+template <> struct Cfg<256, double> : CfgBase<8, 16, 2, 256,   Plan<8, 4, 8>,   Plan<8, 8, 4>,    Plan<8, 4, 4>> {};
+#endif
 template <> struct Cfg<512, double> : CfgBase<8, 8, 1, 256,    Plan<8, 8, 8>,   Plan<8, 8, 8>,    Plan<8, 8, 4>> {};
 
 template <int N, class T>
@@ -252,10 +265,12 @@ __device__ __forceinline__ void zinv_tile(const cpx<T>* const* __restrict__ in,
 template <int N, class T, class Pro>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zinv(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
-           long long out_fs, const cpx<T>* __restrict__ tw, Pro pro)
+           long long out_fs, const cpx<T>* __restrict__ tw_g, Pro pro)
 {
     using G = Geo<N, T>;
     __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     const cpx<T>* in[Pro::FI];
     cpx<T>* out[Pro::FO];
 #pragma unroll
@@ -293,10 +308,12 @@ __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, c
 
 template <int N, class T, int DIR>
 __global__ void __launch_bounds__((Geo<N, T>::YNT))
-    k_ycol(cpx<T>* __restrict__ data0, long long fs, const cpx<T>* __restrict__ tw)
+    k_ycol(cpx<T>* __restrict__ data0, long long fs, const cpx<T>* __restrict__ tw_g)
 {
     using G = Geo<N, T>;
     __shared__ cpx<T> lds[N * G::C::YW];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     ycol_tile<N, T, DIR>(data0 + blockIdx.y * fs, blockIdx.x, lds, tw);
 }
 
@@ -348,7 +365,7 @@ struct OpCopy {
 
 template <int N, class T, class Op>
 struct XGeo {
-    static constexpr int M = N / 2, H = M + 1, HP = H;
+    static constexpr int M = N / 2, H = M + 1, HP = H + Cfg<N, T>::XPAD;
     static constexpr int RB = Cfg<N, T>::RB;
     static constexpr int NTX = Cfg<N, T>::NTX;
     static constexpr int FMAX = Op::FI > Op::FO ? Op::FI : Op::FO;
@@ -356,8 +373,8 @@ struct XGeo {
 };
 
 template <int N, class T, class Op>
-__device__ __forceinline__ void xpass_tile(const void* const* __restrict__ in,
-                                           void* const* __restrict__ out, int blk, cpx<T>* lds,
+__device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long long in_fs,
+                                           cpx<T>* __restrict__ out0, long long out_fs, int blk, cpx<T>* lds,
                                            const cpx<T>* __restrict__ tw, XRed& red)
 {
     using X = XGeo<N, T, Op>;
@@ -368,10 +385,27 @@ __device__ __forceinline__ void xpass_tile(const void* const* __restrict__ in,
 
     // ---- load (+ c2r pre-processing) ----
     if constexpr (Op::IN_SPEC) {
-        for (int e = tid; e < FI * RB * H; e += NTX) {
-            const int f = e / (RB * H), rk = e % (RB * H);
-            lds[f * RB * HP + (rk / H) * HP + rk % H] =
-                static_cast<const cpx<T>*>(in[f])[row0 * H + rk];
+        {
+            // all loads of the block issued before the first LDS store (memory-level parallelism)
+            constexpr int NL = FI * RB * H;
+            constexpr int QL = (NL + NTX - 1) / NTX;
+            cpx<T> buf[QL];
+#pragma unroll
+            for (int q = 0; q < QL; ++q) {
+                const int e = tid + q * NTX;
+                if (NL % NTX == 0 || e < NL) {
+                    const int f = e / (RB * H), rk = e % (RB * H);
+                    buf[q] = in0[f * in_fs + row0 * H + rk];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < QL; ++q) {
+                const int e = tid + q * NTX;
+                if (NL % NTX == 0 || e < NL) {
+                    const int f = e / (RB * H), rk = e % (RB * H);
+                    lds[f * RB * HP + (rk / H) * HP + rk % H] = buf[q];
+                }
+            }
         }
         __syncthreads();
         // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),  k < M;
@@ -408,7 +442,7 @@ __device__ __forceinline__ void xpass_tile(const void* const* __restrict__ in,
         for (int e = tid; e < FI * RB * M; e += NTX) {
             const int f = e / (RB * M), rm = e % (RB * M);
             lds[f * RB * HP + (rm / M) * HP + rm % M] =
-                static_cast<const cpx<T>*>(in[f])[row0 * M + rm];
+                in0[f * in_fs + row0 * M + rm];
         }
         __syncthreads();
     }
@@ -444,12 +478,12 @@ __device__ __forceinline__ void xpass_tile(const void* const* __restrict__ in,
             const cpx<T> zm = conj(row[k == 0 ? 0 : M - k]);
             const cpx<T> a = T(0.5) * (zk + zm);
             const cpx<T> b = (T(0.5) * (zk - zm)) * tw[k];
-            static_cast<cpx<T>*>(out[f])[row0 * H + rk] = a + mul_i<-1>(b);
+            out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
         }
     } else {
         for (int e = tid; e < FO * RB * M; e += NTX) {
             const int f = e / (RB * M), rm = e % (RB * M);
-            static_cast<cpx<T>*>(out[f])[row0 * M + rm] = lds[f * RB * HP + (rm / M) * HP + rm % M];
+            out0[f * out_fs + row0 * M + rm] = lds[f * RB * HP + (rm / M) * HP + rm % M];
         }
     }
 }
@@ -478,22 +512,18 @@ __device__ __forceinline__ void block_max_atomic(XRed& red, unsigned long long* 
 template <int N, class T, class Op>
 __global__ void __launch_bounds__((Cfg<N, T>::NTX))
     k_xpass(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
-            long long out_fs, const cpx<T>* __restrict__ tw, unsigned long long* red_bits,
+            long long out_fs, const cpx<T>* __restrict__ tw_g, unsigned long long* red_bits,
             int signed_last)
 {
     using X = XGeo<N, T, Op>;
     __shared__ cpx<T> lds[X::LDS];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     __shared__ double wsm[4 * 16];
-    const void* in[Op::FI];
-    void* out[Op::FO > 0 ? Op::FO : 1];
-#pragma unroll
-    for (int f = 0; f < Op::FI; ++f) in[f] = in0 + f * in_fs;
-#pragma unroll
-    for (int f = 0; f < Op::FO; ++f) out[f] = out0 + f * out_fs;
     XRed red;
 #pragma unroll
     for (int j = 0; j < 4; ++j) red.m[j] = Op::NRED == 4 && j == 3 ? -1e300 : 0.0;
-    xpass_tile<N, T, Op>(in, out, blockIdx.x, lds, tw, red);
+    xpass_tile<N, T, Op>(in0, in_fs, out0, out_fs, blockIdx.x, lds, tw, red);
     if constexpr (Op::NRED > 0) {
         if (red_bits) block_max_atomic<Op::NRED>(red, red_bits, signed_last != 0, wsm);
     }
@@ -552,25 +582,25 @@ __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long l
     constexpr long long ZS = (long long)N * H;
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
+    // issue the loads of all F fields before the first transform
 #pragma unroll
     for (int f = 0; f < F; ++f) {
-        cpx<T> a[RM];
         const cpx<T>* src = in0 + f * in_fs;
 #pragma unroll
-        for (int q = 0; q < RM; ++q) a[q] = valid ? src[CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
-        CF::run(a, lds, c, g, tw, N);
-#pragma unroll
-        for (int q = 0; q < RM; ++q) O[f][q] = a[q];
+        for (int q = 0; q < RM; ++q) O[f][q] = valid ? src[CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
     }
+#pragma unroll
+    for (int f = 0; f < F; ++f) CF::run(O[f], lds, c, g, tw, N);
 }
 
 // Stepper epilogue on the forward-transformed n_hat = FFT((omega x u)/N^3) of a column
 // tile: mask, pressure projection, band / Kolmogorov forcing (rhs.f90:67-133), then
 // the update of MODE.  When NEXT is set, U receives the input of the next RHS at the
 // same positions (the predicted state for PRED, the new state for CORR).
-template <int N, class T, int MODE, bool NEXT>
+template <int N, class T, int MODE, bool NEXT, bool PREF = false>
 __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZRM], int col, bool valid,
-                                              const EpiArgs<T>& ea, cpx<T> (&U)[3][Geo<N, T>::ZRM])
+                                              const EpiArgs<T>& ea, cpx<T> (&U)[3][Geo<N, T>::ZRM],
+                                              const cpx<T> (*Sp)[Geo<N, T>::ZRM] = nullptr)
 {
     using G = Geo<N, T>;
     using CF = typename G::CFZ_F;
@@ -636,7 +666,7 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
             if constexpr (MODE == EPI_PRED) {  // dnsboxheun.f90:88-114
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    const cpx<T> u = valid ? ea.state[f * ea.fs + idx] : cpx<T>{T(0), T(0)};
+                    const cpx<T> u = PREF ? Sp[f][q] : (valid ? ea.state[f * ea.fs + idx] : cpx<T>{T(0), T(0)});
                     const cpx<T> up = e * (u + tdt * nl[f]);
                     if (NEXT) U[f][q] = up;
                     if (valid) {
@@ -655,7 +685,7 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
             } else if constexpr (MODE == EPI_CORR) {  // dnsboxheun.f90:120-126
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    const cpx<T> u = valid ? ea.state[f * ea.fs + idx] : cpx<T>{T(0), T(0)};
+                    const cpx<T> u = PREF ? Sp[f][q] : (valid ? ea.state[f * ea.fs + idx] : cpx<T>{T(0), T(0)});
                     const cpx<T> un = u + (tdt * T(0.5)) * nl[f];
                     if (NEXT) U[f][q] = un;
                     if (valid) ea.state[f * ea.fs + idx] = un;
@@ -691,11 +721,13 @@ __device__ __forceinline__ void pred_scalars(const BoxScalars* /*unused*/, BoxSc
 
 template <int N, class T, int MODE>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
-    k_zfwd(const cpx<T>* __restrict__ in0, long long in_fs, const cpx<T>* __restrict__ tw,
+    k_zfwd(const cpx<T>* __restrict__ in0, long long in_fs, const cpx<T>* __restrict__ tw_g,
            EpiArgs<T> ea)
 {
     using G = Geo<N, T>;
     __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     const int col = blockIdx.x * G::C::ZW + threadIdx.x % G::C::ZW;
     const bool valid = col < G::ZCOLS;
     cpx<T> O[3][G::ZRM];
@@ -712,16 +744,36 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
 // same columns, so the block reads its whole tile before it writes.
 template <int N, class T, int MODE>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
-    k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw, EpiArgs<T> ea,
+    k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw_g, EpiArgs<T> ea,
                 ProCurl<T> pro)
 {
     using G = Geo<N, T>;
     __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     const int col = blockIdx.x * G::C::ZW + threadIdx.x % G::C::ZW;
     const bool valid = col < G::ZCOLS;
     cpx<T> O[3][G::ZRM];
+#if DNSBOX_Z_PREFETCH
+    // the state at this thread's modes (symmetric z plan: output positions = input positions),
+    // loaded before the transforms so that its latency overlaps them
+    cpx<T> Sp[3][G::ZRM];
+    {
+        using CF = typename G::CFZ_F;
+        constexpr long long ZS = (long long)N * G::H;
+        const int g = threadIdx.x / G::C::ZW;
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+#pragma unroll
+            for (int q = 0; q < G::ZRM; ++q)
+                Sp[f][q] = valid ? ea.state[f * ea.fs + CF::out_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
+    }
+    zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
+    zfwd_epilogue<N, T, MODE, true, true>(O, col, valid, ea, O, Sp);
+#else
     zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
     zfwd_epilogue<N, T, MODE, true>(O, col, valid, ea, O);
+#endif
     cpx<T>* out[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) out[f] = X + f * fs;

@@ -589,6 +589,9 @@ class BoxT final : public BoxBase {
 template <class T>
 BoxBase* make_box_t(const BoxConfig& c)
 {
+#ifdef DNSBOX_ONLY_N
+    return c.n == DNSBOX_ONLY_N ? new BoxT<DNSBOX_ONLY_N, T>(c) : nullptr;
+#endif
     switch (c.n) {
         case 16: return new BoxT<16, T>(c);
         case 32: return new BoxT<32, T>(c);

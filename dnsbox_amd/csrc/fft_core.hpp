@@ -227,6 +227,15 @@ __device__ __forceinline__ void twiddle(cpx<T>* a, int k, int step, const cpx<T>
     }
 }
 
+// copy the twiddle table of length N into LDS (all threads of the block)
+template <int N, class T>
+__device__ __forceinline__ const cpx<T>* stage_twiddles(const cpx<T>* __restrict__ g, cpx<T>* s)
+{
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s[i] = g[i];
+    __syncthreads();
+    return s;
+}
+
 // ---------------------------------------------------------------------------
 // Column FFT: transforms along the slow axis of a tile of W columns.  Thread
 // (c, g) = (tid % W, tid / W) owns butterflies i = g + j*G of column c, G = L/RMAX.
