@@ -38,4 +38,5 @@ def test_fortran_driver_matches_oracle():
         f = o.heun(f, 10, 0.005)
         d, _ = o.diagnostics(f, 0.005)
         assert abs(row[0] - 0.05 * (i + 1)) < 1e-12
-        assert abs(row[1] - d[EKIN]) < 1e-12 * d[EKIN], (i, row[1], d[EKIN])
+        # the driver prints 12 significant digits (e20.12, the Ekin.dat format of io.f90:651)
+        assert abs(row[1] - d[EKIN]) < 1e-11 * d[EKIN], (i, row[1], d[EKIN])
