@@ -44,10 +44,11 @@ def parse():
     return ap.parse_args()
 
 
-def make_box(n, precision, adaptive=True, dt=0.002):
+def make_box(n, precision, adaptive=True, dt=0.002, state_precision=0):
     from dnsbox_amd import Box
     return Box(n=n, precision=precision, nu=0.005, kolmo_amp=1.0, kolmo_k=4, bandlim=0,
-               adaptive_dt=int(adaptive), cfl_min=0.5, cfl_max=0.5, dt_max=0.01, dt=dt)
+               adaptive_dt=int(adaptive), cfl_min=0.5, cfl_max=0.5, dt_max=0.01, dt=dt,
+               state_precision=state_precision)
 
 
 def timed_steps(box, steps, warmup):
@@ -83,23 +84,23 @@ def copy_ceiling_gbs():
     return 2 * n * 4 / (ms * 1e-3) / 1e9
 
 
-def spectrum_error(n, steps, seed=0):
-    """FP32 vs FP64 after `steps` fixed-dt steps from the same seeded field."""
+def run_fixed(n, prec, steps, state_precision=0, seed=0):
+    b = make_box(n, prec, adaptive=False, dt=0.002, state_precision=state_precision)
+    b.init_random(seed, 0, 4.0, 0.5)
+    b.rhs_int_fact(0.002)
+    b.step(steps)
+    out = b.diagnostics()
+    b.close()
+    return out
+
+
+def errors(ref, got):
     import numpy as np
-    boxes = {}
-    for prec in (64, 32):
-        b = make_box(n, prec, adaptive=False, dt=0.002)
-        b.init_random(seed, 0, 4.0, 0.5)
-        b.rhs_int_fact(0.002)
-        b.step(steps)
-        boxes[prec] = b.diagnostics()
-        b.close()
-    (d64, s64), (d32, s32) = boxes[64], boxes[32]
+    (d64, s64), (d32, s32) = ref, got
     nz = s64 > 0
-    spec_err = float(np.max(np.abs(s32[nz] - s64[nz]) / s64[nz]))
-    spec_l2 = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
-    e_err = float(abs(d32[0] - d64[0]) / d64[0])
-    return spec_err, spec_l2, e_err
+    return {"spectrum_max_shell": float(np.max(np.abs(s32[nz] - s64[nz]) / s64[nz])),
+            "spectrum_l2": float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64)),
+            "energy": float(abs(d32[0] - d64[0]) / d64[0])}
 
 
 def cpu_baseline(n, threads):
@@ -161,12 +162,22 @@ def main():
     b64.close()
     print(f"[bench] fp64 {fp64_rate:.1f} steps/s", file=sys.stderr, flush=True)
 
-    # ---- FP32 error vs FP64 ----
-    spec_err = spec_l2 = e_err = None
+    # ---- FP32 state-in-FP64 variant: rate ----
+    b3 = make_box(n, 32, state_precision=64)
+    b3.init_random(0, 0, 4.0, 0.5)
+    secs3 = timed_steps(b3, args.steps, args.warmup)
+    rate_s64 = args.steps / secs3
+    b3.close()
+    print(f"[bench] fp32 + fp64 state {rate_s64:.1f} steps/s", file=sys.stderr, flush=True)
+
+    # ---- FP32 error vs FP64 (same field, same fixed dt, err_steps steps) ----
+    err32 = err_s64 = None
     if args.err_steps > 0:
-        spec_err, spec_l2, e_err = spectrum_error(n, args.err_steps)
-        print(f"[bench] fp32 vs fp64 after {args.err_steps} steps: spectrum max-shell {spec_err:.2e}, "
-              f"L2 {spec_l2:.2e}, energy {e_err:.2e}", file=sys.stderr, flush=True)
+        ref = run_fixed(n, 64, args.err_steps)
+        err32 = errors(ref, run_fixed(n, 32, args.err_steps))
+        err_s64 = errors(ref, run_fixed(n, 32, args.err_steps, state_precision=64))
+        print(f"[bench] vs fp64 after {args.err_steps} steps: fp32 {err32}; fp32+fp64 state {err_s64}",
+              file=sys.stderr, flush=True)
 
     # ---- CPU baseline (oracle, port of the reference structure) ----
     cpu = None
@@ -197,9 +208,10 @@ def main():
                    "n": n, "nu": 0.005, "cfl": 0.5, "stepper": "IF-Heun"},
         "fp64_steps_per_s": fp64_rate,
         "speedup_fp32_vs_fp64": value / fp64_rate,
-        "fp32_vs_fp64_spectrum_rel_err": spec_err,
-        "fp32_vs_fp64_spectrum_rel_l2": spec_l2,
-        "fp32_vs_fp64_energy_rel_err": e_err,
+        "fp32_err_vs_fp64": err32,
+        "fp32_state64_steps_per_s": rate_s64,
+        "fp32_state64_err_vs_fp64": err_s64,
+        "speedup_fp32_state64_vs_fp64": rate_s64 / fp64_rate,
         "err_steps": args.err_steps,
         "roofline": {"bound": "hbm", "achieved": k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": k["gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom,

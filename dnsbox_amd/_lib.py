@@ -28,7 +28,7 @@ class Params(C.Structure):
         ("cfl_max", C.c_double), ("dt_max", C.c_double), ("dt", C.c_double),
         ("n", C.c_int), ("precision", C.c_int), ("kolmo_k", C.c_int), ("bandlim", C.c_int),
         ("spherical", C.c_int), ("parity_mode", C.c_int), ("adaptive_dt", C.c_int),
-        ("reserved", C.c_int),
+        ("state_precision", C.c_int),
     ]
 
 

@@ -240,8 +240,8 @@ __device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<N,
     }
 }
 
-template <int N, class T, class Pro>
-__device__ __forceinline__ void zinv_tile(const cpx<T>* const* __restrict__ in,
+template <int N, class T, class Pro, class TI = T>
+__device__ __forceinline__ void zinv_tile(const cpx<TI>* const* __restrict__ in,
                                           cpx<T>* const* __restrict__ out, int tile,
                                           cpx<T>* lds, const cpx<T>* __restrict__ tw, const Pro& pro)
 {
@@ -258,26 +258,26 @@ __device__ __forceinline__ void zinv_tile(const cpx<T>* const* __restrict__ in,
     for (int f = 0; f < Pro::FI; ++f)
 #pragma unroll
         for (int q = 0; q < RM; ++q)
-            U[f][q] = valid ? in[f][CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
+            U[f][q] = valid ? cvt<T>(in[f][CF::in_pos(g, q) * ZS + col]) : cpx<T>{T(0), T(0)};
     zinv_from_regs<N, T, Pro>(U, out, col, valid, lds, tw, pro);
 }
 
-template <int N, class T, class Pro>
+template <int N, class T, class Pro, class TI = T>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
-    k_zinv(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
+    k_zinv(const cpx<TI>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
            long long out_fs, const cpx<T>* __restrict__ tw_g, Pro pro)
 {
     using G = Geo<N, T>;
     __shared__ cpx<T> lds[N * G::C::ZW];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    const cpx<T>* in[Pro::FI];
+    const cpx<TI>* in[Pro::FI];
     cpx<T>* out[Pro::FO];
 #pragma unroll
     for (int f = 0; f < Pro::FI; ++f) in[f] = in0 + f * in_fs;
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
-    zinv_tile<N, T, Pro>(in, out, blockIdx.x, lds, tw, pro);
+    zinv_tile<N, T, Pro, TI>(in, out, blockIdx.x, lds, tw, pro);
 }
 
 // ===========================================================================
@@ -534,12 +534,15 @@ __global__ void __launch_bounds__((Cfg<N, T>::NTX))
 // ===========================================================================
 enum EpiMode { EPI_RHS = 0, EPI_PRED = 1, EPI_CORR = 2, EPI_AB2 = 3, EPI_STORE = 4, EPI_PREDONLY = 5 };
 
-template <class T>
+// S is the element type of the state (the FFT type T, or double for an FP32 box that
+// keeps its state in FP64); the state update is computed in S.
+template <class T, class S = T>
 struct EpiArgs {
-    cpx<T>* out;          // EPI_RHS: N(u);  EPI_PRED: predicted state;  EPI_STORE: destination
-    cpx<T>* state;        // EPI_PRED/CORR/AB2: the state (updated in place); band forcing input
+    cpx<T>* out;          // EPI_RHS: N(u);  EPI_PRED/PREDONLY: predicted state
+    cpx<S>* state;        // EPI_PRED/CORR/AB2: the state (updated in place); EPI_STORE: destination
     cpx<T>* nold;         // EPI_AB2: N_{n-1} (read, then overwritten with N_n); EPI_PRED: N_0 if set
-    const cpx<T>* rhs_in; // RHS input u (band forcing); may alias state
+    const void* rhs_in;   // RHS input u (band forcing), of type S if rhs_in_s else T
+    int rhs_in_s;
     long long fs;         // field stride of out/state/nold/rhs_in
     double dt_fixed;      // EPI_PREDONLY: step size
     BoxScalars* sc;
@@ -597,10 +600,10 @@ __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long l
 // tile: mask, pressure projection, band / Kolmogorov forcing (rhs.f90:67-133), then
 // the update of MODE.  When NEXT is set, U receives the input of the next RHS at the
 // same positions (the predicted state for PRED, the new state for CORR).
-template <int N, class T, int MODE, bool NEXT, bool PREF = false>
+template <int N, class T, int MODE, bool NEXT, bool PREF = false, class S = T>
 __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZRM], int col, bool valid,
-                                              const EpiArgs<T>& ea, cpx<T> (&U)[3][Geo<N, T>::ZRM],
-                                              const cpx<T> (*Sp)[Geo<N, T>::ZRM] = nullptr)
+                                              const EpiArgs<T, S>& ea, cpx<T> (&U)[3][Geo<N, T>::ZRM],
+                                              const cpx<S> (*Sp)[Geo<N, T>::ZRM] = nullptr)
 {
     using G = Geo<N, T>;
     using CF = typename G::CFZ_F;
@@ -627,7 +630,7 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
         if constexpr (MODE == EPI_STORE) {
             if (valid) {
 #pragma unroll
-                for (int f = 0; f < 3; ++f) ea.out[f * ea.fs + idx] = O[f][q];
+                for (int f = 0; f < 3; ++f) ea.state[f * ea.fs + idx] = cvt<S>(O[f][q]);
             }
             continue;
         } else {
@@ -645,7 +648,11 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
                 nl[2] = T(kz) * pk - O[2][q];
                 if (ph.bandlim && sqrt(double(k2)) <= ph.kcut) {  // rhs.f90:109-118
 #pragma unroll
-                    for (int f = 0; f < 3; ++f) nl[f] = nl[f] + T(fband) * ea.rhs_in[f * ea.fs + idx];
+                    for (int f = 0; f < 3; ++f) {
+                        const cpx<T> ui = ea.rhs_in_s ? cvt<T>(static_cast<const cpx<S>*>(ea.rhs_in)[f * ea.fs + idx])
+                                                      : static_cast<const cpx<T>*>(ea.rhs_in)[f * ea.fs + idx];
+                        nl[f] = nl[f] + T(fband) * ui;
+                    }
                 }
                 if (ph.kolmo_k > 0 && ix == 0 && iz == 0) {  // f = amp sin(kf y) e_x
                     if (iy == ph.kolmo_k) nl[0].y -= T(0.5 * ph.kolmo_amp * n3);
@@ -659,19 +666,20 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
                 for (int f = 0; f < 3; ++f) ea.out[f * ea.fs + idx] = nl[f];
             }
         } else {
-            // rhs.f90:8-16 integrating factor, exponent in FP64
+            // rhs.f90:8-16 integrating factor, exponent in FP64; the update in the state type S
             const double k2d = double(kx * kx + ky * ky + kz * kz);
             const double e64 = exp((-ph.nu * k2d + ph.Q) * dt);
-            const T e = T(e64), tdt = T(dt);
+            const S e = S(e64), sdt = S(dt);
             if constexpr (MODE == EPI_PRED) {  // dnsboxheun.f90:88-114
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    const cpx<T> u = PREF ? Sp[f][q] : (valid ? ea.state[f * ea.fs + idx] : cpx<T>{T(0), T(0)});
-                    const cpx<T> up = e * (u + tdt * nl[f]);
+                    const cpx<S> u = PREF ? Sp[f][q] : (valid ? ea.state[f * ea.fs + idx] : cpx<S>{S(0), S(0)});
+                    const cpx<S> n = cvt<S>(nl[f]);
+                    const cpx<T> up = cvt<T>(e * (u + sdt * n));
                     if (NEXT) U[f][q] = up;
                     if (valid) {
                         if (ea.out) ea.out[f * ea.fs + idx] = up;
-                        ea.state[f * ea.fs + idx] = e * (u + (tdt * T(0.5)) * nl[f]);
+                        ea.state[f * ea.fs + idx] = e * (u + (sdt * S(0.5)) * n);
                         if (ea.nold) ea.nold[f * ea.fs + idx] = nl[f];
                     }
                 }
@@ -679,26 +687,26 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     if (!valid) continue;
-                    const cpx<T> u = ea.state[f * ea.fs + idx];
-                    ea.out[f * ea.fs + idx] = e * (u + tdt * nl[f]);
+                    const cpx<S> u = ea.state[f * ea.fs + idx];
+                    ea.out[f * ea.fs + idx] = cvt<T>(e * (u + sdt * cvt<S>(nl[f])));
                 }
             } else if constexpr (MODE == EPI_CORR) {  // dnsboxheun.f90:120-126
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    const cpx<T> u = PREF ? Sp[f][q] : (valid ? ea.state[f * ea.fs + idx] : cpx<T>{T(0), T(0)});
-                    const cpx<T> un = u + (tdt * T(0.5)) * nl[f];
-                    if (NEXT) U[f][q] = un;
+                    const cpx<S> u = PREF ? Sp[f][q] : (valid ? ea.state[f * ea.fs + idx] : cpx<S>{S(0), S(0)});
+                    const cpx<S> un = u + (sdt * S(0.5)) * cvt<S>(nl[f]);
+                    if (NEXT) U[f][q] = cvt<T>(un);
                     if (valid) ea.state[f * ea.fs + idx] = un;
                 }
             } else if constexpr (MODE == EPI_AB2) {
-                const T e2 = T(e64 * e64);
+                const S e2 = S(e64 * e64);
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     if (!valid) continue;
-                    const cpx<T> u = ea.state[f * ea.fs + idx];
-                    const cpx<T> no = ea.nold[f * ea.fs + idx];
+                    const cpx<S> u = ea.state[f * ea.fs + idx];
+                    const cpx<S> no = cvt<S>(ea.nold[f * ea.fs + idx]);
                     ea.state[f * ea.fs + idx] =
-                        e * u + tdt * (T(1.5) * e * nl[f] - T(0.5) * e2 * no);
+                        e * u + sdt * (S(1.5) * e * cvt<S>(nl[f]) - S(0.5) * e2 * no);
                     ea.nold[f * ea.fs + idx] = nl[f];
                 }
             }
@@ -719,10 +727,10 @@ __device__ __forceinline__ void pred_scalars(const BoxScalars* /*unused*/, BoxSc
     sc->dt = dt;
 }
 
-template <int N, class T, int MODE>
+template <int N, class T, int MODE, class S = T>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zfwd(const cpx<T>* __restrict__ in0, long long in_fs, const cpx<T>* __restrict__ tw_g,
-           EpiArgs<T> ea)
+           EpiArgs<T, S> ea)
 {
     using G = Geo<N, T>;
     __shared__ cpx<T> lds[N * G::C::ZW];
@@ -732,7 +740,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     const bool valid = col < G::ZCOLS;
     cpx<T> O[3][G::ZRM];
     zfwd_regs<N, T>(in0, in_fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, false>(O, col, valid, ea, O);
+    zfwd_epilogue<N, T, MODE, false, false, S>(O, col, valid, ea, O);
     if constexpr (MODE == EPI_PRED) {
         if (blockIdx.x == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
     }
@@ -742,9 +750,9 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
 // of the next RHS: the next RHS input never makes a round trip through HBM.  The
 // input tile (fields 0..2 of X) and the output tile (fields 0..5 of X) cover the
 // same columns, so the block reads its whole tile before it writes.
-template <int N, class T, int MODE>
+template <int N, class T, int MODE, class S = T>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
-    k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw_g, EpiArgs<T> ea,
+    k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw_g, EpiArgs<T, S> ea,
                 ProCurl<T> pro)
 {
     using G = Geo<N, T>;
@@ -757,7 +765,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
 #if DNSBOX_Z_PREFETCH
     // the state at this thread's modes (symmetric z plan: output positions = input positions),
     // loaded before the transforms so that its latency overlaps them
-    cpx<T> Sp[3][G::ZRM];
+    cpx<S> Sp[3][G::ZRM];
     {
         using CF = typename G::CFZ_F;
         constexpr long long ZS = (long long)N * G::H;
@@ -766,13 +774,13 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
         for (int f = 0; f < 3; ++f)
 #pragma unroll
             for (int q = 0; q < G::ZRM; ++q)
-                Sp[f][q] = valid ? ea.state[f * ea.fs + CF::out_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
+                Sp[f][q] = valid ? ea.state[f * ea.fs + CF::out_pos(g, q) * ZS + col] : cpx<S>{S(0), S(0)};
     }
     zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, true, true>(O, col, valid, ea, O, Sp);
+    zfwd_epilogue<N, T, MODE, true, true, S>(O, col, valid, ea, O, Sp);
 #else
     zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, true>(O, col, valid, ea, O);
+    zfwd_epilogue<N, T, MODE, true, false, S>(O, col, valid, ea, O);
 #endif
     cpx<T>* out[6];
 #pragma unroll

@@ -2,5 +2,5 @@
 #include "box_impl.hpp"
 
 namespace dnsbox {
-BoxBase* make_box_f64(const BoxConfig& c) { return make_box_t<double>(c); }
+BoxBase* make_box_f64(const BoxConfig& c) { return make_box_t<double, double>(c); }
 }  // namespace dnsbox

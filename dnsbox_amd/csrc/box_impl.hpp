@@ -1,6 +1,7 @@
 // Host-side launchers of one box: owns the device buffers of a box and
-// sequences the kernel passes of box.hpp.  Instantiated per (N, precision) in
-// box_f32.hip / box_f64.hip; the C-ABI (dnsbox_capi.hip) sees only BoxBase.
+// sequences the kernel passes of box.hpp / adjoint.hpp.  Instantiated per
+// (N, FFT type T, state type S) in box_f32.hip / box_f64.hip; the C-ABI
+// (dnsbox_capi.hip) sees only BoxBase.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,6 +10,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "adjoint.hpp"
@@ -32,6 +34,7 @@ struct HipError : std::runtime_error {
 struct BoxConfig {
     int n = 32;
     int precision = 64;
+    int state_precision = 64;
     BoxPhys ph{};
     double deltak = 1.0;
     double dt = 1e-3;
@@ -46,13 +49,15 @@ class BoxBase {
     hipStream_t stream = nullptr;
     int nspec = 0;
     long long fs = 0;        // complex elements per spectral field
-    size_t cbytes = 0;       // bytes per complex element
-    size_t rbytes = 0;       // bytes per real element
+    size_t cbytes = 0;       // bytes per complex element of the transforms
+    size_t sbytes = 0;       // bytes per complex element of the state
+    size_t rbytes = 0;       // bytes per real element of the transforms
+    bool uniform = true;     // state type == transform type (device-tensor entry points need it)
 
     // device buffers
-    void* state = nullptr;   // 3 fields
+    void* state = nullptr;   // 3 fields (state type)
     void* pred = nullptr;    // 3 fields (IF-Heun predicted state)
-    void* scratch = nullptr; // 6 fields
+    void* scratch = nullptr; // 6 fields (9 for the VJP, 12 for rhsAdj)
     void* nonlin = nullptr;  // 3 fields
     void* nold = nullptr;    // 3 fields (AB2 history), lazily
     void* tw = nullptr;      // twiddle table
@@ -62,41 +67,45 @@ class BoxBase {
     unsigned long long* red_bits = nullptr;  // 4 slots
     int red_blocks = 0;
     bool ab2_started = false;
-    bool x6_valid = false;   // scratch X holds the z-inverse + curl of the current state
-    int scratch_fields = 6;  // capacity of scratch in spectral fields
+    bool x6_valid = false;   // scratch holds the z-inverse + curl of the current state
+    int scratch_fields = 6;
     void* adjbuf = nullptr;  // 9 fields: A, Mu, u~ (adjoint passes), lazily
     double* band2 = nullptr; // band-forcing scalars of the VJP
 
+    virtual void rhs_state() = 0;                 // nonlin <- N(state)
     virtual void rhs(const void* in, void* out) = 0;
     virtual void heun_step() = 0;
     virtual void ab2_step() = 0;
-    virtual void spec_reduce(const void* u) = 0;  // -> red_out[DG_NSCAL + nspec]
-    virtual void phys_reduce(const void* u) = 0;  // -> red_bits[0..3]
+    virtual void spec_reduce() = 0;               // state -> red_out[DG_NSCAL + nspec]
+    virtual void phys_reduce() = 0;               // state -> red_bits[0..3]
     virtual void init_random(unsigned long long seed, int box, double k0, double energy) = 0;
     virtual void project(int dealias, int proj) = 0;
-    virtual void to_physical(const void* spec, void* phys3) = 0;   // device real (3, N^3)
-    virtual void from_physical(const void* phys3, void* spec) = 0; // device real (3, N^3)
-    // average device time of each RHS pass over `reps` back-to-back launches (ms),
-    // measured with HIP events on the box stream; order of PASS_* below
+    virtual void to_physical(void* phys3) = 0;    // state -> device real (3, N^3)
+    virtual void from_physical(const void* phys3) = 0;
+    // average device time of each step pass over `reps` back-to-back launches (ms),
+    // measured with HIP events on the box stream; order of DNSBOX_PASS_*
     virtual void time_passes(int reps, double* ms) = 0;
     // adjoint and caller-buffer entry points (device pointers, internal layout)
     virtual void rhs_vjp(const void* u, const void* lam, void* out) = 0;
     virtual void step_vjp(const void* u, const void* lam, void* lam_out, double dt) = 0;
     virtual void rhs_adj(const void* u, void* out, double lin_sign) = 0;
     virtual void step_ext(void* st, int nsteps) = 0;
-    virtual void convert_in(void* dst, const double* src, long long n) = 0;   // host-staged
-    virtual void convert_out(double* dst, const void* src, long long n) = 0;
+    // FP64 device array -> transform type / state type, and back
+    virtual void convert_in(void* dst, const double* src, long long n, bool state_type) = 0;
+    virtual void convert_out(double* dst, const void* src, long long n, bool state_type) = 0;
 };
 
 BoxBase* make_box_f32(const BoxConfig& c);
 BoxBase* make_box_f64(const BoxConfig& c);
 
-template <int N, class T>
+template <int N, class T, class S = T>
 class BoxT final : public BoxBase {
     using G = Geo<N, T>;
     using C = cpx<T>;
+    using CS = cpx<S>;
     static constexpr int ZNT = G::ZNT;
     static constexpr int YNT = G::YNT;
+    static constexpr bool UNIFORM = std::is_same<S, T>::value;
 
    public:
     explicit BoxT(const BoxConfig& c)
@@ -104,14 +113,16 @@ class BoxT final : public BoxBase {
         cfg = c;
         fs = G::FS;
         cbytes = sizeof(C);
+        sbytes = sizeof(CS);
         rbytes = sizeof(T);
+        uniform = UNIFORM;
         const double ax = (double(N) / 2.0) * (2.0 * 1.0 / 3.0);
         if (cfg.ph.spherical) nspec = int(ax / cfg.deltak) - 1;
         else nspec = int(std::sqrt(3.0 * ax * ax) / cfg.deltak) - 1;
         if (nspec < 1) nspec = 1;
         const size_t fb = size_t(fs) * sizeof(C);
         try {
-            DNSBOX_HIP(hipMalloc(&state, 3 * fb));
+            DNSBOX_HIP(hipMalloc(&state, 3 * size_t(fs) * sizeof(CS)));
             DNSBOX_HIP(hipMalloc(&pred, 3 * fb));
             DNSBOX_HIP(hipMalloc(&scratch, 6 * fb));
             DNSBOX_HIP(hipMalloc(&nonlin, 3 * fb));
@@ -121,7 +132,7 @@ class BoxT final : public BoxBase {
             DNSBOX_HIP(hipMalloc(&red_partial, size_t(red_blocks) * (DG_NSCAL + nspec) * sizeof(double)));
             DNSBOX_HIP(hipMalloc(&red_out, (DG_NSCAL + nspec) * sizeof(double)));
             DNSBOX_HIP(hipMalloc(&red_bits, 4 * sizeof(unsigned long long)));
-            DNSBOX_HIP(hipMemset(state, 0, 3 * fb));
+            DNSBOX_HIP(hipMemset(state, 0, 3 * size_t(fs) * sizeof(CS)));
             DNSBOX_HIP(hipMemset(pred, 0, 3 * fb));
             DNSBOX_HIP(hipMemset(nonlin, 0, 3 * fb));
             std::vector<C> h(N);
@@ -155,52 +166,41 @@ class BoxT final : public BoxBase {
 
     const C* twc() const { return static_cast<const C*>(tw); }
     C* X() const { return static_cast<C*>(scratch); }
+    CS* St() const { return static_cast<CS*>(state); }
+    C* Pr() const { return static_cast<C*>(pred); }
+    static T inv_n3() { return T(1.0 / (double(N) * N * N)); }
 
-    // five passes of one RHS; epilogue selected by MODE
-    template <int MODE>
-    void rhs_passes(const C* in, const EpiArgs<T>& ea, bool track_cfl)
+    void require_uniform(const char* what) const
     {
-        const T s = T(1.0 / (double(N) * N * N));
-        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in, fs,
-                           X(), fs, twc(), ProCurl<T>{s});
-        DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
-        DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
-                           stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
-        DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
-        DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_zfwd<N, T, MODE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
-                           ea);
-        DNSBOX_LAUNCH_CHECK();
+        if (!UNIFORM)
+            throw std::invalid_argument(std::string(what) +
+                                        " needs the state in the transform precision (state_precision = precision)");
     }
 
-    EpiArgs<T> epi() const
+    EpiArgs<T, S> epi() const
     {
-        EpiArgs<T> ea{};
+        EpiArgs<T, S> ea{};
         ea.fs = fs;
         ea.sc = sc;
         ea.ph = cfg.ph;
         return ea;
     }
 
-    void eband(const void* u)
+    template <class E>
+    void eband(const cpx<E>* u)
     {
         if (!cfg.ph.bandlim) return;
         const double sm = 1.0 / (double(N) * N * N);
-        hipLaunchKernelGGL((k_eband<N, T>), dim3(1), dim3(256), 0, stream, static_cast<const C*>(u), fs,
-                           cfg.ph.kcut, sm * sm, sc);
+        hipLaunchKernelGGL((k_eband<N, E>), dim3(1), dim3(256), 0, stream, u, fs, cfg.ph.kcut, sm * sm, sc);
         DNSBOX_LAUNCH_CHECK();
     }
 
-    void rhs(const void* in, void* out) override
+    template <class E>
+    void zinv_curl(const cpx<E>* in)
     {
-        eband(in);
-        EpiArgs<T> ea = epi();
-        ea.out = static_cast<C*>(out);
-        ea.rhs_in = static_cast<const C*>(in);
-        rhs_passes<EPI_RHS>(static_cast<const C*>(in), ea, false);
+        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>, E>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in, fs, X(), fs,
+                           twc(), ProCurl<T>{inv_n3()});
+        DNSBOX_LAUNCH_CHECK();
     }
 
     // middle passes of an RHS: y-inverse (6), x cross product, y-forward (3)
@@ -215,36 +215,68 @@ class BoxT final : public BoxBase {
         DNSBOX_LAUNCH_CHECK();
     }
 
+    // five passes of one RHS with an unfused z-forward epilogue of MODE
+    template <int MODE, class E>
+    void rhs_passes(const cpx<E>* in, const EpiArgs<T, S>& ea, bool track_cfl)
+    {
+        zinv_curl(in);
+        rhs_middle(track_cfl);
+        hipLaunchKernelGGL((k_zfwd<N, T, MODE, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), ea);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void rhs_state() override
+    {
+        x6_valid = false;
+        eband(St());
+        EpiArgs<T, S> ea = epi();
+        ea.out = static_cast<C*>(nonlin);
+        ea.rhs_in = St();
+        ea.rhs_in_s = 1;
+        rhs_passes<EPI_RHS>(St(), ea, false);
+    }
+
+    void rhs(const void* in_, void* out) override
+    {
+        require_uniform("rhs on device buffers");
+        x6_valid = false;
+        const C* in = static_cast<const C*>(in_);
+        eband(in);
+        EpiArgs<T, S> ea = epi();
+        ea.out = static_cast<C*>(out);
+        ea.rhs_in = in;
+        ea.rhs_in_s = 0;
+        rhs_passes<EPI_RHS>(in, ea, false);
+    }
+
     // IF-Heun step (dnsboxheun.f90:84-126) as a pipeline: the forward z pass of each
     // RHS carries the stepper epilogue and the z-inverse + curl of the next RHS.
     void heun_step() override
     {
-        C* S = static_cast<C*>(state);
-        C* P = static_cast<C*>(pred);
-        const T s = T(1.0 / (double(N) * N * N));
+        const ProCurl<T> pro{inv_n3()};
         if (!x6_valid) {
-            eband(S);
-            hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, S, fs,
-                               X(), fs, twc(), ProCurl<T>{s});
-            DNSBOX_LAUNCH_CHECK();
+            eband(St());
+            zinv_curl(St());
         }
         rhs_middle(true);
-        EpiArgs<T> ea = epi();
-        ea.out = cfg.ph.bandlim ? P : nullptr;  // the predicted state is kept only for band forcing
-        ea.state = S;
-        ea.rhs_in = S;
-        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                           twc(), ea, ProCurl<T>{s});
+        EpiArgs<T, S> ea = epi();
+        ea.out = cfg.ph.bandlim ? Pr() : nullptr;  // the predicted state is kept only for band forcing
+        ea.state = St();
+        ea.rhs_in = St();
+        ea.rhs_in_s = 1;
+        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
+                           twc(), ea, pro);
         DNSBOX_LAUNCH_CHECK();
-        eband(P);
+        eband(Pr());
         rhs_middle(false);
-        EpiArgs<T> eb = epi();
-        eb.state = S;
-        eb.rhs_in = P;
-        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                           twc(), eb, ProCurl<T>{s});
+        EpiArgs<T, S> eb = epi();
+        eb.state = St();
+        eb.rhs_in = Pr();
+        eb.rhs_in_s = 0;
+        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
+                           twc(), eb, pro);
         DNSBOX_LAUNCH_CHECK();
-        eband(S);
+        eband(St());
         hipLaunchKernelGGL(k_step_end<0>, dim3(1), dim3(64), 0, stream, sc);
         DNSBOX_LAUNCH_CHECK();
         x6_valid = true;
@@ -253,110 +285,112 @@ class BoxT final : public BoxBase {
     void ab2_step() override
     {
         x6_valid = false;
-        C* S = static_cast<C*>(state);
         if (!nold) DNSBOX_HIP(hipMalloc(&nold, 3 * size_t(fs) * sizeof(C)));
         if (!ab2_started) {
             // first step: IF-Heun, keeping N(u_0) as the AB2 history (dnsbox.f90:87-141)
-            C* P = static_cast<C*>(pred);
-            eband(S);
-            EpiArgs<T> ea = epi();
-            ea.out = P;
-            ea.state = S;
-            ea.rhs_in = S;
+            eband(St());
+            EpiArgs<T, S> ea = epi();
+            ea.out = Pr();
+            ea.state = St();
+            ea.rhs_in = St();
+            ea.rhs_in_s = 1;
             ea.nold = static_cast<C*>(nold);
-            rhs_passes<EPI_PRED>(S, ea, true);
-            eband(P);
-            EpiArgs<T> eb = epi();
-            eb.state = S;
-            eb.rhs_in = P;
-            rhs_passes<EPI_CORR>(P, eb, false);
+            rhs_passes<EPI_PRED>(St(), ea, true);
+            eband(Pr());
+            EpiArgs<T, S> eb = epi();
+            eb.state = St();
+            eb.rhs_in = Pr();
+            eb.rhs_in_s = 0;
+            rhs_passes<EPI_CORR>(Pr(), eb, false);
             ab2_started = true;
         } else {
-            eband(S);
-            EpiArgs<T> ea = epi();
-            ea.state = S;
-            ea.rhs_in = S;
+            eband(St());
+            EpiArgs<T, S> ea = epi();
+            ea.state = St();
+            ea.rhs_in = St();
+            ea.rhs_in_s = 1;
             ea.nold = static_cast<C*>(nold);
-            rhs_passes<EPI_AB2>(S, ea, true);
+            rhs_passes<EPI_AB2>(St(), ea, true);
         }
         hipLaunchKernelGGL(k_step_end<0>, dim3(1), dim3(64), 0, stream, sc);
         DNSBOX_LAUNCH_CHECK();
     }
 
-    void spec_reduce(const void* u) override
+    void spec_reduce() override
     {
         const double sm = 1.0 / (double(N) * N * N);
         const int nslot = DG_NSCAL + nspec;
-        hipLaunchKernelGGL((k_spec_reduce<N, T>), dim3(red_blocks), dim3(256), nspec * sizeof(double),
-                           stream, static_cast<const C*>(u), fs, cfg.ph, cfg.deltak, nspec, red_partial);
+        hipLaunchKernelGGL((k_spec_reduce<N, S>), dim3(red_blocks), dim3(256), nspec * sizeof(double), stream,
+                           St(), fs, cfg.ph, cfg.deltak, nspec, red_partial);
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_spec_finish<0>, dim3(1), dim3(256), 0, stream, red_partial, red_blocks, nslot,
                            sm * sm, cfg.ph.nu, cfg.deltak, red_out, (BoxScalars*)nullptr);
         DNSBOX_LAUNCH_CHECK();
     }
 
-    void phys_reduce(const void* u) override
+    void phys_reduce() override
     {
+        x6_valid = false;
         DNSBOX_HIP(hipMemsetAsync(red_bits, 0, 4 * sizeof(unsigned long long), stream));
-        const T s = T(1.0 / (double(N) * N * N));
-        hipLaunchKernelGGL((k_zinv<N, T, ProVelDiv<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream,
-                           static_cast<const C*>(u), fs, X(), fs, twc(), ProVelDiv<T>{s});
+        hipLaunchKernelGGL((k_zinv<N, T, ProVelDiv<T>, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, St(), fs, X(),
+                           fs, twc(), ProVelDiv<T>{inv_n3()});
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 4), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_xpass<N, T, OpVelDivMax<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
-                           stream, X(), fs, X(), fs, twc(), red_bits, 1);
+        hipLaunchKernelGGL((k_xpass<N, T, OpVelDivMax<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream,
+                           X(), fs, X(), fs, twc(), red_bits, 1);
         DNSBOX_LAUNCH_CHECK();
     }
 
     void init_random(unsigned long long seed, int box, double k0, double energy) override
     {
-        const long long tot = fs;
-        hipLaunchKernelGGL((k_init_solenoidal<N, T>), dim3((tot + 255) / 256), dim3(256), 0, stream,
-                           static_cast<C*>(state), fs, cfg.ph, seed, box, k0);
+        x6_valid = false;
+        hipLaunchKernelGGL((k_init_solenoidal<N, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
+                           cfg.ph, seed, box, k0);
         DNSBOX_LAUNCH_CHECK();
-        spec_reduce(state);
-        hipLaunchKernelGGL((k_scale_to_energy<T>), dim3(1024), dim3(256), 0, stream,
-                           static_cast<C*>(state), 3 * fs, (const double*)(red_out + DG_EKIN), energy);
+        spec_reduce();
+        hipLaunchKernelGGL((k_scale_to_energy<S>), dim3(1024), dim3(256), 0, stream, St(), 3 * fs,
+                           (const double*)(red_out + DG_EKIN), energy);
         DNSBOX_LAUNCH_CHECK();
     }
 
     void project(int dealias, int proj) override
     {
-        hipLaunchKernelGGL((k_project_dealias<N, T>), dim3((fs + 255) / 256), dim3(256), 0, stream,
-                           static_cast<C*>(state), fs, cfg.ph, dealias, proj);
+        x6_valid = false;
+        hipLaunchKernelGGL((k_project_dealias<N, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
+                           cfg.ph, dealias, proj);
         DNSBOX_LAUNCH_CHECK();
     }
 
-    void to_physical(const void* spec, void* phys3) override
+    void to_physical(void* phys3) override
     {
-        const T s = T(1.0 / (double(N) * N * N));
-        hipLaunchKernelGGL((k_zinv<N, T, ProScale<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream,
-                           static_cast<const C*>(spec), fs, X(), fs, twc(), ProScale<T>{s});
+        x6_valid = false;
+        hipLaunchKernelGGL((k_zinv<N, T, ProScale<T>, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, St(), fs, X(),
+                           fs, twc(), ProScale<T>{inv_n3()});
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
         // real output field f at phys3 + f * N^3 reals
         const long long pf = (long long)N * N * N / 2;  // complex pairs per real field
-        hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, true, false>>), dim3(G::XBLOCKS),
-                           dim3(Cfg<N, T>::NTX), 0, stream, X(), fs, static_cast<C*>(phys3), pf, twc(),
-                           (unsigned long long*)nullptr, 0);
+        hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, true, false>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
+                           stream, X(), fs, static_cast<C*>(phys3), pf, twc(), (unsigned long long*)nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
     }
 
-    void from_physical(const void* phys3, void* spec) override
+    void from_physical(const void* phys3) override
     {
+        x6_valid = false;
         const long long pf = (long long)N * N * N / 2;
-        hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, false, true>>), dim3(G::XBLOCKS),
-                           dim3(Cfg<N, T>::NTX), 0, stream, static_cast<const C*>(phys3), pf, X(), fs,
-                           twc(), (unsigned long long*)nullptr, 0);
+        hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, false, true>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
+                           stream, static_cast<const C*>(phys3), pf, X(), fs, twc(), (unsigned long long*)nullptr,
+                           0);
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
-        EpiArgs<T> ea = epi();
-        ea.out = static_cast<C*>(spec);
-        hipLaunchKernelGGL((k_zfwd<N, T, EPI_STORE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                           twc(), ea);
+        EpiArgs<T, S> ea = epi();
+        ea.state = St();
+        hipLaunchKernelGGL((k_zfwd<N, T, EPI_STORE, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
+                           ea);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -414,6 +448,7 @@ class BoxT final : public BoxBase {
 
     void rhs_vjp(const void* u, const void* lam, void* out) override
     {
+        require_uniform("the RHS adjoint");
         x6_valid = false;
         ensure_scratch(9);
         ensure_adj();
@@ -425,25 +460,23 @@ class BoxT final : public BoxBase {
     // lam_out = (d u_{n+1} / d u_n)^T lam for one fixed-dt IF-Heun step from u
     void step_vjp(const void* u_, const void* lam_, void* lam_out, double dt) override
     {
+        require_uniform("the step adjoint");
         x6_valid = false;
         ensure_scratch(9);
         ensure_adj();
         const C* u = static_cast<const C*>(u_);
         const C* lam = static_cast<const C*>(lam_);
-        C* P = static_cast<C*>(pred);
-        const T s = T(1.0 / (double(N) * N * N));
         // u* = E (u + dt N(u))
         eband(u);
-        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, u, fs, X(), fs,
-                           twc(), ProCurl<T>{s});
-        DNSBOX_LAUNCH_CHECK();
+        zinv_curl(u);
         rhs_middle(false);
-        EpiArgs<T> ea = epi();
-        ea.out = P;
-        ea.state = const_cast<C*>(u);
+        EpiArgs<T, S> ea = epi();
+        ea.out = Pr();
+        ea.state = reinterpret_cast<CS*>(const_cast<C*>(u));
         ea.rhs_in = u;
+        ea.rhs_in_s = 0;
         ea.dt_fixed = dt;
-        hipLaunchKernelGGL((k_zfwd<N, T, EPI_PREDONLY>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
+        hipLaunchKernelGGL((k_zfwd<N, T, EPI_PREDONLY, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
                            ea);
         DNSBOX_LAUNCH_CHECK();
         // lam_* = J(u*)^T (dt/2 lam);  A = E (lam + lam_*),  Mu = E (dt/2 lam + dt lam_*)
@@ -452,7 +485,7 @@ class BoxT final : public BoxBase {
         v1.A = adjA();
         v1.Mu = adjMu();
         v1.dt = dt;
-        vjp_passes<VJP_S1>(P, lam, 0.5 * dt, v1);
+        vjp_passes<VJP_S1>(Pr(), lam, 0.5 * dt, v1);
         // lam_n = A + J(u)^T Mu
         VjpArgs<T> v2{};
         v2.out = static_cast<C*>(lam_out);
@@ -464,11 +497,12 @@ class BoxT final : public BoxBase {
     // rhsAdj.f90:64-381 (phatadj read as phat)
     void rhs_adj(const void* u_, void* out, double lin_sign) override
     {
+        require_uniform("rhsAdj");
         x6_valid = false;
         ensure_scratch(12);
         ensure_adj();
         const C* u = static_cast<const C*>(u_);
-        const T s = T(1.0 / (double(N) * N * N));
+        const T s = inv_n3();
         AdjArgs<T> aa{};
         aa.out = static_cast<C*>(out);
         aa.u = u;
@@ -508,6 +542,7 @@ class BoxT final : public BoxBase {
     // IF-Heun steps on a caller-owned state buffer
     void step_ext(void* st, int nsteps) override
     {
+        require_uniform("stepping a device buffer");
         void* own = state;
         x6_valid = false;
         state = st;
@@ -524,9 +559,11 @@ class BoxT final : public BoxBase {
 
     void time_passes(int reps, double* ms) override
     {
-        const T s = T(1.0 / (double(N) * N * N));
-        C* P = static_cast<C*>(pred);
-        const C* S = static_cast<const C*>(state);
+        x6_valid = false;
+        // the fused kernels update a state: give them a scratch one
+        CS* tmp = nullptr;
+        DNSBOX_HIP(hipMalloc(&tmp, 3 * size_t(fs) * sizeof(CS)));
+        DNSBOX_HIP(hipMemsetAsync(tmp, 0, 3 * size_t(fs) * sizeof(CS), stream));
         hipEvent_t e0, e1;
         DNSBOX_HIP(hipEventCreate(&e0));
         DNSBOX_HIP(hipEventCreate(&e1));
@@ -540,10 +577,8 @@ class BoxT final : public BoxBase {
             DNSBOX_HIP(hipEventElapsedTime(&t, e0, e1));
             ms[which] = double(t) / reps;
         };
-        timed(0, [&] {
-            hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, S, fs, X(),
-                               fs, twc(), ProCurl<T>{s});
-        });
+        const ProCurl<T> pro{inv_n3()};
+        timed(0, [&] { zinv_curl(static_cast<const CS*>(tmp)); });
         timed(1, [&] {
             hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
         });
@@ -554,52 +589,60 @@ class BoxT final : public BoxBase {
         timed(3, [&] {
             hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
         });
-        EpiArgs<T> ea = epi();
+        EpiArgs<T, S> ea = epi();
         ea.out = nullptr;
-        ea.state = P;
-        ea.rhs_in = P;
+        ea.state = tmp;
+        ea.rhs_in = tmp;
+        ea.rhs_in_s = 1;
         timed(4, [&] {
-            hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                               twc(), ea, ProCurl<T>{s});
+            hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
+                               twc(), ea, pro);
         });
         timed(5, [&] {
-            hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                               twc(), ea, ProCurl<T>{s});
+            hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
+                               twc(), ea, pro);
         });
         DNSBOX_LAUNCH_CHECK();
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        DNSBOX_HIP(hipStreamSynchronize(stream));
+        DNSBOX_HIP(hipFree(tmp));
     }
 
-    void convert_in(void* dst, const double* src, long long n) override
+    void convert_in(void* dst, const double* src, long long n, bool state_type) override
     {
-        // src: device FP64 array of n reals
-        hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, stream, static_cast<T*>(dst),
-                           src, n);
+        if (state_type)
+            hipLaunchKernelGGL((k_convert<S, double>), dim3(1024), dim3(256), 0, stream, static_cast<S*>(dst), src, n);
+        else
+            hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, stream, static_cast<T*>(dst), src, n);
         DNSBOX_LAUNCH_CHECK();
     }
-    void convert_out(double* dst, const void* src, long long n) override
+    void convert_out(double* dst, const void* src, long long n, bool state_type) override
     {
-        hipLaunchKernelGGL((k_convert<double, T>), dim3(1024), dim3(256), 0, stream, dst,
-                           static_cast<const T*>(src), n);
+        if (state_type)
+            hipLaunchKernelGGL((k_convert<double, S>), dim3(1024), dim3(256), 0, stream, dst,
+                               static_cast<const S*>(src), n);
+        else
+            hipLaunchKernelGGL((k_convert<double, T>), dim3(1024), dim3(256), 0, stream, dst,
+                               static_cast<const T*>(src), n);
         DNSBOX_LAUNCH_CHECK();
     }
 };
 
-template <class T>
+template <class T, class S>
 BoxBase* make_box_t(const BoxConfig& c)
 {
 #ifdef DNSBOX_ONLY_N
-    return c.n == DNSBOX_ONLY_N ? new BoxT<DNSBOX_ONLY_N, T>(c) : nullptr;
+    return c.n == DNSBOX_ONLY_N ? new BoxT<DNSBOX_ONLY_N, T, S>(c) : nullptr;
 #endif
     switch (c.n) {
-        case 16: return new BoxT<16, T>(c);
-        case 32: return new BoxT<32, T>(c);
-        case 48: return new BoxT<48, T>(c);
-        case 64: return new BoxT<64, T>(c);
-        case 128: return new BoxT<128, T>(c);
-        case 256: return new BoxT<256, T>(c);
-        case 512: return new BoxT<512, T>(c);
+        case 16: return new BoxT<16, T, S>(c);
+        case 32: return new BoxT<32, T, S>(c);
+        case 48: return new BoxT<48, T, S>(c);
+        case 64: return new BoxT<64, T, S>(c);
+        case 128: return new BoxT<128, T, S>(c);
+        case 256: return new BoxT<256, T, S>(c);
+        case 512: return new BoxT<512, T, S>(c);
         default: return nullptr;
     }
 }

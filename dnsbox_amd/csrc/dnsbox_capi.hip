@@ -89,26 +89,29 @@ void dev_to_ref_layout(const std::vector<double>& src, double* dst, int n)
 }
 
 // copy 3 host FP64 arrays (already in device layout, nper reals each) into dev (context precision)
-void upload3(dnsbox_ctx* c, void* dev, const std::vector<double>* h, long long nper, long long dstride_bytes)
+void upload3(dnsbox_ctx* c, void* dev, const std::vector<double>* h, long long nper, long long dstride_bytes,
+             bool state_type)
 {
     BoxBase* b = c->box.get();
     DevBuf tmp(size_t(nper) * sizeof(double));
     for (int f = 0; f < 3; ++f) {
         DNSBOX_HIP(hipMemcpyAsync(tmp.p, h[f].data(), size_t(nper) * sizeof(double), hipMemcpyHostToDevice,
                                   b->stream));
-        b->convert_in(static_cast<char*>(dev) + f * dstride_bytes, static_cast<const double*>(tmp.p), nper);
+        b->convert_in(static_cast<char*>(dev) + f * dstride_bytes, static_cast<const double*>(tmp.p), nper,
+                      state_type);
     }
     DNSBOX_HIP(hipStreamSynchronize(b->stream));
 }
 
 void download3(dnsbox_ctx* c, const void* dev, std::vector<double>* h, long long nper,
-               long long sstride_bytes)
+               long long sstride_bytes, bool state_type)
 {
     BoxBase* b = c->box.get();
     DevBuf tmp(size_t(nper) * sizeof(double));
     for (int f = 0; f < 3; ++f) {
         h[f].resize(nper);
-        b->convert_out(static_cast<double*>(tmp.p), static_cast<const char*>(dev) + f * sstride_bytes, nper);
+        b->convert_out(static_cast<double*>(tmp.p), static_cast<const char*>(dev) + f * sstride_bytes, nper,
+                       state_type);
         DNSBOX_HIP(hipMemcpyAsync(h[f].data(), tmp.p, size_t(nper) * sizeof(double), hipMemcpyDeviceToHost,
                                   b->stream));
         DNSBOX_HIP(hipStreamSynchronize(b->stream));
@@ -159,6 +162,10 @@ int dnsbox_create(const dnsbox_params* p, dnsbox_ctx** out)
                                       " (supported: 16, 32, 48, 64, 128, 256, 512)");
     if (p->precision != 32 && p->precision != 64)
         return fail(DNSBOX_E_ARG, "precision must be 32 or 64");
+    if (p->state_precision != 0 && p->state_precision != 32 && p->state_precision != 64)
+        return fail(DNSBOX_E_ARG, "state_precision must be 0 (= precision), 32 or 64");
+    if (p->state_precision == 32 && p->precision == 64)
+        return fail(DNSBOX_E_ARG, "an FP64 box keeps an FP64 state");
     if (p->kolmo_k < 0 || p->kolmo_k >= p->n / 2) return fail(DNSBOX_E_ARG, "kolmo_k out of range");
     if (p->bandlim && !(p->kcut > 0.0)) return fail(DNSBOX_E_ARG, "bandlim needs kcut > 0");
     if (!(p->dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
@@ -171,6 +178,7 @@ int dnsbox_create(const dnsbox_params* p, dnsbox_ctx** out)
         BoxConfig bc;
         bc.n = p->n;
         bc.precision = p->precision;
+        bc.state_precision = p->state_precision == 64 ? 64 : p->precision;
         bc.deltak = p->deltak > 0 ? p->deltak : 1.0;
         bc.dt = p->dt;
         dnsbox::BoxPhys& ph = bc.ph;
@@ -244,7 +252,7 @@ int dnsbox_set_state(dnsbox_ctx* c, const double* u, const double* v, const doub
         const double* src[3] = {u, v, w};
         for (int f = 0; f < 3; ++f) ref_to_dev_layout(src[f], h[f], n);
         BoxBase* b = c->box.get();
-        upload3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->cbytes);
+        upload3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->sbytes, true);
         b->ab2_started = false;
         return 0;
     });
@@ -257,7 +265,7 @@ int dnsbox_get_state(dnsbox_ctx* c, double* u, double* v, double* w)
     return guarded([&]() -> int {
         BoxBase* b = c->box.get();
         std::vector<double> h[3];
-        download3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->cbytes);
+        download3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->sbytes, true);
         double* dst[3] = {u, v, w};
         for (int f = 0; f < 3; ++f) dev_to_ref_layout(h[f], dst[f], c->p.n);
         return 0;
@@ -271,7 +279,7 @@ int dnsbox_get_nonlin(dnsbox_ctx* c, double* u, double* v, double* w)
     return guarded([&]() -> int {
         BoxBase* b = c->box.get();
         std::vector<double> h[3];
-        download3(c, b->nonlin, h, 2 * b->fs, b->fs * (long long)b->cbytes);
+        download3(c, b->nonlin, h, 2 * b->fs, b->fs * (long long)b->cbytes, false);
         double* dst[3] = {u, v, w};
         for (int f = 0; f < 3; ++f) dev_to_ref_layout(h[f], dst[f], c->p.n);
         return 0;
@@ -290,8 +298,8 @@ int dnsbox_set_state_physical(dnsbox_ctx* c, const double* u, const double* v, c
         const double* src[3] = {u, v, w};
         for (int f = 0; f < 3; ++f) h[f].assign(src[f], src[f] + n3);
         DevBuf phys(size_t(3 * n3) * b->rbytes);
-        upload3(c, phys.p, h, n3, n3 * (long long)b->rbytes);
-        b->from_physical(phys.p, b->state);
+        upload3(c, phys.p, h, n3, n3 * (long long)b->rbytes, false);
+        b->from_physical(phys.p);
         DNSBOX_HIP(hipStreamSynchronize(b->stream));
         b->ab2_started = false;
         return 0;
@@ -307,9 +315,9 @@ int dnsbox_get_state_physical(dnsbox_ctx* c, double* u, double* v, double* w)
         BoxBase* b = c->box.get();
         const long long n3 = (long long)c->p.n * c->p.n * c->p.n;
         DevBuf phys(size_t(3 * n3) * b->rbytes);
-        b->to_physical(b->state, phys.p);
+        b->to_physical(phys.p);
         std::vector<double> h[3];
-        download3(c, phys.p, h, n3, n3 * (long long)b->rbytes);
+        download3(c, phys.p, h, n3, n3 * (long long)b->rbytes, false);
         double* dst[3] = {u, v, w};
         for (int f = 0; f < 3; ++f) std::memcpy(dst[f], h[f].data(), size_t(n3) * sizeof(double));
         return 0;
@@ -359,7 +367,7 @@ int dnsbox_rhs_nonlinear(dnsbox_ctx* c)
     if (int e = check_ctx(c)) return e;
     c->box->x6_valid = false;
     return guarded([&]() -> int {
-        c->box->rhs(c->box->state, c->box->nonlin);
+        c->box->rhs_state();
         return 0;
     });
 }
@@ -418,8 +426,8 @@ int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
     if (!diag) return fail(DNSBOX_E_ARG, "null diag");
     return guarded([&]() -> int {
         BoxBase* b = c->box.get();
-        b->spec_reduce(b->state);
-        b->phys_reduce(b->state);
+        b->spec_reduce();
+        b->phys_reduce();
         std::vector<double> red(dnsbox::DG_NSCAL + b->nspec);
         unsigned long long bits[4];
         dnsbox::BoxScalars s{};

@@ -33,6 +33,11 @@ template <class T>
 __device__ __forceinline__ cpx<T> operator*(T s, cpx<T> a) { return {s * a.x, s * a.y}; }
 template <class T>
 __device__ __forceinline__ cpx<T> conj(cpx<T> a) { return {a.x, -a.y}; }
+template <class TO, class TI>
+__device__ __forceinline__ cpx<TO> cvt(cpx<TI> a)
+{
+    return cpx<TO>{TO(a.x), TO(a.y)};
+}
 // multiply by s*i, s = +-1
 template <int S, class T>
 __device__ __forceinline__ cpx<T> mul_i(cpx<T> a)

@@ -22,7 +22,7 @@ module dnsbox_gpu
         real(c_double) :: nu, Q, Pin, kcut, kolmo_amp, deltak
         real(c_double) :: cfl_min, cfl_max, dt_max, dt
         integer(c_int) :: n, precision, kolmo_k, bandlim, spherical, parity_mode
-        integer(c_int) :: adaptive_dt, reserved
+        integer(c_int) :: adaptive_dt, state_precision
     end type dnsbox_params
 
     interface

@@ -21,7 +21,8 @@
  *     uhat(Nz, Ny, Nx/2+1) complex(8) (p3dfft STRIDE1, z fastest), one array per
  *     velocity component.  Host physical arrays: Fortran u(Nx, Ny, Nz) real(8).
  *   - Device-pointer entry points (suffix _dev) use the internal layout:
- *     (3, Nz, Ny, Nx/2+1) complex in the context precision, kx fastest.
+ *     (3, Nz, Ny, Nx/2+1) complex in the context precision, kx fastest; they need
+ *     state_precision equal to precision.
  */
 #ifndef DNSBOX_H
 #define DNSBOX_H
@@ -73,7 +74,7 @@ typedef struct dnsbox_params {
     int parity_mode;    /* 1: reference k-table (Nyquist wavenumber 0, Nyquist modes kept by
                            the mask, variables.f90:229,245,259); 0: Nyquist modes removed */
     int adaptive_dt;    /* 1: CFL controller (tStepFix = .false., parameters.f90:37) */
-    int reserved;
+    int state_precision;/* 0 = precision; 64 with precision 32: FP64 state, FP32 transforms */
 } dnsbox_params;
 
 typedef struct dnsbox_ctx dnsbox_ctx;

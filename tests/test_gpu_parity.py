@@ -221,3 +221,21 @@ def test_fp32_heun_close_to_fp64(gpu_lib):
           f"spectrum max {rel.max():.2e}", flush=True)
     assert abs(d32[0] - d64[0]) / d64[0] < 1e-5
     assert rel[:20].max() < 1e-4
+
+
+def test_fp32_state64_closer_to_fp64(gpu_lib):
+    """FP32 transforms with the state kept in FP64 track the FP64 run more closely than an
+    FP32 state does."""
+    n = 64
+    kw = dict(nu=0.01, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=0.004)
+    res = {}
+    for name, prec, sp in (("f64", 64, 0), ("f32", 32, 0), ("f32s64", 32, 64)):
+        with make(n, precision=prec, state_precision=sp, **kw) as b:
+            b.init_random(2, 0, 4.0, 0.5)
+            b.step(200)
+            res[name] = b.get_state()
+    e32 = rel_l2(res["f32"], res["f64"])
+    e64s = rel_l2(res["f32s64"], res["f64"])
+    print(f"state error vs fp64 after 200 steps: fp32 state {e32:.2e}, fp64 state {e64s:.2e}", flush=True)
+    assert e64s < e32
+    assert e64s < 1e-5
