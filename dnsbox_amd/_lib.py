@@ -68,6 +68,12 @@ SIGNATURES = {
     "dnsbox_rhs_vjp_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "dnsbox_step_vjp_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
     "dnsbox_rhs_adj_dev": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "dnsbox_group_create": (C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(_vp)]),
+    "dnsbox_group_destroy": (C.c_int, [_vp]),
+    "dnsbox_group_size": (C.c_int, [_vp]),
+    "dnsbox_group_box": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "dnsbox_group_step": (C.c_int, [_vp, C.c_int]),
+    "dnsbox_group_sync": (C.c_int, [_vp]),
 }
 
 _lib = None

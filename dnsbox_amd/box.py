@@ -57,14 +57,27 @@ class Box:
         h = C.c_void_p()
         check(self.lib.dnsbox_create(C.byref(self.params), C.byref(h)))
         self._h = h
+        self._owned = True
         self.n = n
         self.precision = precision
 
+    @classmethod
+    def _view(cls, handle, params):
+        """A borrowed box of a group (destroyed with the group)."""
+        b = cls.__new__(cls)
+        b.lib = load()
+        b.params = params
+        b._h = handle
+        b._owned = False
+        b.n = params.n
+        b.precision = params.precision
+        return b
+
     # -- lifetime -------------------------------------------------------------------
     def close(self):
-        if getattr(self, "_h", None) and self._h.value:
+        if getattr(self, "_h", None) and self._h.value and getattr(self, "_owned", True):
             check(self.lib.dnsbox_destroy(self._h))
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -227,3 +240,54 @@ class Box:
         self._on_torch_stream()
         check(self.lib.dnsbox_rhs_adj_dev(self._h, self._check_tensor(u), self._check_tensor(out),
                                           lin_sign))
+
+
+class BoxGroup:
+    """Many independent boxes (N in {16, 32, 48, 64, 128}, one precision) stepped together:
+    one launch per pass for all boxes, the step captured in a hipGraph."""
+
+    def __init__(self, params):
+        """params: list of dicts of Box keyword arguments (n, precision, nu, ...)."""
+        self.lib = load()
+        arr = (Params * len(params))()
+        for i, kw in enumerate(params):
+            arr[i] = default_params(**kw)
+        h = C.c_void_p()
+        check(self.lib.dnsbox_group_create(arr, len(params), C.byref(h)))
+        self._h = h
+        self.boxes = []
+        for i in range(len(params)):
+            bh = C.c_void_p()
+            check(self.lib.dnsbox_group_box(h, i, C.byref(bh)))
+            self.boxes.append(Box._view(bh, arr[i]))
+
+    def __len__(self):
+        return len(self.boxes)
+
+    def __getitem__(self, i):
+        return self.boxes[i]
+
+    def step(self, nsteps: int = 1):
+        check(self.lib.dnsbox_group_step(self._h, nsteps))
+
+    def sync(self):
+        check(self.lib.dnsbox_group_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            for b in self.boxes:
+                b._h = C.c_void_p()
+            check(self.lib.dnsbox_group_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -127,7 +127,7 @@ template <> struct Cfg<256, float> : CfgBase<16, 64, 4, 256,  Plan<8, 4, 8>,    
 template <> struct Cfg<512, float> : CfgBase<8, 16, 2, 256,   Plan<8, 8, 8>,    Plan<16, 16, 2>,  Plan<16, 16>> {};
 template <> struct Cfg<16, double>  : CfgBase<64, 64, 32, 256, Plan<4, 4>,      Plan<4, 4>,       Plan<8>>       {};
 template <> struct Cfg<32, double>  : CfgBase<32, 64, 16, 256, Plan<4, 2, 4>,   Plan<8, 4>,       Plan<4, 4>>    {};
-template <> struct Cfg<48, double>  : CfgBase<64, 32, 8, 256,  Plan<2, 12, 2>,  Plan<12, 4>,      Plan<8, 3>>    {};
+template <> struct Cfg<48, double>  : CfgBase<64, 64, 8, 256,  Plan<2, 12, 2>,  Plan<12, 4>,      Plan<8, 3>>    {};
 template <> struct Cfg<64, double>  : CfgBase<16, 32, 8, 256,  Plan<4, 4, 4>,   Plan<8, 8>,       Plan<8, 4>>    {};
 template <> struct Cfg<128, double> : CfgBase<16, 16, 4, 256,  Plan<4, 8, 4>,   Plan<8, 4, 4>,    Plan<8, 8>>    {};
 #ifdef DNSBOX_CFG256D
@@ -751,15 +751,11 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
 // input tile (fields 0..2 of X) and the output tile (fields 0..5 of X) cover the
 // same columns, so the block reads its whole tile before it writes.
 template <int N, class T, int MODE, class S = T>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
-    k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw_g, EpiArgs<T, S> ea,
-                ProCurl<T> pro)
+__device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long fs, int tile, cpx<T>* lds,
+                                               const cpx<T>* tw, const EpiArgs<T, S>& ea, const ProCurl<T>& pro)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::ZW];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    const int col = blockIdx.x * G::C::ZW + threadIdx.x % G::C::ZW;
+    const int col = tile * G::C::ZW + threadIdx.x % G::C::ZW;
     const bool valid = col < G::ZCOLS;
     cpx<T> O[3][G::ZRM];
 #if DNSBOX_Z_PREFETCH
@@ -787,8 +783,20 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     for (int f = 0; f < 6; ++f) out[f] = X + f * fs;
     zinv_from_regs<N, T, ProCurl<T>>(O, out, col, valid, lds, tw, pro);
     if constexpr (MODE == EPI_PRED) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
+        if (tile == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
     }
+}
+
+template <int N, class T, int MODE, class S = T>
+__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+    k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw_g, EpiArgs<T, S> ea,
+                ProCurl<T> pro)
+{
+    using G = Geo<N, T>;
+    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    zfwd_zinv_tile<N, T, MODE, S>(X, fs, blockIdx.x, lds, tw, ea, pro);
 }
 
 // end of step bookkeeping: t += dt, dt_prev = dt, clear the CFL maxima

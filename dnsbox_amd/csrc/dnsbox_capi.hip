@@ -11,14 +11,28 @@
 
 #include "../../include/dnsbox.h"
 #include "box_impl.hpp"
+#include "group.hpp"
 
 using dnsbox::BoxBase;
 using dnsbox::BoxConfig;
 
+namespace dnsbox {
+GroupBase* make_group(std::vector<std::unique_ptr<BoxBase>>&& boxes, int precision);
+}
+
+struct dnsbox_group;
+
 struct dnsbox_ctx {
     dnsbox_params p;
-    std::unique_ptr<BoxBase> box;
+    BoxBase* box = nullptr;
+    bool owned = true;              // false: a view of one box of a group
+    dnsbox_group* group = nullptr;  // the group the box belongs to
     int device = 0;
+};
+
+struct dnsbox_group {
+    std::unique_ptr<dnsbox::GroupBase> g;
+    std::vector<dnsbox_ctx> views;
 };
 
 namespace {
@@ -92,7 +106,7 @@ void dev_to_ref_layout(const std::vector<double>& src, double* dst, int n)
 void upload3(dnsbox_ctx* c, void* dev, const std::vector<double>* h, long long nper, long long dstride_bytes,
              bool state_type)
 {
-    BoxBase* b = c->box.get();
+    BoxBase* b = c->box;
     DevBuf tmp(size_t(nper) * sizeof(double));
     for (int f = 0; f < 3; ++f) {
         DNSBOX_HIP(hipMemcpyAsync(tmp.p, h[f].data(), size_t(nper) * sizeof(double), hipMemcpyHostToDevice,
@@ -106,7 +120,7 @@ void upload3(dnsbox_ctx* c, void* dev, const std::vector<double>* h, long long n
 void download3(dnsbox_ctx* c, const void* dev, std::vector<double>* h, long long nper,
                long long sstride_bytes, bool state_type)
 {
-    BoxBase* b = c->box.get();
+    BoxBase* b = c->box;
     DevBuf tmp(size_t(nper) * sizeof(double));
     for (int f = 0; f < 3; ++f) {
         h[f].resize(nper);
@@ -121,6 +135,60 @@ void download3(dnsbox_ctx* c, const void* dev, std::vector<double>* h, long long
 int check_ctx(dnsbox_ctx* c)
 {
     if (!c || !c->box) return fail(DNSBOX_E_ARG, "null context");
+    return 0;
+}
+
+// the box's step pipeline (and its group's) no longer matches its state
+void invalidate(dnsbox_ctx* c)
+{
+    c->box->x6_valid = false;
+    if (c->group) c->group->g->invalidate();
+}
+
+dnsbox::BoxConfig make_config(const dnsbox_params* p)
+{
+    BoxConfig bc;
+    bc.n = p->n;
+    bc.precision = p->precision;
+    bc.state_precision = p->state_precision == 64 ? 64 : p->precision;
+    bc.deltak = p->deltak > 0 ? p->deltak : 1.0;
+    bc.dt = p->dt;
+    dnsbox::BoxPhys& ph = bc.ph;
+    ph.nu = p->nu;
+    ph.Q = p->Q;
+    ph.Pin = p->Pin;
+    ph.kcut = p->kcut;
+    ph.kolmo_amp = p->kolmo_amp;
+    ph.mask_thr = (double(p->n) / 2.0) * (2.0 * 1.0 / 3.0);
+    const double ts = (double(p->n) / 2.0) * 0.5 * 1.0;
+    ph.sph_thr2 = ts * ts;
+    ph.cmin = p->cfl_min;
+    ph.cmax = p->cfl_max;
+    ph.dtmax = p->dt_max;
+    ph.kolmo_k = p->kolmo_amp != 0.0 ? p->kolmo_k : 0;
+    ph.bandlim = p->bandlim;
+    ph.spherical = p->spherical;
+    ph.parity = p->parity_mode;
+    ph.adaptive = p->adaptive_dt;
+    return bc;
+}
+
+int check_params(const dnsbox_params* p)
+{
+    if (!supported_n(p->n))
+        return fail(DNSBOX_E_ARG, "unsupported n=" + std::to_string(p->n) +
+                                      " (supported: 16, 32, 48, 64, 128, 256, 512)");
+    if (p->precision != 32 && p->precision != 64)
+        return fail(DNSBOX_E_ARG, "precision must be 32 or 64");
+    if (p->state_precision != 0 && p->state_precision != 32 && p->state_precision != 64)
+        return fail(DNSBOX_E_ARG, "state_precision must be 0 (= precision), 32 or 64");
+    if (p->state_precision == 32 && p->precision == 64)
+        return fail(DNSBOX_E_ARG, "an FP64 box keeps an FP64 state");
+    if (p->kolmo_k < 0 || p->kolmo_k >= p->n / 2) return fail(DNSBOX_E_ARG, "kolmo_k out of range");
+    if (p->bandlim && !(p->kcut > 0.0)) return fail(DNSBOX_E_ARG, "bandlim needs kcut > 0");
+    if (!(p->dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+    if (p->adaptive_dt && !(p->cfl_max >= p->cfl_min && p->cfl_min > 0.0 && p->dt_max > 0.0))
+        return fail(DNSBOX_E_ARG, "adaptive dt needs 0 < cfl_min <= cfl_max and dt_max > 0");
     return 0;
 }
 
@@ -157,51 +225,16 @@ int dnsbox_create(const dnsbox_params* p, dnsbox_ctx** out)
 {
     if (!p || !out) return fail(DNSBOX_E_ARG, "null argument");
     *out = nullptr;
-    if (!supported_n(p->n))
-        return fail(DNSBOX_E_ARG, "unsupported n=" + std::to_string(p->n) +
-                                      " (supported: 16, 32, 48, 64, 128, 256, 512)");
-    if (p->precision != 32 && p->precision != 64)
-        return fail(DNSBOX_E_ARG, "precision must be 32 or 64");
-    if (p->state_precision != 0 && p->state_precision != 32 && p->state_precision != 64)
-        return fail(DNSBOX_E_ARG, "state_precision must be 0 (= precision), 32 or 64");
-    if (p->state_precision == 32 && p->precision == 64)
-        return fail(DNSBOX_E_ARG, "an FP64 box keeps an FP64 state");
-    if (p->kolmo_k < 0 || p->kolmo_k >= p->n / 2) return fail(DNSBOX_E_ARG, "kolmo_k out of range");
-    if (p->bandlim && !(p->kcut > 0.0)) return fail(DNSBOX_E_ARG, "bandlim needs kcut > 0");
-    if (!(p->dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
-    if (p->adaptive_dt && !(p->cfl_max >= p->cfl_min && p->cfl_min > 0.0 && p->dt_max > 0.0))
-        return fail(DNSBOX_E_ARG, "adaptive dt needs 0 < cfl_min <= cfl_max and dt_max > 0");
+    if (int e = check_params(p)) return e;
     return guarded([&]() -> int {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
             return fail(DNSBOX_E_HIP, "no HIP device available");
-        BoxConfig bc;
-        bc.n = p->n;
-        bc.precision = p->precision;
-        bc.state_precision = p->state_precision == 64 ? 64 : p->precision;
-        bc.deltak = p->deltak > 0 ? p->deltak : 1.0;
-        bc.dt = p->dt;
-        dnsbox::BoxPhys& ph = bc.ph;
-        ph.nu = p->nu;
-        ph.Q = p->Q;
-        ph.Pin = p->Pin;
-        ph.kcut = p->kcut;
-        ph.kolmo_amp = p->kolmo_amp;
-        ph.mask_thr = (double(p->n) / 2.0) * (2.0 * 1.0 / 3.0);
-        const double ts = (double(p->n) / 2.0) * 0.5 * 1.0;
-        ph.sph_thr2 = ts * ts;
-        ph.cmin = p->cfl_min;
-        ph.cmax = p->cfl_max;
-        ph.dtmax = p->dt_max;
-        ph.kolmo_k = p->kolmo_amp != 0.0 ? p->kolmo_k : 0;
-        ph.bandlim = p->bandlim;
-        ph.spherical = p->spherical;
-        ph.parity = p->parity_mode;
-        ph.adaptive = p->adaptive_dt;
+        BoxConfig bc = make_config(p);
         auto c = std::make_unique<dnsbox_ctx>();
         c->p = *p;
         (void)hipGetDevice(&c->device);
-        c->box.reset(p->precision == 64 ? dnsbox::make_box_f64(bc) : dnsbox::make_box_f32(bc));
+        c->box = p->precision == 64 ? dnsbox::make_box_f64(bc) : dnsbox::make_box_f32(bc);
         if (!c->box) return fail(DNSBOX_E_ARG, "unsupported configuration");
         *out = c.release();
         return 0;
@@ -211,9 +244,11 @@ int dnsbox_create(const dnsbox_params* p, dnsbox_ctx** out)
 int dnsbox_destroy(dnsbox_ctx* c)
 {
     if (!c) return 0;
+    if (!c->owned) return fail(DNSBOX_E_STATE, "a box of a group is destroyed with its group");
     return guarded([&]() -> int {
         if (c->box && c->box->stream) (void)hipStreamSynchronize(c->box->stream);
         else (void)hipDeviceSynchronize();
+        delete c->box;
         delete c;
         return 0;
     });
@@ -244,14 +279,14 @@ int dnsbox_nspec(dnsbox_ctx* c)
 int dnsbox_set_state(dnsbox_ctx* c, const double* u, const double* v, const double* w)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null state array");
     return guarded([&]() -> int {
         const int n = c->p.n;
         std::vector<double> h[3];
         const double* src[3] = {u, v, w};
         for (int f = 0; f < 3; ++f) ref_to_dev_layout(src[f], h[f], n);
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         upload3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->sbytes, true);
         b->ab2_started = false;
         return 0;
@@ -263,7 +298,7 @@ int dnsbox_get_state(dnsbox_ctx* c, double* u, double* v, double* w)
     if (int e = check_ctx(c)) return e;
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null state array");
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         std::vector<double> h[3];
         download3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->sbytes, true);
         double* dst[3] = {u, v, w};
@@ -277,7 +312,7 @@ int dnsbox_get_nonlin(dnsbox_ctx* c, double* u, double* v, double* w)
     if (int e = check_ctx(c)) return e;
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         std::vector<double> h[3];
         download3(c, b->nonlin, h, 2 * b->fs, b->fs * (long long)b->cbytes, false);
         double* dst[3] = {u, v, w};
@@ -289,10 +324,10 @@ int dnsbox_get_nonlin(dnsbox_ctx* c, double* u, double* v, double* w)
 int dnsbox_set_state_physical(dnsbox_ctx* c, const double* u, const double* v, const double* w)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         const long long n3 = (long long)c->p.n * c->p.n * c->p.n;
         std::vector<double> h[3];
         const double* src[3] = {u, v, w};
@@ -309,10 +344,10 @@ int dnsbox_set_state_physical(dnsbox_ctx* c, const double* u, const double* v, c
 int dnsbox_get_state_physical(dnsbox_ctx* c, double* u, double* v, double* w)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         const long long n3 = (long long)c->p.n * c->p.n * c->p.n;
         DevBuf phys(size_t(3 * n3) * b->rbytes);
         b->to_physical(phys.p);
@@ -327,7 +362,7 @@ int dnsbox_get_state_physical(dnsbox_ctx* c, double* u, double* v, double* w)
 int dnsbox_init_random(dnsbox_ctx* c, uint64_t seed, int box, double k0, double energy)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!(k0 > 0.0) || !(energy >= 0.0)) return fail(DNSBOX_E_ARG, "k0 must be > 0, energy >= 0");
     return guarded([&]() -> int {
         c->box->init_random(seed, box, k0, energy);
@@ -339,7 +374,7 @@ int dnsbox_init_random(dnsbox_ctx* c, uint64_t seed, int box, double k0, double 
 int dnsbox_project(dnsbox_ctx* c, int dealias, int project)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     return guarded([&]() -> int {
         c->box->project(dealias, project);
         return 0;
@@ -351,7 +386,7 @@ int dnsbox_int_fact(dnsbox_ctx* c, double dt)
     if (int e = check_ctx(c)) return e;
     if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         DNSBOX_HIP(hipStreamSynchronize(b->stream));
         dnsbox::BoxScalars s{};
         DNSBOX_HIP(hipMemcpy(&s, b->sc, sizeof(s), hipMemcpyDeviceToHost));
@@ -365,7 +400,7 @@ int dnsbox_int_fact(dnsbox_ctx* c, double dt)
 int dnsbox_rhs_nonlinear(dnsbox_ctx* c)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     return guarded([&]() -> int {
         c->box->rhs_state();
         return 0;
@@ -379,8 +414,9 @@ int dnsbox_step(dnsbox_ctx* c, int nsteps, int stepper)
     if (stepper != 1 && stepper != 2) return fail(DNSBOX_E_ARG, "stepper must be 1 (IF-Heun) or 2 (IF-AB2)");
     if (stepper == 2 && c->p.adaptive_dt)
         return fail(DNSBOX_E_STATE, "IF-AB2 runs at constant dt; create the box with adaptive_dt = 0");
+    if (c->group) c->group->g->invalidate();
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         for (int i = 0; i < nsteps; ++i) {
             if (stepper == 1) b->heun_step();
             else b->ab2_step();
@@ -393,7 +429,7 @@ int dnsbox_get_time(dnsbox_ctx* c, double* t, double* dt)
 {
     if (int e = check_ctx(c)) return e;
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         DNSBOX_HIP(hipStreamSynchronize(b->stream));
         dnsbox::BoxScalars s{};
         DNSBOX_HIP(hipMemcpy(&s, b->sc, sizeof(s), hipMemcpyDeviceToHost));
@@ -408,7 +444,7 @@ int dnsbox_set_time(dnsbox_ctx* c, double t, double dt)
     if (int e = check_ctx(c)) return e;
     if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         DNSBOX_HIP(hipStreamSynchronize(b->stream));
         dnsbox::BoxScalars s{};
         DNSBOX_HIP(hipMemcpy(&s, b->sc, sizeof(s), hipMemcpyDeviceToHost));
@@ -422,10 +458,10 @@ int dnsbox_set_time(dnsbox_ctx* c, double t, double dt)
 int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!diag) return fail(DNSBOX_E_ARG, "null diag");
     return guarded([&]() -> int {
-        BoxBase* b = c->box.get();
+        BoxBase* b = c->box;
         b->spec_reduce();
         b->phys_reduce();
         std::vector<double> red(dnsbox::DG_NSCAL + b->nspec);
@@ -464,7 +500,7 @@ int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
 int dnsbox_time_passes(dnsbox_ctx* c, int reps, double* ms)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (reps < 1 || !ms) return fail(DNSBOX_E_ARG, "reps >= 1 and an output array are required");
     return guarded([&]() -> int {
         c->box->time_passes(reps, ms);
@@ -483,7 +519,7 @@ int dnsbox_state_ptr(dnsbox_ctx* c, void** p)
 int dnsbox_rhs_dev(dnsbox_ctx* c, const void* in, void* out)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!in || !out) return fail(DNSBOX_E_ARG, "null device pointer");
     return guarded([&]() -> int {
         c->box->rhs(in, out);
@@ -494,7 +530,7 @@ int dnsbox_rhs_dev(dnsbox_ctx* c, const void* in, void* out)
 int dnsbox_step_dev(dnsbox_ctx* c, void* state, int nsteps)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!state || nsteps < 0) return fail(DNSBOX_E_ARG, "null state or nsteps < 0");
     return guarded([&]() -> int {
         c->box->step_ext(state, nsteps);
@@ -505,7 +541,7 @@ int dnsbox_step_dev(dnsbox_ctx* c, void* state, int nsteps)
 int dnsbox_rhs_vjp_dev(dnsbox_ctx* c, const void* u, const void* lam, void* out)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!u || !lam || !out) return fail(DNSBOX_E_ARG, "null device pointer");
     return guarded([&]() -> int {
         c->box->rhs_vjp(u, lam, out);
@@ -516,7 +552,7 @@ int dnsbox_rhs_vjp_dev(dnsbox_ctx* c, const void* u, const void* lam, void* out)
 int dnsbox_step_vjp_dev(dnsbox_ctx* c, const void* u, const void* lam, void* lam_out, double dt)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!u || !lam || !lam_out) return fail(DNSBOX_E_ARG, "null device pointer");
     if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
     return guarded([&]() -> int {
@@ -528,11 +564,93 @@ int dnsbox_step_vjp_dev(dnsbox_ctx* c, const void* u, const void* lam, void* lam
 int dnsbox_rhs_adj_dev(dnsbox_ctx* c, const void* u, void* out, int lin_sign)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->x6_valid = false;
+    invalidate(c);
     if (!u || !out) return fail(DNSBOX_E_ARG, "null device pointer");
     if (lin_sign != 1 && lin_sign != -1) return fail(DNSBOX_E_ARG, "lin_sign must be +1 or -1");
     return guarded([&]() -> int {
         c->box->rhs_adj(u, out, double(lin_sign));
+        return 0;
+    });
+}
+
+int dnsbox_group_create(const dnsbox_params* params, int nbox, dnsbox_group** out)
+{
+    if (!params || !out || nbox < 1) return fail(DNSBOX_E_ARG, "null argument or nbox < 1");
+    *out = nullptr;
+    const int prec = params[0].precision;
+    for (int i = 0; i < nbox; ++i) {
+        const dnsbox_params* p = params + i;
+        if (int e = check_params(p)) return e;
+        if (!(p->n == 16 || p->n == 32 || p->n == 48 || p->n == 64 || p->n == 128))
+            return fail(DNSBOX_E_ARG, "group boxes need n in {16, 32, 48, 64, 128}");
+        if (p->precision != prec || (p->state_precision != 0 && p->state_precision != prec))
+            return fail(DNSBOX_E_ARG, "group boxes share one precision (state included)");
+        if (p->bandlim) return fail(DNSBOX_E_ARG, "group boxes do not support band-limited forcing");
+    }
+    return guarded([&]() -> int {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            return fail(DNSBOX_E_HIP, "no HIP device available");
+        std::vector<std::unique_ptr<BoxBase>> boxes;
+        for (int i = 0; i < nbox; ++i) {
+            BoxConfig bc = make_config(params + i);
+            boxes.emplace_back(prec == 64 ? dnsbox::make_box_f64(bc) : dnsbox::make_box_f32(bc));
+        }
+        auto grp = std::make_unique<dnsbox_group>();
+        grp->g.reset(dnsbox::make_group(std::move(boxes), prec));
+        grp->views.resize(nbox);
+        for (int i = 0; i < nbox; ++i) {
+            dnsbox_ctx& v = grp->views[i];
+            v.p = params[i];
+            v.box = grp->g->boxes[i].get();
+            v.owned = false;
+            v.group = grp.get();
+            (void)hipGetDevice(&v.device);
+        }
+        *out = grp.release();
+        return 0;
+    });
+}
+
+int dnsbox_group_destroy(dnsbox_group* g)
+{
+    if (!g) return 0;
+    return guarded([&]() -> int {
+        (void)hipDeviceSynchronize();
+        delete g;
+        return 0;
+    });
+}
+
+int dnsbox_group_box(dnsbox_group* g, int i, dnsbox_ctx** box)
+{
+    if (!g || !box || i < 0 || i >= int(g->views.size())) return fail(DNSBOX_E_ARG, "bad group or index");
+    *box = &g->views[i];
+    return 0;
+}
+
+int dnsbox_group_size(dnsbox_group* g)
+{
+    if (!g) return fail(DNSBOX_E_ARG, "null group");
+    return int(g->views.size());
+}
+
+int dnsbox_group_step(dnsbox_group* g, int nsteps)
+{
+    if (!g) return fail(DNSBOX_E_ARG, "null group");
+    if (nsteps < 0) return fail(DNSBOX_E_ARG, "nsteps < 0");
+    return guarded([&]() -> int {
+        g->g->step(nsteps);
+        for (auto& b : g->g->boxes) b->x6_valid = false;
+        return 0;
+    });
+}
+
+int dnsbox_group_sync(dnsbox_group* g)
+{
+    if (!g) return fail(DNSBOX_E_ARG, "null group");
+    return guarded([&]() -> int {
+        DNSBOX_HIP(hipDeviceSynchronize());
         return 0;
     });
 }

@@ -158,6 +158,20 @@ int dnsbox_step_vjp_dev(dnsbox_ctx* ctx, const void* u, const void* lam, void* l
    direction -DF^+(u~) = -L u~ - P n_adj of J = 1/2 ||F(u)||^2_{H^-1}. */
 int dnsbox_rhs_adj_dev(dnsbox_ctx* ctx, const void* u, void* out, int lin_sign);
 
+/* Groups: many independent boxes (N in {16, 32, 48, 64, 128}, one precision, each with its
+   own parameters) advanced together, one launch per pass for all of them, the step captured
+   in a hipGraph.  Batched counterpart of dnsbox_step for adjoint-descent guesses.  No
+   reference counterpart (the reference runs one box per MPI job). */
+typedef struct dnsbox_group dnsbox_group;
+int dnsbox_group_create(const dnsbox_params* params, int nbox, dnsbox_group** out);
+int dnsbox_group_destroy(dnsbox_group* g);
+int dnsbox_group_size(dnsbox_group* g);
+/* borrowed context of box i (set/get state, init, diagnostics, ...; not destroyable) */
+int dnsbox_group_box(dnsbox_group* g, int i, dnsbox_ctx** box);
+/* nsteps IF-Heun steps of every box of the group (dnsboxheun.f90:84-157 per box) */
+int dnsbox_group_step(dnsbox_group* g, int nsteps);
+int dnsbox_group_sync(dnsbox_group* g);
+
 #ifdef __cplusplus
 }
 #endif
