@@ -68,6 +68,120 @@ class _StepMap(torch.autograd.Function):
         return lam, None, None, None, None
 
 
+def _copy_all(dst, src):
+    torch._foreach_copy_(dst, src)
+
+
+class _GroupStepMap(torch.autograd.Function):
+    """Phi_T of every box of a BoxGroup at once (one launch per pass for all boxes)."""
+
+    @staticmethod
+    def forward(ctx, group, T, dts, k, *u0s):
+        for b, dt in zip(group.boxes, dts):
+            b.rhs_int_fact(dt)
+        us = [u.detach().clone().contiguous() for u in u0s]
+        ckpts = []
+        for t in range(T):
+            if t % k == 0:
+                c = [torch.empty_like(u) for u in us]
+                _copy_all(c, us)
+                ckpts.append(c)
+            group.step_dev(us, 1)
+        ctx.group, ctx.T, ctx.dts, ctx.k, ctx.ckpts = group, T, list(dts), k, ckpts
+        return tuple(us)
+
+    @staticmethod
+    def backward(ctx, *gs):
+        group, T, dts, k = ctx.group, ctx.T, ctx.dts, ctx.k
+        for b, dt in zip(group.boxes, dts):
+            b.rhs_int_fact(dt)
+        lams = [torch.zeros_like(c) if g is None else g.detach().clone().contiguous()
+                for g, c in zip(gs, ctx.ckpts[0])]
+        outs = [torch.empty_like(l) for l in lams]
+        seg_states = None
+        for seg in reversed(range(len(ctx.ckpts))):
+            start = seg * k
+            stop = min(start + k, T)
+            if seg_states is None:
+                seg_states = [[torch.empty_like(l) for l in lams] for _ in range(k)]
+            states = seg_states
+            _copy_all(states[0], ctx.ckpts[seg])
+            for j in range(1, stop - start):
+                _copy_all(states[j], states[j - 1])
+                group.step_dev(states[j], 1)
+            for j in reversed(range(stop - start)):
+                group.step_vjp_dev(states[j], lams, outs, dts)
+                lams, outs = outs, lams
+        return (None, None, None, None, *lams)
+
+
+def group_step_map(group, u0s, T: int, dts, k: int = 20):
+    """[Phi_T(u0_i)] for every box i of a BoxGroup, differentiable, checkpointed every k."""
+    return _GroupStepMap.apply(group, T, list(dts), k, *u0s)
+
+
+def group_residuals(group, u0s, T, dts, k=20):
+    """[J_i] = [1/2 <|Phi_T(u0_i) - u0_i|^2>] for every box (FP64 scalars, one tensor)."""
+    uTs = group_step_map(group, u0s, T, dts, k)
+    return torch.stack([0.5 * mean_square(uT - u0, b.n)
+                        for uT, u0, b in zip(uTs, u0s, group.boxes)])
+
+
+def group_residual_grads(group, u0s, T, dts, k=20):
+    """(J, [dJ_i/du0_i]): the residuals and their Euclidean gradients, all boxes at once."""
+    xs = [u.detach().clone().requires_grad_(True) for u in u0s]
+    J = group_residuals(group, xs, T, dts, k)
+    gs = torch.autograd.grad(J.sum(), xs)
+    return J.detach(), list(gs)
+
+
+def group_adjoint_descent(group, u0s, T, dts, k=20, iters=10, step0=1.0, c1=1e-4, shrink=0.5,
+                          max_backtrack=20, callback=None):
+    """adjoint_descent for every box of a group at once: one batched gradient per iteration,
+    an FP64 Armijo backtracking line search per box (each box keeps its own step length;
+    trial residuals of all boxes are evaluated in one batched forward).  Returns (us, hist)
+    with hist[it] the list of residuals of the boxes."""
+    ws = [hermitian_weight(b.n, u.device) for b, u in zip(group.boxes, u0s)]
+    us = [u.detach().clone() for u in u0s]
+    nb = len(us)
+    alpha = [step0] * nb
+    hist = []
+    for it in range(iters):
+        J, gs = group_residual_grads(group, us, T, dts, k)
+        J0 = J.tolist()
+        hist.append(J0)
+        if callback:
+            callback(it, J0)
+        ds, slopes = [], []
+        for b, g, w in zip(group.boxes, gs, ws):
+            gm = g.to(torch.complex128) / w * float(b.n) ** 6
+            d = -project_masked(gm, b.n).to(g.dtype)
+            ds.append(d)
+            slopes.append(float((g.to(torch.complex128).conj() * d.to(torch.complex128)).real.sum()))
+        pending = list(range(nb))
+        trial = list(us)
+        for _ in range(max_backtrack):
+            if not pending:
+                break
+            for i in pending:
+                trial[i] = us[i] + alpha[i] * ds[i]
+            with torch.no_grad():
+                Jt = group_residuals(group, trial, T, dts, k).tolist()
+            still = []
+            for i in pending:
+                if Jt[i] <= J0[i] + c1 * alpha[i] * slopes[i]:
+                    us[i] = trial[i]
+                    alpha[i] *= 2.0
+                else:
+                    alpha[i] *= shrink
+                    trial[i] = us[i]
+                    still.append(i)
+            pending = still
+    with torch.no_grad():
+        hist.append(group_residuals(group, us, T, dts, k).tolist())
+    return us, hist
+
+
 def step_map(box, u0, T: int, dt: float, k: int = 20):
     """Phi_T(u0): T fixed-dt IF-Heun steps, differentiable, checkpointed every k steps."""
     return _StepMap.apply(u0, box, T, dt, k)

@@ -204,6 +204,18 @@ class Box:
                              f"{t.dtype} {tuple(t.shape)} on {t.device}")
         return C.c_void_p(t.data_ptr())
 
+    def state_tensor(self):
+        """A device-tensor copy of the state, internal layout (3, N, N, N/2+1)."""
+        import torch
+        f = np.stack([np.transpose(a, (2, 1, 0)) for a in self.get_state()])
+        dt = torch.complex128 if self.precision == 64 else torch.complex64
+        return torch.from_numpy(np.ascontiguousarray(f)).to(dt).cuda().contiguous()
+
+    def set_state_tensor(self, t):
+        """Set the state from a device tensor of the internal layout."""
+        a = t.detach().cpu().numpy().astype(np.complex128)
+        self.set_state(*[np.transpose(a[i], (2, 1, 0)) for i in range(3)])
+
     def empty_state(self):
         import torch
         dt = torch.complex128 if self.precision == 64 else torch.complex64
@@ -268,10 +280,43 @@ class BoxGroup:
         return self.boxes[i]
 
     def step(self, nsteps: int = 1):
+        """nsteps IF-Heun steps of every box's own state (hipGraph replay)."""
+        self._on_torch_stream()
         check(self.lib.dnsbox_group_step(self._h, nsteps))
 
     def sync(self):
         check(self.lib.dnsbox_group_sync(self._h))
+
+    def set_stream(self, stream_ptr: int):
+        check(self.lib.dnsbox_group_set_stream(self._h, C.c_void_p(stream_ptr)))
+
+    def _on_torch_stream(self):
+        import torch
+        if torch.cuda.is_available():
+            self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def empty_states(self):
+        return [b.empty_state() for b in self.boxes]
+
+    def _ptrs(self, tensors):
+        if len(tensors) != len(self.boxes):
+            raise ValueError(f"expected {len(self.boxes)} tensors, got {len(tensors)}")
+        arr = (C.c_void_p * len(tensors))()
+        for i, (b, t) in enumerate(zip(self.boxes, tensors)):
+            arr[i] = b._check_tensor(t).value
+        return arr
+
+    def step_dev(self, states, nsteps: int = 1):
+        """IF-Heun steps in place on one device tensor per box (each box's own dt)."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_group_step_dev(self._h, self._ptrs(states), nsteps))
+
+    def step_vjp_dev(self, us, lams, outs, dts):
+        """outs[i] <- (d u_{n+1}/d u_n)^T lams[i] for one fixed-dt step of box i from us[i]."""
+        self._on_torch_stream()
+        d = (C.c_double * len(self.boxes))(*[float(x) for x in dts])
+        check(self.lib.dnsbox_group_step_vjp_dev(self._h, self._ptrs(us), self._ptrs(lams),
+                                                 self._ptrs(outs), d))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -184,17 +184,14 @@ struct VjpArgs {
 };
 
 template <int N, class T, int MODE>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
-    k_zfwd_vjp(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, VjpArgs<T> va)
+__device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long long xfs, int tile, cpx<T>* lds,
+                                              const cpx<T>* tw, const VjpArgs<T>& va)
 {
     using G = Geo<N, T>;
     using CF = typename G::CFZ_F;
     constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
     constexpr long long ZS = (long long)N * H;
-    __shared__ cpx<T> lds[N * W];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    const int col = blockIdx.x * W + threadIdx.x % W;
+    const int col = tile * W + threadIdx.x % W;
     const int g = threadIdx.x / W;
     const bool valid = col < G::ZCOLS;
     cpx<T> O[6][RM];
@@ -252,6 +249,17 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
             for (int f = 0; f < 3; ++f) va.out[f * va.fs + idx] = va.A[f * va.fs + idx] + gg[f];
         }
     }
+}
+
+template <int N, class T, int MODE>
+__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+    k_zfwd_vjp(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, VjpArgs<T> va)
+{
+    using G = Geo<N, T>;
+    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> stw_[N];
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    zfwd_vjp_tile<N, T, MODE>(X, xfs, blockIdx.x, lds, tw, va);
 }
 
 // [Eband(u), lam_scale * sum_{|k| <= kcut, unmasked} Re(conj(lamN) u)]  (one block)

@@ -728,6 +728,21 @@ __device__ __forceinline__ void pred_scalars(const BoxScalars* /*unused*/, BoxSc
 }
 
 template <int N, class T, int MODE, class S = T>
+__device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long long in_fs, int tile, cpx<T>* lds,
+                                          const cpx<T>* tw, const EpiArgs<T, S>& ea)
+{
+    using G = Geo<N, T>;
+    const int col = tile * G::C::ZW + threadIdx.x % G::C::ZW;
+    const bool valid = col < G::ZCOLS;
+    cpx<T> O[3][G::ZRM];
+    zfwd_regs<N, T>(in0, in_fs, col, valid, lds, tw, O);
+    zfwd_epilogue<N, T, MODE, false, false, S>(O, col, valid, ea, O);
+    if constexpr (MODE == EPI_PRED) {
+        if (tile == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
+    }
+}
+
+template <int N, class T, int MODE, class S = T>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zfwd(const cpx<T>* __restrict__ in0, long long in_fs, const cpx<T>* __restrict__ tw_g,
            EpiArgs<T, S> ea)
@@ -736,14 +751,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     __shared__ cpx<T> lds[N * G::C::ZW];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    const int col = blockIdx.x * G::C::ZW + threadIdx.x % G::C::ZW;
-    const bool valid = col < G::ZCOLS;
-    cpx<T> O[3][G::ZRM];
-    zfwd_regs<N, T>(in0, in_fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, false, false, S>(O, col, valid, ea, O);
-    if constexpr (MODE == EPI_PRED) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
-    }
+    zfwd_tile<N, T, MODE, S>(in0, in_fs, blockIdx.x, lds, tw, ea);
 }
 
 // Forward z pass + stepper epilogue (PRED or CORR) fused with the z-inverse + curl

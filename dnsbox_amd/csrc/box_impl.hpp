@@ -93,6 +93,8 @@ class BoxBase {
     // FP64 device array -> transform type / state type, and back
     virtual void convert_in(void* dst, const double* src, long long n, bool state_type) = 0;
     virtual void convert_out(double* dst, const void* src, long long n, bool state_type) = 0;
+    // allocate at least nf scratch fields (and the adjoint buffers) now
+    virtual void reserve(int nf, bool adj) = 0;
 };
 
 BoxBase* make_box_f32(const BoxConfig& c);
@@ -409,6 +411,11 @@ class BoxT final : public BoxBase {
         if (adjbuf) return;
         DNSBOX_HIP(hipMalloc(&adjbuf, 9 * size_t(fs) * sizeof(C)));
         DNSBOX_HIP(hipMalloc(&band2, 2 * sizeof(double)));
+    }
+    void reserve(int nf, bool adj) override
+    {
+        ensure_scratch(nf);
+        if (adj) ensure_adj();
     }
     C* adjA() const { return static_cast<C*>(adjbuf); }
     C* adjMu() const { return static_cast<C*>(adjbuf) + 3 * fs; }

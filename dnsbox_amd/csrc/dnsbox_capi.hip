@@ -655,4 +655,41 @@ int dnsbox_group_sync(dnsbox_group* g)
     });
 }
 
+int dnsbox_group_set_stream(dnsbox_group* g, void* stream)
+{
+    if (!g) return fail(DNSBOX_E_ARG, "null group");
+    g->g->stream = static_cast<hipStream_t>(stream);
+    return 0;
+}
+
+int dnsbox_group_step_dev(dnsbox_group* g, void* const* states, int nsteps)
+{
+    if (!g || !states) return fail(DNSBOX_E_ARG, "null argument");
+    if (nsteps < 0) return fail(DNSBOX_E_ARG, "nsteps < 0");
+    for (size_t i = 0; i < g->views.size(); ++i)
+        if (!states[i]) return fail(DNSBOX_E_ARG, "null state pointer");
+    return guarded([&]() -> int {
+        g->g->step_ext(states, nsteps);
+        for (auto& b : g->g->boxes) b->x6_valid = false;
+        return 0;
+    });
+}
+
+int dnsbox_group_step_vjp_dev(dnsbox_group* g, const void* const* u, const void* const* lam, void* const* lam_out,
+                              const double* dt)
+{
+    if (!g || !u || !lam || !lam_out || !dt) return fail(DNSBOX_E_ARG, "null argument");
+    for (size_t i = 0; i < g->views.size(); ++i) {
+        if (!u[i] || !lam[i] || !lam_out[i]) return fail(DNSBOX_E_ARG, "null device pointer");
+        if (!(dt[i] > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+        if (lam_out[i] == u[i] || lam_out[i] == lam[i])
+            return fail(DNSBOX_E_ARG, "lam_out must not alias u or lam");
+    }
+    return guarded([&]() -> int {
+        g->g->step_vjp(u, lam, lam_out, dt);
+        for (auto& b : g->g->boxes) b->x6_valid = false;
+        return 0;
+    });
+}
+
 }  // extern "C"

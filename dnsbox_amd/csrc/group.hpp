@@ -10,7 +10,10 @@
 // captured once in a hipGraph and replayed.
 #pragma once
 
+#include <cstring>
+#include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "box_impl.hpp"
@@ -22,11 +25,19 @@ struct GBox {
     int n;
     int zt0, yt0, xb0;  // first global z tile, y tile and x block of this box
     long long fs;
-    cpx<T>* st;
-    cpx<T>* X;
+    cpx<T>* st;          // state advanced in place by the forward step
+    cpx<T>* X;           // transform scratch (9 fields)
     const cpx<T>* tw;
     BoxScalars* sc;
     BoxPhys ph;
+    // step adjoint: out = (d u_{n+1} / d u_n)^T lam at u, fixed step dt
+    const cpx<T>* u;
+    const cpx<T>* lam;
+    cpx<T>* out;
+    cpx<T>* pred;        // u* of the step
+    cpx<T>* A;           // E (lam + lam_*)
+    cpx<T>* Mu;          // E (dt/2 lam + dt lam_*)
+    double dt;
 };
 
 constexpr int kGroupThreads = 256;
@@ -44,12 +55,13 @@ struct GroupLds {
     static constexpr int z() { return N * Cfg<N, T>::ZW; }
     template <int N>
     static constexpr int y() { return N * Cfg<N, T>::YW; }
-    template <int N>
-    static constexpr int x() { return XGeo<N, T, OpCross<T>>::LDS; }
+    template <int N, class Op>
+    static constexpr int x() { return XGeo<N, T, Op>::LDS; }
     static constexpr int mx(int a, int b) { return a > b ? a : b; }
     static constexpr int Z = mx(mx(mx(z<16>(), z<32>()), mx(z<48>(), z<64>())), z<128>());
     static constexpr int Y = mx(mx(mx(y<16>(), y<32>()), mx(y<48>(), y<64>())), y<128>());
-    static constexpr int X = mx(mx(mx(x<16>(), x<32>()), mx(x<48>(), x<64>())), x<128>());
+    template <class Op>
+    static constexpr int X = mx(mx(mx(x<16, Op>(), x<32, Op>()), mx(x<48, Op>(), x<64, Op>())), x<128, Op>());
     static_assert(group_size_ok<16, T>() && group_size_ok<32, T>() && group_size_ok<48, T>() &&
                       group_size_ok<64, T>() && group_size_ok<128, T>(),
                   "group sizes must share one block shape");
@@ -86,8 +98,9 @@ __device__ __forceinline__ const cpx<T>* stage_tw_n(const cpx<T>* __restrict__ g
         default: CALL(128); break;   \
     }
 
+// z-inverse + curl of the state (from_u = 0) or of the step-adjoint point u (from_u = 1)
 template <class T>
-__global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restrict__ gb, int nbox)
+__global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restrict__ gb, int nbox, int from_u)
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
@@ -95,7 +108,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restr
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
     const int tile = blockIdx.x - bx.zt0;
-    const cpx<T>* in[3] = {bx.st, bx.st + bx.fs, bx.st + 2 * bx.fs};
+    const cpx<T>* src = from_u ? bx.u : bx.st;
+    const cpx<T>* in[3] = {src, src + bx.fs, src + 2 * bx.fs};
     cpx<T>* out[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) out[f] = bx.X + f * bx.fs;
@@ -120,10 +134,11 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restr
 #undef DNSBOX_G_YCOL
 }
 
-template <class T>
-__global__ void __launch_bounds__(kGroupThreads) k_g_xcross(const GBox<T>* __restrict__ gb, int nbox, int track)
+// x pass of Op in place on the scratch; OpCross can track the CFL maxima
+template <class T, class Op>
+__global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __restrict__ gb, int nbox, int track)
 {
-    __shared__ cpx<T> lds[GroupLds<T>::X];
+    __shared__ cpx<T> lds[GroupLds<T>::template X<Op>];
     __shared__ cpx<T> stw[128];
     __shared__ double wsm[4 * 16];
     const int b = find_box(gb, nbox, blockIdx.x, 2);
@@ -133,10 +148,12 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_xcross(const GBox<T>* __res
     XRed red;
 #pragma unroll
     for (int j = 0; j < 4; ++j) red.m[j] = 0.0;
-#define DNSBOX_G_X(NN) xpass_tile<NN, T, OpCross<T>>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
+#define DNSBOX_G_X(NN) xpass_tile<NN, T, Op>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_X)
 #undef DNSBOX_G_X
-    if (track) block_max_atomic<3>(red, bx.sc->maxu, false, wsm);
+    if constexpr (Op::NRED == 3) {
+        if (track) block_max_atomic<3>(red, bx.sc->maxu, false, wsm);
+    }
 }
 
 template <class T, int MODE>
@@ -160,6 +177,81 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
 #define DNSBOX_G_Z(NN) zfwd_zinv_tile<NN, T, MODE, T>(bx.X, bx.fs, tile, lds, tw, ea, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_Z)
 #undef DNSBOX_G_Z
+}
+
+// u* = E (u + dt N(u)) of the step adjoint, into pred
+template <class T>
+__global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_predonly(const GBox<T>* __restrict__ gb, int nbox)
+{
+    __shared__ cpx<T> lds[GroupLds<T>::Z];
+    __shared__ cpx<T> stw[128];
+    const int b = find_box(gb, nbox, blockIdx.x, 0);
+    const GBox<T> bx = gb[b];
+    const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
+    const int tile = blockIdx.x - bx.zt0;
+    EpiArgs<T, T> ea{};
+    ea.out = bx.pred;
+    ea.state = const_cast<cpx<T>*>(bx.u);  // read only in this mode
+    ea.rhs_in = bx.u;
+    ea.rhs_in_s = 0;
+    ea.fs = bx.fs;
+    ea.dt_fixed = bx.dt;
+    ea.sc = bx.sc;
+    ea.ph = bx.ph;
+#define DNSBOX_G_Z(NN) zfwd_tile<NN, T, EPI_PREDONLY, T>(bx.X, bx.fs, tile, lds, tw, ea)
+    DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_Z)
+#undef DNSBOX_G_Z
+}
+
+// z-inverse of the VJP prologue: STAGE 1 at u* with cotangent dt/2 lam, STAGE 2 at u
+// with cotangent Mu
+template <class T, int STAGE>
+__global__ void __launch_bounds__(kGroupThreads) k_g_zinv_vjp(const GBox<T>* __restrict__ gb, int nbox)
+{
+    __shared__ cpx<T> lds[GroupLds<T>::Z];
+    __shared__ cpx<T> stw[128];
+    const int b = find_box(gb, nbox, blockIdx.x, 0);
+    const GBox<T> bx = gb[b];
+    const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
+    const int tile = blockIdx.x - bx.zt0;
+    const cpx<T>* l = STAGE == 1 ? bx.lam : bx.Mu;
+    const cpx<T>* p = STAGE == 1 ? bx.pred : bx.u;
+    const cpx<T>* in[6] = {l, l + bx.fs, l + 2 * bx.fs, p, p + bx.fs, p + 2 * bx.fs};
+    cpx<T>* out[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) out[f] = bx.X + f * bx.fs;
+    const double sm = 1.0 / (double(bx.n) * bx.n * bx.n);
+    const ProVjpIn<T> pro{T(sm), T(STAGE == 1 ? 0.5 * bx.dt : 1.0), bx.n, bx.ph};
+#define DNSBOX_G_ZV(NN) zinv_tile<NN, T, ProVjpIn<T>>(in, out, tile, lds, tw, pro)
+    DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZV)
+#undef DNSBOX_G_ZV
+}
+
+// z-forward of the VJP with the step-adjoint epilogue (VJP_S1 / VJP_S2)
+template <class T, int MODE>
+__global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_vjp(const GBox<T>* __restrict__ gb, int nbox)
+{
+    __shared__ cpx<T> lds[GroupLds<T>::Z];
+    __shared__ cpx<T> stw[128];
+    const int b = find_box(gb, nbox, blockIdx.x, 0);
+    const GBox<T> bx = gb[b];
+    const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
+    const int tile = blockIdx.x - bx.zt0;
+    VjpArgs<T> va{};
+    va.out = bx.out;
+    va.lam = bx.lam;
+    va.A = bx.A;
+    va.Mu = bx.Mu;
+    va.u = bx.u;
+    va.lamN = MODE == VJP_S1 ? bx.lam : bx.Mu;
+    va.lam_scale = MODE == VJP_S1 ? 0.5 * bx.dt : 1.0;
+    va.dt = bx.dt;
+    va.band = nullptr;  // group boxes have no band forcing
+    va.fs = bx.fs;
+    va.ph = bx.ph;
+#define DNSBOX_G_ZF(NN) zfwd_vjp_tile<NN, T, MODE>(bx.X, bx.fs, tile, lds, tw, va)
+    DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZF)
+#undef DNSBOX_G_ZF
 }
 
 template <class T>
@@ -205,25 +297,36 @@ class GroupBase {
     std::vector<std::unique_ptr<BoxBase>> boxes;
     hipStream_t stream = nullptr;
     virtual void step(int nsteps) = 0;
+    virtual void step_ext(void* const* states, int nsteps) = 0;
+    virtual void step_vjp(const void* const* u, const void* const* lam, void* const* out, const double* dt) = 0;
     virtual void invalidate() = 0;
 };
 
 template <class T>
 class Group final : public GroupBase {
+    using C = cpx<T>;
+
    public:
-    GBox<T>* d_gb = nullptr;
+    GBox<T>* d_gb = nullptr;          // descriptors of the boxes' own states (read by the graph)
     std::vector<GBox<T>> h_gb;
     int zt = 0, yt = 0, xb = 0;
     bool x6_valid = false;
     hipStream_t gstream = nullptr;
     hipGraphExec_t graph = nullptr;
+    // descriptor sets of the device-buffer entry points, keyed by their bytes: the
+    // pointers a caller cycles through (checkpoint segments, ping-pong cotangents) are
+    // uploaded once and reused
+    std::map<std::string, GBox<T>*> dcache;
+    static constexpr size_t kMaxCached = 256;
 
     explicit Group(std::vector<std::unique_ptr<BoxBase>>&& bs)
     {
         boxes = std::move(bs);
         const int prec = sizeof(T) == 8 ? 64 : 32;
         for (auto& b : boxes) {
-            GBox<T> g{};
+            b->reserve(9, true);  // VJP scratch and adjoint buffers up front
+            GBox<T> g;
+            std::memset(&g, 0, sizeof(g));  // descriptor sets are compared by their bytes
             g.n = b->cfg.n;
             g.zt0 = zt;
             g.yt0 = yt;
@@ -231,12 +334,7 @@ class Group final : public GroupBase {
             zt += group_ztiles(g.n, prec);
             yt += group_ytiles(g.n, prec);
             xb += group_xblocks(g.n, prec);
-            g.fs = b->fs;
-            g.st = static_cast<cpx<T>*>(b->state);
-            g.X = static_cast<cpx<T>*>(b->scratch);
-            g.tw = static_cast<const cpx<T>*>(b->tw);
-            g.sc = b->sc;
-            g.ph = b->cfg.ph;
+            fill(g, *b);
             h_gb.push_back(g);
         }
         DNSBOX_HIP(hipMalloc(&d_gb, h_gb.size() * sizeof(GBox<T>)));
@@ -250,23 +348,86 @@ class Group final : public GroupBase {
             (void)hipStreamSynchronize(gstream);
             (void)hipStreamDestroy(gstream);
         }
+        (void)hipDeviceSynchronize();
+        for (auto& kv : dcache) (void)hipFree(kv.second);
         if (d_gb) (void)hipFree(d_gb);
+    }
+
+    static void fill(GBox<T>& g, const BoxBase& b)
+    {
+        g.fs = b.fs;
+        g.st = static_cast<C*>(b.state);
+        g.X = static_cast<C*>(b.scratch);
+        g.tw = static_cast<const C*>(b.tw);
+        g.sc = b.sc;
+        g.ph = b.cfg.ph;
+        g.pred = static_cast<C*>(b.pred);
+        g.A = static_cast<C*>(b.adjbuf);
+        g.Mu = static_cast<C*>(b.adjbuf) + 3 * b.fs;
     }
 
     void invalidate() override { x6_valid = false; }
 
-    void one_step(hipStream_t s)
+    // a box's buffers move when a single-box entry point grows its scratch (rhsAdj):
+    // rewrite the descriptors before the next group launch
+    void refresh()
+    {
+        bool moved = false;
+        for (size_t i = 0; i < boxes.size(); ++i) {
+            const BoxBase& b = *boxes[i];
+            if (h_gb[i].X != b.scratch || h_gb[i].st != b.state || h_gb[i].A != b.adjbuf) {
+                fill(h_gb[i], b);
+                moved = true;
+            }
+        }
+        if (!moved) return;
+        DNSBOX_HIP(hipDeviceSynchronize());
+        DNSBOX_HIP(hipMemcpy(d_gb, h_gb.data(), h_gb.size() * sizeof(GBox<T>), hipMemcpyHostToDevice));
+        drop_cache();
+        x6_valid = false;
+    }
+
+    void drop_cache()
+    {
+        if (dcache.empty()) return;
+        DNSBOX_HIP(hipDeviceSynchronize());
+        for (auto& kv : dcache) (void)hipFree(kv.second);
+        dcache.clear();
+    }
+
+    const GBox<T>* descriptors(const std::vector<GBox<T>>& h)
+    {
+        std::string key(reinterpret_cast<const char*>(h.data()), h.size() * sizeof(GBox<T>));
+        auto it = dcache.find(key);
+        if (it != dcache.end()) return it->second;
+        if (dcache.size() >= kMaxCached) drop_cache();
+        GBox<T>* d = nullptr;
+        DNSBOX_HIP(hipMalloc(&d, h.size() * sizeof(GBox<T>)));
+        DNSBOX_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(GBox<T>), hipMemcpyHostToDevice));
+        dcache.emplace(std::move(key), d);
+        return d;
+    }
+
+    void launch_zinv(hipStream_t s, const GBox<T>* d, int from_u)
+    {
+        hipLaunchKernelGGL((k_g_zinv<T>), dim3(zt), dim3(kGroupThreads), 0, s, d, int(h_gb.size()), from_u);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    // one IF-Heun step of every box, its z-inverse + curl already in the scratch
+    void one_step(hipStream_t s, const GBox<T>* d)
     {
         const int nb = int(h_gb.size());
-        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d_gb, nb);
-        hipLaunchKernelGGL((k_g_xcross<T>), dim3(xb), dim3(kGroupThreads), 0, s, d_gb, nb, 1);
-        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 3), dim3(kGroupThreads), 0, s, d_gb, nb);
-        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_PRED>), dim3(zt), dim3(kGroupThreads), 0, s, d_gb, nb);
-        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d_gb, nb);
-        hipLaunchKernelGGL((k_g_xcross<T>), dim3(xb), dim3(kGroupThreads), 0, s, d_gb, nb, 0);
-        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 3), dim3(kGroupThreads), 0, s, d_gb, nb);
-        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_CORR>), dim3(zt), dim3(kGroupThreads), 0, s, d_gb, nb);
-        hipLaunchKernelGGL((k_g_step_end<T>), dim3((nb + 63) / 64), dim3(64), 0, s, d_gb, nb);
+        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 1);
+        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 3), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_PRED>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 3), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_CORR>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_step_end<T>), dim3((nb + 63) / 64), dim3(64), 0, s, d, nb);
+        DNSBOX_LAUNCH_CHECK();
     }
 
     // IF-Heun steps of every box: the first z-inverse if needed, then the captured
@@ -275,20 +436,19 @@ class Group final : public GroupBase {
     void step(int nsteps) override
     {
         if (nsteps <= 0) return;
+        refresh();
         hipEvent_t ev;
         DNSBOX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         DNSBOX_HIP(hipEventRecord(ev, stream));
         DNSBOX_HIP(hipStreamWaitEvent(gstream, ev, 0));
-        const int nb = int(h_gb.size());
         if (!x6_valid) {
-            hipLaunchKernelGGL((k_g_zinv<T>), dim3(zt), dim3(kGroupThreads), 0, gstream, d_gb, nb);
-            DNSBOX_LAUNCH_CHECK();
+            launch_zinv(gstream, d_gb, 0);
             x6_valid = true;
         }
         if (!graph) {
             hipGraph_t g;
             DNSBOX_HIP(hipStreamBeginCapture(gstream, hipStreamCaptureModeThreadLocal));
-            one_step(gstream);
+            one_step(gstream, d_gb);
             DNSBOX_HIP(hipStreamEndCapture(gstream, &g));
             DNSBOX_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
             DNSBOX_HIP(hipGraphDestroy(g));
@@ -297,6 +457,60 @@ class Group final : public GroupBase {
         DNSBOX_HIP(hipEventRecord(ev, gstream));
         DNSBOX_HIP(hipStreamWaitEvent(stream, ev, 0));
         DNSBOX_HIP(hipEventDestroy(ev));
+    }
+
+    // IF-Heun steps of caller-owned states (device buffers of the boxes' shapes), on
+    // the caller's stream; each box's step size and clock are its own
+    void step_ext(void* const* states, int nsteps) override
+    {
+        if (nsteps <= 0) return;
+        refresh();
+        std::vector<GBox<T>> h = h_gb;
+        for (size_t i = 0; i < h.size(); ++i) h[i].st = static_cast<C*>(states[i]);
+        const GBox<T>* d = descriptors(h);
+        x6_valid = false;
+        launch_zinv(stream, d, 0);
+        for (int i = 0; i < nsteps; ++i) one_step(stream, d);
+    }
+
+    // out_i = (d u_{n+1} / d u_n)^T lam_i for one fixed-dt IF-Heun step of every box
+    void step_vjp(const void* const* u, const void* const* lam, void* const* out, const double* dt) override
+    {
+        refresh();
+        std::vector<GBox<T>> h = h_gb;
+        for (size_t i = 0; i < h.size(); ++i) {
+            h[i].u = static_cast<const C*>(u[i]);
+            h[i].lam = static_cast<const C*>(lam[i]);
+            h[i].out = static_cast<C*>(out[i]);
+            h[i].dt = dt[i];
+        }
+        const GBox<T>* d = descriptors(h);
+        const int nb = int(h.size());
+        hipStream_t s = stream;
+        x6_valid = false;
+        // u* = E (u + dt N(u))
+        launch_zinv(s, d, 1);
+        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 3), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_predonly<T>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
+        DNSBOX_LAUNCH_CHECK();
+        // lam_* = J(u*)^T (dt/2 lam); A = E (lam + lam_*), Mu = E (dt/2 lam + dt lam_*)
+        vjp_passes<VJP_S1>(s, d);
+        // out = A + J(u)^T Mu
+        vjp_passes<VJP_S2>(s, d);
+    }
+
+    template <int MODE>
+    void vjp_passes(hipStream_t s, const GBox<T>* d)
+    {
+        const int nb = int(h_gb.size());
+        hipLaunchKernelGGL((k_g_zinv_vjp<T, MODE>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 9), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpVjp<T>>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_vjp<T, MODE>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
+        DNSBOX_LAUNCH_CHECK();
     }
 };
 

@@ -171,6 +171,16 @@ int dnsbox_group_box(dnsbox_group* g, int i, dnsbox_ctx** box);
 /* nsteps IF-Heun steps of every box of the group (dnsboxheun.f90:84-157 per box) */
 int dnsbox_group_step(dnsbox_group* g, int nsteps);
 int dnsbox_group_sync(dnsbox_group* g);
+/* stream (hipStream_t) the device-buffer group entry points run on, and that
+   dnsbox_group_step is ordered with; NULL = the null stream */
+int dnsbox_group_set_stream(dnsbox_group* g, void* stream);
+/* batched dnsbox_step_dev: states[i] is a device state of box i (internal layout, group
+   precision), advanced in place by nsteps IF-Heun steps at box i's step size */
+int dnsbox_group_step_dev(dnsbox_group* g, void* const* states, int nsteps);
+/* batched dnsbox_step_vjp_dev: lam_out[i] <- (d u_{n+1} / d u_n)^T lam[i] for one IF-Heun
+   step of box i from u[i] at step size dt[i] */
+int dnsbox_group_step_vjp_dev(dnsbox_group* g, const void* const* u, const void* const* lam,
+                              void* const* lam_out, const double* dt);
 
 #ifdef __cplusplus
 }
