@@ -1,23 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X pseudospectral Navier-Stokes step (one GPU).
 
-Workload (the headline metric's configuration): 256^3 box, FP32 FFTs and nonlinear
-term with FP64 integrating factor / CFL control / diagnostics, nu = 0.005,
-Kolmogorov forcing f = (sin 4y, 0, 0), IF-Heun predictor-corrector with the CFL
-controller at Courant 0.5, seeded solenoidal initial field E(k) ~ k^4 exp(-2 (k/4)^2)
-normalised to E = 0.5.  A step is one IF-Heun step (two RHS evaluations) of the
-whole box, with the state resident in HBM.
+Workload of the headline number: 256^3 box, FP32 FFTs and nonlinear term, FP64 state
+update, integrating factor, CFL control and diagnostics; nu = 0.005, Kolmogorov forcing
+f = (sin 4y, 0, 0), IF-Heun predictor-corrector with the CFL controller at Courant 0.5,
+seeded solenoidal initial field E(k) ~ k^4 exp(-2 (k/4)^2) normalised to E = 0.5.  A step
+is one IF-Heun step (two RHS evaluations) of the whole box, state resident in HBM.
 
-Printed as one JSON line: steps/s (FP32), the FP64 step rate beside it, the FP32
-spectrum/energy error against the FP64 run after the same number of fixed-dt steps,
-the HBM roofline fraction of the dominant kernel (HIP-event timed), and the CPU
-oracle timed on the host cores (a reported baseline, not the target).
+Printed as one JSON line:
+  value                  steps/s of that FP32 form
+  fp32_pure_*            the all-FP32 form (FP32 state too): rate and error
+  fp64_steps_per_s       the same workload in FP64
+  *_err_vs_fp64          spectrum (per shell and L2) and energy error vs the FP64 run after
+                         --err-steps fixed-dt steps from the same field
+  roofline               the dominant kernel of the step: algorithmic bytes / HIP-event
+                         launch time vs the 8 TB/s HBM peak; traffic = measured HBM bytes
+                         per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+                         when a measurement of this kernel is on file
+  adjoint                64^3 FP64 checkpointed T-step gradients per second (T = 200,
+                         dt = 0.005, checkpoint every 20) and one finite-difference check
+  batched                48 boxes (16 x 32^3, 16 x 48^3, 16 x 64^3, FP32, own nu and k0):
+                         box-steps/s forward (hipGraph step) and forward + adjoint
+                         (checkpointed gradient of the T = 100 map)
+  cpu_baseline           the FP64 CPU oracle on the host cores (a reported baseline)
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,6 +39,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("time-steps/s at 256³ FP32 vs FP64 (err stated), % HBM roofline, "
           "adjoint grads/s")
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256cubed.txt")
 
 
 def parse():
@@ -36,12 +49,19 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--err-steps", type=int, default=1000,
-                    help="fixed-dt steps of the FP32-vs-FP64 error run (0 = skip)")
-    ap.add_argument("--fp64-steps", type=int, default=200, help="timed FP64 steps")
+                    help="fixed-dt steps of the FP32-vs-FP64 error runs (0 = skip)")
+    ap.add_argument("--fp64-steps", type=int, default=300, help="timed FP64 steps")
+    ap.add_argument("--pure-steps", type=int, default=1000, help="timed all-FP32 steps (0 = skip)")
+    ap.add_argument("--adjoint", type=int, default=1, help="time adjoint gradients (1/0)")
+    ap.add_argument("--batched", type=int, default=1, help="time the 48-box group (1/0)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (1/0)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--reps", type=int, default=20, help="launches per pass for kernel timing")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def make_box(n, precision, adaptive=True, dt=0.002, state_precision=0):
@@ -64,7 +84,7 @@ def timed_steps(box, steps, warmup):
 
 
 def copy_ceiling_gbs():
-    """device-to-device copy of 2 GiB, timed with torch events (read + write bytes)."""
+    """device-to-device copy of 1 GiB, timed with torch events (read + write bytes)."""
     import torch
     n = 1 << 28  # floats: 1 GiB per buffer
     a = torch.empty(n, dtype=torch.float32, device="cuda")
@@ -103,6 +123,108 @@ def errors(ref, got):
             "energy": float(abs(d32[0] - d64[0]) / d64[0])}
 
 
+def kernel_table(box, reps):
+    """HIP-event time and algorithmic GB/s of each step pass of `box`."""
+    from dnsbox_amd import _lib
+    ms = box.time_passes(reps)
+    byts = box.pass_bytes()
+    out = {}
+    for name, per_step in zip(_lib.PASS_NAMES, _lib.PASS_PER_STEP):
+        out[name] = {"ms": ms[name], "alg_bytes": byts[name],
+                     "gbs": byts[name] / (ms[name] * 1e-3) / 1e9, "ms_per_step": ms[name] * per_step}
+    return out
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` for `workload` from the rocprofv3 counter summary on
+    file (lines `workload kernel fetch_bytes write_bytes`), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 4 and p[0] == workload and p[1] == kernel and re.fullmatch(r"[0-9.eE+]+", p[2]):
+                return float(p[2]) + float(p[3])
+    return None
+
+
+def adjoint_rate(reps=3):
+    """64^3 FP64 residual gradients per second (T = 200, dt = 0.005, checkpoint every 20)
+    and a central-difference check of one gradient along a seeded solenoidal direction."""
+    import torch
+    from dnsbox_amd import Box
+    from dnsbox_amd.adjoint import residual
+    n, T, dt, k = 64, 200, 0.005, 20
+    with Box(n=n, precision=64, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        b.init_random(0, 0, 4.0, 0.5)
+        u0 = b.state_tensor()
+        b.init_random(1, 0, 4.0, 0.5)
+        v = b.state_tensor()
+
+        def grad():
+            x = u0.clone().requires_grad_(True)
+            J = residual(b, x, T, dt, k)
+            (g,) = torch.autograd.grad(J, x)
+            return J, g
+
+        J, g = grad()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            J, g = grad()
+        torch.cuda.synchronize()
+        secs = (time.perf_counter() - t0) / reps
+        eps = 1e-6
+        with torch.no_grad():
+            jp = float(residual(b, u0 + eps * v, T, dt, k))
+            jm = float(residual(b, u0 - eps * v, T, dt, k))
+        fd = (jp - jm) / (2 * eps)
+        an = float((g.conj() * v).real.sum())
+    return {"grads_per_s": 1.0 / secs, "ms_per_grad": secs * 1e3, "n": n, "T": T, "dt": dt,
+            "checkpoint_every": k, "dtype": "f64", "J": float(J),
+            "fd_rel_err": abs(fd - an) / abs(fd),
+            "workload": "64^3 FP64, nu=0.02, f=(sin 4y,0,0), J=1/2<|Phi_T(u)-u|^2>"}
+
+
+def batched_rates(steps=500, T=100, k=20, reps=2):
+    """The 48-box group: forward box-steps/s (hipGraph-replayed step of the boxes' own states)
+    and forward + adjoint box-steps/s (checkpointed gradient of the T-step map)."""
+    import numpy as np
+    import torch
+    from dnsbox_amd import BoxGroup
+    from dnsbox_amd.adjoint import group_residual_grads
+    rng = np.random.default_rng(2024)
+    sizes = [32] * 16 + [48] * 16 + [64] * 16
+    nus = rng.uniform(0.01, 0.05, len(sizes))
+    k0s = rng.uniform(2.0, 6.0, len(sizes))
+    dt = 0.005
+    params = [dict(n=n, precision=32, nu=float(nu), kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0,
+                   dt=dt) for n, nu in zip(sizes, nus)]
+    with BoxGroup(params) as g:
+        us = []
+        for i, b in enumerate(g.boxes):
+            b.init_random(100, i, float(k0s[i]), 0.5)
+            b.rhs_int_fact(dt)
+            us.append(b.state_tensor())
+        g.step(10)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.sync()
+        fwd = len(sizes) * steps / (time.perf_counter() - t0)
+        dts = [dt] * len(sizes)
+        group_residual_grads(g, us, T, dts, k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            group_residual_grads(g, us, T, dts, k)
+        torch.cuda.synchronize()
+        secs = (time.perf_counter() - t0) / reps
+    return {"boxes": len(sizes), "sizes": "16x32^3 + 16x48^3 + 16x64^3", "dtype": "f32",
+            "forward_box_steps_per_s": fwd, "fwd_adj_box_steps_per_s": len(sizes) * T / secs,
+            "ms_per_batched_gradient": secs * 1e3, "T": T, "checkpoint_every": k, "dt": dt}
+
+
 def cpu_baseline(n, threads):
     """One IF-Heun step of the FP64 CPU oracle (reference-structure restatement: 15 3-D
     transforms per RHS) at n^3 on `threads` host threads."""
@@ -119,67 +241,66 @@ def cpu_baseline(n, threads):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs one box per GPU on a single GPU; "
-                          "--gpus N > 1 is not supported (multi-GPU sharding is design-only)"}))
+        print(json.dumps({"error": "this build runs on one GPU: the step of one box, or a group of "
+                          "boxes, per GPU; --gpus N > 1 is not supported (sharding is design-only)"}))
         sys.exit(2)
-    import numpy as np
     import torch
-    from dnsbox_amd import _lib
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n = args.n
-    print(f"[bench] n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+    log(f"n={n} steps={args.steps} warmup={args.warmup}")
 
-    # ---- FP32 timed run (the metric) ----
-    b32 = make_box(n, 32)
-    b32.init_random(0, 0, 4.0, 0.5)
-    secs = timed_steps(b32, args.steps, args.warmup)
+    # ---- headline: FP32 transforms + FP64 state, timed ----
+    bh = make_box(n, 32, state_precision=64)
+    bh.init_random(0, 0, 4.0, 0.5)
+    secs = timed_steps(bh, args.steps, args.warmup)
     value = args.steps / secs
-    t32, dt32 = b32.time()
-    print(f"[bench] fp32 {value:.1f} steps/s (t={t32:.4f}, dt={dt32:.3e})", file=sys.stderr, flush=True)
-
-    # ---- kernel passes, HIP events on the box stream ----
-    ms = b32.time_passes(args.reps)
-    fs_bytes = n * n * (n // 2 + 1) * 8  # one FP32 complex spectral field
-    per_step = {"zinv_curl": 0, "yinv6": 2, "xcross": 2, "yfwd3": 2, "zpred_zinv": 1, "zcorr_zinv": 1}
-    kernels = {}
-    for name, (r, w) in zip(_lib.PASS_NAMES, _lib.PASS_FIELDS):
-        byts = (r + w) * fs_bytes
-        kernels[name] = {"ms": ms[name], "alg_bytes": byts,
-                         "gbs": byts / (ms[name] * 1e-3) / 1e9,
-                         "ms_per_step": ms[name] * per_step[name]}
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    t_end, dt_end = bh.time()
+    log(f"fp32 (fp64 state) {value:.1f} steps/s (t={t_end:.4f}, dt={dt_end:.3e})")
+    kernels = kernel_table(bh, args.reps)
+    bh.close()
+    dom = max(kernels, key=lambda q: kernels[q]["ms_per_step"])
     copy_gbs = copy_ceiling_gbs()
-    b32.close()
-    print(f"[bench] passes {json.dumps({k: round(v['gbs']) for k, v in kernels.items()})} "
-          f"copy={copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    log(f"passes {json.dumps({q: round(v['gbs']) for q, v in kernels.items()})} copy={copy_gbs:.0f} GB/s")
 
-    # ---- FP64 timed run (same workload in FP64) ----
+    # ---- all-FP32 form ----
+    pure = None
+    if args.pure_steps > 0:
+        bp = make_box(n, 32)
+        bp.init_random(0, 0, 4.0, 0.5)
+        pure_rate = args.pure_steps / timed_steps(bp, args.pure_steps, args.warmup)
+        pure_kernels = kernel_table(bp, args.reps)
+        bp.close()
+        pure = {"steps_per_s": pure_rate,
+                "kernels": {q: {"ms": round(v["ms"], 5), "gbs": round(v["gbs"], 1)} for q, v in pure_kernels.items()}}
+        log(f"fp32 (fp32 state) {pure_rate:.1f} steps/s")
+
+    # ---- FP64 ----
     b64 = make_box(n, 64)
     b64.init_random(0, 0, 4.0, 0.5)
-    secs64 = timed_steps(b64, args.fp64_steps, args.warmup)
-    fp64_rate = args.fp64_steps / secs64
+    fp64_rate = args.fp64_steps / timed_steps(b64, args.fp64_steps, args.warmup)
     b64.close()
-    print(f"[bench] fp64 {fp64_rate:.1f} steps/s", file=sys.stderr, flush=True)
+    log(f"fp64 {fp64_rate:.1f} steps/s")
 
-    # ---- FP32 state-in-FP64 variant: rate ----
-    b3 = make_box(n, 32, state_precision=64)
-    b3.init_random(0, 0, 4.0, 0.5)
-    secs3 = timed_steps(b3, args.steps, args.warmup)
-    rate_s64 = args.steps / secs3
-    b3.close()
-    print(f"[bench] fp32 + fp64 state {rate_s64:.1f} steps/s", file=sys.stderr, flush=True)
-
-    # ---- FP32 error vs FP64 (same field, same fixed dt, err_steps steps) ----
-    err32 = err_s64 = None
+    # ---- errors vs FP64 (same field, same fixed dt) ----
+    err = err_pure = None
     if args.err_steps > 0:
         ref = run_fixed(n, 64, args.err_steps)
-        err32 = errors(ref, run_fixed(n, 32, args.err_steps))
-        err_s64 = errors(ref, run_fixed(n, 32, args.err_steps, state_precision=64))
-        print(f"[bench] vs fp64 after {args.err_steps} steps: fp32 {err32}; fp32+fp64 state {err_s64}",
-              file=sys.stderr, flush=True)
+        err = errors(ref, run_fixed(n, 32, args.err_steps, state_precision=64))
+        if pure is not None:
+            err_pure = errors(ref, run_fixed(n, 32, args.err_steps))
+            pure["err_vs_fp64"] = err_pure
+        log(f"vs fp64 after {args.err_steps} steps: fp64-state {err}; all-fp32 {err_pure}")
 
-    # ---- CPU baseline (oracle, port of the reference structure) ----
+    adj = adjoint_rate() if args.adjoint else None
+    if adj:
+        log(f"adjoint {adj['grads_per_s']:.2f} grads/s, fd rel err {adj['fd_rel_err']:.2e}")
+    bat = batched_rates() if args.batched else None
+    if bat:
+        log(f"batched {bat['forward_box_steps_per_s']:.0f} box-steps/s forward, "
+            f"{bat['fwd_adj_box_steps_per_s']:.0f} forward+adjoint")
+
+    # ---- CPU baseline (oracle, restatement of the reference structure) ----
     cpu = None
     if args.cpu_baseline:
         threads = min(args.cpu_threads, os.cpu_count() or 1)
@@ -187,9 +308,10 @@ def main():
         cpu = {"value": rate, "unit": "time-steps/s", "cores": threads, "kind": "port",
                "sample": f"1 IF-Heun step of the FP64 CPU oracle at {n}^3 (15 transforms/RHS, "
                          f"OpenMP), {secs_cpu:.1f} s"}
-        print(f"[bench] cpu {rate:.4f} steps/s on {threads} threads", file=sys.stderr, flush=True)
+        log(f"cpu {rate:.4f} steps/s on {threads} threads")
 
-    k = kernels[dom]
+    kd = kernels[dom]
+    workload = f"{n}cubed_fp32_state64"
     out = {
         "metric": METRIC,
         "value": value,
@@ -204,19 +326,21 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded solenoidal field, E(k)~k^4 exp(-2(k/4)^2), E=0.5)",
         "config": {"workload": f"{n}^3 forced box, IF-Heun, CFL 0.5, nu=0.005, f=(sin 4y,0,0), "
-                               "FP32 FFT/nonlinear + FP64 IF/CFL/diagnostics",
+                               "FP32 FFT/nonlinear, FP64 state/IF/CFL/diagnostics",
                    "n": n, "nu": 0.005, "cfl": 0.5, "stepper": "IF-Heun"},
-        "fp64_steps_per_s": fp64_rate,
-        "speedup_fp32_vs_fp64": value / fp64_rate,
-        "fp32_err_vs_fp64": err32,
-        "fp32_state64_steps_per_s": rate_s64,
-        "fp32_state64_err_vs_fp64": err_s64,
-        "speedup_fp32_state64_vs_fp64": rate_s64 / fp64_rate,
+        "err_vs_fp64": err,
         "err_steps": args.err_steps,
-        "roofline": {"bound": "hbm", "achieved": k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": k["gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+        "fp64_steps_per_s": fp64_rate,
+        "speedup_vs_fp64": value / fp64_rate,
+        "fp32_pure": pure,
+        "speedup_fp32_pure_vs_fp64": (pure["steps_per_s"] / fp64_rate) if pure else None,
+        "roofline": {"bound": "hbm", "achieved": kd["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kd["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom, workload),
+                     "kernel": dom, "alg_bytes_per_launch": kd["alg_bytes"], "ms_per_launch": kd["ms"],
                      "copy_ceiling_gbs": copy_gbs},
-        "kernels": {kk: {"ms": round(v["ms"], 5), "gbs": round(v["gbs"], 1)} for kk, v in kernels.items()},
+        "kernels": {q: {"ms": round(v["ms"], 5), "gbs": round(v["gbs"], 1)} for q, v in kernels.items()},
+        "adjoint": adj,
+        "batched": bat,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

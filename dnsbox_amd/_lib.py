@@ -14,8 +14,13 @@ LIB_PATH = os.path.join(PKG, "lib", "libdnsbox_hip.so")
 (D_EKIN, D_DISP, D_PROD, D_COURANT, D_DIVMAX, D_EZERO, D_EONE, D_ESQRT2, D_EBAND, D_TIME,
  D_DT, NDIAG) = range(12)
 PASS_NAMES = ("zinv_curl", "yinv6", "xcross", "yfwd3", "zpred_zinv", "zcorr_zinv")
-# spectral fields (read, written) per launch of each pass
-PASS_FIELDS = ((3, 6), (6, 6), (6, 3), (3, 3), (6, 9), (6, 9))
+# spectral fields per launch of each pass: (transform-type read, transform-type written,
+# state-type read, state-type written); the state type is FP64 for an FP32 box with an
+# FP64 state, else the transform type
+PASS_FIELDS = ((0, 6, 3, 0), (6, 6, 0, 0), (6, 3, 0, 0), (3, 3, 0, 0), (3, 6, 3, 3), (3, 6, 3, 3))
+# launches of each pass per IF-Heun step once the step pipeline is running (the z-inverse of
+# the first RHS is fused into the previous step's last pass)
+PASS_PER_STEP = (0, 2, 2, 2, 1, 1)
 DIAG_NAMES = ("Ekin", "Disp", "Prod", "Courant", "divMax", "EZero", "EOne", "Esqrt2", "Eband",
               "time", "dt")
 

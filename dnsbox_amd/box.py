@@ -189,6 +189,17 @@ class Box:
         check(self.lib.dnsbox_time_passes(self._h, reps, _dp(ms)))
         return dict(zip(_lib.PASS_NAMES, ms.tolist()))
 
+    def pass_bytes(self):
+        """Algorithmic HBM bytes of one launch of each step pass (every field read once and
+        written once), in the order of time_passes."""
+        n = self.n
+        m = n * n * (n // 2 + 1)
+        c = 16 if self.precision == 64 else 8
+        sp = self.params.state_precision or self.precision
+        s = 16 if sp == 64 else 8
+        return {name: m * (c * (xr + xw) + s * (sr + sw))
+                for name, (xr, xw, sr, sw) in zip(_lib.PASS_NAMES, _lib.PASS_FIELDS)}
+
     def state_ptr(self) -> int:
         p = C.c_void_p()
         check(self.lib.dnsbox_state_ptr(self._h, C.byref(p)))

@@ -23,8 +23,27 @@
 #ifndef DNSBOX_Z_PREFETCH
 #define DNSBOX_Z_PREFETCH 1
 #endif
+#ifndef DNSBOX_XCD_REMAP
+#define DNSBOX_XCD_REMAP 1
+#endif
 
 namespace dnsbox {
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Give
+// each XCD a contiguous range of logical tiles, so that tiles whose rows share cache
+// lines (the y pass reads 8-byte-aligned segments of x-lines) meet in one L2.
+// b: linear block id in [0, nb); returns the logical tile id, a bijection on [0, nb).
+__device__ __forceinline__ int xcd_contiguous(int b, int nb)
+{
+#if DNSBOX_XCD_REMAP
+    constexpr int NX = 8;
+    const int xcd = b % NX, idx = b / NX, per = nb / NX, rem = nb % NX;
+    return xcd * per + (xcd < rem ? xcd : rem) + idx;
+#else
+    (void)nb;
+    return b;
+#endif
+}
 
 // ---------------------------------------------------------------------------
 // device-resident per-box scalars (FP64)
@@ -314,7 +333,9 @@ __global__ void __launch_bounds__((Geo<N, T>::YNT))
     __shared__ cpx<T> lds[N * G::C::YW];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    ycol_tile<N, T, DIR>(data0 + blockIdx.y * fs, blockIdx.x, lds, tw);
+    const int L = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int field = L / gridDim.x, tile = L % gridDim.x;
+    ycol_tile<N, T, DIR>(data0 + field * fs, tile, lds, tw);
 }
 
 // ===========================================================================
@@ -392,11 +413,12 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
             cpx<T> buf[QL];
 #pragma unroll
             for (int q = 0; q < QL; ++q) {
-                const int e = tid + q * NTX;
-                if (NL % NTX == 0 || e < NL) {
-                    const int f = e / (RB * H), rk = e % (RB * H);
-                    buf[q] = in0[f * in_fs + row0 * H + rk];
-                }
+                // unconditional loads (the tail re-reads element 0 of the block): a
+                // conditionally written buf is kept in scratch memory by the compiler
+                const int e0 = tid + q * NTX;
+                const int e = (NL % NTX == 0 || e0 < NL) ? e0 : 0;
+                const int f = e / (RB * H), rk = e % (RB * H);
+                buf[q] = in0[f * in_fs + row0 * H + rk];
             }
 #pragma unroll
             for (int q = 0; q < QL; ++q) {

@@ -124,11 +124,13 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restr
 {
     __shared__ cpx<T> lds[GroupLds<T>::Y];
     __shared__ cpx<T> stw[128];
-    const int b = find_box(gb, nbox, blockIdx.x, 1);
+    const int L = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int field = L / gridDim.x, gt = L % gridDim.x;
+    const int b = find_box(gb, nbox, gt, 1);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = blockIdx.x - bx.yt0;
-    cpx<T>* data = bx.X + blockIdx.y * bx.fs;
+    const int tile = gt - bx.yt0;
+    cpx<T>* data = bx.X + field * bx.fs;
 #define DNSBOX_G_YCOL(NN) ycol_tile<NN, T, DIR>(data, tile, lds, tw)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_YCOL)
 #undef DNSBOX_G_YCOL
