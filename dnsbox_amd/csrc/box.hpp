@@ -118,8 +118,11 @@ __device__ __forceinline__ double wave_sum(double v)
 //   YP, YW : the same for the y passes
 //   RP     : radices of the length-N/2 complex row transforms (x; real pairs packed)
 //   RB     : x-lines per block in the x pass, NTX its threads
+//   XPD    : x-pass LDS rows hold element j at j + j/XPD (0: at j)
+//   XPAD   : extra complex elements per x-pass LDS row
+// The x-pass padding comes from a bank model of the pass's LDS accesses.
 // ---------------------------------------------------------------------------
-template <int ZW_, int YW_, int RB_, int NTX_, class ZP_, class YP_, class RP_, int XPAD_ = 0>
+template <int ZW_, int YW_, int RB_, int NTX_, class ZP_, class YP_, class RP_, int XPD_ = 0, int XPAD_ = 0>
 struct CfgBase {
     using ZP = ZP_;
     using YP = YP_;
@@ -128,33 +131,34 @@ struct CfgBase {
     static constexpr int YW = YW_;
     static constexpr int RB = RB_;
     static constexpr int NTX = NTX_;
-    static constexpr int XPAD = XPAD_;  // LDS row padding (complex elements) of the x pass
+    static constexpr int XPD = XPD_;
+    static constexpr int XPAD = XPAD_;
 };
 template <int N, class T>
 struct Cfg;
-//                                        ZW  YW  RB  NTX  z plan              y plan            x (row) plan
-template <> struct Cfg<16, float>  : CfgBase<64, 64, 64, 256, Plan<4, 4>,       Plan<4, 4>,       Plan<8>>      {};
-template <> struct Cfg<32, float>  : CfgBase<32, 64, 32, 256, Plan<4, 2, 4>,    Plan<8, 4>,       Plan<16>>     {};
-template <> struct Cfg<48, float>  : CfgBase<64, 64, 16, 256, Plan<2, 12, 2>,   Plan<12, 4>,      Plan<8, 3>>   {};
-template <> struct Cfg<64, float>  : CfgBase<32, 32, 16, 256, Plan<8, 8>,       Plan<8, 8>,       Plan<8, 4>>   {};
-template <> struct Cfg<128, float> : CfgBase<16, 32, 8, 256,  Plan<4, 8, 4>,    Plan<16, 8>,      Plan<8, 8>>   {};
+//                                        ZW  YW  RB  NTX  z plan              y plan            x (row) plan   XPD XPAD
+template <> struct Cfg<16, float>  : CfgBase<64, 64, 64, 256, Plan<4, 4>,       Plan<4, 4>,       Plan<8>>              {};
+template <> struct Cfg<32, float>  : CfgBase<32, 64, 32, 256, Plan<4, 2, 4>,    Plan<8, 4>,       Plan<16>>             {};
+template <> struct Cfg<48, float>  : CfgBase<64, 64, 16, 256, Plan<2, 12, 2>,   Plan<12, 4>,      Plan<8, 3>>           {};
+template <> struct Cfg<64, float>  : CfgBase<32, 32, 16, 256, Plan<8, 8>,       Plan<8, 8>,       Plan<8, 4>, 0, 4>     {};
+template <> struct Cfg<128, float> : CfgBase<16, 32, 8, 256,  Plan<4, 8, 4>,    Plan<16, 8>,      Plan<8, 8>, 16, 2>    {};
 #ifdef DNSBOX_CFG256F
 template <> struct Cfg<256, float> : DNSBOX_CFG256F {};
 #else
-template <> struct Cfg<256, float> : CfgBase<16, 64, 4, 256,  Plan<8, 4, 8>,    Plan<16, 16>,     Plan<16, 8>>  {};
+template <> struct Cfg<256, float> : CfgBase<16, 64, 4, 256,  Plan<8, 4, 8>,    Plan<16, 16>,     Plan<16, 8>, 32, 2>   {};
 #endif
-template <> struct Cfg<512, float> : CfgBase<8, 16, 2, 256,   Plan<8, 8, 8>,    Plan<16, 16, 2>,  Plan<16, 16>> {};
-template <> struct Cfg<16, double>  : CfgBase<64, 64, 32, 256, Plan<4, 4>,      Plan<4, 4>,       Plan<8>>       {};
-template <> struct Cfg<32, double>  : CfgBase<32, 64, 16, 256, Plan<4, 2, 4>,   Plan<8, 4>,       Plan<4, 4>>    {};
-template <> struct Cfg<48, double>  : CfgBase<64, 64, 8, 256,  Plan<2, 12, 2>,  Plan<12, 4>,      Plan<8, 3>>    {};
-template <> struct Cfg<64, double>  : CfgBase<16, 32, 8, 256,  Plan<4, 4, 4>,   Plan<8, 8>,       Plan<8, 4>>    {};
-template <> struct Cfg<128, double> : CfgBase<16, 16, 4, 256,  Plan<4, 8, 4>,   Plan<8, 4, 4>,    Plan<8, 8>>    {};
+template <> struct Cfg<512, float> : CfgBase<8, 16, 2, 256,   Plan<8, 8, 8>,    Plan<16, 16, 2>,  Plan<16, 16>, 32, 6>  {};
+template <> struct Cfg<16, double>  : CfgBase<64, 64, 32, 256, Plan<4, 4>,      Plan<4, 4>,       Plan<8>>              {};
+template <> struct Cfg<32, double>  : CfgBase<32, 64, 16, 256, Plan<4, 2, 4>,   Plan<8, 4>,       Plan<4, 4>, 8, 0>     {};
+template <> struct Cfg<48, double>  : CfgBase<64, 64, 8, 256,  Plan<2, 12, 2>,  Plan<12, 4>,      Plan<8, 3>, 0, 10>    {};
+template <> struct Cfg<64, double>  : CfgBase<16, 32, 8, 256,  Plan<4, 4, 4>,   Plan<8, 8>,       Plan<8, 4>, 16, 0>    {};
+template <> struct Cfg<128, double> : CfgBase<16, 16, 4, 256,  Plan<4, 8, 4>,   Plan<8, 4, 4>,    Plan<8, 8>, 16, 2>    {};
 #ifdef DNSBOX_CFG256D
 template <> struct Cfg<256, double> : DNSBOX_CFG256D {};
 #else
-template <> struct Cfg<256, double> : CfgBase<8, 16, 2, 256,   Plan<8, 4, 8>,   Plan<8, 8, 4>,    Plan<8, 4, 4>> {};
+template <> struct Cfg<256, double> : CfgBase<8, 16, 2, 256,   Plan<8, 4, 8>,   Plan<8, 8, 4>,    Plan<8, 4, 4>, 16, 5> {};
 #endif
-template <> struct Cfg<512, double> : CfgBase<8, 8, 1, 256,    Plan<8, 8, 8>,   Plan<8, 8, 8>,    Plan<8, 8, 4>> {};
+template <> struct Cfg<512, double> : CfgBase<8, 8, 1, 256,    Plan<8, 8, 8>,   Plan<8, 8, 8>,    Plan<8, 8, 4>, 16, 0> {};
 
 template <int N, class T>
 struct Geo {
@@ -386,7 +390,10 @@ struct OpCopy {
 
 template <int N, class T, class Op>
 struct XGeo {
-    static constexpr int M = N / 2, H = M + 1, HP = H + Cfg<N, T>::XPAD;
+    static constexpr int M = N / 2, H = M + 1;
+    static constexpr int XPD = Cfg<N, T>::XPD;
+    static constexpr int HP = H + (XPD ? H / XPD + 1 : 0) + Cfg<N, T>::XPAD;
+    __device__ __forceinline__ static constexpr int px(int j) { return XPD ? j + j / XPD : j; }
     static constexpr int RB = Cfg<N, T>::RB;
     static constexpr int NTX = Cfg<N, T>::NTX;
     static constexpr int FMAX = Op::FI > Op::FO ? Op::FI : Op::FO;
@@ -425,7 +432,7 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
                 const int e = tid + q * NTX;
                 if (NL % NTX == 0 || e < NL) {
                     const int f = e / (RB * H), rk = e % (RB * H);
-                    lds[f * RB * HP + (rk / H) * HP + rk % H] = buf[q];
+                    lds[f * RB * HP + (rk / H) * HP + X::px(rk % H)] = buf[q];
                 }
             }
         }
@@ -441,7 +448,7 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
             if (NI % NTX == 0 || e < NI) {
                 const int fr = e / M, k = e % M;
                 const cpx<T>* row = lds + fr * HP;
-                cpx<T> A = row[k], B = conj(row[M - k]);
+                cpx<T> A = row[X::px(k)], B = conj(row[X::px(M - k)]);
                 if (k == 0) {
                     A.y = T(0);
                     B.y = T(0);
@@ -456,14 +463,14 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int e = tid + q * NTX;
-            if (NI % NTX == 0 || e < NI) lds[(e / M) * HP + e % M] = z[q];
+            if (NI % NTX == 0 || e < NI) lds[(e / M) * HP + X::px(e % M)] = z[q];
         }
         __syncthreads();
-        RowFFT<typename Cfg<N, T>::RP, FI * RB, HP, NTX, +1, T>::run(lds, tid, tw, N);
+        RowFFT<typename Cfg<N, T>::RP, FI * RB, HP, NTX, +1, T, X::XPD>::run(lds, tid, tw, N);
     } else {
         for (int e = tid; e < FI * RB * M; e += NTX) {
             const int f = e / (RB * M), rm = e % (RB * M);
-            lds[f * RB * HP + (rm / M) * HP + rm % M] =
+            lds[f * RB * HP + (rm / M) * HP + X::px(rm % M)] =
                 in0[f * in_fs + row0 * M + rm];
         }
         __syncthreads();
@@ -475,14 +482,14 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
         T vr[FI], vi[FI], orr[FO > 0 ? FO : 1], oi[FO > 0 ? FO : 1];
 #pragma unroll
         for (int f = 0; f < FI; ++f) {
-            cpx<T> v = lds[f * RB * HP + r * HP + m];
+            cpx<T> v = lds[f * RB * HP + r * HP + X::px(m)];
             vr[f] = v.x;
             vi[f] = v.y;
         }
         Op::apply(vr, orr, red);
         Op::apply(vi, oi, red);
 #pragma unroll
-        for (int f = 0; f < FO; ++f) lds[f * RB * HP + r * HP + m] = cpx<T>{orr[f], oi[f]};
+        for (int f = 0; f < FO; ++f) lds[f * RB * HP + r * HP + X::px(m)] = cpx<T>{orr[f], oi[f]};
     }
     if constexpr (FO == 0) return;
     __syncthreads();
@@ -490,14 +497,14 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
     // ---- r2c or real store ----
     if constexpr (FO == 0) {
     } else if constexpr (Op::OUT_SPEC) {
-        RowFFT<typename Cfg<N, T>::RP, FO * RB, HP, NTX, -1, T>::run(lds, tid, tw, N);
+        RowFFT<typename Cfg<N, T>::RP, FO * RB, HP, NTX, -1, T, X::XPD>::run(lds, tid, tw, N);
         // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]), k <= M
         for (int e = tid; e < FO * RB * H; e += NTX) {
             const int f = e / (RB * H), rk = e % (RB * H);
             const int r = rk / H, k = rk % H;
             const cpx<T>* row = lds + f * RB * HP + r * HP;
-            const cpx<T> zk = row[k == M ? 0 : k];
-            const cpx<T> zm = conj(row[k == 0 ? 0 : M - k]);
+            const cpx<T> zk = row[X::px(k == M ? 0 : k)];
+            const cpx<T> zm = conj(row[X::px(k == 0 ? 0 : M - k)]);
             const cpx<T> a = T(0.5) * (zk + zm);
             const cpx<T> b = (T(0.5) * (zk - zm)) * tw[k];
             out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
@@ -505,7 +512,7 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
     } else {
         for (int e = tid; e < FO * RB * M; e += NTX) {
             const int f = e / (RB * M), rm = e % (RB * M);
-            out0[f * out_fs + row0 * M + rm] = lds[f * RB * HP + (rm / M) * HP + rm % M];
+            out0[f * out_fs + row0 * M + rm] = lds[f * RB * HP + (rm / M) * HP + X::px(rm % M)];
         }
     }
 }

@@ -325,11 +325,14 @@ struct ColFFT {
 
 // ---------------------------------------------------------------------------
 // Row FFT: NB sequences of length L resident in LDS (sequence s at lds + s*BS,
-// unit stride), transformed in place by all NTH threads of the block.
+// element j at offset pad(j) = j + j/PD, or j when PD = 0: one gap every PD elements
+// breaks the power-of-two strides of the Stockham stages across LDS banks),
+// transformed in place by all NTH threads of the block.
 // ---------------------------------------------------------------------------
-template <class P, int NB, int BS, int NTH, int DIR, class T>
+template <class P, int NB, int BS, int NTH, int DIR, class T, int PD = 0>
 struct RowFFT {
     static constexpr int L = P::L;
+    __device__ __forceinline__ static constexpr int pad(int j) { return PD ? j + j / PD : j; }
 
     template <int t>
     __device__ __forceinline__ static void stage(cpx<T>* lds, int tid, const cpx<T>* __restrict__ tw,
@@ -347,7 +350,7 @@ struct RowFFT {
                 if (NBF % NTH == 0 || b < NBF) {
                     const int s = b / (L / R), i = b % (L / R);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) a[q * R + r] = lds[s * BS + i + r * (L / R)];
+                    for (int r = 0; r < R; ++r) a[q * R + r] = lds[s * BS + pad(i + r * (L / R))];
                 }
             }
             __syncthreads();
@@ -361,7 +364,7 @@ struct RowFFT {
                     Dft<R, DIR, T>::run(a + q * R);
                     const int base = (i - k) * R + k;
 #pragma unroll
-                    for (int r = 0; r < R; ++r) lds[s * BS + base + r * p] = a[q * R + r];
+                    for (int r = 0; r < R; ++r) lds[s * BS + pad(base + r * p)] = a[q * R + r];
                 }
             }
             __syncthreads();
