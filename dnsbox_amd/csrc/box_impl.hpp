@@ -40,6 +40,8 @@ struct BoxConfig {
     double dt = 1e-3;
 };
 
+constexpr int kNPass = 6;  // passes reported by time_passes (DNSBOX_PASS_* order)
+
 // Per-box device buffers and the operations the C-ABI exposes.
 class BoxBase {
    public:
@@ -205,16 +207,31 @@ class BoxT final : public BoxBase {
         DNSBOX_LAUNCH_CHECK();
     }
 
+    // pass markers of time_passes: an event after each launch of the step, tagged with
+    // its DNSBOX_PASS_* id (-1: not a pass); inactive (null) outside time_passes
+    std::vector<std::pair<int, hipEvent_t>>* marks = nullptr;
+    void mark(int pass)
+    {
+        if (!marks) return;
+        hipEvent_t e;
+        DNSBOX_HIP(hipEventCreate(&e));
+        marks->emplace_back(pass, e);
+        DNSBOX_HIP(hipEventRecord(e, stream));
+    }
+
     // middle passes of an RHS: y-inverse (6), x cross product, y-forward (3)
     void rhs_middle(bool track_cfl)
     {
         hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
+        mark(1);
         hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
                            stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
+        mark(2);
         hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
+        mark(3);
     }
 
     // five passes of one RHS with an unfused z-forward epilogue of MODE
@@ -259,6 +276,7 @@ class BoxT final : public BoxBase {
         if (!x6_valid) {
             eband(St());
             zinv_curl(St());
+            mark(0);
         }
         rhs_middle(true);
         EpiArgs<T, S> ea = epi();
@@ -269,6 +287,7 @@ class BoxT final : public BoxBase {
         hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
                            twc(), ea, pro);
         DNSBOX_LAUNCH_CHECK();
+        mark(4);
         eband(Pr());
         rhs_middle(false);
         EpiArgs<T, S> eb = epi();
@@ -278,9 +297,11 @@ class BoxT final : public BoxBase {
         hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
                            twc(), eb, pro);
         DNSBOX_LAUNCH_CHECK();
+        mark(5);
         eband(St());
         hipLaunchKernelGGL(k_step_end<0>, dim3(1), dim3(64), 0, stream, sc);
         DNSBOX_LAUNCH_CHECK();
+        mark(-1);
         x6_valid = true;
     }
 
@@ -564,56 +585,69 @@ class BoxT final : public BoxBase {
         x6_valid = false;
     }
 
+    // Average duration of each pass inside real IF-Heun steps, taken on a copy of the state
+    // (the box's state and clock are left as they were): an event after every launch of
+    // `reps` steps, each pass timed from the previous event.  The z-inverse of the first
+    // RHS runs once per pipeline start; it is timed over `reps` back-to-back launches.
     void time_passes(int reps, double* ms) override
     {
-        x6_valid = false;
-        // the fused kernels update a state: give them a scratch one
+        if (reps < 1) reps = 1;
         CS* tmp = nullptr;
         DNSBOX_HIP(hipMalloc(&tmp, 3 * size_t(fs) * sizeof(CS)));
-        DNSBOX_HIP(hipMemsetAsync(tmp, 0, 3 * size_t(fs) * sizeof(CS), stream));
-        hipEvent_t e0, e1;
-        DNSBOX_HIP(hipEventCreate(&e0));
-        DNSBOX_HIP(hipEventCreate(&e1));
-        auto timed = [&](int which, auto&& launch) {
-            launch();  // warm
-            DNSBOX_HIP(hipEventRecord(e0, stream));
-            for (int r = 0; r < reps; ++r) launch();
-            DNSBOX_HIP(hipEventRecord(e1, stream));
-            DNSBOX_HIP(hipEventSynchronize(e1));
-            float t = 0.f;
-            DNSBOX_HIP(hipEventElapsedTime(&t, e0, e1));
-            ms[which] = double(t) / reps;
-        };
-        const ProCurl<T> pro{inv_n3()};
-        timed(0, [&] { zinv_curl(static_cast<const CS*>(tmp)); });
-        timed(1, [&] {
-            hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
-        });
-        timed(2, [&] {
-            hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream,
-                               X(), fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
-        });
-        timed(3, [&] {
-            hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
-        });
-        EpiArgs<T, S> ea = epi();
-        ea.out = nullptr;
-        ea.state = tmp;
-        ea.rhs_in = tmp;
-        ea.rhs_in_s = 1;
-        timed(4, [&] {
-            hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                               twc(), ea, pro);
-        });
-        timed(5, [&] {
-            hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                               twc(), ea, pro);
-        });
-        DNSBOX_LAUNCH_CHECK();
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        BoxScalars saved{};
         DNSBOX_HIP(hipStreamSynchronize(stream));
+        DNSBOX_HIP(hipMemcpy(&saved, sc, sizeof(saved), hipMemcpyDeviceToHost));
+        DNSBOX_HIP(hipMemcpyAsync(tmp, state, 3 * size_t(fs) * sizeof(CS), hipMemcpyDeviceToDevice, stream));
+        void* own = state;
+        std::vector<std::pair<int, hipEvent_t>> mk;
+        double sum[kNPass] = {0};
+        int cnt[kNPass] = {0};
+        try {
+            state = tmp;
+            x6_valid = false;
+            heun_step();  // warm-up, starts the step pipeline
+            marks = &mk;
+            mark(-1);
+            for (int r = 0; r < reps; ++r) heun_step();
+            marks = nullptr;
+            mk.emplace_back(-1, nullptr);
+            DNSBOX_HIP(hipEventCreate(&mk.back().second));
+            // z-inverse + curl, back to back
+            DNSBOX_HIP(hipEventRecord(mk.back().second, stream));
+            for (int r = 0; r < reps; ++r) zinv_curl(static_cast<const CS*>(tmp));
+            hipEvent_t ez;
+            DNSBOX_HIP(hipEventCreate(&ez));
+            mk.emplace_back(-2, ez);
+            DNSBOX_HIP(hipEventRecord(ez, stream));
+            DNSBOX_HIP(hipEventSynchronize(ez));
+            for (size_t k = 1; k < mk.size(); ++k) {
+                float t = 0.f;
+                DNSBOX_HIP(hipEventElapsedTime(&t, mk[k - 1].second, mk[k].second));
+                const int p = mk[k].first;
+                if (p >= 0) {
+                    sum[p] += t;
+                    cnt[p] += 1;
+                } else if (p == -2) {
+                    sum[0] = t;
+                    cnt[0] = reps;
+                }
+            }
+        } catch (...) {
+            marks = nullptr;
+            state = own;
+            x6_valid = false;
+            for (auto& e : mk)
+                if (e.second) (void)hipEventDestroy(e.second);
+            (void)hipFree(tmp);
+            throw;
+        }
+        state = own;
+        x6_valid = false;
+        for (auto& e : mk)
+            if (e.second) (void)hipEventDestroy(e.second);
+        DNSBOX_HIP(hipMemcpy(sc, &saved, sizeof(saved), hipMemcpyHostToDevice));
         DNSBOX_HIP(hipFree(tmp));
+        for (int p = 0; p < kNPass; ++p) ms[p] = cnt[p] ? sum[p] / cnt[p] : 0.0;
     }
 
     void convert_in(void* dst, const double* src, long long n, bool state_type) override

@@ -124,8 +124,10 @@ int dnsbox_get_time(dnsbox_ctx* ctx, double* t, double* dt);
 int dnsbox_set_time(dnsbox_ctx* ctx, double t, double dt);
 
 /* measurement: average duration (ms, HIP events on the context stream) of each kernel pass
-   of the step over `reps` back-to-back launches, in the order of DNSBOX_PASS_*.  No reference
-   counterpart (the reference has no timers). */
+   of the step, in the order of DNSBOX_PASS_*: the passes of `reps` IF-Heun steps taken on a
+   copy of the state (an event after every launch), and DNSBOX_PASS_ZINV (run once per pipeline
+   start) over `reps` back-to-back launches.  State, time and dt of the context are unchanged.
+   No reference counterpart (the reference has no timers). */
 enum {
     DNSBOX_PASS_ZINV = 0,  /* z-inverse FFT + spectral curl: reads 3 fields, writes 6     */
     DNSBOX_PASS_YINV,      /* y-inverse FFT of 6 fields in place: reads 6, writes 6       */

@@ -239,3 +239,26 @@ def test_fp32_state64_closer_to_fp64(gpu_lib):
     print(f"state error vs fp64 after 200 steps: fp32 state {e32:.2e}, fp64 state {e64s:.2e}", flush=True)
     assert e64s < e32
     assert e64s < 1e-5
+
+
+def test_time_passes_leaves_box_unchanged(gpu_lib):
+    """The in-step pass timer works on a copy: state, time and dt are untouched, and every
+    pass gets a positive duration."""
+    from dnsbox_amd import Box
+    with Box(n=32, precision=64, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=1,
+             cfl_min=0.5, cfl_max=0.5, dt_max=0.01, dt=0.004) as b:
+        b.init_random(5, 0, 4.0, 0.5)
+        b.step(3)
+        before = b.get_state()
+        t0 = b.time()
+        ms = b.time_passes(4)
+        after = b.get_state()
+        assert b.time() == t0
+        assert all(np.array_equal(x, y) for x, y in zip(before, after))
+        assert all(v > 0 for v in ms.values()), ms
+        b.step(2)
+        with Box(n=32, precision=64, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=1,
+                 cfl_min=0.5, cfl_max=0.5, dt_max=0.01, dt=0.004) as ref:
+            ref.init_random(5, 0, 4.0, 0.5)
+            ref.step(5)
+            assert all(np.array_equal(x, y) for x, y in zip(b.get_state(), ref.get_state()))
