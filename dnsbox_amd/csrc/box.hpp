@@ -400,121 +400,151 @@ struct XGeo {
     static constexpr int LDS = FMAX * RB * HP;
 };
 
+// The x pass of one block of RB x-lines in three phases:
+//   load    : global -> registers (all loads of the block issued together)
+//   stage   : registers -> padded LDS rows
+//   compute : c2r pre-processing, row FFTs, pointwise op, row FFTs, r2c post-processing
+//             and the global stores
+template <int N, class T, class Op>
+struct XPass {
+    using X = XGeo<N, T, Op>;
+    static constexpr int M = X::M, H = X::H, HP = X::HP, RB = X::RB, NTX = X::NTX;
+    static constexpr int FI = Op::FI, FO = Op::FO;
+    static constexpr int NL = FI * RB * H;
+    static constexpr int QL = (NL + NTX - 1) / NTX;
+
+    __device__ __forceinline__ static void load(const cpx<T>* __restrict__ in0, long long in_fs, int blk,
+                                                cpx<T> (&buf)[QL])
+    {
+        const int tid = threadIdx.x;
+        const long long row0 = (long long)blk * RB;
+#pragma unroll
+        for (int q = 0; q < QL; ++q) {
+            // unconditional loads (the tail re-reads element 0 of the block): a
+            // conditionally written buf is kept in scratch memory by the compiler
+            const int e0 = tid + q * NTX;
+            const int e = (NL % NTX == 0 || e0 < NL) ? e0 : 0;
+            const int f = e / (RB * H), rk = e % (RB * H);
+            buf[q] = in0[f * in_fs + row0 * H + rk];
+        }
+    }
+
+    __device__ __forceinline__ static void stage(const cpx<T> (&buf)[QL], cpx<T>* lds)
+    {
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < QL; ++q) {
+            const int e = tid + q * NTX;
+            if (NL % NTX == 0 || e < NL) {
+                const int f = e / (RB * H), rk = e % (RB * H);
+                lds[f * RB * HP + (rk / H) * HP + X::px(rk % H)] = buf[q];
+            }
+        }
+    }
+
+    // on entry the LDS holds the staged input rows (after a barrier); leaves the LDS in use
+    // until its last read (the caller puts a barrier before the next stage)
+    __device__ __forceinline__ static void compute(cpx<T>* __restrict__ out0, long long out_fs, int blk,
+                                                   cpx<T>* lds, const cpx<T>* __restrict__ tw, XRed& red)
+    {
+        const int tid = threadIdx.x;
+        const long long row0 = (long long)blk * RB;
+        if constexpr (Op::IN_SPEC) {
+            // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),  k < M;
+            // imaginary parts of X[0], X[M] are ignored (c2r convention).
+            constexpr int NI = FI * RB * M;
+            constexpr int Q = (NI + NTX - 1) / NTX;
+            cpx<T> z[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int e = tid + q * NTX;
+                if (NI % NTX == 0 || e < NI) {
+                    const int fr = e / M, k = e % M;
+                    const cpx<T>* row = lds + fr * HP;
+                    cpx<T> A = row[X::px(k)], B = conj(row[X::px(M - k)]);
+                    if (k == 0) {
+                        A.y = T(0);
+                        B.y = T(0);
+                    }
+                    cpx<T> w = tw[k];
+                    w.y = -w.y;
+                    cpx<T> fe = A + B, fo = (A - B) * w;
+                    z[q] = fe + mul_i<1>(fo);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int e = tid + q * NTX;
+                if (NI % NTX == 0 || e < NI) lds[(e / M) * HP + X::px(e % M)] = z[q];
+            }
+            __syncthreads();
+            RowFFT<typename Cfg<N, T>::RP, FI * RB, HP, NTX, +1, T, X::XPD>::run(lds, tid, tw, N);
+        }
+
+        // ---- pointwise op on the physical samples x[2m] = Re z[m], x[2m+1] = Im z[m] ----
+        for (int e = tid; e < RB * M; e += NTX) {
+            const int r = e / M, m = e % M;
+            T vr[FI], vi[FI], orr[FO > 0 ? FO : 1], oi[FO > 0 ? FO : 1];
+#pragma unroll
+            for (int f = 0; f < FI; ++f) {
+                cpx<T> v = lds[f * RB * HP + r * HP + X::px(m)];
+                vr[f] = v.x;
+                vi[f] = v.y;
+            }
+            Op::apply(vr, orr, red);
+            Op::apply(vi, oi, red);
+#pragma unroll
+            for (int f = 0; f < FO; ++f) lds[f * RB * HP + r * HP + X::px(m)] = cpx<T>{orr[f], oi[f]};
+        }
+        if constexpr (FO == 0) return;
+        __syncthreads();
+
+        // ---- r2c or real store ----
+        if constexpr (FO == 0) {
+        } else if constexpr (Op::OUT_SPEC) {
+            RowFFT<typename Cfg<N, T>::RP, FO * RB, HP, NTX, -1, T, X::XPD>::run(lds, tid, tw, N);
+            // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]), k <= M
+            for (int e = tid; e < FO * RB * H; e += NTX) {
+                const int f = e / (RB * H), rk = e % (RB * H);
+                const int r = rk / H, k = rk % H;
+                const cpx<T>* row = lds + f * RB * HP + r * HP;
+                const cpx<T> zk = row[X::px(k == M ? 0 : k)];
+                const cpx<T> zm = conj(row[X::px(k == 0 ? 0 : M - k)]);
+                const cpx<T> a = T(0.5) * (zk + zm);
+                const cpx<T> b = (T(0.5) * (zk - zm)) * tw[k];
+                out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
+            }
+        } else {
+            for (int e = tid; e < FO * RB * M; e += NTX) {
+                const int f = e / (RB * M), rm = e % (RB * M);
+                out0[f * out_fs + row0 * M + rm] = lds[f * RB * HP + (rm / M) * HP + X::px(rm % M)];
+            }
+        }
+    }
+};
+
 template <int N, class T, class Op>
 __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long long in_fs,
                                            cpx<T>* __restrict__ out0, long long out_fs, int blk, cpx<T>* lds,
                                            const cpx<T>* __restrict__ tw, XRed& red)
 {
-    using X = XGeo<N, T, Op>;
-    constexpr int M = X::M, H = X::H, HP = X::HP, RB = X::RB, NTX = X::NTX;
-    constexpr int FI = Op::FI, FO = Op::FO;
-    const int tid = threadIdx.x;
-    const long long row0 = (long long)blk * RB;
-
-    // ---- load (+ c2r pre-processing) ----
+    using XP = XPass<N, T, Op>;
+    using X = typename XP::X;
     if constexpr (Op::IN_SPEC) {
-        {
-            // all loads of the block issued before the first LDS store (memory-level parallelism)
-            constexpr int NL = FI * RB * H;
-            constexpr int QL = (NL + NTX - 1) / NTX;
-            cpx<T> buf[QL];
-#pragma unroll
-            for (int q = 0; q < QL; ++q) {
-                // unconditional loads (the tail re-reads element 0 of the block): a
-                // conditionally written buf is kept in scratch memory by the compiler
-                const int e0 = tid + q * NTX;
-                const int e = (NL % NTX == 0 || e0 < NL) ? e0 : 0;
-                const int f = e / (RB * H), rk = e % (RB * H);
-                buf[q] = in0[f * in_fs + row0 * H + rk];
-            }
-#pragma unroll
-            for (int q = 0; q < QL; ++q) {
-                const int e = tid + q * NTX;
-                if (NL % NTX == 0 || e < NL) {
-                    const int f = e / (RB * H), rk = e % (RB * H);
-                    lds[f * RB * HP + (rk / H) * HP + X::px(rk % H)] = buf[q];
-                }
-            }
-        }
-        __syncthreads();
-        // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),  k < M;
-        // imaginary parts of X[0], X[M] are ignored (c2r convention).
-        constexpr int NI = FI * RB * M;
-        constexpr int Q = (NI + NTX - 1) / NTX;
-        cpx<T> z[Q];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int e = tid + q * NTX;
-            if (NI % NTX == 0 || e < NI) {
-                const int fr = e / M, k = e % M;
-                const cpx<T>* row = lds + fr * HP;
-                cpx<T> A = row[X::px(k)], B = conj(row[X::px(M - k)]);
-                if (k == 0) {
-                    A.y = T(0);
-                    B.y = T(0);
-                }
-                cpx<T> w = tw[k];
-                w.y = -w.y;
-                cpx<T> fe = A + B, fo = (A - B) * w;
-                z[q] = fe + mul_i<1>(fo);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int e = tid + q * NTX;
-            if (NI % NTX == 0 || e < NI) lds[(e / M) * HP + X::px(e % M)] = z[q];
-        }
-        __syncthreads();
-        RowFFT<typename Cfg<N, T>::RP, FI * RB, HP, NTX, +1, T, X::XPD>::run(lds, tid, tw, N);
+        cpx<T> buf[XP::QL];
+        XP::load(in0, in_fs, blk, buf);
+        XP::stage(buf, lds);
     } else {
-        for (int e = tid; e < FI * RB * M; e += NTX) {
+        constexpr int M = X::M, HP = X::HP, RB = X::RB, NTX = X::NTX;
+        const long long row0 = (long long)blk * RB;
+        for (int e = threadIdx.x; e < Op::FI * RB * M; e += NTX) {
             const int f = e / (RB * M), rm = e % (RB * M);
-            lds[f * RB * HP + (rm / M) * HP + X::px(rm % M)] =
-                in0[f * in_fs + row0 * M + rm];
+            lds[f * RB * HP + (rm / M) * HP + X::px(rm % M)] = in0[f * in_fs + row0 * M + rm];
         }
-        __syncthreads();
     }
-
-    // ---- pointwise op on the physical samples x[2m] = Re z[m], x[2m+1] = Im z[m] ----
-    for (int e = tid; e < RB * M; e += NTX) {
-        const int r = e / M, m = e % M;
-        T vr[FI], vi[FI], orr[FO > 0 ? FO : 1], oi[FO > 0 ? FO : 1];
-#pragma unroll
-        for (int f = 0; f < FI; ++f) {
-            cpx<T> v = lds[f * RB * HP + r * HP + X::px(m)];
-            vr[f] = v.x;
-            vi[f] = v.y;
-        }
-        Op::apply(vr, orr, red);
-        Op::apply(vi, oi, red);
-#pragma unroll
-        for (int f = 0; f < FO; ++f) lds[f * RB * HP + r * HP + X::px(m)] = cpx<T>{orr[f], oi[f]};
-    }
-    if constexpr (FO == 0) return;
     __syncthreads();
-
-    // ---- r2c or real store ----
-    if constexpr (FO == 0) {
-    } else if constexpr (Op::OUT_SPEC) {
-        RowFFT<typename Cfg<N, T>::RP, FO * RB, HP, NTX, -1, T, X::XPD>::run(lds, tid, tw, N);
-        // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]), k <= M
-        for (int e = tid; e < FO * RB * H; e += NTX) {
-            const int f = e / (RB * H), rk = e % (RB * H);
-            const int r = rk / H, k = rk % H;
-            const cpx<T>* row = lds + f * RB * HP + r * HP;
-            const cpx<T> zk = row[X::px(k == M ? 0 : k)];
-            const cpx<T> zm = conj(row[X::px(k == 0 ? 0 : M - k)]);
-            const cpx<T> a = T(0.5) * (zk + zm);
-            const cpx<T> b = (T(0.5) * (zk - zm)) * tw[k];
-            out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
-        }
-    } else {
-        for (int e = tid; e < FO * RB * M; e += NTX) {
-            const int f = e / (RB * M), rm = e % (RB * M);
-            out0[f * out_fs + row0 * M + rm] = lds[f * RB * HP + (rm / M) * HP + X::px(rm % M)];
-        }
-    }
+    XP::compute(out0, out_fs, blk, lds, tw, red);
 }
 
 template <int NRED>
