@@ -18,6 +18,8 @@
 //           (IF-Heun predictor / corrector, IF-AB2) in the epilogue
 #pragma once
 
+#include <type_traits>
+
 #include "fft_core.hpp"
 
 #ifndef DNSBOX_Z_PREFETCH
@@ -134,31 +136,22 @@ struct CfgBase {
     static constexpr int XPD = XPD_;
     static constexpr int XPAD = XPAD_;
 };
+// CfgDefault<N, T>: the table chosen by the autotuner (tuned_cfg.hpp).  A tuning build
+// (-DDNSBOX_TUNE_N=..., -DDNSBOX_TUNE_T=float|double, -DDNSBOX_TUNE_CFG=CfgBase<...>)
+// replaces the entry of one size and precision with a candidate.
 template <int N, class T>
-struct Cfg;
-//                                        ZW  YW  RB  NTX  z plan              y plan            x (row) plan   XPD XPAD
-template <> struct Cfg<16, float>  : CfgBase<64, 64, 64, 256, Plan<4, 4>,       Plan<4, 4>,       Plan<8>>              {};
-template <> struct Cfg<32, float>  : CfgBase<32, 64, 32, 256, Plan<4, 2, 4>,    Plan<8, 4>,       Plan<16>>             {};
-template <> struct Cfg<48, float>  : CfgBase<64, 64, 16, 256, Plan<2, 12, 2>,   Plan<12, 4>,      Plan<8, 3>>           {};
-template <> struct Cfg<64, float>  : CfgBase<32, 32, 16, 256, Plan<8, 8>,       Plan<8, 8>,       Plan<8, 4>, 0, 4>     {};
-template <> struct Cfg<128, float> : CfgBase<16, 32, 8, 256,  Plan<4, 8, 4>,    Plan<16, 8>,      Plan<8, 8>, 16, 2>    {};
-#ifdef DNSBOX_CFG256F
-template <> struct Cfg<256, float> : DNSBOX_CFG256F {};
+struct CfgDefault;
+}  // namespace dnsbox
+#include "tuned_cfg.hpp"
+namespace dnsbox {
+#ifdef DNSBOX_TUNE_CFG
+template <int N, class T>
+struct Cfg : std::conditional_t<(N == DNSBOX_TUNE_N && std::is_same<T, DNSBOX_TUNE_T>::value), DNSBOX_TUNE_CFG,
+                                CfgDefault<N, T>> {};
 #else
-template <> struct Cfg<256, float> : CfgBase<16, 64, 4, 256,  Plan<8, 4, 8>,    Plan<16, 16>,     Plan<16, 8>, 32, 2>   {};
+template <int N, class T>
+struct Cfg : CfgDefault<N, T> {};
 #endif
-template <> struct Cfg<512, float> : CfgBase<8, 16, 2, 256,   Plan<8, 8, 8>,    Plan<16, 16, 2>,  Plan<16, 16>, 32, 6>  {};
-template <> struct Cfg<16, double>  : CfgBase<64, 64, 32, 256, Plan<4, 4>,      Plan<4, 4>,       Plan<8>>              {};
-template <> struct Cfg<32, double>  : CfgBase<32, 64, 16, 256, Plan<4, 2, 4>,   Plan<8, 4>,       Plan<4, 4>, 8, 0>     {};
-template <> struct Cfg<48, double>  : CfgBase<64, 64, 8, 256,  Plan<2, 12, 2>,  Plan<12, 4>,      Plan<8, 3>, 0, 10>    {};
-template <> struct Cfg<64, double>  : CfgBase<16, 32, 8, 256,  Plan<4, 4, 4>,   Plan<8, 8>,       Plan<8, 4>, 16, 0>    {};
-template <> struct Cfg<128, double> : CfgBase<16, 16, 4, 256,  Plan<4, 8, 4>,   Plan<8, 4, 4>,    Plan<8, 8>, 16, 2>    {};
-#ifdef DNSBOX_CFG256D
-template <> struct Cfg<256, double> : DNSBOX_CFG256D {};
-#else
-template <> struct Cfg<256, double> : CfgBase<8, 16, 2, 256,   Plan<8, 4, 8>,   Plan<8, 8, 4>,    Plan<8, 4, 4>, 16, 5> {};
-#endif
-template <> struct Cfg<512, double> : CfgBase<8, 8, 1, 256,    Plan<8, 8, 8>,   Plan<8, 8, 8>,    Plan<8, 8, 4>, 16, 0> {};
 
 template <int N, class T>
 struct Geo {

@@ -11,6 +11,7 @@
 
 #include "../../include/dnsbox.h"
 #include "box_impl.hpp"
+#include "config.hpp"
 #include "group.hpp"
 
 using dnsbox::BoxBase;
@@ -145,34 +146,6 @@ void invalidate(dnsbox_ctx* c)
     if (c->group) c->group->g->invalidate();
 }
 
-dnsbox::BoxConfig make_config(const dnsbox_params* p)
-{
-    BoxConfig bc;
-    bc.n = p->n;
-    bc.precision = p->precision;
-    bc.state_precision = p->state_precision == 64 ? 64 : p->precision;
-    bc.deltak = p->deltak > 0 ? p->deltak : 1.0;
-    bc.dt = p->dt;
-    dnsbox::BoxPhys& ph = bc.ph;
-    ph.nu = p->nu;
-    ph.Q = p->Q;
-    ph.Pin = p->Pin;
-    ph.kcut = p->kcut;
-    ph.kolmo_amp = p->kolmo_amp;
-    ph.mask_thr = (double(p->n) / 2.0) * (2.0 * 1.0 / 3.0);
-    const double ts = (double(p->n) / 2.0) * 0.5 * 1.0;
-    ph.sph_thr2 = ts * ts;
-    ph.cmin = p->cfl_min;
-    ph.cmax = p->cfl_max;
-    ph.dtmax = p->dt_max;
-    ph.kolmo_k = p->kolmo_amp != 0.0 ? p->kolmo_k : 0;
-    ph.bandlim = p->bandlim;
-    ph.spherical = p->spherical;
-    ph.parity = p->parity_mode;
-    ph.adaptive = p->adaptive_dt;
-    return bc;
-}
-
 int check_params(const dnsbox_params* p)
 {
     if (!supported_n(p->n))
@@ -230,7 +203,7 @@ int dnsbox_create(const dnsbox_params* p, dnsbox_ctx** out)
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
             return fail(DNSBOX_E_HIP, "no HIP device available");
-        BoxConfig bc = make_config(p);
+        BoxConfig bc = dnsbox::make_config(p);
         auto c = std::make_unique<dnsbox_ctx>();
         c->p = *p;
         (void)hipGetDevice(&c->device);
@@ -593,7 +566,7 @@ int dnsbox_group_create(const dnsbox_params* params, int nbox, dnsbox_group** ou
             return fail(DNSBOX_E_HIP, "no HIP device available");
         std::vector<std::unique_ptr<BoxBase>> boxes;
         for (int i = 0; i < nbox; ++i) {
-            BoxConfig bc = make_config(params + i);
+            BoxConfig bc = dnsbox::make_config(params + i);
             boxes.emplace_back(prec == 64 ? dnsbox::make_box_f64(bc) : dnsbox::make_box_f32(bc));
         }
         auto grp = std::make_unique<dnsbox_group>();
