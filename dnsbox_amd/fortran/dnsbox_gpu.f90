@@ -110,6 +110,32 @@ module dnsbox_gpu
         type(c_ptr) function dnsbox_last_error_c() bind(C, name="dnsbox_last_error")
             import :: c_ptr
         end function
+        ! groups: many boxes advanced by one launch per pass (no reference counterpart)
+        integer(c_int) function dnsbox_group_create(p, nbox, grp) bind(C, name="dnsbox_group_create")
+            import :: c_int, c_ptr, dnsbox_params
+            type(dnsbox_params), intent(in) :: p(*)
+            integer(c_int), value :: nbox
+            type(c_ptr), intent(out) :: grp
+        end function
+        integer(c_int) function dnsbox_group_destroy(grp) bind(C, name="dnsbox_group_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: grp
+        end function
+        integer(c_int) function dnsbox_group_box(grp, i, ctx) bind(C, name="dnsbox_group_box")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: grp
+            integer(c_int), value :: i
+            type(c_ptr), intent(out) :: ctx
+        end function
+        integer(c_int) function dnsbox_group_step(grp, nsteps) bind(C, name="dnsbox_group_step")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: grp
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function dnsbox_group_sync(grp) bind(C, name="dnsbox_group_sync")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: grp
+        end function
     end interface
 
 contains

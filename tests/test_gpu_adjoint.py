@@ -132,7 +132,7 @@ def test_tstep_gradient_vs_finite_differences(gpu_lib, n, T, k):
     dt = 0.005
     nu = 0.02
     o = Oracle(n, nu=nu, kolmo_amp=1.0, kolmo_k=4)
-    ndir = 8 if n == 32 else 2
+    ndir = 8
     with make(n, nu=nu, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
         u0 = dev(o.init_solenoidal(11, 0, 4.0, 0.5))
         x = u0.clone().requires_grad_(True)
