@@ -390,9 +390,10 @@ int dnsbox_step(dnsbox_ctx* c, int nsteps, int stepper)
     if (c->group) c->group->g->invalidate();
     return guarded([&]() -> int {
         BoxBase* b = c->box;
-        for (int i = 0; i < nsteps; ++i) {
-            if (stepper == 1) b->heun_step();
-            else b->ab2_step();
+        if (stepper == 1) {
+            b->heun_steps(nsteps);
+        } else {
+            for (int i = 0; i < nsteps; ++i) b->ab2_step();
         }
         return 0;
     });

@@ -10,6 +10,7 @@
 // captured once in a hipGraph and replayed.
 #pragma once
 
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -314,7 +315,9 @@ class Group final : public GroupBase {
     int zt = 0, yt = 0, xb = 0;
     bool x6_valid = false;
     hipStream_t gstream = nullptr;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;   // graph_steps steps of every box
+    int graph_mode = -1;
+    int graph_steps = 16;
     // descriptor sets of the device-buffer entry points, keyed by their bytes: the
     // pointers a caller cycles through (checkpoint segments, ping-pong cotangents) are
     // uploaded once and reused
@@ -432,9 +435,10 @@ class Group final : public GroupBase {
         DNSBOX_LAUNCH_CHECK();
     }
 
-    // IF-Heun steps of every box: the first z-inverse if needed, then the captured
-    // step replayed.  Work runs on the group's own stream, ordered after the
-    // caller's stream and before anything the caller enqueues next.
+    // IF-Heun steps of every box: the first z-inverse if needed, then graph_steps steps
+    // captured once in a hipGraph and replayed, the remainder launched directly
+    // (DNSBOX_STEP_GRAPH=0 launches every step directly).  Work runs on the group's own
+    // stream, ordered after the caller's stream and before anything it enqueues next.
     void step(int nsteps) override
     {
         if (nsteps <= 0) return;
@@ -447,15 +451,24 @@ class Group final : public GroupBase {
             launch_zinv(gstream, d_gb, 0);
             x6_valid = true;
         }
-        if (!graph) {
+        if (graph_mode < 0) {
+            const char* e = std::getenv("DNSBOX_STEP_GRAPH");
+            graph_mode = (e && std::strcmp(e, "0") == 0) ? 0 : 1;
+            const char* k = std::getenv("DNSBOX_GRAPH_STEPS");
+            if (k && std::atoi(k) > 0) graph_steps = std::atoi(k);
+        }
+        if (graph_mode && nsteps >= graph_steps && !graph) {
             hipGraph_t g;
             DNSBOX_HIP(hipStreamBeginCapture(gstream, hipStreamCaptureModeThreadLocal));
-            one_step(gstream, d_gb);
+            for (int i = 0; i < graph_steps; ++i) one_step(gstream, d_gb);
             DNSBOX_HIP(hipStreamEndCapture(gstream, &g));
             DNSBOX_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
             DNSBOX_HIP(hipGraphDestroy(g));
         }
-        for (int i = 0; i < nsteps; ++i) DNSBOX_HIP(hipGraphLaunch(graph, gstream));
+        int done = 0;
+        if (graph_mode)
+            for (; done + graph_steps <= nsteps; done += graph_steps) DNSBOX_HIP(hipGraphLaunch(graph, gstream));
+        for (; done < nsteps; ++done) one_step(gstream, d_gb);
         DNSBOX_HIP(hipEventRecord(ev, gstream));
         DNSBOX_HIP(hipStreamWaitEvent(stream, ev, 0));
         DNSBOX_HIP(hipEventDestroy(ev));

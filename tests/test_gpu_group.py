@@ -42,3 +42,39 @@ def test_group_matches_single_boxes(gpu_lib, prec):
         print(f"group box {i} n={kw['n']} prec={prec} rel={err:.2e} t={times[i][0]:.6f}", flush=True)
         assert err == 0.0 or err < 1e-14
         assert times[i] == t
+
+
+def test_group_graph_replay_matches_direct_launches(gpu_lib):
+    """Steps replayed from the group's hipGraph (16 steps per graph) equal directly launched
+    ones bit for bit, across replays with per-box work in between."""
+    import os
+    from dnsbox_amd import BoxGroup
+    spec = [(32, 0.02), (48, 0.03), (64, 0.015), (16, 0.05)]
+    params = box_params(32, spec)
+
+    def run(env):
+        old = os.environ.get("DNSBOX_STEP_GRAPH")
+        if env is not None:
+            os.environ["DNSBOX_STEP_GRAPH"] = env
+        try:
+            g = BoxGroup(params)
+            for i, b in enumerate(g.boxes):
+                b.init_random(300 + i, i, 3.0, 0.5)
+            g.step(40)
+        finally:
+            if env is not None:
+                if old is None:
+                    del os.environ["DNSBOX_STEP_GRAPH"]
+                else:
+                    os.environ["DNSBOX_STEP_GRAPH"] = old
+        d = g.boxes[2].diagnostics()[0]
+        g.step(33)
+        g.sync()
+        out = [b.get_state() for b in g.boxes], [b.time() for b in g.boxes], d
+        g.close()
+        return out
+
+    (s0, t0, d0), (s1, t1, d1) = run("0"), run(None)
+    assert t0 == t1 and np.array_equal(d0, d1)
+    for a, b in zip(s0, s1):
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))

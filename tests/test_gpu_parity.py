@@ -262,3 +262,40 @@ def test_time_passes_leaves_box_unchanged(gpu_lib):
             ref.init_random(5, 0, 4.0, 0.5)
             ref.step(5)
             assert all(np.array_equal(x, y) for x, y in zip(b.get_state(), ref.get_state()))
+
+
+@pytest.mark.parametrize("prec,sp", [(64, 0), (32, 64)])
+def test_graph_replay_matches_eager_steps(gpu_lib, prec, sp):
+    """The hipGraph-replayed step equals the eager step bit for bit, over several replays
+    with eager work (diagnostics, a state round trip) and synchronisations in between."""
+    import os
+    from dnsbox_amd import Box
+    kw = dict(n=32, precision=prec, state_precision=sp, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0,
+              adaptive_dt=1, cfl_min=0.5, cfl_max=0.5, dt_max=0.01, dt=0.004)
+    eager = Box(**kw)
+    eager.init_random(9, 0, 3.0, 0.5)
+    eager.step(1)
+    os.environ["DNSBOX_STEP_GRAPH"] = "1"      # read at the box's first step
+    try:
+        graph = Box(**kw)
+        graph.init_random(9, 0, 3.0, 0.5)
+        graph.step(1)
+    finally:
+        del os.environ["DNSBOX_STEP_GRAPH"]
+    for chunk in (16, 3, 40, 17):   # 16 steps per graph: replays, remainders, eager chunks
+        eager.step(chunk)
+        graph.step(chunk)
+        d_e, s_e = eager.diagnostics()
+        d_g, s_g = graph.diagnostics()
+        # shell sums are accumulated with LDS atomics (order not fixed): equal to rounding
+        assert np.array_equal(d_e, d_g) and np.allclose(s_e, s_g, rtol=1e-13, atol=0)
+        a, b = eager.get_state(), graph.get_state()
+        assert all(np.array_equal(x, y) for x, y in zip(a, b)), chunk
+        assert eager.time() == graph.time()
+    graph.set_state(*graph.get_state())     # restarts the pipeline: the next step runs eagerly first
+    eager.set_state(*eager.get_state())
+    eager.step(33)
+    graph.step(33)
+    assert all(np.array_equal(x, y) for x, y in zip(eager.get_state(), graph.get_state()))
+    eager.close()
+    graph.close()
