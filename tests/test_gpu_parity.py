@@ -299,3 +299,35 @@ def test_graph_replay_matches_eager_steps(gpu_lib, prec, sp):
     assert all(np.array_equal(x, y) for x, y in zip(eager.get_state(), graph.get_state()))
     eager.close()
     graph.close()
+
+
+def test_128_fp64_adaptive_forced_box_matches_oracle(gpu_lib):
+    """128^3 FP64 forced box, nu = 0.01, f = (sin 4y, 0, 0), CFL 0.5 controller: state, time, dt,
+    energy, dissipation, production, Courant number and the shell spectrum against the oracle at
+    checkpoints every 10 steps (20 steps: the CPU oracle takes ~1 s per step at this size)."""
+    n = 128
+    o = Oracle(n, nu=0.01, kolmo_amp=1.0, kolmo_k=4, nthreads=16)
+    ref = o.init_solenoidal(0, 0, 4.0, 0.5)
+    dt_ref, t_ref = 0.002, 0.0
+    with make(n, nu=0.01, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=1, cfl_min=0.5,
+              cfl_max=0.5, dt_max=0.01, dt=0.002) as b:
+        b.set_state(*ref)
+        for chk in (10, 20):
+            ref, dt_ref, t_ref = o.heun_adaptive(ref, 10, dt_ref, 0.5, 0.5, 0.01, t_ref)
+            b.step(10)
+            got = b.get_state()
+            t, dt = b.time()
+            err = rel_l2(got, ref)
+            od, os_ = o.diagnostics(ref, dt_ref)
+            gd, gs = b.diagnostics()
+            # shells past the dealiasing cutoff hold only rounding-level energy: compare them
+            # against the largest shell
+            floor = 1e-13 * os_.max()
+            shell = np.max(np.abs(gs - os_) / np.maximum(os_, floor))
+            print(f"128^3 step {chk}: rel_l2={err:.2e} t={t:.12f} ({t_ref:.12f}) dt={dt:.3e} "
+                  f"Ekin={gd[0]:.12e} ({od[0]:.12e}) max shell err={shell:.2e}", flush=True)
+            assert err < 1e-10
+            assert abs(t - t_ref) < 1e-12 and abs(dt - dt_ref) < 1e-13
+            for i in (0, 1, 2, 3):   # Ekin, Disp, Prod, Courant
+                assert abs(gd[i] - od[i]) <= 1e-10 * max(abs(od[i]), 1e-300) + 1e-14, (i, gd[i], od[i])
+            np.testing.assert_allclose(gs, os_, rtol=1e-10, atol=floor)
