@@ -111,7 +111,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     cpx<T>* out[Pro::FO];
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
-    zinv_tile<N, T, Pro>(in.p, out, blockIdx.x, lds, tw, pro);
+    zinv_tile<N, T, Pro>(in.p, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
 }
 
 // ---------------------------------------------------------------------------
@@ -259,7 +259,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     __shared__ cpx<T> lds[N * G::C::ZW];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    zfwd_vjp_tile<N, T, MODE>(X, xfs, blockIdx.x, lds, tw, va);
+    zfwd_vjp_tile<N, T, MODE>(X, xfs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, va);
 }
 
 // [Eband(u), lam_scale * sum_{|k| <= kcut, unmasked} Re(conj(lamN) u)]  (one block)
@@ -335,7 +335,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     __shared__ cpx<T> lds[N * W];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    const int col = blockIdx.x * W + threadIdx.x % W;
+    const int col = xcd_contiguous(blockIdx.x, gridDim.x) * W + threadIdx.x % W;
     const int g = threadIdx.x / W;
     const bool valid = col < G::ZCOLS;
     cpx<T> O[3][RM];

@@ -33,7 +33,8 @@ namespace dnsbox {
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Give
 // each XCD a contiguous range of logical tiles, so that tiles whose rows share cache
-// lines (the y pass reads 8-byte-aligned segments of x-lines) meet in one L2.
+// lines (the y pass reads 8-byte-aligned segments of x-lines; a z tile narrower than a
+// cache line shares its lines with its neighbours) meet in one L2.
 // b: linear block id in [0, nb); returns the logical tile id, a bijection on [0, nb).
 __device__ __forceinline__ int xcd_contiguous(int b, int nb)
 {
@@ -293,7 +294,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     for (int f = 0; f < Pro::FI; ++f) in[f] = in0 + f * in_fs;
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
-    zinv_tile<N, T, Pro, TI>(in, out, blockIdx.x, lds, tw, pro);
+    zinv_tile<N, T, Pro, TI>(in, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
 }
 
 // ===========================================================================
@@ -803,7 +804,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     __shared__ cpx<T> lds[N * G::C::ZW];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    zfwd_tile<N, T, MODE, S>(in0, in_fs, blockIdx.x, lds, tw, ea);
+    zfwd_tile<N, T, MODE, S>(in0, in_fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea);
 }
 
 // Forward z pass + stepper epilogue (PRED or CORR) fused with the z-inverse + curl
@@ -856,7 +857,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     __shared__ cpx<T> lds[N * G::C::ZW];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    zfwd_zinv_tile<N, T, MODE, S>(X, fs, blockIdx.x, lds, tw, ea, pro);
+    zfwd_zinv_tile<N, T, MODE, S>(X, fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea, pro);
 }
 
 // end of step bookkeeping: t += dt, dt_prev = dt, clear the CFL maxima

@@ -105,10 +105,11 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restr
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int b = find_box(gb, nbox, blockIdx.x, 0);
+    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = blockIdx.x - bx.zt0;
+    const int tile = gt - bx.zt0;
     const cpx<T>* src = from_u ? bx.u : bx.st;
     const cpx<T>* in[3] = {src, src + bx.fs, src + 2 * bx.fs};
     cpx<T>* out[6];
@@ -164,10 +165,11 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int b = find_box(gb, nbox, blockIdx.x, 0);
+    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = blockIdx.x - bx.zt0;
+    const int tile = gt - bx.zt0;
     EpiArgs<T, T> ea{};
     ea.out = nullptr;
     ea.state = bx.st;
@@ -188,10 +190,11 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_predonly(const GBox<T>
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int b = find_box(gb, nbox, blockIdx.x, 0);
+    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = blockIdx.x - bx.zt0;
+    const int tile = gt - bx.zt0;
     EpiArgs<T, T> ea{};
     ea.out = bx.pred;
     ea.state = const_cast<cpx<T>*>(bx.u);  // read only in this mode
@@ -213,10 +216,11 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv_vjp(const GBox<T>* __r
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int b = find_box(gb, nbox, blockIdx.x, 0);
+    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = blockIdx.x - bx.zt0;
+    const int tile = gt - bx.zt0;
     const cpx<T>* l = STAGE == 1 ? bx.lam : bx.Mu;
     const cpx<T>* p = STAGE == 1 ? bx.pred : bx.u;
     const cpx<T>* in[6] = {l, l + bx.fs, l + 2 * bx.fs, p, p + bx.fs, p + 2 * bx.fs};
@@ -236,10 +240,11 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_vjp(const GBox<T>* __r
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int b = find_box(gb, nbox, blockIdx.x, 0);
+    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = blockIdx.x - bx.zt0;
+    const int tile = gt - bx.zt0;
     VjpArgs<T> va{};
     va.out = bx.out;
     va.lam = bx.lam;
