@@ -384,7 +384,8 @@ int dnsbox_step(dnsbox_ctx* c, int nsteps, int stepper)
 {
     if (int e = check_ctx(c)) return e;
     if (nsteps < 0) return fail(DNSBOX_E_ARG, "nsteps < 0");
-    if (stepper != 1 && stepper != 2) return fail(DNSBOX_E_ARG, "stepper must be 1 (IF-Heun) or 2 (IF-AB2)");
+    if (stepper < 1 || stepper > 3)
+        return fail(DNSBOX_E_ARG, "stepper must be 1 (IF-Heun), 2 (IF-AB2) or 3 (IF-Heun + projection)");
     if (stepper == 2 && c->p.adaptive_dt)
         return fail(DNSBOX_E_STATE, "IF-AB2 runs at constant dt; create the box with adaptive_dt = 0");
     if (c->group) c->group->g->invalidate();
@@ -392,8 +393,13 @@ int dnsbox_step(dnsbox_ctx* c, int nsteps, int stepper)
         BoxBase* b = c->box;
         if (stepper == 1) {
             b->heun_steps(nsteps);
-        } else {
+        } else if (stepper == 2) {
             for (int i = 0; i < nsteps; ++i) b->ab2_step();
+        } else {
+            for (int i = 0; i < nsteps; ++i) {   // dnsboxband.f90:91-156
+                b->heun_step();
+                b->project(1, 1);
+            }
         }
         return 0;
     });

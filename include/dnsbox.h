@@ -114,7 +114,8 @@ int dnsbox_int_fact(dnsbox_ctx* ctx, double dt);
 int dnsbox_rhs_nonlinear(dnsbox_ctx* ctx);
 int dnsbox_get_nonlin(dnsbox_ctx* ctx, double* nuhat, double* nvhat, double* nwhat);
 /* the time loop body of dnsboxheun.f90:81-157: stepper 1 = IF-Heun predictor-corrector
-   (dnsboxheun.f90:84-126), 2 = IF-AB2 (dnsbox.f90:143-193, consistent weights), with the
+   (dnsboxheun.f90:84-126), 2 = IF-AB2 (dnsbox.f90:143-193, consistent weights), 3 = IF-Heun
+   with the state projected and dealiased after every step (dnsboxband.f90:91-156), with the
    CFL controller set_time_step (dnsboxheun.f90:227-246) when adaptive_dt */
 int dnsbox_step(dnsbox_ctx* ctx, int nsteps, int stepper);
 /* io_save_stats (io.f90:628-661) + state_spectrum (state.f90:350-411): diag[DNSBOX_NDIAG],
