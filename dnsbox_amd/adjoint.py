@@ -40,10 +40,9 @@ class _StepMap(torch.autograd.Function):
         box.rhs_int_fact(dt)
         u = u0.detach().clone().contiguous()
         ckpts = []
-        for t in range(T):
-            if t % k == 0:
-                ckpts.append(u.clone())
-            box.step_dev(u, 1)
+        for t in range(0, T, k):   # k steps per call: the step pipeline runs through a segment
+            ckpts.append(u.clone())
+            box.step_dev(u, min(k, T - t))
         ctx.box, ctx.T, ctx.dt, ctx.k = box, T, dt, k
         ctx.ckpts = ckpts
         return u
@@ -81,12 +80,11 @@ class _GroupStepMap(torch.autograd.Function):
             b.rhs_int_fact(dt)
         us = [u.detach().clone().contiguous() for u in u0s]
         ckpts = []
-        for t in range(T):
-            if t % k == 0:
-                c = [torch.empty_like(u) for u in us]
-                _copy_all(c, us)
-                ckpts.append(c)
-            group.step_dev(us, 1)
+        for t in range(0, T, k):   # k steps per call: the step pipeline runs through a segment
+            c = [torch.empty_like(u) for u in us]
+            _copy_all(c, us)
+            ckpts.append(c)
+            group.step_dev(us, min(k, T - t))
         ctx.group, ctx.T, ctx.dts, ctx.k, ctx.ckpts = group, T, list(dts), k, ckpts
         return tuple(us)
 

@@ -105,7 +105,9 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restr
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
+    // all XCDs, which balances boxes of uneven cost
+    const int gt = blockIdx.x;
     const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
@@ -165,7 +167,9 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
+    // all XCDs, which balances boxes of uneven cost
+    const int gt = blockIdx.x;
     const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
@@ -190,7 +194,9 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_predonly(const GBox<T>
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
+    // all XCDs, which balances boxes of uneven cost
+    const int gt = blockIdx.x;
     const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
@@ -216,7 +222,9 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv_vjp(const GBox<T>* __r
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
+    // all XCDs, which balances boxes of uneven cost
+    const int gt = blockIdx.x;
     const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
@@ -240,7 +248,9 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_vjp(const GBox<T>* __r
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
-    const int gt = xcd_contiguous(blockIdx.x, gridDim.x);
+    // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
+    // all XCDs, which balances boxes of uneven cost
+    const int gt = blockIdx.x;
     const int b = find_box(gb, nbox, gt, 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);

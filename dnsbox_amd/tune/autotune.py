@@ -102,18 +102,19 @@ def x_candidates(n, prec, base, limit=8):
     plans = _plans(n // 2, _radices(n // 2), 3)
     plans = [p for p in plans if len(p) == min(len(q) for q in plans)][:3]
     for rp in plans:
-        for rb in (1, 2, 4, 8, 16, 32, 64):
-            # 2 to 16 input loads per thread of a 256-thread block
-            if rows % rb or rb * h > 256 * 16 or (rb * h < 512 and rb * 2 <= rows // 256):
-                continue
-            lay = best_layouts(n, prec, rb, 256, rp, top=1, periods=(0, 16, 32), extra=8)
-            xpd, xpad, hp, _ = lay[0]
-            if 12 * rb * hp * esz > LDS_MAX or 6 * rb * hp * esz > 64 * 1024:
-                continue
-            c = base.with_(rb=rb, ntx=256, rp=rp, xpd=xpd, xpad=xpad)
-            if c not in out:
-                out.append(c)
-    out.sort(key=lambda c: (c != base, abs(c.rb * h - 4 * 129), len(c.rp)))
+        for ntx in ((256,) if n in GROUP_SIZES else (128, 256)):
+            for rb in (1, 2, 4, 8, 16, 32, 64):
+                # 2 to 16 input loads per thread
+                if rows % rb or rb * h > ntx * 16 or (rb * h < 2 * ntx and rb * 2 <= rows // 256):
+                    continue
+                lay = best_layouts(n, prec, rb, ntx, rp, top=1, periods=(0, 16, 32), extra=8)
+                xpd, xpad, hp, _ = lay[0]
+                if 12 * rb * hp * esz > LDS_MAX or 6 * rb * hp * esz > 64 * 1024:
+                    continue
+                c = base.with_(rb=rb, ntx=ntx, rp=rp, xpd=xpd, xpad=xpad)
+                if c not in out:
+                    out.append(c)
+    out.sort(key=lambda c: (c != base, abs(c.rb * h * 256 // c.ntx - 4 * 129), len(c.rp)))
     return out[:limit]
 
 
