@@ -28,6 +28,9 @@
 #ifndef DNSBOX_XCD_REMAP
 #define DNSBOX_XCD_REMAP 1
 #endif
+#ifndef DNSBOX_X_PRELOAD
+#define DNSBOX_X_PRELOAD 1
+#endif
 
 namespace dnsbox {
 
@@ -436,14 +439,71 @@ struct XPass {
         }
     }
 
-    // on entry the LDS holds the staged input rows (after a barrier); leaves the LDS in use
-    // until its last read (the caller puts a barrier before the next stage)
+    static constexpr int NI = FI * RB * M;
+    static constexpr int QZ = (NI + NTX - 1) / NTX;
+
+    // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),  k < M;
+    // imaginary parts of X[0], X[M] are ignored (c2r convention)
+    __device__ __forceinline__ static cpx<T> pre(cpx<T> A, cpx<T> Bm, int k, const cpx<T>* __restrict__ tw)
+    {
+        cpx<T> B = conj(Bm);
+        if (k == 0) {
+            A.y = T(0);
+            B.y = T(0);
+        }
+        cpx<T> w = tw[k];
+        w.y = -w.y;
+        const cpx<T> fe = A + B, fo = (A - B) * w;
+        return fe + mul_i<1>(fo);
+    }
+
+    // the c2r pre-processing applied on the way in: both loads of a Z value (X[k] and its
+    // mirror X[M-k], coalesced in reverse) issued together, straight from global memory
+    __device__ __forceinline__ static void load_pre(const cpx<T>* __restrict__ in0, long long in_fs, int blk,
+                                                    const cpx<T>* __restrict__ tw, cpx<T> (&z)[QZ])
+    {
+        const int tid = threadIdx.x;
+        const long long row0 = (long long)blk * RB;
+        cpx<T> A[QZ], B[QZ];
+#pragma unroll
+        for (int q = 0; q < QZ; ++q) {
+            const int e0 = tid + q * NTX;
+            const int e = (NI % NTX == 0 || e0 < NI) ? e0 : 0;   // unconditional, as in load()
+            const int fr = e / M, k = e % M;
+            const cpx<T>* row = in0 + (fr / RB) * in_fs + (row0 + fr % RB) * H;
+            A[q] = row[k];
+            B[q] = row[M - k];
+        }
+#pragma unroll
+        for (int q = 0; q < QZ; ++q) {
+            const int e0 = tid + q * NTX;
+            const int e = (NI % NTX == 0 || e0 < NI) ? e0 : 0;
+            z[q] = pre(A[q], B[q], e % M, tw);
+        }
+    }
+
+    __device__ __forceinline__ static void stage_z(const cpx<T> (&z)[QZ], cpx<T>* lds)
+    {
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < QZ; ++q) {
+            const int e = tid + q * NTX;
+            if (NI % NTX == 0 || e < NI) lds[(e / M) * HP + X::px(e % M)] = z[q];
+        }
+    }
+
+    // on entry the LDS holds the staged input rows (after a barrier) — the pre-processed
+    // Z rows when Z_STAGED; leaves the LDS in use until its last read (the caller puts a
+    // barrier before the next stage)
+    template <bool Z_STAGED = false>
     __device__ __forceinline__ static void compute(cpx<T>* __restrict__ out0, long long out_fs, int blk,
                                                    cpx<T>* lds, const cpx<T>* __restrict__ tw, XRed& red)
     {
         const int tid = threadIdx.x;
         const long long row0 = (long long)blk * RB;
-        if constexpr (Op::IN_SPEC) {
+        if constexpr (Op::IN_SPEC && Z_STAGED) {
+            RowFFT<typename Cfg<N, T>::RP, FI * RB, HP, NTX, +1, T, X::XPD>::run(lds, tid, tw, N);
+        } else if constexpr (Op::IN_SPEC) {
             // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),  k < M;
             // imaginary parts of X[0], X[M] are ignored (c2r convention).
             constexpr int NI = FI * RB * M;
@@ -525,7 +585,14 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
 {
     using XP = XPass<N, T, Op>;
     using X = typename XP::X;
-    if constexpr (Op::IN_SPEC) {
+    if constexpr (Op::IN_SPEC && DNSBOX_X_PRELOAD) {
+        cpx<T> z[XP::QZ];
+        XP::load_pre(in0, in_fs, blk, tw, z);
+        XP::stage_z(z, lds);
+        __syncthreads();
+        XP::template compute<true>(out0, out_fs, blk, lds, tw, red);
+        return;
+    } else if constexpr (Op::IN_SPEC) {
         cpx<T> buf[XP::QL];
         XP::load(in0, in_fs, blk, buf);
         XP::stage(buf, lds);
