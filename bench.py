@@ -13,7 +13,9 @@ Printed as one JSON line:
   fp64_steps_per_s       the same workload in FP64
   *_err_vs_fp64          spectrum (per shell and L2) and energy error vs the FP64 run after
                          --err-steps fixed-dt steps from the same field
-  roofline               the dominant kernel of the step: algorithmic bytes / HIP-event
+  unpruned_steps_per_s   the headline form with the transforms unpruned (the step prunes its
+                         passes to the 2/3-rule cube; results agree to rounding)
+  roofline               the dominant kernel of the step: algorithmic bytes (pruned) / HIP-event
                          launch time vs the 8 TB/s HBM peak; traffic = measured HBM bytes
                          per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
                          when a measurement of this kernel is on file
@@ -53,6 +55,8 @@ def parse():
                     help="fixed-dt steps of the FP32-vs-FP64 error runs (0 = skip)")
     ap.add_argument("--fp64-steps", type=int, default=300, help="timed FP64 steps")
     ap.add_argument("--pure-steps", type=int, default=1000, help="timed all-FP32 steps (0 = skip)")
+    ap.add_argument("--unpruned-steps", type=int, default=300,
+                    help="timed steps of the headline form with unpruned transforms (0 = skip)")
     ap.add_argument("--adjoint", type=int, default=1, help="time adjoint gradients (1/0)")
     ap.add_argument("--batched", type=int, default=1, help="time the 48-box group (1/0)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (1/0)")
@@ -271,8 +275,19 @@ def main():
     value = args.steps / secs
     t_end, dt_end = bh.time()
     log(f"fp32 (fp64 state) {value:.1f} steps/s (t={t_end:.4f}, dt={dt_end:.3e})")
+    pruned = bh.pruning()
     kernels = kernel_table(bh, args.reps)
     bh.close()
+
+    # ---- the same form with the transforms unpruned (same results to rounding) ----
+    unpruned = None
+    if args.unpruned_steps > 0:
+        bu = make_box(n, 32, state_precision=64)
+        bu.init_random(0, 0, 4.0, 0.5)
+        bu.set_pruning(False)
+        unpruned = args.unpruned_steps / timed_steps(bu, args.unpruned_steps, args.warmup)
+        bu.close()
+        log(f"fp32 (fp64 state), unpruned transforms: {unpruned:.1f} steps/s")
     dom = max(kernels, key=lambda q: kernels[q]["ms_per_step"])
     copy_gbs = copy_ceiling_gbs()
     log(f"passes {json.dumps({q: round(v['gbs']) for q, v in kernels.items()})} copy={copy_gbs:.0f} GB/s")
@@ -341,7 +356,9 @@ def main():
         "data": "synthetic (seeded solenoidal field, E(k)~k^4 exp(-2(k/4)^2), E=0.5)",
         "config": {"workload": f"{n}^3 forced box, IF-Heun, CFL 0.5, nu={NU}, f=(sin 4y,0,0), "
                                "FP32 FFT/nonlinear, FP64 state/IF/CFL/diagnostics",
-                   "n": n, "nu": NU, "cfl": 0.5, "stepper": "IF-Heun"},
+                   "n": n, "nu": NU, "cfl": 0.5, "stepper": "IF-Heun",
+                   "transforms": "pruned to the 2/3-rule cube" if pruned else "full"},
+        "unpruned_steps_per_s": unpruned,
         "err_vs_fp64": err,
         "err_steps": args.err_steps,
         "err_dt": err_dt(n),
@@ -352,7 +369,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": kd["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kd["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom, workload),
                      "kernel": dom, "alg_bytes_per_launch": kd["alg_bytes"], "ms_per_launch": kd["ms"],
-                     "copy_ceiling_gbs": copy_gbs},
+                     "copy_ceiling_gbs": copy_gbs,
+                     "note": ("the x pass is bound by VALU issue (about 70 M wave64 VALU instructions per "
+                              "launch), not by HBM" if dom == "xcross" else None)},
         "kernels": {q: {"ms": round(v["ms"], 5), "gbs": round(v["gbs"], 1)} for q, v in kernels.items()},
         "adjoint": adj,
         "batched": bat,

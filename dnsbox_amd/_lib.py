@@ -14,10 +14,6 @@ LIB_PATH = os.path.join(PKG, "lib", "libdnsbox_hip.so")
 (D_EKIN, D_DISP, D_PROD, D_COURANT, D_DIVMAX, D_EZERO, D_EONE, D_ESQRT2, D_EBAND, D_TIME,
  D_DT, NDIAG) = range(12)
 PASS_NAMES = ("zinv_curl", "yinv6", "xcross", "yfwd3", "zpred_zinv", "zcorr_zinv")
-# spectral fields per launch of each pass: (transform-type read, transform-type written,
-# state-type read, state-type written); the state type is FP64 for an FP32 box with an
-# FP64 state, else the transform type
-PASS_FIELDS = ((0, 6, 3, 0), (6, 6, 0, 0), (6, 3, 0, 0), (3, 3, 0, 0), (3, 6, 3, 3), (3, 6, 3, 3))
 # launches of each pass per IF-Heun step once the step pipeline is running (the z-inverse of
 # the first RHS is fused into the previous step's last pass)
 PASS_PER_STEP = (0, 2, 2, 2, 1, 1)
@@ -68,6 +64,9 @@ SIGNATURES = {
     "dnsbox_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
     "dnsbox_time_passes": (C.c_int, [_vp, C.c_int, _dp]),
     "dnsbox_state_ptr": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "dnsbox_pass_bytes": (C.c_int, [_vp, _dp]),
+    "dnsbox_set_pruning": (C.c_int, [_vp, C.c_int]),
+    "dnsbox_get_pruning": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "dnsbox_rhs_dev": (C.c_int, [_vp, _vp, _vp]),
     "dnsbox_step_dev": (C.c_int, [_vp, _vp, C.c_int]),
     "dnsbox_rhs_vjp_dev": (C.c_int, [_vp, _vp, _vp, _vp]),

@@ -190,15 +190,22 @@ class Box:
         return dict(zip(_lib.PASS_NAMES, ms.tolist()))
 
     def pass_bytes(self):
-        """Algorithmic HBM bytes of one launch of each step pass (every field read once and
-        written once), in the order of time_passes."""
-        n = self.n
-        m = n * n * (n // 2 + 1)
-        c = 16 if self.precision == 64 else 8
-        sp = self.params.state_precision or self.precision
-        s = 16 if sp == 64 else 8
-        return {name: m * (c * (xr + xw) + s * (sr + sw))
-                for name, (xr, xw, sr, sw) in zip(_lib.PASS_NAMES, _lib.PASS_FIELDS)}
+        """Algorithmic HBM bytes of one launch of each step pass as the next step runs it (every
+        element the pass must read or write, once), in the order of time_passes."""
+        b = np.zeros(len(_lib.PASS_NAMES))
+        check(self.lib.dnsbox_pass_bytes(self._h, _dp(b)))
+        return dict(zip(_lib.PASS_NAMES, b.tolist()))
+
+    def set_pruning(self, enable: bool = True):
+        """Run the step's transforms pruned to the 2/3-rule cube while the state allows it
+        (default on; only exact zeros are skipped, the results agree to rounding)."""
+        check(self.lib.dnsbox_set_pruning(self._h, int(bool(enable))))
+
+    def pruning(self) -> bool:
+        """Whether the next step runs pruned."""
+        v = C.c_int()
+        check(self.lib.dnsbox_get_pruning(self._h, C.byref(v)))
+        return bool(v.value)
 
     def state_ptr(self) -> int:
         p = C.c_void_p()

@@ -121,7 +121,7 @@ template <class T>
 struct OpVjp {  // (ybar, u, w) -> (ybar x w, u x ybar)
     static constexpr int FI = 9, FO = 6, NRED = 0;
     static constexpr bool IN_SPEC = true, OUT_SPEC = true;
-    __device__ __forceinline__ static void apply(const T* in, T* out, XRed&)
+    __device__ __forceinline__ static void apply(const T* in, T* out, XRed<T>&)
     {
         const T y0 = in[0], y1 = in[1], y2 = in[2];
         const T u0 = in[3], u1 = in[4], u2 = in[5];
@@ -139,7 +139,7 @@ template <class T>
 struct OpConv {  // (u, d_j u_i) -> (u . grad) u          rhsAdj.f90:83-122
     static constexpr int FI = 12, FO = 3, NRED = 0;
     static constexpr bool IN_SPEC = true, OUT_SPEC = true;
-    __device__ __forceinline__ static void apply(const T* in, T* out, XRed&)
+    __device__ __forceinline__ static void apply(const T* in, T* out, XRed<T>&)
     {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -151,7 +151,7 @@ template <class T>
 struct OpAdjN {  // (u, d_j u~_i) -> sum_j (d_i u~_j + d_j u~_i) u_j      rhsAdj.f90:271-309
     static constexpr int FI = 12, FO = 3, NRED = 0;
     static constexpr bool IN_SPEC = true, OUT_SPEC = true;
-    __device__ __forceinline__ static void apply(const T* in, T* out, XRed&)
+    __device__ __forceinline__ static void apply(const T* in, T* out, XRed<T>&)
     {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {

@@ -92,6 +92,38 @@ __device__ __forceinline__ bool masked(int ix, int iy, int iz, int kx, int ky, i
     return false;
 }
 
+// ---------------------------------------------------------------------------
+// Transforms pruned to the 2/3-rule cube.  The mask above zeroes every mode whose table
+// wavenumber has |k_i| >= N/3 on some axis (the threshold is (N/2)(2/3) <= KMAX + 1).  A state
+// with no energy on those modes keeps none through a step -- the masked nonlinear term is zero
+// there and the integrating factor only scales -- so the passes of the step need only the kept
+// index range of each axis, |kval(i)| <= KMAX (the Nyquist index, table wavenumber 0, is kept).
+// A pruned pass loads zeros in place of the other modes and does not store them; it skips only
+// exact zeros, so its results agree with the unpruned pass to rounding (the two instantiations
+// may contract multiply-adds differently).
+//   z passes : columns (iy, ix) over Ky x Kx, state modes over Kz
+//   y passes : columns (iz, ix) over all iz x Kx; inverse reads ky in Ky, forward stores ky in Ky
+//   x pass   : lines over all (y, z); c2r reads kx in Kx, r2c stores kx in Kx
+// ---------------------------------------------------------------------------
+template <int N>
+struct Keep {
+    static constexpr int KMAX = (N - 1) / 3;    // largest integer below N/3
+    static constexpr int H = N / 2 + 1;
+    static constexpr int KX = KMAX + 2;         // kx indices 0..KMAX and N/2
+    static constexpr int KY = 2 * KMAX + 2;     // ky (kz) indices 0..KMAX, N-KMAX..N-1 and N/2
+    static constexpr int ZCOLS = KY * KX;       // z-pass columns
+    static constexpr int YCOLS = N * KX;        // y-pass columns
+    static_assert(KMAX + 1 < N / 2, "kept ranges must not meet the Nyquist index");
+    __device__ __forceinline__ static bool kept(int i) { return abs(kval(i, N)) <= KMAX; }
+    __device__ __forceinline__ static int ix(int x) { return x <= KMAX ? x : N / 2; }
+    __device__ __forceinline__ static int iy(int y)
+    {
+        return y <= KMAX ? y : (y <= 2 * KMAX ? y + N - 2 * KMAX - 1 : N / 2);
+    }
+    // column c of the pruned z pass -> column index iy * H + ix of the full layout
+    __device__ __forceinline__ static int zcol(int c) { return iy(c / KX) * H + ix(c % KX); }
+};
+
 __device__ __forceinline__ unsigned long long dbl_bits(double v)
 {
     return (unsigned long long)__double_as_longlong(v);
@@ -260,7 +292,8 @@ __device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<N,
     }
 }
 
-template <int N, class T, class Pro, class TI = T>
+// PR: pruned to the 2/3-rule cube (Keep): tile columns over Ky x Kx, inputs read over Kz
+template <int N, class T, class Pro, class TI = T, bool PR = false>
 __device__ __forceinline__ void zinv_tile(const cpx<TI>* const* __restrict__ in,
                                           cpx<T>* const* __restrict__ out, int tile,
                                           cpx<T>* lds, const cpx<T>* __restrict__ tw, const Pro& pro)
@@ -271,18 +304,33 @@ __device__ __forceinline__ void zinv_tile(const cpx<TI>* const* __restrict__ in,
     constexpr long long ZS = (long long)N * H;
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
-    const int col = tile * W + c;
-    const bool valid = col < G::ZCOLS;
+    const int tc = tile * W + c;
+    const bool valid = tc < (PR ? Keep<N>::ZCOLS : G::ZCOLS);
+    const int col = PR ? Keep<N>::zcol(valid ? tc : 0) : tc;
     cpx<T> U[Pro::FI][RM];
 #pragma unroll
     for (int f = 0; f < Pro::FI; ++f)
 #pragma unroll
-        for (int q = 0; q < RM; ++q)
-            U[f][q] = valid ? cvt<T>(in[f][CF::in_pos(g, q) * ZS + col]) : cpx<T>{T(0), T(0)};
+        for (int q = 0; q < RM; ++q) {
+            const int iz = CF::in_pos(g, q);
+            U[f][q] = valid && (!PR || Keep<N>::kept(iz)) ? cvt<T>(in[f][iz * ZS + col]) : cpx<T>{T(0), T(0)};
+        }
     zinv_from_regs<N, T, Pro>(U, out, col, valid, lds, tw, pro);
 }
 
-template <int N, class T, class Pro, class TI = T>
+// tiles of the z passes
+template <int N, class T, bool PR>
+constexpr int ztiles()
+{
+    return ((PR ? Keep<N>::ZCOLS : Geo<N, T>::ZCOLS) + Cfg<N, T>::ZW - 1) / Cfg<N, T>::ZW;
+}
+template <int N, class T, bool PR>
+constexpr int ytiles()
+{
+    return ((PR ? Keep<N>::YCOLS : Geo<N, T>::YCOLS) + Cfg<N, T>::YW - 1) / Cfg<N, T>::YW;
+}
+
+template <int N, class T, class Pro, class TI = T, bool PR = false>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zinv(const cpx<TI>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
            long long out_fs, const cpx<T>* __restrict__ tw_g, Pro pro)
@@ -297,36 +345,45 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     for (int f = 0; f < Pro::FI; ++f) in[f] = in0 + f * in_fs;
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
-    zinv_tile<N, T, Pro, TI>(in, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
+    zinv_tile<N, T, Pro, TI, PR>(in, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
 }
 
 // ===========================================================================
 // y pass (plain, in place), one field per blockIdx.y
 // ===========================================================================
-template <int N, class T, int DIR>
+// PR: pruned (Keep): columns over all iz x Kx; the inverse (DIR = +1) reads ky in Ky only,
+// the forward (DIR = -1) stores ky in Ky only
+template <int N, class T, int DIR, bool PR = false>
 __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, cpx<T>* lds,
                                           const cpx<T>* __restrict__ tw)
 {
     using G = Geo<N, T>;
     using CF = ColFFT<typename G::C::YP, G::C::YW, G::C::YW, DIR, T>;
     constexpr int W = G::C::YW, RM = CF::RM, H = G::H;
+    constexpr int CW = PR ? Keep<N>::KX : H;   // columns per z-plane
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
     const int col = tile * W + c;
-    const bool valid = col < G::YCOLS;
-    const int ix = col % H, iz = col / H;
+    const bool valid = col < (PR ? Keep<N>::YCOLS : G::YCOLS);
+    const int ix = PR ? Keep<N>::ix(col % CW) : col % H, iz = col / CW;
     cpx<T>* base = data + (long long)iz * N * H + ix;
     cpx<T> a[RM];
 #pragma unroll
-    for (int q = 0; q < RM; ++q) a[q] = valid ? base[CF::in_pos(g, q) * H] : cpx<T>{T(0), T(0)};
+    for (int q = 0; q < RM; ++q) {
+        const int iy = CF::in_pos(g, q);
+        a[q] = valid && (!PR || DIR < 0 || Keep<N>::kept(iy)) ? base[iy * H] : cpx<T>{T(0), T(0)};
+    }
     CF::run(a, lds, c, g, tw, N);
     if (valid) {
 #pragma unroll
-        for (int q = 0; q < RM; ++q) base[CF::out_pos(g, q) * H] = a[q];
+        for (int q = 0; q < RM; ++q) {
+            const int iy = CF::out_pos(g, q);
+            if (!PR || DIR > 0 || Keep<N>::kept(iy)) base[iy * H] = a[q];
+        }
     }
 }
 
-template <int N, class T, int DIR>
+template <int N, class T, int DIR, bool PR = false>
 __global__ void __launch_bounds__((Geo<N, T>::YNT))
     k_ycol(cpx<T>* __restrict__ data0, long long fs, const cpx<T>* __restrict__ tw_g)
 {
@@ -336,49 +393,50 @@ __global__ void __launch_bounds__((Geo<N, T>::YNT))
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     const int L = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int field = L / gridDim.x, tile = L % gridDim.x;
-    ycol_tile<N, T, DIR>(data0 + field * fs, tile, lds, tw);
+    ycol_tile<N, T, DIR, PR>(data0 + field * fs, tile, lds, tw);
 }
 
 // ===========================================================================
 // x pass: FI inputs (half-spectrum lines -> c2r, or real lines) -> pointwise op
 // in physical space -> FO outputs (r2c half-spectrum lines, or real lines)
 // ===========================================================================
-struct XRed {  // reductions carried by the x pass
-    double m[4];
+template <class T>
+struct XRed {  // reductions carried by the x pass (maxima: exact in the transform type)
+    T m[4];
 };
 
 template <class T>
 struct OpCross {  // rhs.f90:39-65: (omega x u); io_Courant's max |u_i| on the way
     static constexpr int FI = 6, FO = 3, NRED = 3;
     static constexpr bool IN_SPEC = true, OUT_SPEC = true;
-    __device__ __forceinline__ static void apply(const T* in, T* out, XRed& r)
+    __device__ __forceinline__ static void apply(const T* in, T* out, XRed<T>& r)
     {
         const T u = in[0], v = in[1], w = in[2], ox = in[3], oy = in[4], oz = in[5];
         out[0] = oy * w - oz * v;
         out[1] = oz * u - ox * w;
         out[2] = ox * v - oy * u;
-        r.m[0] = fmax(r.m[0], (double)fabs(u));
-        r.m[1] = fmax(r.m[1], (double)fabs(v));
-        r.m[2] = fmax(r.m[2], (double)fabs(w));
+        r.m[0] = fmax(r.m[0], fabs(u));
+        r.m[1] = fmax(r.m[1], fabs(v));
+        r.m[2] = fmax(r.m[2], fabs(w));
     }
 };
 template <class T>
 struct OpVelDivMax {  // max|u|, max|v|, max|w|, signed max(div u); no output
     static constexpr int FI = 4, FO = 0, NRED = 4;
     static constexpr bool IN_SPEC = true, OUT_SPEC = true;
-    __device__ __forceinline__ static void apply(const T* in, T*, XRed& r)
+    __device__ __forceinline__ static void apply(const T* in, T*, XRed<T>& r)
     {
-        r.m[0] = fmax(r.m[0], (double)fabs(in[0]));
-        r.m[1] = fmax(r.m[1], (double)fabs(in[1]));
-        r.m[2] = fmax(r.m[2], (double)fabs(in[2]));
-        r.m[3] = fmax(r.m[3], (double)in[3]);
+        r.m[0] = fmax(r.m[0], fabs(in[0]));
+        r.m[1] = fmax(r.m[1], fabs(in[1]));
+        r.m[2] = fmax(r.m[2], fabs(in[2]));
+        r.m[3] = fmax(r.m[3], in[3]);
     }
 };
 template <class T, int F, bool INS, bool OUTS>
 struct OpCopy {
     static constexpr int FI = F, FO = F, NRED = 0;
     static constexpr bool IN_SPEC = INS, OUT_SPEC = OUTS;
-    __device__ __forceinline__ static void apply(const T* in, T* out, XRed&)
+    __device__ __forceinline__ static void apply(const T* in, T* out, XRed<T>&)
     {
 #pragma unroll
         for (int f = 0; f < F; ++f) out[f] = in[f];
@@ -402,7 +460,8 @@ struct XGeo {
 //   stage   : registers -> padded LDS rows
 //   compute : c2r pre-processing, row FFTs, pointwise op, row FFTs, r2c post-processing
 //             and the global stores
-template <int N, class T, class Op>
+// PR: pruned (Keep): the c2r reads kx in Kx only, the r2c stores kx in Kx only
+template <int N, class T, class Op, bool PR = false>
 struct XPass {
     using X = XGeo<N, T, Op>;
     static constexpr int M = X::M, H = X::H, HP = X::HP, RB = X::RB, NTX = X::NTX;
@@ -471,8 +530,13 @@ struct XPass {
             const int e = (NI % NTX == 0 || e0 < NI) ? e0 : 0;   // unconditional, as in load()
             const int fr = e / M, k = e % M;
             const cpx<T>* row = in0 + (fr / RB) * in_fs + (row0 + fr % RB) * H;
-            A[q] = row[k];
-            B[q] = row[M - k];
+            if constexpr (PR) {   // X[k], k < M, is kept for k <= KMAX; X[M - k] for M - k <= KMAX or k = 0
+                A[q] = k <= Keep<N>::KMAX ? row[k] : cpx<T>{T(0), T(0)};
+                B[q] = (k == 0 || M - k <= Keep<N>::KMAX) ? row[M - k] : cpx<T>{T(0), T(0)};
+            } else {
+                A[q] = row[k];
+                B[q] = row[M - k];
+            }
         }
 #pragma unroll
         for (int q = 0; q < QZ; ++q) {
@@ -497,7 +561,7 @@ struct XPass {
     // barrier before the next stage)
     template <bool Z_STAGED = false>
     __device__ __forceinline__ static void compute(cpx<T>* __restrict__ out0, long long out_fs, int blk,
-                                                   cpx<T>* lds, const cpx<T>* __restrict__ tw, XRed& red)
+                                                   cpx<T>* lds, const cpx<T>* __restrict__ tw, XRed<T>& red)
     {
         const int tid = threadIdx.x;
         const long long row0 = (long long)blk * RB;
@@ -567,7 +631,7 @@ struct XPass {
                 const cpx<T> zm = conj(row[X::px(k == 0 ? 0 : M - k)]);
                 const cpx<T> a = T(0.5) * (zk + zm);
                 const cpx<T> b = (T(0.5) * (zk - zm)) * tw[k];
-                out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
+                if (!PR || k <= Keep<N>::KMAX || k == M) out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
             }
         } else {
             for (int e = tid; e < FO * RB * M; e += NTX) {
@@ -578,13 +642,14 @@ struct XPass {
     }
 };
 
-template <int N, class T, class Op>
+template <int N, class T, class Op, bool PR = false>
 __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long long in_fs,
                                            cpx<T>* __restrict__ out0, long long out_fs, int blk, cpx<T>* lds,
-                                           const cpx<T>* __restrict__ tw, XRed& red)
+                                           const cpx<T>* __restrict__ tw, XRed<T>& red)
 {
-    using XP = XPass<N, T, Op>;
+    using XP = XPass<N, T, Op, PR>;
     using X = typename XP::X;
+    static_assert(!PR || (Op::IN_SPEC && Op::OUT_SPEC && DNSBOX_X_PRELOAD), "pruned x pass: spectral in and out");
     if constexpr (Op::IN_SPEC && DNSBOX_X_PRELOAD) {
         cpx<T> z[XP::QZ];
         XP::load_pre(in0, in_fs, blk, tw, z);
@@ -608,15 +673,15 @@ __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long 
     XP::compute(out0, out_fs, blk, lds, tw, red);
 }
 
-template <int NRED>
-__device__ __forceinline__ void block_max_atomic(XRed& red, unsigned long long* dst_bits,
+template <int NRED, class T>
+__device__ __forceinline__ void block_max_atomic(XRed<T>& red, unsigned long long* dst_bits,
                                                  bool signed_last, double* wsm)
 {
     // dst_bits[j]: max of |.| (plain bits) or, for the last entry when signed_last, ordered bits
     const int lane = threadIdx.x % 64, wid = threadIdx.x / 64, nw = blockDim.x / 64;
 #pragma unroll
     for (int j = 0; j < NRED; ++j) {
-        double v = wave_max(red.m[j]);
+        double v = wave_max(double(red.m[j]));
         if (lane == 0) wsm[j * 16 + wid] = v;
     }
     __syncthreads();
@@ -629,7 +694,7 @@ __device__ __forceinline__ void block_max_atomic(XRed& red, unsigned long long* 
     }
 }
 
-template <int N, class T, class Op>
+template <int N, class T, class Op, bool PR = false>
 __global__ void __launch_bounds__((Cfg<N, T>::NTX))
     k_xpass(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
             long long out_fs, const cpx<T>* __restrict__ tw_g, unsigned long long* red_bits,
@@ -640,10 +705,10 @@ __global__ void __launch_bounds__((Cfg<N, T>::NTX))
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     __shared__ double wsm[4 * 16];
-    XRed red;
+    XRed<T> red;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) red.m[j] = Op::NRED == 4 && j == 3 ? -1e300 : 0.0;
-    xpass_tile<N, T, Op>(in0, in_fs, out0, out_fs, blockIdx.x, lds, tw, red);
+    for (int j = 0; j < 4; ++j) red.m[j] = Op::NRED == 4 && j == 3 ? T(-1e300) : T(0);   // -1e300: -inf in FP32
+    xpass_tile<N, T, Op, PR>(in0, in_fs, out0, out_fs, blockIdx.x, lds, tw, red);
     if constexpr (Op::NRED > 0) {
         if (red_bits) block_max_atomic<Op::NRED>(red, red_bits, signed_last != 0, wsm);
     }
@@ -720,7 +785,8 @@ __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long l
 // tile: mask, pressure projection, band / Kolmogorov forcing (rhs.f90:67-133), then
 // the update of MODE.  When NEXT is set, U receives the input of the next RHS at the
 // same positions (the predicted state for PRED, the new state for CORR).
-template <int N, class T, int MODE, bool NEXT, bool PREF = false, class S = T>
+// PR: pruned (Keep): state (and predicted-state) modes outside Kz are neither read nor written
+template <int N, class T, int MODE, bool NEXT, bool PREF = false, class S = T, bool PR = false>
 __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZRM], int col, bool valid,
                                               const EpiArgs<T, S>& ea, cpx<T> (&U)[3][Geo<N, T>::ZRM],
                                               const cpx<S> (*Sp)[Geo<N, T>::ZRM] = nullptr)
@@ -790,14 +856,15 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
             const double k2d = double(kx * kx + ky * ky + kz * kz);
             const double e64 = exp((-ph.nu * k2d + ph.Q) * dt);
             const S e = S(e64), sdt = S(dt);
+            const bool live = valid && (!PR || Keep<N>::kept(iz));
             if constexpr (MODE == EPI_PRED) {  // dnsboxheun.f90:88-114
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    const cpx<S> u = PREF ? Sp[f][q] : (valid ? ea.state[f * ea.fs + idx] : cpx<S>{S(0), S(0)});
+                    const cpx<S> u = PREF ? Sp[f][q] : (live ? ea.state[f * ea.fs + idx] : cpx<S>{S(0), S(0)});
                     const cpx<S> n = cvt<S>(nl[f]);
                     const cpx<T> up = cvt<T>(e * (u + sdt * n));
                     if (NEXT) U[f][q] = up;
-                    if (valid) {
+                    if (live) {
                         if (ea.out) ea.out[f * ea.fs + idx] = up;
                         ea.state[f * ea.fs + idx] = e * (u + (sdt * S(0.5)) * n);
                         if (ea.nold) ea.nold[f * ea.fs + idx] = nl[f];
@@ -813,10 +880,10 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
             } else if constexpr (MODE == EPI_CORR) {  // dnsboxheun.f90:120-126
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
-                    const cpx<S> u = PREF ? Sp[f][q] : (valid ? ea.state[f * ea.fs + idx] : cpx<S>{S(0), S(0)});
+                    const cpx<S> u = PREF ? Sp[f][q] : (live ? ea.state[f * ea.fs + idx] : cpx<S>{S(0), S(0)});
                     const cpx<S> un = u + (sdt * S(0.5)) * cvt<S>(nl[f]);
                     if (NEXT) U[f][q] = cvt<T>(un);
-                    if (valid) ea.state[f * ea.fs + idx] = un;
+                    if (live) ea.state[f * ea.fs + idx] = un;
                 }
             } else if constexpr (MODE == EPI_AB2) {
                 const S e2 = S(e64 * e64);
@@ -878,13 +945,15 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
 // of the next RHS: the next RHS input never makes a round trip through HBM.  The
 // input tile (fields 0..2 of X) and the output tile (fields 0..5 of X) cover the
 // same columns, so the block reads its whole tile before it writes.
-template <int N, class T, int MODE, class S = T>
+// PR: pruned (Keep): tile columns over Ky x Kx, state modes over Kz
+template <int N, class T, int MODE, class S = T, bool PR = false>
 __device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long fs, int tile, cpx<T>* lds,
                                                const cpx<T>* tw, const EpiArgs<T, S>& ea, const ProCurl<T>& pro)
 {
     using G = Geo<N, T>;
-    const int col = tile * G::C::ZW + threadIdx.x % G::C::ZW;
-    const bool valid = col < G::ZCOLS;
+    const int tc = tile * G::C::ZW + threadIdx.x % G::C::ZW;
+    const bool valid = tc < (PR ? Keep<N>::ZCOLS : G::ZCOLS);
+    const int col = PR ? Keep<N>::zcol(valid ? tc : 0) : tc;
     cpx<T> O[3][G::ZRM];
 #if DNSBOX_Z_PREFETCH
     // the state at this thread's modes (symmetric z plan: output positions = input positions),
@@ -897,14 +966,17 @@ __device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long
 #pragma unroll
         for (int f = 0; f < 3; ++f)
 #pragma unroll
-            for (int q = 0; q < G::ZRM; ++q)
-                Sp[f][q] = valid ? ea.state[f * ea.fs + CF::out_pos(g, q) * ZS + col] : cpx<S>{S(0), S(0)};
+            for (int q = 0; q < G::ZRM; ++q) {
+                const int iz = CF::out_pos(g, q);
+                Sp[f][q] = valid && (!PR || Keep<N>::kept(iz)) ? ea.state[f * ea.fs + iz * ZS + col]
+                                                                 : cpx<S>{S(0), S(0)};
+            }
     }
     zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, true, true, S>(O, col, valid, ea, O, Sp);
+    zfwd_epilogue<N, T, MODE, true, true, S, PR>(O, col, valid, ea, O, Sp);
 #else
     zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, true, false, S>(O, col, valid, ea, O);
+    zfwd_epilogue<N, T, MODE, true, false, S, PR>(O, col, valid, ea, O);
 #endif
     cpx<T>* out[6];
 #pragma unroll
@@ -915,7 +987,7 @@ __device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long
     }
 }
 
-template <int N, class T, int MODE, class S = T>
+template <int N, class T, int MODE, class S = T, bool PR = false>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw_g, EpiArgs<T, S> ea,
                 ProCurl<T> pro)
@@ -924,7 +996,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     __shared__ cpx<T> lds[N * G::C::ZW];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    zfwd_zinv_tile<N, T, MODE, S>(X, fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea, pro);
+    zfwd_zinv_tile<N, T, MODE, S, PR>(X, fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea, pro);
 }
 
 // end of step bookkeeping: t += dt, dt_prev = dt, clear the CFL maxima

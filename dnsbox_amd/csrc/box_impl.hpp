@@ -71,6 +71,12 @@ class BoxBase {
     int red_blocks = 0;
     bool ab2_started = false;
     bool x6_valid = false;   // scratch holds the z-inverse + curl of the current state
+    bool x6_pruned = false;  // ... written by the pruned passes (columns outside the kept cube stale)
+    // the state has no energy outside the 2/3-rule cube (Keep in box.hpp): true for the zero
+    // state, after init_random, state_dealias and every step; set_state checks the field
+    bool dealiased = true;
+    bool prune = true;       // run the step's passes pruned to the kept cube when dealiased
+    bool pruned() const { return prune && dealiased; }
     int scratch_fields = 6;
     void* adjbuf = nullptr;  // 9 fields: A, Mu, u~ (adjoint passes), lazily
     double* band2 = nullptr; // band-forcing scalars of the VJP
@@ -90,6 +96,9 @@ class BoxBase {
     // average device time of each step pass over `reps` back-to-back launches (ms),
     // measured with HIP events on the box stream; order of DNSBOX_PASS_*
     virtual void time_passes(int reps, double* ms) = 0;
+    // algorithmic HBM bytes of one launch of each step pass (DNSBOX_PASS_* order): every field
+    // element a pass must read or write, once, for the passes the next step would run
+    virtual void pass_bytes(double* bytes) const = 0;
     // adjoint and caller-buffer entry points (device pointers, internal layout)
     virtual void rhs_vjp(const void* u, const void* lam, void* out) = 0;
     virtual void step_vjp(const void* u, const void* lam, void* lam_out, double dt) = 0;
@@ -208,11 +217,11 @@ class BoxT final : public BoxBase {
         DNSBOX_LAUNCH_CHECK();
     }
 
-    template <class E>
+    template <class E, bool PR = false>
     void zinv_curl(const cpx<E>* in)
     {
-        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>, E>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in, fs, X(), fs,
-                           twc(), ProCurl<T>{inv_n3()});
+        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>, E, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream, in,
+                           fs, X(), fs, twc(), ProCurl<T>{inv_n3()});
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -229,16 +238,19 @@ class BoxT final : public BoxBase {
     }
 
     // middle passes of an RHS: y-inverse (6), x cross product, y-forward (3)
+    template <bool PR = false>
     void rhs_middle(bool track_cfl)
     {
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<N, T, +1, PR>), dim3(ytiles<N, T, PR>(), 6), dim3(YNT), 0, stream, X(), fs,
+                           twc());
         DNSBOX_LAUNCH_CHECK();
         mark(1);
-        hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
+        hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>, PR>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
                            stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
         mark(2);
-        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<N, T, -1, PR>), dim3(ytiles<N, T, PR>(), 3), dim3(YNT), 0, stream, X(), fs,
+                           twc());
         DNSBOX_LAUNCH_CHECK();
         mark(3);
     }
@@ -278,33 +290,41 @@ class BoxT final : public BoxBase {
     }
 
     // IF-Heun step (dnsboxheun.f90:84-126) as a pipeline: the forward z pass of each
-    // RHS carries the stepper epilogue and the z-inverse + curl of the next RHS.
+    // RHS carries the stepper epilogue and the z-inverse + curl of the next RHS.  The passes
+    // are pruned to the 2/3-rule cube while the state has no energy outside it.
     void heun_step() override
     {
+        if (pruned()) heun_step_t<true>();
+        else heun_step_t<false>();
+    }
+
+    template <bool PR>
+    void heun_step_t()
+    {
         const ProCurl<T> pro{inv_n3()};
-        if (!x6_valid) {
+        if (!x6_valid || x6_pruned != PR) {
             eband(St());
-            zinv_curl(St());
+            zinv_curl<S, PR>(St());
             mark(0);
         }
-        rhs_middle(true);
+        rhs_middle<PR>(true);
         EpiArgs<T, S> ea = epi();
         ea.out = cfg.ph.bandlim ? Pr() : nullptr;  // the predicted state is kept only for band forcing
         ea.state = St();
         ea.rhs_in = St();
         ea.rhs_in_s = 1;
-        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                           twc(), ea, pro);
+        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED, S, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream,
+                           X(), fs, twc(), ea, pro);
         DNSBOX_LAUNCH_CHECK();
         mark(4);
         eband(Pr());
-        rhs_middle(false);
+        rhs_middle<PR>(false);
         EpiArgs<T, S> eb = epi();
         eb.state = St();
         eb.rhs_in = Pr();
         eb.rhs_in_s = 0;
-        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs,
-                           twc(), eb, pro);
+        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream,
+                           X(), fs, twc(), eb, pro);
         DNSBOX_LAUNCH_CHECK();
         mark(5);
         eband(St());
@@ -312,6 +332,7 @@ class BoxT final : public BoxBase {
         DNSBOX_LAUNCH_CHECK();
         mark(-1);
         x6_valid = true;
+        x6_pruned = PR;
     }
 
     // Optional replay of the steady-state IF-Heun steps from a hipGraph (DNSBOX_STEP_GRAPH=1
@@ -449,6 +470,7 @@ class BoxT final : public BoxBase {
     void init_random(unsigned long long seed, int box, double k0, double energy) override
     {
         x6_valid = false;
+        dealiased = true;   // the generator leaves every masked mode empty
         hipLaunchKernelGGL((k_init_solenoidal<N, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
                            cfg.ph, seed, box, k0);
         DNSBOX_LAUNCH_CHECK();
@@ -461,6 +483,7 @@ class BoxT final : public BoxBase {
     void project(int dealias, int proj) override
     {
         x6_valid = false;
+        if (dealias) dealiased = true;   // state_dealias zeroes every mode outside the kept cube
         hipLaunchKernelGGL((k_project_dealias<N, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
                            cfg.ph, dealias, proj);
         DNSBOX_LAUNCH_CHECK();
@@ -484,6 +507,7 @@ class BoxT final : public BoxBase {
     void from_physical(const void* phys3) override
     {
         x6_valid = false;
+        dealiased = false;   // a physical field may hold energy on every mode
         const long long pf = (long long)N * N * N / 2;
         hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, false, true>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
                            stream, static_cast<const C*>(phys3), pf, X(), fs, twc(), (unsigned long long*)nullptr,
@@ -670,6 +694,29 @@ class BoxT final : public BoxBase {
     // (the box's state and clock are left as they were): an event after every launch of
     // `reps` steps, each pass timed from the previous event.  The z-inverse of the first
     // RHS runs once per pipeline start; it is timed over `reps` back-to-back launches.
+    void pass_bytes(double* b) const override
+    {
+        const double cb = double(sizeof(C)), sb = double(sizeof(CS)), n = double(N);
+        if (pruned()) {
+            const double kx = Keep<N>::KX, ky = Keep<N>::KY, zc = kx * ky;
+            const double pr = cfg.ph.bandlim ? 3.0 * zc * ky * cb : 0.0;   // predicted state (band forcing)
+            b[0] = 3.0 * zc * ky * sb + 6.0 * zc * n * cb;                 // state over Kz -> 6 fields, all z
+            b[1] = 6.0 * n * kx * (ky + n) * cb;                           // ky in Ky read, all y written
+            b[2] = 9.0 * n * n * kx * cb;                                  // 6 lines read, 3 written, kx in Kx
+            b[3] = 3.0 * n * kx * (n + ky) * cb;                           // all y read, ky in Ky written
+            b[4] = 9.0 * zc * n * cb + 6.0 * zc * ky * sb + pr;            // 3 + 6 fields all z, state over Kz
+            b[5] = 9.0 * zc * n * cb + 6.0 * zc * ky * sb;
+        } else {
+            const double f = double(G::FS);
+            b[0] = f * (6.0 * cb + 3.0 * sb);
+            b[1] = f * 12.0 * cb;
+            b[2] = f * 9.0 * cb;
+            b[3] = f * 6.0 * cb;
+            b[4] = f * (9.0 * cb + 6.0 * sb) + (cfg.ph.bandlim ? f * 3.0 * cb : 0.0);
+            b[5] = f * (9.0 * cb + 6.0 * sb);
+        }
+    }
+
     void time_passes(int reps, double* ms) override
     {
         if (reps < 1) reps = 1;
@@ -695,7 +742,10 @@ class BoxT final : public BoxBase {
             DNSBOX_HIP(hipEventCreate(&mk.back().second));
             // z-inverse + curl, back to back
             DNSBOX_HIP(hipEventRecord(mk.back().second, stream));
-            for (int r = 0; r < reps; ++r) zinv_curl(static_cast<const CS*>(tmp));
+            for (int r = 0; r < reps; ++r) {
+                if (pruned()) zinv_curl<S, true>(static_cast<const CS*>(tmp));
+                else zinv_curl(static_cast<const CS*>(tmp));
+            }
             hipEvent_t ez;
             DNSBOX_HIP(hipEventCreate(&ez));
             mk.emplace_back(-2, ez);

@@ -90,6 +90,26 @@ void ref_to_dev_layout(const double* src, std::vector<double>& dst, int n)
             }
 }
 
+// true when the 3 device-layout fields hold no energy outside the kept cube of the 2/3 rule
+// (|kval| <= (n-1)/3 on every axis; the step's pruned passes rely on it, box.hpp Keep)
+bool zero_outside_kept_cube(const std::vector<double>* h, int n)
+{
+    const int hh = n / 2 + 1, kmax = (n - 1) / 3;
+    auto kept = [n, kmax](int i) {
+        const int k = i < n / 2 ? i : (i == n / 2 ? 0 : i - n);
+        return (k < 0 ? -k : k) <= kmax;
+    };
+    for (int iz = 0; iz < n; ++iz)
+        for (int iy = 0; iy < n; ++iy)
+            for (int ix = 0; ix < hh; ++ix) {
+                if (kept(iz) && kept(iy) && kept(ix)) continue;
+                const size_t d = (size_t(iz) * n + iy) * hh + ix;
+                for (int f = 0; f < 3; ++f)
+                    if (h[f][2 * d] != 0.0 || h[f][2 * d + 1] != 0.0) return false;
+            }
+    return true;
+}
+
 void dev_to_ref_layout(const std::vector<double>& src, double* dst, int n)
 {
     const int h = n / 2 + 1;
@@ -261,6 +281,7 @@ int dnsbox_set_state(dnsbox_ctx* c, const double* u, const double* v, const doub
         for (int f = 0; f < 3; ++f) ref_to_dev_layout(src[f], h[f], n);
         BoxBase* b = c->box;
         upload3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->sbytes, true);
+        b->dealiased = zero_outside_kept_cube(h, n);
         b->ab2_started = false;
         return 0;
     });
@@ -492,7 +513,35 @@ int dnsbox_state_ptr(dnsbox_ctx* c, void** p)
 {
     if (int e = check_ctx(c)) return e;
     if (!p) return fail(DNSBOX_E_ARG, "null pointer");
+    invalidate(c);
+    // the caller may write the state: the step no longer assumes it empty outside the kept
+    // cube until dnsbox_project(..., dealias = 1), dnsbox_init_random or dnsbox_set_state
+    c->box->dealiased = false;
     *p = c->box->state;
+    return 0;
+}
+
+int dnsbox_pass_bytes(dnsbox_ctx* c, double* bytes)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!bytes) return fail(DNSBOX_E_ARG, "null pointer");
+    c->box->pass_bytes(bytes);
+    return 0;
+}
+
+int dnsbox_set_pruning(dnsbox_ctx* c, int enable)
+{
+    if (int e = check_ctx(c)) return e;
+    invalidate(c);
+    c->box->prune = enable != 0;
+    return 0;
+}
+
+int dnsbox_get_pruning(dnsbox_ctx* c, int* active)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!active) return fail(DNSBOX_E_ARG, "null pointer");
+    *active = c->box->pruned() ? 1 : 0;
     return 0;
 }
 

@@ -151,9 +151,9 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __rest
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
     const int blk = blockIdx.x - bx.xb0;
-    XRed red;
+    XRed<T> red;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) red.m[j] = 0.0;
+    for (int j = 0; j < 4; ++j) red.m[j] = T(0);
 #define DNSBOX_G_X(NN) xpass_tile<NN, T, Op>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_X)
 #undef DNSBOX_G_X

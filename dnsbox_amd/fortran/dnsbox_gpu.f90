@@ -73,6 +73,16 @@ module dnsbox_gpu
             integer(c_int), value :: box
             real(c_double), value :: k0, energy
         end function
+        integer(c_int) function dnsbox_set_pruning(ctx, enable) bind(C, name="dnsbox_set_pruning")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: enable
+        end function
+        integer(c_int) function dnsbox_get_pruning(ctx, active) bind(C, name="dnsbox_get_pruning")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(out) :: active
+        end function
         integer(c_int) function dnsbox_project(ctx, dealias, project) bind(C, name="dnsbox_project")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

@@ -141,9 +141,25 @@ enum {
     DNSBOX_NPASS
 };
 int dnsbox_time_passes(dnsbox_ctx* ctx, int reps, double* ms);
+/* algorithmic HBM bytes of one launch of each pass (DNSBOX_PASS_* order) as the next step would
+   run it: every element the pass must read or write, once.  No reference counterpart. */
+int dnsbox_pass_bytes(dnsbox_ctx* ctx, double* bytes);
+
+/* Transforms pruned to the 2/3-rule cube.  The mask of rhs_nonlinear (rhs.f90:74-88) zeroes
+   every mode with |k_i| >= N/3 on some axis, so a state with no energy there keeps none through
+   a step.  While that holds (after dnsbox_init_random, dnsbox_project with dealias = 1, every
+   step, and dnsbox_set_state of such a field) the IF-Heun step skips the loads and stores of
+   those modes; it skips only exact zeros, so the results agree with the unpruned step to
+   rounding.  Taking dnsbox_state_ptr or setting a
+   physical field ends it until the state is dealiased again.  set_pruning(ctx, 0) turns it off
+   (default on); get_pruning reports whether the next step runs pruned.  No reference
+   counterpart (p3dfft transforms whole fields). */
+int dnsbox_set_pruning(dnsbox_ctx* ctx, int enable);
+int dnsbox_get_pruning(dnsbox_ctx* ctx, int* active);
 
 /* device-pointer entry points (internal layout, context precision) */
-int dnsbox_state_ptr(dnsbox_ctx* ctx, void** dev_state);   /* the context's state buffer */
+int dnsbox_state_ptr(dnsbox_ctx* ctx, void** dev_state);   /* the context's state buffer (ends
+                                                              pruning until the next dealias) */
 /* rhs_nonlinear on device buffers: out <- N(in) */
 int dnsbox_rhs_dev(dnsbox_ctx* ctx, const void* in, void* out);
 /* the time loop of dnsboxheun.f90:81-126 on a caller-owned device state (in place) */

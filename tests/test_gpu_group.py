@@ -1,5 +1,6 @@
 """Groups of uneven boxes stepped by one launch per pass reproduce the same boxes stepped
-one by one (the group kernels run the same tile functions)."""
+one by one (the group kernels run the same tile functions; the single boxes run their
+unpruned passes, as the group does)."""
 import numpy as np
 import pytest
 
@@ -32,6 +33,7 @@ def test_group_matches_single_boxes(gpu_lib, prec):
         times = [b.time() for b in g.boxes]
     for i, kw in enumerate(params):
         with Box(**kw) as b:
+            b.set_pruning(False)   # the group runs the unpruned passes
             b.init_random(100 + i, i, k0s[i], 0.5)
             b.step(15)
             ref = b.get_state()

@@ -143,6 +143,85 @@ def test_fp32_rhs_close_to_fp64(gpu_lib):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("state_precision", [0, 64])
+def test_fp32_256_matches_oracle(gpu_lib, state_precision):
+    """The 256^3 FP32 step (the headline size; the step's passes run pruned) against the FP64
+    oracle: the RHS, then two fixed-dt IF-Heun steps."""
+    n, dt = 256, 0.002
+    kw = dict(nu=0.005, kolmo_amp=1.0, kolmo_k=4)
+    o = Oracle(n, **kw, nthreads=16)
+    f = o.init_solenoidal(3, 0, 4.0, 0.5)
+    ref = o.rhs(f)
+    with make(n, precision=32, state_precision=state_precision, bandlim=0, adaptive_dt=0, dt=dt, **kw) as b:
+        b.set_state(*f)
+        got = b.rhs_nonlinear()
+        err = rel_l2(got, ref)
+        print(f"fp32 256 rhs rel_l2={err:.3e}", flush=True)
+        assert err < 1e-5
+        b.set_state(*f)
+        b.rhs_int_fact(dt)
+        b.step(2)
+        got = b.get_state()
+    ref = o.heun(f, 2, dt)
+    err = rel_l2(got, ref)
+    print(f"fp32 256 (state {state_precision or 32}) heun x2 rel_l2={err:.3e}", flush=True)
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("n,prec,sp,band", [(32, 64, 0, 0), (32, 64, 0, 1), (48, 32, 0, 0), (64, 64, 0, 0),
+                                             (128, 32, 64, 0), (256, 32, 64, 0)])
+def test_pruned_step_matches_unpruned(gpu_lib, n, prec, sp, band):
+    """The step's passes pruned to the 2/3-rule cube skip only exact zeros: the results agree
+    with the unpruned passes to rounding (state, clock, adaptive dt, diagnostics).  Not to the
+    bit: the two kernel instantiations may contract multiply-adds differently."""
+    kw = dict(nu=0.01, kolmo_amp=1.0, kolmo_k=4, bandlim=band, Pin=0.1, kcut=2.0, adaptive_dt=1,
+              cfl_min=0.5, cfl_max=0.5, dt=0.002)
+    tol = 1e-13 if prec == 64 else (1e-8 if sp == 64 else 1e-6)
+    with make(n, precision=prec, state_precision=sp, **kw) as a, \
+            make(n, precision=prec, state_precision=sp, **kw) as b:
+        b.set_pruning(False)
+        for x in (a, b):
+            x.init_random(5, 0, 4.0, 0.5)
+        assert a.pruning() and not b.pruning()
+        for x in (a, b):
+            x.step(12)
+        sa, sb = a.get_state(), b.get_state()
+        err = rel_l2(sa, sb)
+        (da, ea), (db, eb) = a.diagnostics(), b.diagnostics()
+        derr = float(np.linalg.norm(ea - eb) / np.linalg.norm(eb))
+        (ta, dta), (tb, dtb) = a.time(), b.time()
+        print(f"pruned n={n} prec={prec} state={sp or prec} band={band}: state rel_l2={err:.3e} "
+              f"spectrum rel_l2={derr:.3e} t {ta!r} vs {tb!r}", flush=True)
+        assert err <= tol
+        assert derr <= tol
+        tt = 1e-12 if prec == 64 else 1e-6    # the CFL step follows max|u| of the transform type
+        assert abs(ta - tb) <= tt * tb and abs(dta - dtb) <= tt * dtb
+
+
+def test_pruning_follows_the_state(gpu_lib):
+    """A field with energy outside the kept cube turns pruning off (the step then matches the
+    oracle, masked modes included); state_dealias turns it back on."""
+    n, dt = 32, 0.005
+    o = Oracle(n, nu=0.05, kolmo_amp=1.0, kolmo_k=4)
+    f = o.init_solenoidal(0, 0, 4.0, 0.5)
+    g = [x.copy() for x in f]
+    g[0][12, 3, 5] = 1e-3 + 2e-3j   # kx = 12 > (32 - 1) // 3: outside the kept cube
+    with make(n, nu=0.05, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        b.set_state(*f)
+        assert b.pruning()
+        b.set_state(*g)
+        assert not b.pruning()
+        b.rhs_int_fact(dt)
+        b.step(3)
+        err = rel_l2(b.get_state(), o.heun(g, 3, dt))
+        print(f"unpruned step with masked-mode energy: rel_l2={err:.3e}", flush=True)
+        assert err <= 1e-12
+        b.state_dealias()
+        assert b.pruning()
+        b.state_ptr()
+        assert not b.pruning()
+
+
 def test_adaptive_heun_matches_oracle(gpu_lib):
     """CFL controller (dnsboxheun.f90:153-157, 227-246) evaluated on the current field."""
     n = 32
