@@ -53,14 +53,15 @@ class _StepMap(torch.autograd.Function):
         box.rhs_int_fact(dt)
         lam = g.detach().clone().contiguous()
         out = torch.empty_like(lam)
+        traj = torch.empty((max(k - 1, 1),) + tuple(lam.shape), dtype=lam.dtype, device=lam.device)
         for seg in reversed(range(len(ctx.ckpts))):
             start = seg * k
             stop = min(start + k, T)
-            states = [ctx.ckpts[seg]]
-            for _ in range(start, stop - 1):
-                u = states[-1].clone()
-                box.step_dev(u, 1)
-                states.append(u)
+            # the segment's states u_start .. u_{stop-1}, recomputed in one call
+            if stop - start > 1:
+                u = ctx.ckpts[seg].clone()
+                box.step_dev_record(u, stop - start - 1, traj)
+            states = [ctx.ckpts[seg]] + [traj[j] for j in range(stop - start - 1)]
             for j in reversed(range(stop - start)):
                 box.step_vjp_dev(states[j], lam, out, dt)
                 lam, out = out, lam

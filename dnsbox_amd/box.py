@@ -253,6 +253,20 @@ class Box:
         self._on_torch_stream()
         check(self.lib.dnsbox_step_dev(self._h, self._check_tensor(state), nsteps))
 
+    def step_dev_record(self, state, nsteps: int, traj):
+        """step_dev, storing the state after step j + 1 in traj[j] (traj: a contiguous device
+        tensor of shape (>= nsteps, 3, N, N, N/2+1) of the context precision)."""
+        import torch
+        want = torch.complex128 if self.precision == 64 else torch.complex64
+        shape = (3, self.n, self.n, self.n // 2 + 1)
+        if not (traj.is_cuda and traj.dtype == want and traj.is_contiguous() and tuple(traj.shape[1:]) == shape
+                and traj.shape[0] >= nsteps):
+            raise ValueError(f"expected a contiguous cuda {want} tensor of shape (>= {nsteps}, {shape}), "
+                             f"got {traj.dtype} {tuple(traj.shape)}")
+        self._on_torch_stream()
+        check(self.lib.dnsbox_step_dev_record(self._h, self._check_tensor(state), nsteps,
+                                              C.c_void_p(traj.data_ptr())))
+
     def rhs_vjp_dev(self, u, lam, out):
         """out <- J_N(u)^T lam (Euclidean inner product of the stored coordinates)."""
         self._on_torch_stream()

@@ -518,12 +518,37 @@ struct XPass {
 
     // the c2r pre-processing applied on the way in: both loads of a Z value (X[k] and its
     // mirror X[M-k], coalesced in reverse) issued together, straight from global memory
+    // Fixed-k thread layout: when NTX is a multiple of M and RB of NTX/M, the e-th element of
+    // a thread (e = tid + q NTX) is column k = tid % M of line row (tid / M) + (NTX/M) q, so
+    // k is the same for all q and every load / store offset is a compile-time constant plus a
+    // per-thread base (no index arithmetic per element).
+    static constexpr bool FIXED_K = NTX % M == 0 && RB % (NTX / M) == 0;
+    static constexpr int FPQ = NTX / M;     // lines advanced per q in the fixed-k layout
+
     __device__ __forceinline__ static void load_pre(const cpx<T>* __restrict__ in0, long long in_fs, int blk,
                                                     const cpx<T>* __restrict__ tw, cpx<T> (&z)[QZ])
     {
         const int tid = threadIdx.x;
         const long long row0 = (long long)blk * RB;
         cpx<T> A[QZ], B[QZ];
+        if constexpr (FIXED_K && NI % NTX == 0) {
+            // spectral inputs are whole half-spectrum fields, N^2 (N/2+1) elements apart
+            constexpr long long FS = (long long)N * N * H;
+            const int k = tid % M;
+            const cpx<T>* base = in0 + (row0 + tid / M) * H;
+            const bool ka = !PR || k <= Keep<N>::KMAX;                 // X[k] kept
+            const bool kb = !PR || k == 0 || M - k <= Keep<N>::KMAX;   // X[M - k] kept
+#pragma unroll
+            for (int q = 0; q < QZ; ++q) {
+                const int fr = FPQ * q;   // line index of this q relative to the thread's first line
+                const cpx<T>* row = base + (fr / RB) * FS + (fr % RB) * H;
+                A[q] = ka ? row[k] : cpx<T>{T(0), T(0)};
+                B[q] = kb ? row[M - k] : cpx<T>{T(0), T(0)};
+            }
+#pragma unroll
+            for (int q = 0; q < QZ; ++q) z[q] = pre(A[q], B[q], k, tw);
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < QZ; ++q) {
             const int e0 = tid + q * NTX;
@@ -623,6 +648,36 @@ struct XPass {
         } else if constexpr (Op::OUT_SPEC) {
             RowFFT<typename Cfg<N, T>::RP, FO * RB, HP, NTX, -1, T, X::XPD>::run(lds, tid, tw, N);
             // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]), k <= M
+            if constexpr (FIXED_K && (FO * RB * M) % NTX == 0) {
+                // k < M in the fixed-k layout, then k = M (one entry per line) by the first FO RB threads
+                constexpr long long FS = (long long)N * N * H;
+                const int k = tid % M;
+                const bool keep = !PR || k <= Keep<N>::KMAX;
+                if (keep) {
+                    const cpx<T> w = tw[k];
+                    const int r0 = tid / M;
+                    cpx<T>* dst = out0 + (row0 + r0) * H + k;
+#pragma unroll
+                    for (int q = 0; q < FO * RB * M / NTX; ++q) {
+                        const int fr = r0 + FPQ * q;   // line within the FO x RB lines of the block
+                        const cpx<T>* row = lds + fr * HP;
+                        const cpx<T> zk = row[X::px(k)];
+                        const cpx<T> zm = conj(row[X::px(k == 0 ? 0 : M - k)]);
+                        const cpx<T> a = T(0.5) * (zk + zm);
+                        const cpx<T> b = (T(0.5) * (zk - zm)) * w;
+                        dst[((FPQ * q) / RB) * FS + ((FPQ * q) % RB) * H] = a + mul_i<-1>(b);
+                    }
+                }
+                if (tid < FO * RB) {
+                    const int f = tid / RB, r = tid % RB;
+                    const cpx<T> z0 = lds[(f * RB + r) * HP + X::px(0)];
+                    const cpx<T> zm = conj(z0);
+                    const cpx<T> a = T(0.5) * (z0 + zm);
+                    const cpx<T> b = (T(0.5) * (z0 - zm)) * tw[M];
+                    out0[f * FS + (row0 + r) * H + M] = a + mul_i<-1>(b);
+                }
+                return;
+            }
             for (int e = tid; e < FO * RB * H; e += NTX) {
                 const int f = e / (RB * H), rk = e % (RB * H);
                 const int r = rk / H, k = rk % H;
@@ -1236,6 +1291,30 @@ __global__ void k_scale_to_energy(cpx<T>* __restrict__ u0, long long n, const do
         cpx<T> a = u0[i];
         u0[i] = cpx<T>{s * a.x, s * a.y};
     }
+}
+
+// flag |= 1 when any of the three fields holds a nonzero mode outside the kept cube (Keep):
+// decides whether a step of a caller's buffer may run the pruned passes
+template <int N, class T>
+__global__ void __launch_bounds__(256)
+    k_outside_cube(const cpx<T>* __restrict__ u0, long long fs, unsigned int* flag)
+{
+    constexpr int H = N / 2 + 1;
+    constexpr long long TOT = (long long)N * N * H;
+    bool nz = false;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < TOT;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int ix = int(idx % H);
+        const int iy = int((idx / H) % N);
+        const int iz = int(idx / ((long long)H * N));
+        if (Keep<N>::kept(ix) && Keep<N>::kept(iy) && Keep<N>::kept(iz)) continue;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const cpx<T> a = u0[f * fs + idx];
+            nz = nz || a.x != T(0) || a.y != T(0);
+        }
+    }
+    if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(flag, 1u);
 }
 
 // state.f90:120-179 state_project / state.f90:181-209 state_dealias, in place

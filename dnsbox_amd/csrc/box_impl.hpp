@@ -80,6 +80,7 @@ class BoxBase {
     int scratch_fields = 6;
     void* adjbuf = nullptr;  // 9 fields: A, Mu, u~ (adjoint passes), lazily
     double* band2 = nullptr; // band-forcing scalars of the VJP
+    unsigned int* chk = nullptr;   // flag of k_outside_cube
 
     virtual void rhs_state() = 0;                 // nonlin <- N(state)
     virtual void rhs(const void* in, void* out) = 0;
@@ -104,6 +105,8 @@ class BoxBase {
     virtual void step_vjp(const void* u, const void* lam, void* lam_out, double dt) = 0;
     virtual void rhs_adj(const void* u, void* out, double lin_sign) = 0;
     virtual void step_ext(void* st, int nsteps) = 0;
+    // step_ext that also stores the state after step j + 1 in traj + j * (3 state fields)
+    virtual void step_ext_record(void* st, int nsteps, void* traj) = 0;
     // FP64 device array -> transform type / state type, and back
     virtual void convert_in(void* dst, const double* src, long long n, bool state_type) = 0;
     virtual void convert_out(double* dst, const void* src, long long n, bool state_type) = 0;
@@ -148,6 +151,7 @@ class BoxT final : public BoxBase {
             DNSBOX_HIP(hipMalloc(&red_partial, size_t(red_blocks) * (DG_NSCAL + nspec) * sizeof(double)));
             DNSBOX_HIP(hipMalloc(&red_out, (DG_NSCAL + nspec) * sizeof(double)));
             DNSBOX_HIP(hipMalloc(&red_bits, 4 * sizeof(unsigned long long)));
+            DNSBOX_HIP(hipMalloc(&chk, sizeof(unsigned int)));
             DNSBOX_HIP(hipMemset(state, 0, 3 * size_t(fs) * sizeof(CS)));
             DNSBOX_HIP(hipMemset(pred, 0, 3 * fb));
             DNSBOX_HIP(hipMemset(nonlin, 0, 3 * fb));
@@ -176,7 +180,7 @@ class BoxT final : public BoxBase {
             (void)hipStreamDestroy(gstream);
             gstream = nullptr;
         }
-        void* ps[] = {state, pred, scratch, nonlin, nold, tw, sc, red_partial, red_out, red_bits, adjbuf, band2};
+        void* ps[] = {state, pred, scratch, nonlin, nold, tw, sc, red_partial, red_out, red_bits, adjbuf, band2, chk};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         state = pred = scratch = nonlin = nold = tw = adjbuf = nullptr;
@@ -184,6 +188,7 @@ class BoxT final : public BoxBase {
         sc = nullptr;
         red_partial = red_out = nullptr;
         red_bits = nullptr;
+        chk = nullptr;
     }
 
     const C* twc() const { return static_cast<const C*>(tw); }
@@ -673,27 +678,51 @@ class BoxT final : public BoxBase {
     }
 
     // IF-Heun steps on a caller-owned state buffer
-    void step_ext(void* st, int nsteps) override
+    // whether a caller's state buffer holds no energy outside the kept cube (synchronises)
+    bool buffer_dealiased(const CS* st)
+    {
+        DNSBOX_HIP(hipMemsetAsync(chk, 0, sizeof(unsigned int), stream));
+        hipLaunchKernelGGL((k_outside_cube<N, S>), dim3((fs + 255) / 256 < 1024 ? (fs + 255) / 256 : 1024), dim3(256),
+                           0, stream, st, fs, chk);
+        DNSBOX_LAUNCH_CHECK();
+        unsigned int h = 1;
+        DNSBOX_HIP(hipMemcpyAsync(&h, chk, sizeof(h), hipMemcpyDeviceToHost, stream));
+        DNSBOX_HIP(hipStreamSynchronize(stream));
+        return h == 0;
+    }
+
+    // IF-Heun steps of a caller's buffer, pruned when the buffer allows it (checked on the
+    // device); after each step the state is copied to traj + j * 3 fields when traj is set
+    void step_buffer(void* st, int nsteps, void* traj)
     {
         require_uniform("stepping a device buffer");
+        if (nsteps <= 0) return;
         void* own = state;
+        const bool own_dealiased = dealiased;
         x6_valid = false;
-        state = st;
         try {
-            for (int i = 0; i < nsteps; ++i) heun_step();
+            dealiased = prune && buffer_dealiased(static_cast<const CS*>(st));
+            state = st;
+            const size_t sb3 = 3 * size_t(fs) * sizeof(CS);
+            for (int i = 0; i < nsteps; ++i) {
+                heun_step();
+                if (traj)
+                    DNSBOX_HIP(hipMemcpyAsync(static_cast<char*>(traj) + i * sb3, st, sb3, hipMemcpyDeviceToDevice,
+                                              stream));
+            }
         } catch (...) {
             state = own;
+            dealiased = own_dealiased;
             x6_valid = false;
             throw;
         }
         state = own;
+        dealiased = own_dealiased;
         x6_valid = false;
     }
+    void step_ext(void* st, int nsteps) override { step_buffer(st, nsteps, nullptr); }
+    void step_ext_record(void* st, int nsteps, void* traj) override { step_buffer(st, nsteps, traj); }
 
-    // Average duration of each pass inside real IF-Heun steps, taken on a copy of the state
-    // (the box's state and clock are left as they were): an event after every launch of
-    // `reps` steps, each pass timed from the previous event.  The z-inverse of the first
-    // RHS runs once per pipeline start; it is timed over `reps` back-to-back launches.
     void pass_bytes(double* b) const override
     {
         const double cb = double(sizeof(C)), sb = double(sizeof(CS)), n = double(N);
@@ -717,6 +746,10 @@ class BoxT final : public BoxBase {
         }
     }
 
+    // Average duration of each pass inside real IF-Heun steps, taken on a copy of the state
+    // (the box's state and clock are left as they were): an event after every launch of
+    // `reps` steps, each pass timed from the previous event.  The z-inverse of the first
+    // RHS runs once per pipeline start; it is timed over `reps` back-to-back launches.
     void time_passes(int reps, double* ms) override
     {
         if (reps < 1) reps = 1;

@@ -567,6 +567,17 @@ int dnsbox_step_dev(dnsbox_ctx* c, void* state, int nsteps)
     });
 }
 
+int dnsbox_step_dev_record(dnsbox_ctx* c, void* state, int nsteps, void* traj)
+{
+    if (int e = check_ctx(c)) return e;
+    invalidate(c);
+    if (!state || !traj || nsteps < 0) return fail(DNSBOX_E_ARG, "null buffer or nsteps < 0");
+    return guarded([&]() -> int {
+        c->box->step_ext_record(state, nsteps, traj);
+        return 0;
+    });
+}
+
 int dnsbox_rhs_vjp_dev(dnsbox_ctx* c, const void* u, const void* lam, void* out)
 {
     if (int e = check_ctx(c)) return e;

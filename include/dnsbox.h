@@ -162,8 +162,13 @@ int dnsbox_state_ptr(dnsbox_ctx* ctx, void** dev_state);   /* the context's stat
                                                               pruning until the next dealias) */
 /* rhs_nonlinear on device buffers: out <- N(in) */
 int dnsbox_rhs_dev(dnsbox_ctx* ctx, const void* in, void* out);
-/* the time loop of dnsboxheun.f90:81-126 on a caller-owned device state (in place) */
+/* the time loop of dnsboxheun.f90:81-126 on a caller-owned device state (in place); the passes
+   run pruned when a device check finds the buffer empty outside the 2/3-rule cube */
 int dnsbox_step_dev(dnsbox_ctx* ctx, void* state, int nsteps);
+/* dnsbox_step_dev that also copies the state after step j + 1 to traj + j * (3 fields of the
+   internal layout), j < nsteps: the recomputed trajectory of an adjoint checkpoint segment in one
+   call (the step pipeline runs through it).  No reference counterpart. */
+int dnsbox_step_dev_record(dnsbox_ctx* ctx, void* state, int nsteps, void* traj);
 /* exact discrete adjoint of rhs_nonlinear (rhs.f90:18-135): out <- J_N(u)^T lam in the
    Euclidean inner product of the stored coordinates, sum Re(conj(a) b).  No reference
    counterpart: the reference's adjoint (rhsAdj.f90) is the continuous one below. */

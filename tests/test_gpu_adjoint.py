@@ -123,6 +123,39 @@ def test_step_dev_matches_internal_state(gpu_lib):
     assert rel_l2(host(t), ref) < 1e-14
 
 
+def test_step_dev_prunes_only_dealiased_buffers(gpu_lib):
+    """step_dev checks the caller's buffer on the device: a field with energy outside the
+    kept cube is stepped with the full passes (matches the oracle, masked modes included)
+    even when the box's own state is dealiased; step_dev_record stores the trajectory."""
+    import torch
+    n, dt = 32, 0.005
+    o = Oracle(n, nu=0.05, kolmo_amp=1.0, kolmo_k=4)
+    f = o.init_solenoidal(0, 0, 4.0, 0.5)
+    g = [x.copy() for x in f]
+    g[0][12, 3, 5] = 1e-3 + 2e-3j   # kx = 12 > (32 - 1) // 3
+    with make(n, nu=0.05, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        b.init_random(1, 0, 4.0, 0.5)
+        assert b.pruning()
+        b.rhs_int_fact(dt)
+        for field, name in ((f, "dealiased"), (g, "masked-mode energy")):
+            t = dev(field)
+            b.step_dev(t, 3)
+            err = rel_l2(host(t), o.heun(field, 3, dt))
+            print(f"step_dev {name}: rel_l2 vs oracle {err:.3e}", flush=True)
+            assert err <= 1e-12
+        t0 = dev(f)
+        traj = torch.empty((4,) + tuple(t0.shape), dtype=t0.dtype, device=t0.device)
+        u = t0.clone()
+        b.step_dev_record(u, 4, traj)
+        ref = t0.clone()
+        for j in range(4):
+            b.step_dev(ref, 1)
+            e = rel_l2(host(traj[j]), host(ref))
+            print(f"record step {j + 1}: rel_l2 {e:.3e}", flush=True)
+            assert e <= 1e-14
+        assert torch.equal(u, traj[3])
+
+
 @pytest.mark.parametrize("n,T,k", [(32, 20, 5), (64, 200, 20)])
 def test_tstep_gradient_vs_finite_differences(gpu_lib, n, T, k):
     """d/de J(u + e v) by central differences (e = 1e-6) along seeded solenoidal directions
