@@ -25,6 +25,7 @@ template <class T>
 struct GBox {
     int n;
     int zt0, yt0, xb0;  // first global z tile, y tile and x block of this box
+    int zt0p, yt0p;     // first z and y tiles of the passes pruned to the 2/3-rule cube
     long long fs;
     cpx<T>* st;          // state advanced in place by the forward step
     cpx<T>* X;           // transform scratch (9 fields)
@@ -69,13 +70,18 @@ struct GroupLds {
 };
 
 // box owning global block `blk`, by the first-block offsets selected by `which`
+// (0 z tiles, 1 y tiles, 2 x blocks, 3 pruned z tiles, 4 pruned y tiles)
 template <class T>
 __device__ __forceinline__ int find_box(const GBox<T>* __restrict__ gb, int nbox, int blk, int which)
 {
     int lo = 0, hi = nbox - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) / 2;
-        const int start = which == 0 ? gb[mid].zt0 : (which == 1 ? gb[mid].yt0 : gb[mid].xb0);
+        const int start = which == 0 ? gb[mid].zt0
+                          : which == 1 ? gb[mid].yt0
+                          : which == 2 ? gb[mid].xb0
+                          : which == 3 ? gb[mid].zt0p
+                                       : gb[mid].yt0p;
         if (start <= blk) lo = mid;
         else hi = mid - 1;
     }
@@ -100,7 +106,8 @@ __device__ __forceinline__ const cpx<T>* stage_tw_n(const cpx<T>* __restrict__ g
     }
 
 // z-inverse + curl of the state (from_u = 0) or of the step-adjoint point u (from_u = 1)
-template <class T>
+// PR: the pruned passes (box.hpp Keep; the states must be empty outside the kept cube)
+template <class T, bool PR = false>
 __global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restrict__ gb, int nbox, int from_u)
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
@@ -108,40 +115,40 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restr
     // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
     // all XCDs, which balances boxes of uneven cost
     const int gt = blockIdx.x;
-    const int b = find_box(gb, nbox, gt, 0);
+    const int b = find_box(gb, nbox, gt, PR ? 3 : 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = gt - bx.zt0;
+    const int tile = gt - (PR ? bx.zt0p : bx.zt0);
     const cpx<T>* src = from_u ? bx.u : bx.st;
     const cpx<T>* in[3] = {src, src + bx.fs, src + 2 * bx.fs};
     cpx<T>* out[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) out[f] = bx.X + f * bx.fs;
     const ProCurl<T> pro{T(1.0 / (double(bx.n) * bx.n * bx.n))};
-#define DNSBOX_G_ZINV(NN) zinv_tile<NN, T, ProCurl<T>>(in, out, tile, lds, tw, pro)
+#define DNSBOX_G_ZINV(NN) zinv_tile<NN, T, ProCurl<T>, T, PR>(in, out, tile, lds, tw, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZINV)
 #undef DNSBOX_G_ZINV
 }
 
-template <class T, int DIR>
+template <class T, int DIR, bool PR = false>
 __global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restrict__ gb, int nbox)
 {
     __shared__ cpx<T> lds[GroupLds<T>::Y];
     __shared__ cpx<T> stw[128];
     const int L = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int field = L / gridDim.x, gt = L % gridDim.x;
-    const int b = find_box(gb, nbox, gt, 1);
+    const int b = find_box(gb, nbox, gt, PR ? 4 : 1);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = gt - bx.yt0;
+    const int tile = gt - (PR ? bx.yt0p : bx.yt0);
     cpx<T>* data = bx.X + field * bx.fs;
-#define DNSBOX_G_YCOL(NN) ycol_tile<NN, T, DIR>(data, tile, lds, tw)
+#define DNSBOX_G_YCOL(NN) ycol_tile<NN, T, DIR, PR>(data, tile, lds, tw)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_YCOL)
 #undef DNSBOX_G_YCOL
 }
 
 // x pass of Op in place on the scratch; OpCross can track the CFL maxima
-template <class T, class Op>
+template <class T, class Op, bool PR = false>
 __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __restrict__ gb, int nbox, int track)
 {
     __shared__ cpx<T> lds[GroupLds<T>::template X<Op>];
@@ -154,7 +161,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __rest
     XRed<T> red;
 #pragma unroll
     for (int j = 0; j < 4; ++j) red.m[j] = T(0);
-#define DNSBOX_G_X(NN) xpass_tile<NN, T, Op>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
+#define DNSBOX_G_X(NN) xpass_tile<NN, T, Op, PR>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_X)
 #undef DNSBOX_G_X
     if constexpr (Op::NRED == 3) {
@@ -162,7 +169,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __rest
     }
 }
 
-template <class T, int MODE>
+template <class T, int MODE, bool PR = false>
 __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __restrict__ gb, int nbox)
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
@@ -170,10 +177,10 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
     // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
     // all XCDs, which balances boxes of uneven cost
     const int gt = blockIdx.x;
-    const int b = find_box(gb, nbox, gt, 0);
+    const int b = find_box(gb, nbox, gt, PR ? 3 : 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = gt - bx.zt0;
+    const int tile = gt - (PR ? bx.zt0p : bx.zt0);
     EpiArgs<T, T> ea{};
     ea.out = nullptr;
     ea.state = bx.st;
@@ -183,7 +190,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
     ea.sc = bx.sc;
     ea.ph = bx.ph;
     const ProCurl<T> pro{T(1.0 / (double(bx.n) * bx.n * bx.n))};
-#define DNSBOX_G_Z(NN) zfwd_zinv_tile<NN, T, MODE, T>(bx.X, bx.fs, tile, lds, tw, ea, pro)
+#define DNSBOX_G_Z(NN) zfwd_zinv_tile<NN, T, MODE, T, PR>(bx.X, bx.fs, tile, lds, tw, ea, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_Z)
 #undef DNSBOX_G_Z
 }
@@ -300,6 +307,22 @@ inline int group_ytiles(int n, int prec)
 #undef DNSBOX_GT
     return (n * (n / 2 + 1) + w - 1) / w;
 }
+inline int group_ztiles_pruned(int n, int prec)
+{
+    int w = 0, c = 0;
+#define DNSBOX_GT(NN) w = prec == 64 ? Cfg<NN, double>::ZW : Cfg<NN, float>::ZW, c = Keep<NN>::ZCOLS
+    DNSBOX_GROUP_SWITCH(n, DNSBOX_GT)
+#undef DNSBOX_GT
+    return (c + w - 1) / w;
+}
+inline int group_ytiles_pruned(int n, int prec)
+{
+    int w = 0, c = 0;
+#define DNSBOX_GT(NN) w = prec == 64 ? Cfg<NN, double>::YW : Cfg<NN, float>::YW, c = Keep<NN>::YCOLS
+    DNSBOX_GROUP_SWITCH(n, DNSBOX_GT)
+#undef DNSBOX_GT
+    return (c + w - 1) / w;
+}
 inline int group_xblocks(int n, int prec)
 {
     int rb = 0;
@@ -328,7 +351,10 @@ class Group final : public GroupBase {
     GBox<T>* d_gb = nullptr;          // descriptors of the boxes' own states (read by the graph)
     std::vector<GBox<T>> h_gb;
     int zt = 0, yt = 0, xb = 0;
+    int ztp = 0, ytp = 0;             // tiles of the pruned passes
     bool x6_valid = false;
+    bool x6_pruned = false;           // the scratch was written by the pruned passes
+    int graph_pruned = -1;            // mode the graph was captured in
     hipStream_t gstream = nullptr;
     hipGraphExec_t graph = nullptr;   // graph_steps steps of every box
     int graph_mode = -1;
@@ -351,9 +377,13 @@ class Group final : public GroupBase {
             g.zt0 = zt;
             g.yt0 = yt;
             g.xb0 = xb;
+            g.zt0p = ztp;
+            g.yt0p = ytp;
             zt += group_ztiles(g.n, prec);
             yt += group_ytiles(g.n, prec);
             xb += group_xblocks(g.n, prec);
+            ztp += group_ztiles_pruned(g.n, prec);
+            ytp += group_ytiles_pruned(g.n, prec);
             fill(g, *b);
             h_gb.push_back(g);
         }
@@ -428,26 +458,43 @@ class Group final : public GroupBase {
         return d;
     }
 
+    template <bool PR = false>
     void launch_zinv(hipStream_t s, const GBox<T>* d, int from_u)
     {
-        hipLaunchKernelGGL((k_g_zinv<T>), dim3(zt), dim3(kGroupThreads), 0, s, d, int(h_gb.size()), from_u);
+        hipLaunchKernelGGL((k_g_zinv<T, PR>), dim3(PR ? ztp : zt), dim3(kGroupThreads), 0, s, d, int(h_gb.size()),
+                           from_u);
         DNSBOX_LAUNCH_CHECK();
     }
 
     // one IF-Heun step of every box, its z-inverse + curl already in the scratch
+    template <bool PR = false>
     void one_step(hipStream_t s, const GBox<T>* d)
     {
         const int nb = int(h_gb.size());
-        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
-        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 1);
-        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 3), dim3(kGroupThreads), 0, s, d, nb);
-        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_PRED>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
-        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
-        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
-        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 3), dim3(kGroupThreads), 0, s, d, nb);
-        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_CORR>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
+        const int z = PR ? ztp : zt, y = PR ? ytp : yt;
+        hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(y, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>, PR>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 1);
+        hipLaunchKernelGGL((k_g_ycol<T, -1, PR>), dim3(y, 3), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_PRED, PR>), dim3(z), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(y, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>, PR>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, -1, PR>), dim3(y, 3), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_CORR, PR>), dim3(z), dim3(kGroupThreads), 0, s, d, nb);
         hipLaunchKernelGGL((k_g_step_end<T>), dim3((nb + 63) / 64), dim3(64), 0, s, d, nb);
         DNSBOX_LAUNCH_CHECK();
+    }
+    void one_step(hipStream_t s, const GBox<T>* d, bool pr)
+    {
+        if (pr) one_step<true>(s, d);
+        else one_step<false>(s, d);
+    }
+
+    // the boxes' own states run the pruned passes when every box's state allows it
+    bool pruned() const
+    {
+        for (auto& b : boxes)
+            if (!b->pruned()) return false;
+        return true;
     }
 
     // IF-Heun steps of every box: the first z-inverse if needed, then graph_steps steps
@@ -462,9 +509,12 @@ class Group final : public GroupBase {
         DNSBOX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         DNSBOX_HIP(hipEventRecord(ev, stream));
         DNSBOX_HIP(hipStreamWaitEvent(gstream, ev, 0));
-        if (!x6_valid) {
-            launch_zinv(gstream, d_gb, 0);
+        const bool pr = pruned();
+        if (!x6_valid || x6_pruned != pr) {
+            if (pr) launch_zinv<true>(gstream, d_gb, 0);
+            else launch_zinv<false>(gstream, d_gb, 0);
             x6_valid = true;
+            x6_pruned = pr;
         }
         if (graph_mode < 0) {
             const char* e = std::getenv("DNSBOX_STEP_GRAPH");
@@ -472,18 +522,24 @@ class Group final : public GroupBase {
             const char* k = std::getenv("DNSBOX_GRAPH_STEPS");
             if (k && std::atoi(k) > 0) graph_steps = std::atoi(k);
         }
+        if (graph && graph_pruned != int(pr)) {
+            DNSBOX_HIP(hipStreamSynchronize(gstream));
+            DNSBOX_HIP(hipGraphExecDestroy(graph));
+            graph = nullptr;
+        }
         if (graph_mode && nsteps >= graph_steps && !graph) {
             hipGraph_t g;
             DNSBOX_HIP(hipStreamBeginCapture(gstream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < graph_steps; ++i) one_step(gstream, d_gb);
+            for (int i = 0; i < graph_steps; ++i) one_step(gstream, d_gb, pr);
             DNSBOX_HIP(hipStreamEndCapture(gstream, &g));
             DNSBOX_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
             DNSBOX_HIP(hipGraphDestroy(g));
+            graph_pruned = int(pr);
         }
         int done = 0;
         if (graph_mode)
             for (; done + graph_steps <= nsteps; done += graph_steps) DNSBOX_HIP(hipGraphLaunch(graph, gstream));
-        for (; done < nsteps; ++done) one_step(gstream, d_gb);
+        for (; done < nsteps; ++done) one_step(gstream, d_gb, pr);
         DNSBOX_HIP(hipEventRecord(ev, gstream));
         DNSBOX_HIP(hipStreamWaitEvent(stream, ev, 0));
         DNSBOX_HIP(hipEventDestroy(ev));
@@ -500,7 +556,7 @@ class Group final : public GroupBase {
         const GBox<T>* d = descriptors(h);
         x6_valid = false;
         launch_zinv(stream, d, 0);
-        for (int i = 0; i < nsteps; ++i) one_step(stream, d);
+        for (int i = 0; i < nsteps; ++i) one_step<false>(stream, d);
     }
 
     // out_i = (d u_{n+1} / d u_n)^T lam_i for one fixed-dt IF-Heun step of every box

@@ -1,6 +1,6 @@
 """Groups of uneven boxes stepped by one launch per pass reproduce the same boxes stepped
-one by one (the group kernels run the same tile functions; the single boxes run their
-unpruned passes, as the group does)."""
+one by one (the group kernels run the same tile functions, pruned to the 2/3-rule cube when
+every member's state allows it, as a single box does), to rounding."""
 import numpy as np
 import pytest
 
@@ -33,7 +33,6 @@ def test_group_matches_single_boxes(gpu_lib, prec):
         times = [b.time() for b in g.boxes]
     for i, kw in enumerate(params):
         with Box(**kw) as b:
-            b.set_pruning(False)   # the group runs the unpruned passes
             b.init_random(100 + i, i, k0s[i], 0.5)
             b.step(15)
             ref = b.get_state()
@@ -41,9 +40,12 @@ def test_group_matches_single_boxes(gpu_lib, prec):
         num = sum(np.linalg.norm((x - y).ravel()) ** 2 for x, y in zip(got[i], ref))
         den = sum(np.linalg.norm(y.ravel()) ** 2 for y in ref)
         err = float(np.sqrt(num / den))
-        print(f"group box {i} n={kw['n']} prec={prec} rel={err:.2e} t={times[i][0]:.6f}", flush=True)
-        assert err == 0.0 or err < 1e-14
-        assert times[i] == t
+        print(f"group box {i} n={kw['n']} prec={prec} rel={err:.2e} t={times[i][0]!r} vs {t[0]!r}", flush=True)
+        # the group kernels run the single-box tile functions in other kernels: the compiler may
+        # contract multiply-adds differently, so the agreement is to rounding
+        tol = 1e-14 if prec == 64 else 1e-6
+        assert err <= tol
+        assert abs(times[i][0] - t[0]) <= tol * t[0] and abs(times[i][1] - t[1]) <= tol * t[1]
 
 
 def test_group_graph_replay_matches_direct_launches(gpu_lib):
