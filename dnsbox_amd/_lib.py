@@ -65,7 +65,8 @@ SIGNATURES = {
     "dnsbox_time_passes": (C.c_int, [_vp, C.c_int, _dp]),
     "dnsbox_state_ptr": (C.c_int, [_vp, C.POINTER(_vp)]),
     "dnsbox_pass_bytes": (C.c_int, [_vp, _dp]),
-    "dnsbox_step_dev_record": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    "dnsbox_step_dev_record": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
+    "dnsbox_step_vjp_pred_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double]),
     "dnsbox_set_pruning": (C.c_int, [_vp, C.c_int]),
     "dnsbox_get_pruning": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "dnsbox_rhs_dev": (C.c_int, [_vp, _vp, _vp]),

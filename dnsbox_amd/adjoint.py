@@ -53,17 +53,23 @@ class _StepMap(torch.autograd.Function):
         box.rhs_int_fact(dt)
         lam = g.detach().clone().contiguous()
         out = torch.empty_like(lam)
-        traj = torch.empty((max(k - 1, 1),) + tuple(lam.shape), dtype=lam.dtype, device=lam.device)
+        shape = (max(k - 1, 1),) + tuple(lam.shape)
+        traj = torch.empty(shape, dtype=lam.dtype, device=lam.device)
+        preds = torch.empty(shape, dtype=lam.dtype, device=lam.device)
         for seg in reversed(range(len(ctx.ckpts))):
             start = seg * k
             stop = min(start + k, T)
-            # the segment's states u_start .. u_{stop-1}, recomputed in one call
+            # the segment's states u_start .. u_{stop-1} and the predicted states of its steps but
+            # the last, recomputed in one call; the last step's VJP recomputes its own
             if stop - start > 1:
                 u = ctx.ckpts[seg].clone()
-                box.step_dev_record(u, stop - start - 1, traj)
+                box.step_dev_record(u, stop - start - 1, traj, preds)
             states = [ctx.ckpts[seg]] + [traj[j] for j in range(stop - start - 1)]
             for j in reversed(range(stop - start)):
-                box.step_vjp_dev(states[j], lam, out, dt)
+                if j < stop - start - 1:
+                    box.step_vjp_pred_dev(states[j], preds[j], lam, out, dt)
+                else:
+                    box.step_vjp_dev(states[j], lam, out, dt)
                 lam, out = out, lam
         return lam, None, None, None, None
 

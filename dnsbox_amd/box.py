@@ -253,9 +253,7 @@ class Box:
         self._on_torch_stream()
         check(self.lib.dnsbox_step_dev(self._h, self._check_tensor(state), nsteps))
 
-    def step_dev_record(self, state, nsteps: int, traj):
-        """step_dev, storing the state after step j + 1 in traj[j] (traj: a contiguous device
-        tensor of shape (>= nsteps, 3, N, N, N/2+1) of the context precision)."""
+    def _check_traj(self, traj, nsteps):
         import torch
         want = torch.complex128 if self.precision == 64 else torch.complex64
         shape = (3, self.n, self.n, self.n // 2 + 1)
@@ -263,9 +261,22 @@ class Box:
                 and traj.shape[0] >= nsteps):
             raise ValueError(f"expected a contiguous cuda {want} tensor of shape (>= {nsteps}, {shape}), "
                              f"got {traj.dtype} {tuple(traj.shape)}")
+        return C.c_void_p(traj.data_ptr())
+
+    def step_dev_record(self, state, nsteps: int, traj, traj_pred=None):
+        """step_dev, storing the state after step j + 1 in traj[j] and, if given, that step's
+        predicted state u* in traj_pred[j] (contiguous device tensors of shape
+        (>= nsteps, 3, N, N, N/2+1) of the context precision)."""
         self._on_torch_stream()
+        tp = self._check_traj(traj_pred, nsteps) if traj_pred is not None else None
         check(self.lib.dnsbox_step_dev_record(self._h, self._check_tensor(state), nsteps,
-                                              C.c_void_p(traj.data_ptr())))
+                                              self._check_traj(traj, nsteps), tp))
+
+    def step_vjp_pred_dev(self, u, ustar, lam, out, dt: float):
+        """step_vjp_dev with the step's predicted state u* supplied (from step_dev_record)."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_step_vjp_pred_dev(self._h, self._check_tensor(u), self._check_tensor(ustar),
+                                                self._check_tensor(lam), self._check_tensor(out), dt))
 
     def rhs_vjp_dev(self, u, lam, out):
         """out <- J_N(u)^T lam (Euclidean inner product of the stored coordinates)."""

@@ -77,6 +77,7 @@ class BoxBase {
     bool dealiased = true;
     bool prune = true;       // run the step's passes pruned to the kept cube when dealiased
     bool pruned() const { return prune && dealiased; }
+    void* pred_out_ = nullptr;   // where the IF-Heun step stores u* (null: internal buffer)
     int scratch_fields = 6;
     void* adjbuf = nullptr;  // 9 fields: A, Mu, u~ (adjoint passes), lazily
     double* band2 = nullptr; // band-forcing scalars of the VJP
@@ -105,8 +106,11 @@ class BoxBase {
     virtual void step_vjp(const void* u, const void* lam, void* lam_out, double dt) = 0;
     virtual void rhs_adj(const void* u, void* out, double lin_sign) = 0;
     virtual void step_ext(void* st, int nsteps) = 0;
-    // step_ext that also stores the state after step j + 1 in traj + j * (3 state fields)
-    virtual void step_ext_record(void* st, int nsteps, void* traj) = 0;
+    // step_ext that also stores the state after step j + 1 in traj + j * (3 state fields) and,
+    // when traj_pred is set, the predicted state u* of step j + 1 in traj_pred + j * 3 fields
+    virtual void step_ext_record(void* st, int nsteps, void* traj, void* traj_pred) = 0;
+    // step_vjp at u with the step's predicted state u* = E (u + dt N(u)) given
+    virtual void step_vjp_pred(const void* u, const void* ustar, const void* lam, void* lam_out, double dt) = 0;
     // FP64 device array -> transform type / state type, and back
     virtual void convert_in(void* dst, const double* src, long long n, bool state_type) = 0;
     virtual void convert_out(double* dst, const void* src, long long n, bool state_type) = 0;
@@ -314,7 +318,9 @@ class BoxT final : public BoxBase {
         }
         rhs_middle<PR>(true);
         EpiArgs<T, S> ea = epi();
-        ea.out = cfg.ph.bandlim ? Pr() : nullptr;  // the predicted state is kept only for band forcing
+        // the predicted state is kept for band forcing, or in the caller's buffer when recorded
+        C* pst = pred_out_ ? static_cast<C*>(pred_out_) : Pr();
+        ea.out = (cfg.ph.bandlim || pred_out_) ? pst : nullptr;
         ea.state = St();
         ea.rhs_in = St();
         ea.rhs_in_s = 1;
@@ -322,11 +328,11 @@ class BoxT final : public BoxBase {
                            X(), fs, twc(), ea, pro);
         DNSBOX_LAUNCH_CHECK();
         mark(4);
-        eband(Pr());
+        eband(pst);
         rhs_middle<PR>(false);
         EpiArgs<T, S> eb = epi();
         eb.state = St();
-        eb.rhs_in = Pr();
+        eb.rhs_in = pst;
         eb.rhs_in_s = 0;
         hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream,
                            X(), fs, twc(), eb, pro);
@@ -603,7 +609,6 @@ class BoxT final : public BoxBase {
         ensure_scratch(9);
         ensure_adj();
         const C* u = static_cast<const C*>(u_);
-        const C* lam = static_cast<const C*>(lam_);
         // u* = E (u + dt N(u))
         eband(u);
         zinv_curl(u);
@@ -617,13 +622,29 @@ class BoxT final : public BoxBase {
         hipLaunchKernelGGL((k_zfwd<N, T, EPI_PREDONLY, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
                            ea);
         DNSBOX_LAUNCH_CHECK();
+        step_vjp_at(u, Pr(), lam_, lam_out, dt);
+    }
+
+    void step_vjp_pred(const void* u_, const void* ustar, const void* lam_, void* lam_out, double dt) override
+    {
+        require_uniform("the step adjoint");
+        x6_valid = false;
+        ensure_scratch(9);
+        ensure_adj();
+        step_vjp_at(static_cast<const C*>(u_), static_cast<const C*>(ustar), lam_, lam_out, dt);
+    }
+
+    // the two VJP sequences of one step at u with its predicted state us
+    void step_vjp_at(const C* u, const C* us, const void* lam_, void* lam_out, double dt)
+    {
+        const C* lam = static_cast<const C*>(lam_);
         // lam_* = J(u*)^T (dt/2 lam);  A = E (lam + lam_*),  Mu = E (dt/2 lam + dt lam_*)
         VjpArgs<T> v1{};
         v1.lam = lam;
         v1.A = adjA();
         v1.Mu = adjMu();
         v1.dt = dt;
-        vjp_passes<VJP_S1>(Pr(), lam, 0.5 * dt, v1);
+        vjp_passes<VJP_S1>(us, lam, 0.5 * dt, v1);
         // lam_n = A + J(u)^T Mu
         VjpArgs<T> v2{};
         v2.out = static_cast<C*>(lam_out);
@@ -692,19 +713,23 @@ class BoxT final : public BoxBase {
     }
 
     // IF-Heun steps of a caller's buffer, pruned when the buffer allows it (checked on the
-    // device); after each step the state is copied to traj + j * 3 fields when traj is set
-    void step_buffer(void* st, int nsteps, void* traj)
+    // device); after step j the state is copied to traj + j * 3 fields when traj is set, and
+    // the step's predicted state is stored at traj_pred + j * 3 fields when that is set (the
+    // pruned step writes only the kept modes, so the slots are cleared first)
+    void step_buffer(void* st, int nsteps, void* traj, void* traj_pred)
     {
         require_uniform("stepping a device buffer");
         if (nsteps <= 0) return;
         void* own = state;
         const bool own_dealiased = dealiased;
         x6_valid = false;
+        const size_t sb3 = 3 * size_t(fs) * sizeof(CS);
         try {
             dealiased = prune && buffer_dealiased(static_cast<const CS*>(st));
             state = st;
-            const size_t sb3 = 3 * size_t(fs) * sizeof(CS);
+            if (traj_pred && dealiased) DNSBOX_HIP(hipMemsetAsync(traj_pred, 0, nsteps * sb3, stream));
             for (int i = 0; i < nsteps; ++i) {
+                pred_out_ = traj_pred ? static_cast<char*>(traj_pred) + i * sb3 : nullptr;
                 heun_step();
                 if (traj)
                     DNSBOX_HIP(hipMemcpyAsync(static_cast<char*>(traj) + i * sb3, st, sb3, hipMemcpyDeviceToDevice,
@@ -712,16 +737,21 @@ class BoxT final : public BoxBase {
             }
         } catch (...) {
             state = own;
+            pred_out_ = nullptr;
             dealiased = own_dealiased;
             x6_valid = false;
             throw;
         }
         state = own;
+        pred_out_ = nullptr;
         dealiased = own_dealiased;
         x6_valid = false;
     }
-    void step_ext(void* st, int nsteps) override { step_buffer(st, nsteps, nullptr); }
-    void step_ext_record(void* st, int nsteps, void* traj) override { step_buffer(st, nsteps, traj); }
+    void step_ext(void* st, int nsteps) override { step_buffer(st, nsteps, nullptr, nullptr); }
+    void step_ext_record(void* st, int nsteps, void* traj, void* traj_pred) override
+    {
+        step_buffer(st, nsteps, traj, traj_pred);
+    }
 
     void pass_bytes(double* b) const override
     {

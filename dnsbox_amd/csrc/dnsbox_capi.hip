@@ -567,13 +567,29 @@ int dnsbox_step_dev(dnsbox_ctx* c, void* state, int nsteps)
     });
 }
 
-int dnsbox_step_dev_record(dnsbox_ctx* c, void* state, int nsteps, void* traj)
+int dnsbox_step_dev_record(dnsbox_ctx* c, void* state, int nsteps, void* traj, void* traj_pred)
 {
     if (int e = check_ctx(c)) return e;
     invalidate(c);
     if (!state || !traj || nsteps < 0) return fail(DNSBOX_E_ARG, "null buffer or nsteps < 0");
+    if (traj_pred == traj || traj_pred == state) return fail(DNSBOX_E_ARG, "traj_pred must not alias traj or state");
     return guarded([&]() -> int {
-        c->box->step_ext_record(state, nsteps, traj);
+        c->box->step_ext_record(state, nsteps, traj, traj_pred);
+        return 0;
+    });
+}
+
+int dnsbox_step_vjp_pred_dev(dnsbox_ctx* c, const void* u, const void* ustar, const void* lam, void* lam_out,
+                             double dt)
+{
+    if (int e = check_ctx(c)) return e;
+    invalidate(c);
+    if (!u || !ustar || !lam || !lam_out) return fail(DNSBOX_E_ARG, "null device pointer");
+    if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+    if (lam_out == u || lam_out == lam || lam_out == ustar)
+        return fail(DNSBOX_E_ARG, "lam_out must not alias u, ustar or lam");
+    return guarded([&]() -> int {
+        c->box->step_vjp_pred(u, ustar, lam, lam_out, dt);
         return 0;
     });
 }

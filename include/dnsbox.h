@@ -166,9 +166,10 @@ int dnsbox_rhs_dev(dnsbox_ctx* ctx, const void* in, void* out);
    run pruned when a device check finds the buffer empty outside the 2/3-rule cube */
 int dnsbox_step_dev(dnsbox_ctx* ctx, void* state, int nsteps);
 /* dnsbox_step_dev that also copies the state after step j + 1 to traj + j * (3 fields of the
-   internal layout), j < nsteps: the recomputed trajectory of an adjoint checkpoint segment in one
-   call (the step pipeline runs through it).  No reference counterpart. */
-int dnsbox_step_dev_record(dnsbox_ctx* ctx, void* state, int nsteps, void* traj);
+   internal layout), j < nsteps, and (traj_pred not NULL) stores that step's predicted state
+   u* = E (u + dt N(u)) at traj_pred + j * 3 fields: the recomputed trajectory of an adjoint
+   checkpoint segment in one call (the step pipeline runs through it).  No reference counterpart. */
+int dnsbox_step_dev_record(dnsbox_ctx* ctx, void* state, int nsteps, void* traj, void* traj_pred);
 /* exact discrete adjoint of rhs_nonlinear (rhs.f90:18-135): out <- J_N(u)^T lam in the
    Euclidean inner product of the stored coordinates, sum Re(conj(a) b).  No reference
    counterpart: the reference's adjoint (rhsAdj.f90) is the continuous one below. */
@@ -176,6 +177,10 @@ int dnsbox_rhs_vjp_dev(dnsbox_ctx* ctx, const void* u, const void* lam, void* ou
 /* adjoint of one fixed-dt IF-Heun step (dnsboxheun.f90:84-126) taken from state u:
    lam_out <- (d u_{n+1} / d u_n)^T lam.  Backward step of the T-step map Phi_T. */
 int dnsbox_step_vjp_dev(dnsbox_ctx* ctx, const void* u, const void* lam, void* lam_out, double dt);
+/* dnsbox_step_vjp_dev with the step's predicted state ustar = E (u + dt N(u)) supplied (as
+   recorded by dnsbox_step_dev_record), so it is not recomputed.  No reference counterpart. */
+int dnsbox_step_vjp_pred_dev(dnsbox_ctx* ctx, const void* u, const void* ustar, const void* lam, void* lam_out,
+                             double dt);
 /* rhsAdj (rhsAdj.f90:356-381 via rhsNonlinearAdj :261-354, rhsupptilde :177-213, convective
    rhsNonlinear :64-175), with the never-assigned phatadj (:343-348) read as phat (:335-340).
    lin_sign = +1 reproduces the reference's L u~ - P n_adj; lin_sign = -1 gives the descent

@@ -145,14 +145,23 @@ def test_step_dev_prunes_only_dealiased_buffers(gpu_lib):
             assert err <= 1e-12
         t0 = dev(f)
         traj = torch.empty((4,) + tuple(t0.shape), dtype=t0.dtype, device=t0.device)
+        preds = torch.full_like(traj, 7.0)   # stale values outside the kept cube must not survive
         u = t0.clone()
-        b.step_dev_record(u, 4, traj)
+        b.step_dev_record(u, 4, traj, preds)
         ref = t0.clone()
+        lam = dev(o.init_solenoidal(5, 0, 4.0, 0.5))
         for j in range(4):
+            prev = ref.clone()
             b.step_dev(ref, 1)
             e = rel_l2(host(traj[j]), host(ref))
-            print(f"record step {j + 1}: rel_l2 {e:.3e}", flush=True)
-            assert e <= 1e-14
+            # the recorded predicted state gives the step adjoint of the recomputed one
+            o1, o2 = torch.empty_like(lam), torch.empty_like(lam)
+            b.step_vjp_dev(prev, lam, o1, dt)
+            b.step_vjp_pred_dev(prev, preds[j], lam, o2, dt)
+            ev = rel_l2(host(o2), host(o1))
+            print(f"record step {j + 1}: state rel_l2 {e:.3e}, step-VJP with recorded u* rel_l2 {ev:.3e}",
+                  flush=True)
+            assert e <= 1e-14 and ev <= 1e-14
         assert torch.equal(u, traj[3])
 
 
