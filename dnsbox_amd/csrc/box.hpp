@@ -75,7 +75,10 @@ struct BoxPhys {
     int kolmo_k, bandlim, spherical, parity, adaptive, pad;
 };
 
-__device__ __forceinline__ int kval(int i, int n) { return i < n / 2 ? i : (i == n / 2 ? 0 : i - n); }
+__host__ __device__ __forceinline__ constexpr int kval(int i, int n)
+{
+    return i < n / 2 ? i : (i == n / 2 ? 0 : i - n);
+}
 
 // rhs.f90:74-88: cubic ".ge." 2/3 rule on the table wavenumbers (Nyquist k = 0),
 // spherical option, k = 0; parity = 0 also removes the Nyquist-index modes.
@@ -120,9 +123,68 @@ struct Keep {
     {
         return y <= KMAX ? y : (y <= 2 * KMAX ? y + N - 2 * KMAX - 1 : N / 2);
     }
-    // column c of the pruned z pass -> column index iy * H + ix of the full layout
-    __device__ __forceinline__ static int zcol(int c) { return iy(c / KX) * H + ix(c % KX); }
+    // column iy * H + ix of the full layout is a kept (ky, kx) column of the z passes
+    __host__ __device__ __forceinline__ static constexpr bool kept_col(int col)
+    {
+        if (col < 0 || col >= N * H) return false;
+        const int ky = kval(col / H, N), x = col % H;
+        return ky <= KMAX && ky >= -KMAX && (x <= KMAX || x == N / 2);
+    }
 };
+
+// Tiles of the pruned z passes: the W-column chunks [W j, W j + W) of the full (ky, kx)
+// column index that hold a kept column.  A kept row of the z-plane holds a run of KMAX + 1
+// columns and its Nyquist column (the row's last, just before the next row's run); rows are
+// N/2 + 1 columns long, so their runs start at every offset within a cache line.  Tiles that
+// follow the chunks keep every row of a tile in one aligned segment of W columns (one 128-byte
+// line at W = 16 FP32 columns) at the price of idle lanes (a fill of 85 % at 256^3, W = 16).
+template <int N, int W>
+constexpr bool chunk_kept(int j)
+{
+    bool any = false;
+    for (int c = 0; c < W; ++c) any = any || Keep<N>::kept_col(j * W + c);
+    return any;
+}
+template <int N, int W>
+constexpr int kept_chunk_count()
+{
+    int n = 0;
+    for (int j = 0; j * W < N * (N / 2 + 1); ++j) n += chunk_kept<N, W>(j) ? 1 : 0;
+    return n;
+}
+template <int N, int W>
+struct KeptChunks {
+    static constexpr int NCH = kept_chunk_count<N, W>();
+    static_assert(N * (N / 2 + 1) / W < 65536, "chunk index fits 16 bits");
+    unsigned short j[NCH];
+};
+template <int N, int W>
+constexpr KeptChunks<N, W> make_kept_chunks()
+{
+    KeptChunks<N, W> t{};
+    int n = 0;
+    for (int j = 0; j * W < N * (N / 2 + 1); ++j)
+        if (chunk_kept<N, W>(j)) t.j[n++] = (unsigned short)j;
+    return t;
+}
+template <int N, int W>
+__device__ constexpr KeptChunks<N, W> kept_chunks = make_kept_chunks<N, W>();
+
+// column of lane c in tile `tile` of a z pass (full layout index iy * H + ix) and whether it
+// is one the pass transforms; an idle lane gets its chunk's first column (in range)
+template <int N, int W, bool PR>
+__device__ __forceinline__ int ztile_col(int tile, int c, bool& valid)
+{
+    if constexpr (PR) {
+        const int base = W * int(kept_chunks<N, W>.j[tile]);
+        valid = Keep<N>::kept_col(base + c);
+        return valid ? base + c : base;
+    } else {
+        const int col = tile * W + c;
+        valid = col < N * (N / 2 + 1);
+        return col;
+    }
+}
 
 __device__ __forceinline__ unsigned long long dbl_bits(double v)
 {
@@ -304,9 +366,8 @@ __device__ __forceinline__ void zinv_tile(const cpx<TI>* const* __restrict__ in,
     constexpr long long ZS = (long long)N * H;
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
-    const int tc = tile * W + c;
-    const bool valid = tc < (PR ? Keep<N>::ZCOLS : G::ZCOLS);
-    const int col = PR ? Keep<N>::zcol(valid ? tc : 0) : tc;
+    bool valid;
+    const int col = ztile_col<N, W, PR>(tile, c, valid);
     cpx<T> U[Pro::FI][RM];
 #pragma unroll
     for (int f = 0; f < Pro::FI; ++f)
@@ -322,7 +383,8 @@ __device__ __forceinline__ void zinv_tile(const cpx<TI>* const* __restrict__ in,
 template <int N, class T, bool PR>
 constexpr int ztiles()
 {
-    return ((PR ? Keep<N>::ZCOLS : Geo<N, T>::ZCOLS) + Cfg<N, T>::ZW - 1) / Cfg<N, T>::ZW;
+    if constexpr (PR) return KeptChunks<N, Cfg<N, T>::ZW>::NCH;
+    else return (Geo<N, T>::ZCOLS + Cfg<N, T>::ZW - 1) / Cfg<N, T>::ZW;
 }
 template <int N, class T, bool PR>
 constexpr int ytiles()
@@ -1006,9 +1068,8 @@ __device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long
                                                const cpx<T>* tw, const EpiArgs<T, S>& ea, const ProCurl<T>& pro)
 {
     using G = Geo<N, T>;
-    const int tc = tile * G::C::ZW + threadIdx.x % G::C::ZW;
-    const bool valid = tc < (PR ? Keep<N>::ZCOLS : G::ZCOLS);
-    const int col = PR ? Keep<N>::zcol(valid ? tc : 0) : tc;
+    bool valid;
+    const int col = ztile_col<N, G::C::ZW, PR>(tile, threadIdx.x % G::C::ZW, valid);
     cpx<T> O[3][G::ZRM];
 #if DNSBOX_Z_PREFETCH
     // the state at this thread's modes (symmetric z plan: output positions = input positions),

@@ -309,11 +309,11 @@ inline int group_ytiles(int n, int prec)
 }
 inline int group_ztiles_pruned(int n, int prec)
 {
-    int w = 0, c = 0;
-#define DNSBOX_GT(NN) w = prec == 64 ? Cfg<NN, double>::ZW : Cfg<NN, float>::ZW, c = Keep<NN>::ZCOLS
+    int t = 0;
+#define DNSBOX_GT(NN) t = prec == 64 ? ztiles<NN, double, true>() : ztiles<NN, float, true>()
     DNSBOX_GROUP_SWITCH(n, DNSBOX_GT)
 #undef DNSBOX_GT
-    return (c + w - 1) / w;
+    return t;
 }
 inline int group_ytiles_pruned(int n, int prec)
 {
