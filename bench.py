@@ -19,6 +19,8 @@ Printed as one JSON line:
                          launch time vs the 8 TB/s HBM peak; traffic = measured HBM bytes
                          per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
                          when a measurement of this kernel is on file
+  large_box              the same precision form at 512^3 (nu = 0.002): FP32-form and FP64
+                         steps/s and the FP32 form's error vs FP64 after 1000 fixed-dt steps
   adjoint                64^3 FP64 checkpointed T-step gradients per second (T = 200,
                          dt = 0.005, checkpoint every 20) and one finite-difference check
   batched                48 boxes (16 x 32^3, 16 x 48^3, 16 x 64^3, FP32, own nu and k0):
@@ -62,6 +64,11 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (1/0)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--reps", type=int, default=20, help="launches per pass for kernel timing")
+    ap.add_argument("--large-n", type=int, default=512,
+                    help="the larger precision-form box (nu = 0.002), reported beside the headline (0 = skip)")
+    ap.add_argument("--large-steps", type=int, default=100, help="timed steps of the larger box")
+    ap.add_argument("--large-err-steps", type=int, default=1000,
+                    help="fixed-dt steps of the larger box's FP32-vs-FP64 error runs (0 = skip)")
     return ap.parse_args()
 
 
@@ -137,6 +144,30 @@ def errors(ref, got):
     return {"spectrum_max_shell": float(np.max(np.abs(s32[nz] - s64[nz]) / s64[nz])),
             "spectrum_l2": float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64)),
             "energy": float(abs(d32[0] - d64[0]) / d64[0])}
+
+
+def large_form(n, steps, err_steps):
+    """The precision form at a larger size (nu = 0.002): steps/s of the FP32 form with the FP64
+    state and of FP64 (adaptive dt, CFL 0.5), and the error of the former against the latter after
+    err_steps fixed-dt steps from the same field."""
+    global NU
+    saved, NU = NU, 0.002
+    try:
+        rates = {}
+        for key, prec, sp, k in (("fp32_state64", 32, 64, steps), ("fp64", 64, 0, max(1, steps // 2))):
+            b = make_box(n, prec, state_precision=sp)
+            b.init_random(0, 0, 4.0, 0.5)
+            rates[key] = k / timed_steps(b, k, 10)
+            b.close()
+        out = {"n": n, "nu": NU, "steps_per_s": rates["fp32_state64"], "fp64_steps_per_s": rates["fp64"],
+               "speedup_vs_fp64": rates["fp32_state64"] / rates["fp64"]}
+        if err_steps > 0:
+            ref = run_fixed(n, 64, err_steps)
+            out["err_vs_fp64"] = errors(ref, run_fixed(n, 32, err_steps, state_precision=64))
+            out["err_steps"], out["err_dt"] = err_steps, err_dt(n)
+        return out
+    finally:
+        NU = saved
 
 
 def kernel_table(box, reps):
@@ -321,6 +352,12 @@ def main():
             pure["err_vs_fp64"] = err_pure
         log(f"vs fp64 after {args.err_steps} steps: fp64-state {err}; all-fp32 {err_pure}")
 
+    large = None
+    if args.large_n > 0 and args.large_n != n:
+        large = large_form(args.large_n, args.large_steps, args.large_err_steps)
+        log(f"{args.large_n}^3 (nu 0.002): fp32 (fp64 state) {large['steps_per_s']:.1f} steps/s, fp64 "
+            f"{large['fp64_steps_per_s']:.1f}; vs fp64 {large.get('err_vs_fp64')}")
+
     adj = adjoint_rate() if args.adjoint else None
     if adj:
         log(f"adjoint {adj['grads_per_s']:.2f} grads/s, fd rel err {adj['fd_rel_err']:.2e}")
@@ -374,6 +411,7 @@ def main():
                               "LDS instructions per launch at 256^3 FP32), not by HBM"
                               if dom == "xcross" else None)},
         "kernels": {q: {"ms": round(v["ms"], 5), "gbs": round(v["gbs"], 1)} for q, v in kernels.items()},
+        "large_box": large,
         "adjoint": adj,
         "batched": bat,
         "cpu_baseline": cpu,

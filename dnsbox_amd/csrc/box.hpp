@@ -386,10 +386,20 @@ constexpr int ztiles()
     if constexpr (PR) return KeptChunks<N, Cfg<N, T>::ZW>::NCH;
     else return (Geo<N, T>::ZCOLS + Cfg<N, T>::ZW - 1) / Cfg<N, T>::ZW;
 }
+// The pruned y passes tile each z-plane's KX columns separately when that keeps at least 85 %
+// of the lanes busy (256^3, 512^3); otherwise (a tile wider than a plane's kept columns, as at
+// 32^3 and 48^3) the tiles run over the planes' columns back to back.
+template <int N, class T>
+constexpr bool ytiles_by_plane()
+{
+    constexpr int W = Cfg<N, T>::YW, TPP = (Keep<N>::KX + W - 1) / W;
+    return 100 * Keep<N>::KX >= 85 * TPP * W;
+}
 template <int N, class T, bool PR>
 constexpr int ytiles()
 {
-    return ((PR ? Keep<N>::YCOLS : Geo<N, T>::YCOLS) + Cfg<N, T>::YW - 1) / Cfg<N, T>::YW;
+    if constexpr (PR && ytiles_by_plane<N, T>()) return N * ((Keep<N>::KX + Cfg<N, T>::YW - 1) / Cfg<N, T>::YW);
+    else return ((PR ? Keep<N>::YCOLS : Geo<N, T>::YCOLS) + Cfg<N, T>::YW - 1) / Cfg<N, T>::YW;
 }
 
 template <int N, class T, class Pro, class TI = T, bool PR = false>
@@ -413,8 +423,8 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
 // ===========================================================================
 // y pass (plain, in place), one field per blockIdx.y
 // ===========================================================================
-// PR: pruned (Keep): columns over all iz x Kx; the inverse (DIR = +1) reads ky in Ky only,
-// the forward (DIR = -1) stores ky in Ky only
+// PR: pruned (Keep): columns over all iz x Kx (tiled plane by plane where ytiles_by_plane); the
+// inverse (DIR = +1) reads ky in Ky only, the forward (DIR = -1) stores ky in Ky only
 template <int N, class T, int DIR, bool PR = false>
 __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, cpx<T>* lds,
                                           const cpx<T>* __restrict__ tw)
@@ -425,9 +435,21 @@ __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, c
     constexpr int CW = PR ? Keep<N>::KX : H;   // columns per z-plane
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
-    const int col = tile * W + c;
-    const bool valid = col < (PR ? Keep<N>::YCOLS : G::YCOLS);
-    const int ix = PR ? Keep<N>::ix(col % CW) : col % H, iz = col / CW;
+    int iz, ix;
+    bool valid;
+    if constexpr (PR && ytiles_by_plane<N, T>()) {
+        // tiles within one z-plane (no tile spans two planes' column runs)
+        constexpr int TPP = (CW + W - 1) / W;   // tiles per z-plane
+        const int x = (tile % TPP) * W + c;
+        iz = tile / TPP;
+        valid = x < CW;
+        ix = Keep<N>::ix(x);
+    } else {
+        const int col = tile * W + c;
+        valid = col < (PR ? Keep<N>::YCOLS : G::YCOLS);
+        ix = PR ? Keep<N>::ix(col % CW) : col % H;
+        iz = col / CW;
+    }
     cpx<T>* base = data + (long long)iz * N * H + ix;
     cpx<T> a[RM];
 #pragma unroll

@@ -317,11 +317,11 @@ inline int group_ztiles_pruned(int n, int prec)
 }
 inline int group_ytiles_pruned(int n, int prec)
 {
-    int w = 0, c = 0;
-#define DNSBOX_GT(NN) w = prec == 64 ? Cfg<NN, double>::YW : Cfg<NN, float>::YW, c = Keep<NN>::YCOLS
+    int t = 0;
+#define DNSBOX_GT(NN) t = prec == 64 ? ytiles<NN, double, true>() : ytiles<NN, float, true>()
     DNSBOX_GROUP_SWITCH(n, DNSBOX_GT)
 #undef DNSBOX_GT
-    return (c + w - 1) / w;
+    return t;
 }
 inline int group_xblocks(int n, int prec)
 {
