@@ -168,8 +168,9 @@ def test_fp32_256_matches_oracle(gpu_lib, state_precision):
     assert err < 1e-6
 
 
-@pytest.mark.parametrize("n,prec,sp,band", [(32, 64, 0, 0), (32, 64, 0, 1), (48, 32, 0, 0), (64, 64, 0, 0),
-                                             (128, 32, 64, 0), (256, 32, 64, 0)])
+@pytest.mark.parametrize("n,prec,sp,band", [(16, 64, 0, 0), (32, 64, 0, 0), (32, 64, 0, 1), (48, 32, 0, 0),
+                                             (64, 64, 0, 0), (128, 32, 64, 0), (256, 32, 64, 0), (256, 64, 0, 0),
+                                             (512, 32, 64, 0)])
 def test_pruned_step_matches_unpruned(gpu_lib, n, prec, sp, band):
     """The step's passes pruned to the 2/3-rule cube skip only exact zeros: the results agree
     with the unpruned passes to rounding (state, clock, adaptive dt, diagnostics).  Not to the
