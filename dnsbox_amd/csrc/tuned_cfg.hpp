@@ -25,36 +25,39 @@ namespace dnsbox {
 //   y: 0.1083 ms/step (table 0.1083, 8 candidates)
 //   x: 0.1075 ms/step (table 0.1129, 4 candidates)
 //   -> CfgBase<16, 16, 8, 256, Plan<8, 2, 8>, Plan<8, 4, 4>, Plan<16, 4>, 16, 6>
-// N=256 FP32 (state FP64): table entry step 1.6263 ms
-//   z: 0.7347 ms/step (table 0.7347, 8 candidates)
-//   y: 0.4687 ms/step (table 0.4941, 8 candidates)
-//   x: 0.4127 ms/step (table 0.4127, 4 candidates)
-//   -> CfgBase<16, 32, 4, 256, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 32, 2>
-// N=256 FP64 (state FP64): table entry step 2.8914 ms
-//   z: 1.1854 ms/step (table 1.1854, 3 candidates)
-//   y: 0.8418 ms/step (table 0.8541, 8 candidates)
-//   x: 0.7916 ms/step (table 0.7916, 3 candidates)
-//   -> CfgBase<8, 16, 2, 256, Plan<8, 4, 8>, Plan<16, 16>, Plan<8, 4, 4>, 16, 5>
-// N=512 FP32 (state FP64): table entry step 13.6997 ms
-//   z: 6.6195 ms/step (table 6.6195, 6 candidates)
-//   y: 3.8948 ms/step (table 3.8948, 8 candidates)
-//   x: 3.1373 ms/step (table 3.1373, 2 candidates)
-//   -> CfgBase<8, 16, 2, 256, Plan<8, 8, 8>, Plan<16, 16, 2>, Plan<16, 16>, 32, 6>
-// tuned 2026-12-03
+// N=256 FP32 (state FP64), passes pruned to the 2/3-rule cube: table entry step 1.0058 ms
+//   z: 0.3217 ms/step (table 0.3217, 9 candidates)
+//   y: 0.3224 ms/step (table 0.3224, 10 candidates)
+//   x: 0.3393 ms/step (table 0.3736, 10 candidates)
+//   -> CfgBase<16, 32, 8, 512, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 32, 2>
+//   (before pruning: step 1.6263 ms, z 0.7347, y 0.4687, x 0.4127 with RB 4, 256 threads)
+// N=256 FP64 (state FP64), passes pruned to the 2/3-rule cube: table entry step 1.9779 ms
+//   z: 0.6024 ms/step (table 0.6024, 3 candidates)
+//   y: 0.6425 ms/step (table 0.6425, 10 candidates)
+//   x: 0.5935 ms/step (table 0.7346, 7 candidates)
+//   -> CfgBase<8, 16, 4, 256, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 32, 2>
+//   (before pruning: step 2.8914 ms, z 1.1854, y 0.8418, x 0.7916 with RB 2, Plan<8, 4, 4>)
+// N=512 FP32 (state FP64), passes pruned to the 2/3-rule cube: table entry step 7.9484 ms
+//   z: 2.4014 ms/step (table 2.4014, 6 candidates)
+//   y: 2.7309 ms/step (table 2.7309, 10 candidates)
+//   x: 2.5708 ms/step (table 2.8147, 6 candidates)
+//   -> CfgBase<8, 16, 4, 512, Plan<8, 8, 8>, Plan<16, 16, 2>, Plan<16, 16>, 32, 6>
+//   (before pruning: step 13.6997 ms, z 6.6195, y 3.8948, x 3.1373 with RB 2, 256 threads)
+// tuned 2026-12-03; N=256 FP32 / FP64 and N=512 FP32 re-tuned 2026-12-04 on the pruned step
 
 template <> struct CfgDefault<16, float> : CfgBase<64, 64, 64, 256, Plan<4, 4>, Plan<4, 4>, Plan<8>, 0, 0> {};
 template <> struct CfgDefault<32, float> : CfgBase<32, 64, 8, 256, Plan<4, 2, 4>, Plan<2, 2, 8>, Plan<16>, 0, 0> {};
 template <> struct CfgDefault<48, float> : CfgBase<64, 64, 16, 256, Plan<2, 12, 2>, Plan<4, 12>, Plan<8, 3>, 0, 0> {};
 template <> struct CfgDefault<64, float> : CfgBase<16, 32, 16, 256, Plan<4, 4, 4>, Plan<8, 8>, Plan<16, 2>, 0, 0> {};
 template <> struct CfgDefault<128, float> : CfgBase<16, 32, 8, 256, Plan<4, 8, 4>, Plan<16, 8>, Plan<8, 8>, 16, 2> {};
-template <> struct CfgDefault<256, float> : CfgBase<16, 32, 4, 256, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 32, 2> {};
-template <> struct CfgDefault<512, float> : CfgBase<8, 16, 2, 256, Plan<8, 8, 8>, Plan<16, 16, 2>, Plan<16, 16>, 32, 6> {};
+template <> struct CfgDefault<256, float> : CfgBase<16, 32, 8, 512, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 32, 2> {};
+template <> struct CfgDefault<512, float> : CfgBase<8, 16, 4, 512, Plan<8, 8, 8>, Plan<16, 16, 2>, Plan<16, 16>, 32, 6> {};
 template <> struct CfgDefault<16, double> : CfgBase<64, 64, 32, 256, Plan<4, 4>, Plan<4, 4>, Plan<8>, 0, 0> {};
 template <> struct CfgDefault<32, double> : CfgBase<32, 64, 16, 256, Plan<4, 2, 4>, Plan<8, 4>, Plan<4, 4>, 8, 0> {};
 template <> struct CfgDefault<48, double> : CfgBase<64, 64, 8, 256, Plan<2, 12, 2>, Plan<12, 4>, Plan<8, 3>, 0, 10> {};
 template <> struct CfgDefault<64, double> : CfgBase<16, 32, 8, 256, Plan<4, 4, 4>, Plan<8, 8>, Plan<8, 4>, 16, 0> {};
 template <> struct CfgDefault<128, double> : CfgBase<16, 16, 8, 256, Plan<8, 2, 8>, Plan<8, 4, 4>, Plan<16, 4>, 16, 6> {};
-template <> struct CfgDefault<256, double> : CfgBase<8, 16, 2, 256, Plan<8, 4, 8>, Plan<16, 16>, Plan<8, 4, 4>, 16, 5> {};
+template <> struct CfgDefault<256, double> : CfgBase<8, 16, 4, 256, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 32, 2> {};
 template <> struct CfgDefault<512, double> : CfgBase<8, 8, 1, 256, Plan<8, 8, 8>, Plan<8, 8, 8>, Plan<8, 8, 4>, 16, 0> {};
 
 }  // namespace dnsbox

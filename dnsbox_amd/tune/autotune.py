@@ -102,7 +102,7 @@ def x_candidates(n, prec, base, limit=8):
     plans = _plans(n // 2, _radices(n // 2), 3)
     plans = [p for p in plans if len(p) == min(len(q) for q in plans)][:3]
     for rp in plans:
-        for ntx in ((256,) if n in GROUP_SIZES else (128, 256)):
+        for ntx in ((256,) if n in GROUP_SIZES else (128, 256, 512)):
             for rb in (1, 2, 4, 8, 16, 32, 64):
                 # 2 to 16 input loads per thread
                 if rows % rb or rb * h > ntx * 16 or (rb * h < 2 * ntx and rb * 2 <= rows // 256):
