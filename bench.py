@@ -228,7 +228,7 @@ def adjoint_rate(reps=3):
         fd = (jp - jm) / (2 * eps)
         an = float((g.conj() * v).real.sum())
     return {"grads_per_s": 1.0 / secs, "ms_per_grad": secs * 1e3, "n": n, "T": T, "dt": dt,
-            "checkpoint_every": k, "dtype": "f64", "J": float(J),
+            "checkpoint_every": k, "dtype": "f64", "J": float(J.detach()),
             "fd_rel_err": abs(fd - an) / abs(fd),
             "workload": "64^3 FP64, nu=0.02, f=(sin 4y,0,0), J=1/2<|Phi_T(u)-u|^2>"}
 

@@ -53,24 +53,53 @@ class _StepMap(torch.autograd.Function):
         box.rhs_int_fact(dt)
         lam = g.detach().clone().contiguous()
         out = torch.empty_like(lam)
+        nseg = len(ctx.ckpts)
         shape = (max(k - 1, 1),) + tuple(lam.shape)
-        traj = torch.empty(shape, dtype=lam.dtype, device=lam.device)
-        preds = torch.empty(shape, dtype=lam.dtype, device=lam.device)
-        for seg in reversed(range(len(ctx.ckpts))):
-            start = seg * k
-            stop = min(start + k, T)
-            # the segment's states u_start .. u_{stop-1} and the predicted states of its steps but
-            # the last, recomputed in one call; the last step's VJP recomputes its own
-            if stop - start > 1:
-                u = ctx.ckpts[seg].clone()
-                box.step_dev_record(u, stop - start - 1, traj, preds)
+        # Segment s's states u_start .. u_{stop-1} and the predicted states of its steps but the
+        # last are recomputed by a twin box (same parameters, own buffers) on a second stream
+        # into one of two buffer pairs, while this box runs the step adjoints of segment s + 1
+        # from the other: at small N neither fills the GPU, so the two overlap.
+        bufs = [(torch.empty(shape, dtype=lam.dtype, device=lam.device),
+                 torch.empty(shape, dtype=lam.dtype, device=lam.device)) for _ in range(min(2, nseg))]
+        rec = box.twin()
+        rec.rhs_int_fact(dt)
+        main = torch.cuda.current_stream(lam.device)
+        side = torch.cuda.Stream(device=lam.device)
+        side.wait_stream(main)            # the checkpoints and buffers are ready
+        recorded = [None] * nseg          # segment s recomputed into bufs[s % 2]
+        adjointed = [None] * nseg         # segment s's step adjoints done with bufs[s % 2]
+
+        def span(seg):
+            return seg * k, min(seg * k + k, T)
+
+        def recompute(seg):
+            start, stop = span(seg)
+            with torch.cuda.stream(side):
+                if seg + 2 < nseg:
+                    side.wait_event(adjointed[seg + 2])   # its buffers were segment seg + 2's
+                if stop - start > 1:
+                    traj, preds = bufs[seg % 2]
+                    rec.step_dev_record(ctx.ckpts[seg].clone(), stop - start - 1, traj, preds)
+                recorded[seg] = torch.cuda.Event()
+                recorded[seg].record(side)
+
+        recompute(nseg - 1)
+        for seg in reversed(range(nseg)):
+            main.wait_event(recorded[seg])
+            start, stop = span(seg)
+            traj, preds = bufs[seg % 2]
             states = [ctx.ckpts[seg]] + [traj[j] for j in range(stop - start - 1)]
+            # the recorded u* of every step but the segment's last, whose VJP recomputes its own
             for j in reversed(range(stop - start)):
                 if j < stop - start - 1:
                     box.step_vjp_pred_dev(states[j], preds[j], lam, out, dt)
                 else:
                     box.step_vjp_dev(states[j], lam, out, dt)
                 lam, out = out, lam
+            adjointed[seg] = torch.cuda.Event()
+            adjointed[seg].record(main)
+            if seg > 0:
+                recompute(seg - 1)
         return lam, None, None, None, None
 
 
@@ -103,20 +132,46 @@ class _GroupStepMap(torch.autograd.Function):
         lams = [torch.zeros_like(c) if g is None else g.detach().clone().contiguous()
                 for g, c in zip(gs, ctx.ckpts[0])]
         outs = [torch.empty_like(l) for l in lams]
-        seg_states = None
-        for seg in reversed(range(len(ctx.ckpts))):
-            start = seg * k
-            stop = min(start + k, T)
-            if seg_states is None:
-                seg_states = [[torch.empty_like(l) for l in lams] for _ in range(k)]
-            states = seg_states
-            _copy_all(states[0], ctx.ckpts[seg])
-            for j in range(1, stop - start):
-                _copy_all(states[j], states[j - 1])
-                group.step_dev(states[j], 1)
+        nseg = len(ctx.ckpts)
+        # as in _StepMap.backward: a twin group recomputes segment s on a second stream into one
+        # of two sets of state buffers while this group runs the adjoints of segment s + 1
+        bufs = [[[torch.empty_like(l) for l in lams] for _ in range(k)] for _ in range(min(2, nseg))]
+        rec = group.twin()
+        for b, dt in zip(rec.boxes, dts):
+            b.rhs_int_fact(dt)
+        main = torch.cuda.current_stream(lams[0].device)
+        side = torch.cuda.Stream(device=lams[0].device)
+        side.wait_stream(main)
+        recorded, adjointed = [None] * nseg, [None] * nseg
+
+        def span(seg):
+            return seg * k, min(seg * k + k, T)
+
+        def recompute(seg):
+            start, stop = span(seg)
+            states = bufs[seg % 2]
+            with torch.cuda.stream(side):
+                if seg + 2 < nseg:
+                    side.wait_event(adjointed[seg + 2])
+                _copy_all(states[0], ctx.ckpts[seg])
+                for j in range(1, stop - start):
+                    _copy_all(states[j], states[j - 1])
+                    rec.step_dev(states[j], 1)
+                recorded[seg] = torch.cuda.Event()
+                recorded[seg].record(side)
+
+        recompute(nseg - 1)
+        for seg in reversed(range(nseg)):
+            main.wait_event(recorded[seg])
+            start, stop = span(seg)
+            states = bufs[seg % 2]
             for j in reversed(range(stop - start)):
                 group.step_vjp_dev(states[j], lams, outs, dts)
                 lams, outs = outs, lams
+            adjointed[seg] = torch.cuda.Event()
+            adjointed[seg].record(main)
+            if seg > 0:
+                recompute(seg - 1)
         return (None, None, None, None, *lams)
 
 

@@ -73,8 +73,28 @@ class Box:
         b.precision = params.precision
         return b
 
+    def twin(self) -> "Box":
+        """A new box with this box's parameters and pruning setting (its own buffers, state and
+        clock), kept with this box and closed with it: the checkpointed backward recomputes
+        segments on it while this box runs the step adjoints."""
+        tw = getattr(self, "_twin", None)
+        if tw is None:
+            tw = Box.__new__(Box)
+            tw.lib = self.lib
+            tw.params = type(self.params).from_buffer_copy(self.params)
+            h = C.c_void_p()
+            check(self.lib.dnsbox_create(C.byref(tw.params), C.byref(h)))
+            tw._h, tw._owned, tw.n, tw.precision = h, True, self.n, self.precision
+            self._twin = tw
+        tw.set_pruning(getattr(self, "_prune", True))
+        return tw
+
     # -- lifetime -------------------------------------------------------------------
     def close(self):
+        tw = getattr(self, "_twin", None)
+        if tw is not None:
+            self._twin = None
+            tw.close()
         if getattr(self, "_h", None) and self._h.value and getattr(self, "_owned", True):
             check(self.lib.dnsbox_destroy(self._h))
         self._h = C.c_void_p()
@@ -200,6 +220,7 @@ class Box:
         """Run the step's transforms pruned to the 2/3-rule cube while the state allows it
         (default on; only exact zeros are skipped, the results agree to rounding)."""
         check(self.lib.dnsbox_set_pruning(self._h, int(bool(enable))))
+        self._prune = bool(enable)
 
     def pruning(self) -> bool:
         """Whether the next step runs pruned."""
@@ -303,18 +324,35 @@ class BoxGroup:
 
     def __init__(self, params):
         """params: list of dicts of Box keyword arguments (n, precision, nu, ...)."""
-        self.lib = load()
         arr = (Params * len(params))()
         for i, kw in enumerate(params):
             arr[i] = default_params(**kw)
+        self._create(arr)
+
+    def _create(self, arr):
+        self.lib = load()
         h = C.c_void_p()
-        check(self.lib.dnsbox_group_create(arr, len(params), C.byref(h)))
+        check(self.lib.dnsbox_group_create(arr, len(arr), C.byref(h)))
         self._h = h
         self.boxes = []
-        for i in range(len(params)):
+        for i in range(len(arr)):
             bh = C.c_void_p()
             check(self.lib.dnsbox_group_box(h, i, C.byref(bh)))
             self.boxes.append(Box._view(bh, arr[i]))
+
+    def twin(self) -> "BoxGroup":
+        """A new group with this group's box parameters (own buffers, states and clocks), kept
+        with this group and closed with it: the checkpointed backward recomputes segments on it
+        while this group runs the step adjoints."""
+        tw = getattr(self, "_twin", None)
+        if tw is None:
+            arr = (Params * len(self.boxes))()
+            for i, b in enumerate(self.boxes):
+                arr[i] = Params.from_buffer_copy(b.params)
+            tw = BoxGroup.__new__(BoxGroup)
+            tw._create(arr)
+            self._twin = tw
+        return tw
 
     def __len__(self):
         return len(self.boxes)
@@ -362,6 +400,10 @@ class BoxGroup:
                                                  self._ptrs(outs), d))
 
     def close(self):
+        tw = getattr(self, "_twin", None)
+        if tw is not None:
+            self._twin = None
+            tw.close()
         if getattr(self, "_h", None) and self._h.value:
             for b in self.boxes:
                 b._h = C.c_void_p()
