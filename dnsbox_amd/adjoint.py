@@ -133,9 +133,13 @@ class _GroupStepMap(torch.autograd.Function):
                 for g, c in zip(gs, ctx.ckpts[0])]
         outs = [torch.empty_like(l) for l in lams]
         nseg = len(ctx.ckpts)
-        # as in _StepMap.backward: a twin group recomputes segment s on a second stream into one
-        # of two sets of state buffers while this group runs the adjoints of segment s + 1
-        bufs = [[[torch.empty_like(l) for l in lams] for _ in range(k)] for _ in range(min(2, nseg))]
+        # as in _StepMap.backward: a twin group recomputes segment s on a second stream (states
+        # and predicted states recorded in one call) into one of two buffer sets while this group
+        # runs the adjoints of segment s + 1 from the other
+        m = max(k - 1, 1)
+        bufs = [[(torch.empty((m,) + tuple(l.shape), dtype=l.dtype, device=l.device),
+                  torch.empty((m,) + tuple(l.shape), dtype=l.dtype, device=l.device)) for l in lams]
+                for _ in range(min(2, nseg))]
         rec = group.twin()
         for b, dt in zip(rec.boxes, dts):
             b.rhs_int_fact(dt)
@@ -149,14 +153,13 @@ class _GroupStepMap(torch.autograd.Function):
 
         def recompute(seg):
             start, stop = span(seg)
-            states = bufs[seg % 2]
             with torch.cuda.stream(side):
                 if seg + 2 < nseg:
                     side.wait_event(adjointed[seg + 2])
-                _copy_all(states[0], ctx.ckpts[seg])
-                for j in range(1, stop - start):
-                    _copy_all(states[j], states[j - 1])
-                    rec.step_dev(states[j], 1)
+                if stop - start > 1:
+                    us = [c.clone() for c in ctx.ckpts[seg]]
+                    rec.step_dev_record(us, stop - start - 1, [t for t, _ in bufs[seg % 2]],
+                                        [p for _, p in bufs[seg % 2]])
                 recorded[seg] = torch.cuda.Event()
                 recorded[seg].record(side)
 
@@ -164,9 +167,13 @@ class _GroupStepMap(torch.autograd.Function):
         for seg in reversed(range(nseg)):
             main.wait_event(recorded[seg])
             start, stop = span(seg)
-            states = bufs[seg % 2]
+            buf = bufs[seg % 2]
             for j in reversed(range(stop - start)):
-                group.step_vjp_dev(states[j], lams, outs, dts)
+                states = ctx.ckpts[seg] if j == 0 else [t[j - 1] for t, _ in buf]
+                if j < stop - start - 1:
+                    group.step_vjp_pred_dev(states, [p[j] for _, p in buf], lams, outs, dts)
+                else:
+                    group.step_vjp_dev(states, lams, outs, dts)
                 lams, outs = outs, lams
             adjointed[seg] = torch.cuda.Event()
             adjointed[seg].record(main)

@@ -399,6 +399,30 @@ class BoxGroup:
         check(self.lib.dnsbox_group_step_vjp_dev(self._h, self._ptrs(us), self._ptrs(lams),
                                                  self._ptrs(outs), d))
 
+    def _traj_ptrs(self, trajs, nsteps):
+        if len(trajs) != len(self.boxes):
+            raise ValueError(f"expected {len(self.boxes)} tensors, got {len(trajs)}")
+        arr = (C.c_void_p * len(trajs))()
+        for i, (b, t) in enumerate(zip(self.boxes, trajs)):
+            arr[i] = b._check_traj(t, nsteps).value
+        return arr
+
+    def step_dev_record(self, states, nsteps: int, trajs, traj_preds=None):
+        """step_dev, copying box i's state after step j + 1 to trajs[i][j] and, if given, storing
+        that step's predicted state u* in traj_preds[i][j] (per box a contiguous device tensor of
+        shape (>= nsteps, 3, N, N, N/2+1) of the group precision)."""
+        self._on_torch_stream()
+        tp = self._traj_ptrs(traj_preds, nsteps) if traj_preds is not None else None
+        check(self.lib.dnsbox_group_step_dev_record(self._h, self._ptrs(states), nsteps,
+                                                    self._traj_ptrs(trajs, nsteps), tp))
+
+    def step_vjp_pred_dev(self, us, ustars, lams, outs, dts):
+        """step_vjp_dev with each box's predicted state u* supplied (from step_dev_record)."""
+        self._on_torch_stream()
+        d = (C.c_double * len(self.boxes))(*[float(x) for x in dts])
+        check(self.lib.dnsbox_group_step_vjp_pred_dev(self._h, self._ptrs(us), self._ptrs(ustars),
+                                                      self._ptrs(lams), self._ptrs(outs), d))
+
     def close(self):
         tw = getattr(self, "_twin", None)
         if tw is not None:

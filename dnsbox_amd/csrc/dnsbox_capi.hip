@@ -731,6 +731,40 @@ int dnsbox_group_step_dev(dnsbox_group* g, void* const* states, int nsteps)
     });
 }
 
+int dnsbox_group_step_dev_record(dnsbox_group* g, void* const* states, int nsteps, void* const* traj,
+                                 void* const* traj_pred)
+{
+    if (!g || !states || !traj) return fail(DNSBOX_E_ARG, "null argument");
+    if (nsteps < 0) return fail(DNSBOX_E_ARG, "nsteps < 0");
+    for (size_t i = 0; i < g->views.size(); ++i) {
+        if (!states[i] || !traj[i]) return fail(DNSBOX_E_ARG, "null state or trajectory pointer");
+        if (traj_pred && (!traj_pred[i] || traj_pred[i] == traj[i] || traj_pred[i] == states[i]))
+            return fail(DNSBOX_E_ARG, "traj_pred[i] must be set and must not alias traj[i] or states[i]");
+    }
+    return guarded([&]() -> int {
+        g->g->step_ext_record(states, nsteps, traj, traj_pred);
+        for (auto& b : g->g->boxes) b->x6_valid = false;
+        return 0;
+    });
+}
+
+int dnsbox_group_step_vjp_pred_dev(dnsbox_group* g, const void* const* u, const void* const* ustar,
+                                   const void* const* lam, void* const* lam_out, const double* dt)
+{
+    if (!g || !u || !ustar || !lam || !lam_out || !dt) return fail(DNSBOX_E_ARG, "null argument");
+    for (size_t i = 0; i < g->views.size(); ++i) {
+        if (!u[i] || !ustar[i] || !lam[i] || !lam_out[i]) return fail(DNSBOX_E_ARG, "null device pointer");
+        if (!(dt[i] > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+        if (lam_out[i] == u[i] || lam_out[i] == lam[i] || lam_out[i] == ustar[i])
+            return fail(DNSBOX_E_ARG, "lam_out must not alias u, ustar or lam");
+    }
+    return guarded([&]() -> int {
+        g->g->step_vjp_pred(u, ustar, lam, lam_out, dt);
+        for (auto& b : g->g->boxes) b->x6_valid = false;
+        return 0;
+    });
+}
+
 int dnsbox_group_step_vjp_dev(dnsbox_group* g, const void* const* u, const void* const* lam, void* const* lam_out,
                               const double* dt)
 {

@@ -37,6 +37,8 @@ struct GBox {
     const cpx<T>* lam;
     cpx<T>* out;
     cpx<T>* pred;        // u* of the step
+    cpx<T>* pout;        // forward step: where the IF-Heun predictor stores u* (null: not stored)
+    cpx<T>* tout;        // forward step: where k_g_copy_state copies the new state (null: none)
     cpx<T>* A;           // E (lam + lam_*)
     cpx<T>* Mu;          // E (dt/2 lam + dt lam_*)
     double dt;
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
     const int tile = gt - (PR ? bx.zt0p : bx.zt0);
     EpiArgs<T, T> ea{};
-    ea.out = nullptr;
+    ea.out = MODE == EPI_PRED ? bx.pout : nullptr;
     ea.state = bx.st;
     ea.rhs_in = bx.st;
     ea.rhs_in_s = 1;
@@ -279,6 +281,17 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_vjp(const GBox<T>* __r
 #undef DNSBOX_G_ZF
 }
 
+// state of box blockIdx.y -> its trajectory slot (boxes without one return at once)
+template <class T>
+__global__ void __launch_bounds__(256) k_g_copy_state(const GBox<T>* __restrict__ gb, int nbox)
+{
+    const GBox<T>& bx = gb[blockIdx.y];
+    if (!bx.tout) return;
+    const long long n = 3 * bx.fs;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        bx.tout[i] = bx.st[i];
+}
+
 template <class T>
 __global__ void k_g_step_end(const GBox<T>* __restrict__ gb, int nbox)
 {
@@ -340,6 +353,9 @@ class GroupBase {
     virtual void step(int nsteps) = 0;
     virtual void step_ext(void* const* states, int nsteps) = 0;
     virtual void step_vjp(const void* const* u, const void* const* lam, void* const* out, const double* dt) = 0;
+    virtual void step_ext_record(void* const* states, int nsteps, void* const* traj, void* const* traj_pred) = 0;
+    virtual void step_vjp_pred(const void* const* u, const void* const* ustar, const void* const* lam,
+                               void* const* out, const double* dt) = 0;
     virtual void invalidate() = 0;
 };
 
@@ -412,6 +428,8 @@ class Group final : public GroupBase {
         g.sc = b.sc;
         g.ph = b.cfg.ph;
         g.pred = static_cast<C*>(b.pred);
+        g.pout = nullptr;
+        g.tout = nullptr;
         g.A = static_cast<C*>(b.adjbuf);
         g.Mu = static_cast<C*>(b.adjbuf) + 3 * b.fs;
     }
@@ -559,6 +577,30 @@ class Group final : public GroupBase {
         for (int i = 0; i < nsteps; ++i) one_step<false>(stream, d);
     }
 
+    // step_ext that also copies box i's state after step j + 1 to traj[i] + j * 3 fields and, when
+    // traj_pred is set, stores that step's predicted state u* at traj_pred[i] + j * 3 fields
+    void step_ext_record(void* const* states, int nsteps, void* const* traj, void* const* traj_pred) override
+    {
+        if (nsteps <= 0) return;
+        refresh();
+        std::vector<GBox<T>> h = h_gb;
+        for (size_t i = 0; i < h.size(); ++i) h[i].st = static_cast<C*>(states[i]);
+        x6_valid = false;
+        launch_zinv(stream, descriptors(h), 0);
+        const int nb = int(h.size());
+        for (int j = 0; j < nsteps; ++j) {
+            for (size_t i = 0; i < h.size(); ++i) {
+                const long long off = (long long)j * 3 * h[i].fs;
+                h[i].pout = traj_pred ? static_cast<C*>(traj_pred[i]) + off : nullptr;
+                h[i].tout = static_cast<C*>(traj[i]) + off;
+            }
+            const GBox<T>* d = descriptors(h);
+            one_step<false>(stream, d);
+            hipLaunchKernelGGL((k_g_copy_state<T>), dim3(64, nb), dim3(256), 0, stream, d, nb);
+            DNSBOX_LAUNCH_CHECK();
+        }
+    }
+
     // out_i = (d u_{n+1} / d u_n)^T lam_i for one fixed-dt IF-Heun step of every box
     void step_vjp(const void* const* u, const void* const* lam, void* const* out, const double* dt) override
     {
@@ -585,6 +627,25 @@ class Group final : public GroupBase {
         vjp_passes<VJP_S1>(s, d);
         // out = A + J(u)^T Mu
         vjp_passes<VJP_S2>(s, d);
+    }
+
+    // step_vjp with each box's predicted state u* given (recorded by step_ext_record)
+    void step_vjp_pred(const void* const* u, const void* const* ustar, const void* const* lam, void* const* out,
+                       const double* dt) override
+    {
+        refresh();
+        std::vector<GBox<T>> h = h_gb;
+        for (size_t i = 0; i < h.size(); ++i) {
+            h[i].u = static_cast<const C*>(u[i]);
+            h[i].pred = const_cast<C*>(static_cast<const C*>(ustar[i]));   // read only by the VJP
+            h[i].lam = static_cast<const C*>(lam[i]);
+            h[i].out = static_cast<C*>(out[i]);
+            h[i].dt = dt[i];
+        }
+        const GBox<T>* d = descriptors(h);
+        x6_valid = false;
+        vjp_passes<VJP_S1>(stream, d);
+        vjp_passes<VJP_S2>(stream, d);
     }
 
     template <int MODE>

@@ -210,6 +210,17 @@ int dnsbox_group_step_dev(dnsbox_group* g, void* const* states, int nsteps);
    step of box i from u[i] at step size dt[i] */
 int dnsbox_group_step_vjp_dev(dnsbox_group* g, const void* const* u, const void* const* lam,
                               void* const* lam_out, const double* dt);
+/* batched dnsbox_step_dev_record: box i's state after step j + 1 is copied to traj[i] + j * 3
+   fields and, when traj_pred is not NULL, that step's predicted state u* is stored at
+   traj_pred[i] + j * 3 fields (device buffers of the group precision, internal layout).  No
+   reference counterpart (it serves the checkpointed adjoint of the T-step map). */
+int dnsbox_group_step_dev_record(dnsbox_group* g, void* const* states, int nsteps, void* const* traj,
+                                 void* const* traj_pred);
+/* batched dnsbox_step_vjp_pred_dev: dnsbox_group_step_vjp_dev with each box's predicted state
+   u* = E (u + dt N(u)) given (as recorded by dnsbox_group_step_dev_record).  No reference
+   counterpart. */
+int dnsbox_group_step_vjp_pred_dev(dnsbox_group* g, const void* const* u, const void* const* ustar,
+                                   const void* const* lam, void* const* lam_out, const double* dt);
 
 #ifdef __cplusplus
 }

@@ -126,3 +126,40 @@ def test_group_adjoint_descent_decreases_every_residual(gpu_lib):
     for i in range(len(us)):
         print(f"descent box {i}: J {first[i]:.4e} -> {last[i]:.4e}", flush=True)
         assert last[i] < first[i]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_group_record_and_vjp_with_recorded_prediction(gpu_lib, prec):
+    """dnsbox_group_step_dev_record stores every box's state after each step (equal to single
+    group steps) and each step's predicted state u*; the step adjoint given that u* equals the
+    one that recomputes it."""
+    import torch
+    from dnsbox_amd import BoxGroup
+    tol = 1e-14 if prec == 64 else 1e-6
+    nst = 4
+    us = fields(prec, 50)
+    lams = rand_like(us, 9)
+    with BoxGroup([kw(n, nu, prec) for n, nu in zip(SIZES, NUS)]) as g:
+        for b in g.boxes:
+            b.rhs_int_fact(DT)
+        trajs = [torch.empty((nst,) + tuple(u.shape), dtype=u.dtype, device=u.device) for u in us]
+        preds = [torch.full_like(t, 7.0) for t in trajs]   # every slot must be written
+        st = [u.clone() for u in us]
+        g.step_dev_record(st, nst, trajs, preds)
+        ref = [u.clone() for u in us]
+        dts = [DT] * len(us)
+        worst_s = worst_v = 0.0
+        for j in range(nst):
+            g.step_dev(ref, 1)
+            uj = us if j == 0 else [t[j - 1] for t in trajs]
+            a = [torch.empty_like(u) for u in us]
+            b = [torch.empty_like(u) for u in us]
+            g.step_vjp_pred_dev(uj, [p[j] for p in preds], lams, a, dts)
+            g.step_vjp_dev(uj, lams, b, dts)
+            torch.cuda.synchronize()
+            for i in range(len(us)):
+                worst_s = max(worst_s, rel_l2(host(trajs[i][j]), host(ref[i])))
+                worst_v = max(worst_v, rel_l2(host(a[i]), host(b[i])))
+        print(f"group record prec={prec}: states vs single steps {worst_s:.2e}, "
+              f"VJP with recorded u* vs recomputed {worst_v:.2e}", flush=True)
+        assert worst_s <= tol and worst_v <= tol
