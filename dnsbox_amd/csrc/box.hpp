@@ -1378,15 +1378,16 @@ __global__ void k_scale_to_energy(cpx<T>* __restrict__ u0, long long n, const do
 
 // flag |= 1 when any of the three fields holds a nonzero mode outside the kept cube (Keep):
 // decides whether a step of a caller's buffer may run the pruned passes
+// whether this thread's share (start, start + stride, ...) of the modes of the 3 fields at u0
+// outside the kept cube holds a nonzero value
 template <int N, class T>
-__global__ void __launch_bounds__(256)
-    k_outside_cube(const cpx<T>* __restrict__ u0, long long fs, unsigned int* flag)
+__device__ __forceinline__ bool outside_cube_nonzero(const cpx<T>* __restrict__ u0, long long fs, long long start,
+                                                     long long stride)
 {
     constexpr int H = N / 2 + 1;
     constexpr long long TOT = (long long)N * N * H;
     bool nz = false;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < TOT;
-         idx += (long long)gridDim.x * blockDim.x) {
+    for (long long idx = start; idx < TOT; idx += stride) {
         const int ix = int(idx % H);
         const int iy = int((idx / H) % N);
         const int iz = int(idx / ((long long)H * N));
@@ -1397,6 +1398,15 @@ __global__ void __launch_bounds__(256)
             nz = nz || a.x != T(0) || a.y != T(0);
         }
     }
+    return nz;
+}
+
+template <int N, class T>
+__global__ void __launch_bounds__(256)
+    k_outside_cube(const cpx<T>* __restrict__ u0, long long fs, unsigned int* flag)
+{
+    const bool nz = outside_cube_nonzero<N, T>(u0, fs, blockIdx.x * (long long)blockDim.x + threadIdx.x,
+                                               (long long)gridDim.x * blockDim.x);
     if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(flag, 1u);
 }
 

@@ -281,6 +281,20 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_vjp(const GBox<T>* __r
 #undef DNSBOX_G_ZF
 }
 
+// *flag |= 1 when the state of some box (box blockIdx.y per block row) has a nonzero mode outside
+// its kept cube: whether a step of caller buffers may run the pruned passes
+template <class T>
+__global__ void __launch_bounds__(256) k_g_outside_cube(const GBox<T>* __restrict__ gb, int nbox, unsigned int* flag)
+{
+    const GBox<T>& bx = gb[blockIdx.y];
+    bool nz = false;
+#define DNSBOX_G_OC(NN) nz = outside_cube_nonzero<NN, T>(bx.st, bx.fs, blockIdx.x * (long long)blockDim.x + threadIdx.x, \
+                                                         (long long)gridDim.x * blockDim.x)
+    DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_OC)
+#undef DNSBOX_G_OC
+    if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(flag, 1u);
+}
+
 // state of box blockIdx.y -> its trajectory slot (boxes without one return at once)
 template <class T>
 __global__ void __launch_bounds__(256) k_g_copy_state(const GBox<T>* __restrict__ gb, int nbox)
@@ -563,8 +577,26 @@ class Group final : public GroupBase {
         DNSBOX_HIP(hipEventDestroy(ev));
     }
 
+    // whether every caller state of the descriptors d is confined to its kept cube (checked on
+    // the device; one 4-byte read back), so that their steps may run the pruned passes
+    bool buffers_dealiased(const GBox<T>* d)
+    {
+        for (auto& b : boxes)
+            if (!b->prune) return false;
+        unsigned int* flag = boxes[0]->chk;
+        const int nb = int(h_gb.size());
+        DNSBOX_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), stream));
+        hipLaunchKernelGGL((k_g_outside_cube<T>), dim3(32, nb), dim3(256), 0, stream, d, nb, flag);
+        DNSBOX_LAUNCH_CHECK();
+        unsigned int hf = 1;
+        DNSBOX_HIP(hipMemcpyAsync(&hf, flag, sizeof(hf), hipMemcpyDeviceToHost, stream));
+        DNSBOX_HIP(hipStreamSynchronize(stream));
+        return hf == 0;
+    }
+
     // IF-Heun steps of caller-owned states (device buffers of the boxes' shapes), on
-    // the caller's stream; each box's step size and clock are its own
+    // the caller's stream; each box's step size and clock are its own.  The passes are
+    // pruned when every state is confined to its kept cube (checked on the device).
     void step_ext(void* const* states, int nsteps) override
     {
         if (nsteps <= 0) return;
@@ -573,8 +605,10 @@ class Group final : public GroupBase {
         for (size_t i = 0; i < h.size(); ++i) h[i].st = static_cast<C*>(states[i]);
         const GBox<T>* d = descriptors(h);
         x6_valid = false;
-        launch_zinv(stream, d, 0);
-        for (int i = 0; i < nsteps; ++i) one_step<false>(stream, d);
+        const bool pr = buffers_dealiased(d);
+        if (pr) launch_zinv<true>(stream, d, 0);
+        else launch_zinv<false>(stream, d, 0);
+        for (int i = 0; i < nsteps; ++i) one_step(stream, d, pr);
     }
 
     // step_ext that also copies box i's state after step j + 1 to traj[i] + j * 3 fields and, when
@@ -586,7 +620,18 @@ class Group final : public GroupBase {
         std::vector<GBox<T>> h = h_gb;
         for (size_t i = 0; i < h.size(); ++i) h[i].st = static_cast<C*>(states[i]);
         x6_valid = false;
-        launch_zinv(stream, descriptors(h), 0);
+        const GBox<T>* d0 = descriptors(h);
+        const bool pr = buffers_dealiased(d0);
+        if (pr) {
+            // the pruned predictor stores u* at the kept modes only: the rest of each slot is zero
+            for (size_t i = 0; i < h.size(); ++i)
+                if (traj_pred)
+                    DNSBOX_HIP(hipMemsetAsync(traj_pred[i], 0, size_t(nsteps) * 3 * size_t(h[i].fs) * sizeof(C),
+                                              stream));
+            launch_zinv<true>(stream, d0, 0);
+        } else {
+            launch_zinv<false>(stream, d0, 0);
+        }
         const int nb = int(h.size());
         for (int j = 0; j < nsteps; ++j) {
             for (size_t i = 0; i < h.size(); ++i) {
@@ -595,7 +640,7 @@ class Group final : public GroupBase {
                 h[i].tout = static_cast<C*>(traj[i]) + off;
             }
             const GBox<T>* d = descriptors(h);
-            one_step<false>(stream, d);
+            one_step(stream, d, pr);
             hipLaunchKernelGGL((k_g_copy_state<T>), dim3(64, nb), dim3(256), 0, stream, d, nb);
             DNSBOX_LAUNCH_CHECK();
         }
