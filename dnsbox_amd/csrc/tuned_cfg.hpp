@@ -31,6 +31,10 @@ namespace dnsbox {
 //   x: 0.3393 ms/step (table 0.3736, 10 candidates)
 //   -> CfgBase<16, 32, 8, 512, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 32, 2>
 //   (before pruning: step 1.6263 ms, z 0.7347, y 0.4687, x 0.4127 with RB 4, 256 threads)
+//   LDS rows measured beyond the bank model's ranking (x ms/step): XPD 64 / XPAD 1 0.3112-0.3135,
+//   64 / 0 0.3112, 32 / 2 0.3399, 0 / 4 0.3443, 32 / 6 0.3500, 128 / 1 0.3425, 64 / 2 0.3674
+//   -> CfgBase<16, 32, 8, 512, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 64, 1>
+//   (256^3 FP64 and 512^3 FP32 keep their layouts: XPD 64 was 8 % and 6 % slower there)
 // N=256 FP64 (state FP64), passes pruned to the 2/3-rule cube: table entry step 1.9779 ms
 //   z: 0.6024 ms/step (table 0.6024, 3 candidates)
 //   y: 0.6425 ms/step (table 0.6425, 10 candidates)
@@ -50,7 +54,7 @@ template <> struct CfgDefault<32, float> : CfgBase<32, 64, 8, 256, Plan<4, 2, 4>
 template <> struct CfgDefault<48, float> : CfgBase<64, 64, 16, 256, Plan<2, 12, 2>, Plan<4, 12>, Plan<8, 3>, 0, 0> {};
 template <> struct CfgDefault<64, float> : CfgBase<16, 32, 16, 256, Plan<4, 4, 4>, Plan<8, 8>, Plan<16, 2>, 0, 0> {};
 template <> struct CfgDefault<128, float> : CfgBase<16, 32, 8, 256, Plan<4, 8, 4>, Plan<16, 8>, Plan<8, 8>, 16, 2> {};
-template <> struct CfgDefault<256, float> : CfgBase<16, 32, 8, 512, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 32, 2> {};
+template <> struct CfgDefault<256, float> : CfgBase<16, 32, 8, 512, Plan<8, 4, 8>, Plan<16, 16>, Plan<16, 8>, 64, 1> {};
 template <> struct CfgDefault<512, float> : CfgBase<8, 16, 4, 512, Plan<8, 8, 8>, Plan<16, 16, 2>, Plan<16, 16>, 32, 6> {};
 template <> struct CfgDefault<16, double> : CfgBase<64, 64, 32, 256, Plan<4, 4>, Plan<4, 4>, Plan<8>, 0, 0> {};
 template <> struct CfgDefault<32, double> : CfgBase<32, 64, 16, 256, Plan<4, 2, 4>, Plan<8, 4>, Plan<4, 4>, 8, 0> {};

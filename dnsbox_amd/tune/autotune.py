@@ -111,9 +111,12 @@ def x_candidates(n, prec, base, limit=8):
                 xpd, xpad, hp, _ = lay[0]
                 if 12 * rb * hp * esz > LDS_MAX or 6 * rb * hp * esz > 64 * 1024:
                     continue
-                c = base.with_(rb=rb, ntx=ntx, rp=rp, xpd=xpd, xpad=xpad)
-                if c not in out:
-                    out.append(c)
+                # the bank model's best layout, and one pad every 64 elements (measured faster
+                # than the model's choice for 256^3 FP32, slower elsewhere)
+                for xd, xa in ((xpd, xpad), (64, 1)):
+                    c = base.with_(rb=rb, ntx=ntx, rp=rp, xpd=xd, xpad=xa)
+                    if c not in out:
+                        out.append(c)
     out.sort(key=lambda c: (c != base, abs(c.rb * h * 256 // c.ntx - 4 * 129), len(c.rp)))
     return out[:limit]
 
