@@ -193,3 +193,41 @@ def test_tstep_gradient_vs_finite_differences(gpu_lib, n, T, k):
             worst = max(worst, rel)
             print(f"T-step grad n={n} T={T} dir={d}: fd={fd:.10e} adj={an:.10e} rel={rel:.2e}", flush=True)
         assert worst < 1e-6
+
+
+@pytest.mark.parametrize("n,T,k", [(32, 40, 7), (64, 60, 20)])
+def test_overlapped_backward_equals_one_stream_loop(gpu_lib, n, T, k):
+    """The checkpointed backward (segments recomputed by the twin box on a second stream while the
+    box runs the step adjoints of the next segment) gives the gradient of a plain loop that
+    recomputes every state with single steps and back-propagates in one stream, to rounding
+    (single steps restart the z-inverse from the state; a race would show as an O(1) error)."""
+    import torch
+    from dnsbox_amd.adjoint import step_map
+    dt = 0.005
+    o = Oracle(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4)
+    with make(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        u0 = dev(o.init_solenoidal(3, 0, 4.0, 0.5))
+        g = dev(rand_spec(o, 4))
+        x = u0.clone().requires_grad_(True)
+        (got,) = torch.autograd.grad(step_map(b, x, T, dt, k), x, g)
+        # the same backward, one stream, no twin
+        b.rhs_int_fact(dt)
+        ckpts, u = [], u0.clone()
+        for t in range(0, T, k):
+            ckpts.append(u.clone())
+            b.step_dev(u, min(k, T - t))
+        lam, out = g.clone(), torch.empty_like(g)
+        for seg in reversed(range(len(ckpts))):
+            m = min(k, T - seg * k)
+            states = [ckpts[seg]]
+            for _ in range(m - 1):
+                v = states[-1].clone()
+                b.step_dev(v, 1)
+                states.append(v)
+            for j in reversed(range(m)):
+                b.step_vjp_dev(states[j], lam, out, dt)
+                lam, out = out, lam
+        torch.cuda.synchronize()
+        err = rel_l2(host(got), host(lam))
+        print(f"overlapped backward n={n} T={T} k={k}: rel_l2 vs one-stream loop {err:.3e}", flush=True)
+        assert err <= 1e-14

@@ -163,3 +163,39 @@ def test_group_record_and_vjp_with_recorded_prediction(gpu_lib, prec):
         print(f"group record prec={prec}: states vs single steps {worst_s:.2e}, "
               f"VJP with recorded u* vs recomputed {worst_v:.2e}", flush=True)
         assert worst_s <= tol and worst_v <= tol
+
+
+def test_group_overlapped_backward_equals_one_stream_loop(gpu_lib):
+    """The batched checkpointed backward (twin group on a second stream, recorded u*) gives the
+    gradients of a plain one-stream loop of single group steps and recomputing step adjoints."""
+    import torch
+    from dnsbox_amd import BoxGroup
+    from dnsbox_amd.adjoint import group_step_map
+    T, k = 30, 7
+    us = fields(64, 70)
+    gs = rand_like(us, 11)
+    with BoxGroup([kw(n, nu, 64) for n, nu in zip(SIZES, NUS)]) as g:
+        dts = [DT] * len(us)
+        xs = [u.clone().requires_grad_(True) for u in us]
+        got = torch.autograd.grad(group_step_map(g, xs, T, dts, k), xs, gs)
+        for b in g.boxes:
+            b.rhs_int_fact(DT)
+        ckpts, cur = [], [u.clone() for u in us]
+        for t in range(0, T, k):
+            ckpts.append([c.clone() for c in cur])
+            g.step_dev(cur, min(k, T - t))
+        lams, outs = [x.clone() for x in gs], [torch.empty_like(x) for x in gs]
+        for seg in reversed(range(len(ckpts))):
+            m = min(k, T - seg * k)
+            states = [ckpts[seg]]
+            for _ in range(m - 1):
+                v = [c.clone() for c in states[-1]]
+                g.step_dev(v, 1)
+                states.append(v)
+            for j in reversed(range(m)):
+                g.step_vjp_dev(states[j], lams, outs, dts)
+                lams, outs = outs, lams
+        torch.cuda.synchronize()
+    worst = max(rel_l2(host(a), host(b)) for a, b in zip(got, lams))
+    print(f"group overlapped backward: worst rel_l2 vs one-stream loop {worst:.3e}", flush=True)
+    assert worst <= 1e-14
