@@ -10,7 +10,9 @@ is one IF-Heun step (two RHS evaluations) of the whole box, state resident in HB
 Printed as one JSON line:
   value                  steps/s of that FP32 form
   fp32_pure_*            the all-FP32 form (FP32 state too): rate and error
-  fp64_steps_per_s       the same workload in FP64
+  fp64_steps_per_s       the same workload in FP64 (speedup_vs_fp64: value over it)
+  fp64_unpruned_*        the FP64 step with whole-field transforms, the reference's transform
+                         structure (speedup_vs_fp64_unpruned: value over it)
   *_err_vs_fp64          spectrum (per shell and L2) and energy error vs the FP64 run after
                          --err-steps fixed-dt steps from the same field
   unpruned_steps_per_s   the headline form with the transforms unpruned (the step prunes its
@@ -343,6 +345,16 @@ def main():
     fp64_rate = args.fp64_steps / timed_steps(b64, args.fp64_steps, args.warmup)
     b64.close()
     log(f"fp64 {fp64_rate:.1f} steps/s")
+    # the FP64 step with whole-field transforms, as the reference transforms (p3dfft)
+    fp64_full = None
+    if args.unpruned_steps > 0:
+        b64 = make_box(n, 64)
+        b64.init_random(0, 0, 4.0, 0.5)
+        b64.set_pruning(False)
+        fp64_full = min(args.unpruned_steps, args.fp64_steps) / timed_steps(
+            b64, min(args.unpruned_steps, args.fp64_steps), args.warmup)
+        b64.close()
+        log(f"fp64, unpruned transforms {fp64_full:.1f} steps/s")
 
     # ---- errors vs FP64 (same field, same fixed dt) ----
     err = err_pure = None
@@ -403,6 +415,8 @@ def main():
         "err_dt": err_dt(n),
         "fp64_steps_per_s": fp64_rate,
         "speedup_vs_fp64": value / fp64_rate,
+        "fp64_unpruned_steps_per_s": fp64_full,
+        "speedup_vs_fp64_unpruned": (value / fp64_full) if fp64_full else None,
         "fp32_pure": pure,
         "speedup_fp32_pure_vs_fp64": (pure["steps_per_s"] / fp64_rate) if pure else None,
         "roofline": {"bound": "hbm", "achieved": kd["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
