@@ -69,6 +69,10 @@ SIGNATURES = {
     "dnsbox_step_vjp_pred_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double]),
     "dnsbox_set_pruning": (C.c_int, [_vp, C.c_int]),
     "dnsbox_get_pruning": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "dnsbox_set_vjp_cube": (C.c_int, [_vp, C.c_int]),
+    "dnsbox_get_last_step_pruned": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "dnsbox_group_set_vjp_cube": (C.c_int, [_vp, C.c_int]),
+    "dnsbox_group_get_last_step_pruned": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "dnsbox_rhs_dev": (C.c_int, [_vp, _vp, _vp]),
     "dnsbox_step_dev": (C.c_int, [_vp, _vp, C.c_int]),
     "dnsbox_rhs_vjp_dev": (C.c_int, [_vp, _vp, _vp, _vp]),

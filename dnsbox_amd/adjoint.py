@@ -40,11 +40,14 @@ class _StepMap(torch.autograd.Function):
         box.rhs_int_fact(dt)
         u = u0.detach().clone().contiguous()
         ckpts = []
+        cube = True
         for t in range(0, T, k):   # k steps per call: the step pipeline runs through a segment
             ckpts.append(u.clone())
             box.step_dev(u, min(k, T - t))
+            cube = cube and box.last_step_pruned()
         ctx.box, ctx.T, ctx.dt, ctx.k = box, T, dt, k
         ctx.ckpts = ckpts
+        ctx.cube = cube   # every state of the trajectory lies in the kept cube
         return u
 
     @staticmethod
@@ -84,22 +87,26 @@ class _StepMap(torch.autograd.Function):
                 recorded[seg].record(side)
 
         recompute(nseg - 1)
-        for seg in reversed(range(nseg)):
-            main.wait_event(recorded[seg])
-            start, stop = span(seg)
-            traj, preds = bufs[seg % 2]
-            states = [ctx.ckpts[seg]] + [traj[j] for j in range(stop - start - 1)]
-            # the recorded u* of every step but the segment's last, whose VJP recomputes its own
-            for j in reversed(range(stop - start)):
-                if j < stop - start - 1:
-                    box.step_vjp_pred_dev(states[j], preds[j], lam, out, dt)
-                else:
-                    box.step_vjp_dev(states[j], lam, out, dt)
-                lam, out = out, lam
-            adjointed[seg] = torch.cuda.Event()
-            adjointed[seg].record(main)
-            if seg > 0:
-                recompute(seg - 1)
+        box.set_vjp_cube(ctx.cube)   # states made by pruned steps: the adjoints read the cube only
+        try:
+            for seg in reversed(range(nseg)):
+                main.wait_event(recorded[seg])
+                start, stop = span(seg)
+                traj, preds = bufs[seg % 2]
+                states = [ctx.ckpts[seg]] + [traj[j] for j in range(stop - start - 1)]
+                # the recorded u* of every step but the segment's last, whose VJP recomputes its own
+                for j in reversed(range(stop - start)):
+                    if j < stop - start - 1:
+                        box.step_vjp_pred_dev(states[j], preds[j], lam, out, dt)
+                    else:
+                        box.step_vjp_dev(states[j], lam, out, dt)
+                    lam, out = out, lam
+                adjointed[seg] = torch.cuda.Event()
+                adjointed[seg].record(main)
+                if seg > 0:
+                    recompute(seg - 1)
+        finally:
+            box.set_vjp_cube(False)
         return lam, None, None, None, None
 
 
@@ -116,12 +123,15 @@ class _GroupStepMap(torch.autograd.Function):
             b.rhs_int_fact(dt)
         us = [u.detach().clone().contiguous() for u in u0s]
         ckpts = []
+        cube = True
         for t in range(0, T, k):   # k steps per call: the step pipeline runs through a segment
             c = [torch.empty_like(u) for u in us]
             _copy_all(c, us)
             ckpts.append(c)
             group.step_dev(us, min(k, T - t))
+            cube = cube and group.last_step_pruned()
         ctx.group, ctx.T, ctx.dts, ctx.k, ctx.ckpts = group, T, list(dts), k, ckpts
+        ctx.cube = cube
         return tuple(us)
 
     @staticmethod
@@ -164,21 +174,25 @@ class _GroupStepMap(torch.autograd.Function):
                 recorded[seg].record(side)
 
         recompute(nseg - 1)
-        for seg in reversed(range(nseg)):
-            main.wait_event(recorded[seg])
-            start, stop = span(seg)
-            buf = bufs[seg % 2]
-            for j in reversed(range(stop - start)):
-                states = ctx.ckpts[seg] if j == 0 else [t[j - 1] for t, _ in buf]
-                if j < stop - start - 1:
-                    group.step_vjp_pred_dev(states, [p[j] for _, p in buf], lams, outs, dts)
-                else:
-                    group.step_vjp_dev(states, lams, outs, dts)
-                lams, outs = outs, lams
-            adjointed[seg] = torch.cuda.Event()
-            adjointed[seg].record(main)
-            if seg > 0:
-                recompute(seg - 1)
+        group.set_vjp_cube(ctx.cube)
+        try:
+            for seg in reversed(range(nseg)):
+                main.wait_event(recorded[seg])
+                start, stop = span(seg)
+                buf = bufs[seg % 2]
+                for j in reversed(range(stop - start)):
+                    states = ctx.ckpts[seg] if j == 0 else [t[j - 1] for t, _ in buf]
+                    if j < stop - start - 1:
+                        group.step_vjp_pred_dev(states, [p[j] for _, p in buf], lams, outs, dts)
+                    else:
+                        group.step_vjp_dev(states, lams, outs, dts)
+                    lams, outs = outs, lams
+                adjointed[seg] = torch.cuda.Event()
+                adjointed[seg].record(main)
+                if seg > 0:
+                    recompute(seg - 1)
+        finally:
+            group.set_vjp_cube(False)
         return (None, None, None, None, *lams)
 
 

@@ -228,6 +228,19 @@ class Box:
         check(self.lib.dnsbox_get_pruning(self._h, C.byref(v)))
         return bool(v.value)
 
+    def last_step_pruned(self) -> bool:
+        """Whether the last step_dev / step_dev_record found its buffer confined to the kept cube
+        (and so ran the pruned passes)."""
+        v = C.c_int()
+        check(self.lib.dnsbox_get_last_step_pruned(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def set_vjp_cube(self, confined: bool):
+        """Guarantee (True) that the states the step / RHS adjoints linearise about lie in the kept
+        cube, so that their first passes read the cube only; a state outside it then gives a wrong
+        adjoint.  Default False."""
+        check(self.lib.dnsbox_set_vjp_cube(self._h, int(bool(confined))))
+
     def state_ptr(self) -> int:
         p = C.c_void_p()
         check(self.lib.dnsbox_state_ptr(self._h, C.byref(p)))
@@ -339,6 +352,16 @@ class BoxGroup:
             bh = C.c_void_p()
             check(self.lib.dnsbox_group_box(h, i, C.byref(bh)))
             self.boxes.append(Box._view(bh, arr[i]))
+
+    def last_step_pruned(self) -> bool:
+        """Whether the last step_dev / step_dev_record found every buffer in its kept cube."""
+        v = C.c_int()
+        check(self.lib.dnsbox_group_get_last_step_pruned(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def set_vjp_cube(self, confined: bool):
+        """Box.set_vjp_cube for every box of the group."""
+        check(self.lib.dnsbox_group_set_vjp_cube(self._h, int(bool(confined))))
 
     def twin(self) -> "BoxGroup":
         """A new group with this group's box parameters (own buffers, states and clocks), kept

@@ -99,7 +99,9 @@ struct ProAdj12 {  // (u~, u) -> (s u_i, s i k_j u~_i)
     }
 };
 
-template <int N, class T, class Pro>
+// PR: tiles over the kept (ky, kx) columns, inputs read at kept kz only (inputs confined to the
+// kept cube); every z of those columns is written
+template <int N, class T, class Pro, bool PR = false>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zinv_p(Ptrs<T, Pro::FI> in, cpx<T>* __restrict__ out0, long long out_fs,
              const cpx<T>* __restrict__ tw_g, Pro pro)
@@ -111,7 +113,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     cpx<T>* out[Pro::FO];
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
-    zinv_tile<N, T, Pro>(in.p, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
+    zinv_tile<N, T, Pro, T, PR>(in.p, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
 }
 
 // ---------------------------------------------------------------------------

@@ -544,8 +544,9 @@ struct XGeo {
 //   stage   : registers -> padded LDS rows
 //   compute : c2r pre-processing, row FFTs, pointwise op, row FFTs, r2c post-processing
 //             and the global stores
-// PR: pruned (Keep): the c2r reads kx in Kx only, the r2c stores kx in Kx only
-template <int N, class T, class Op, bool PR = false>
+// PR: pruned (Keep): the c2r reads kx in Kx only; PO: the r2c stores kx in Kx only (the forward
+// step prunes both; the step adjoint, whose outputs fill the whole spectrum, only its inputs)
+template <int N, class T, class Op, bool PR = false, bool PO = PR>
 struct XPass {
     using X = XGeo<N, T, Op>;
     static constexpr int M = X::M, H = X::H, HP = X::HP, RB = X::RB, NTX = X::NTX;
@@ -736,7 +737,7 @@ struct XPass {
                 // k < M in the fixed-k layout, then k = M (one entry per line) by the first FO RB threads
                 constexpr long long FS = (long long)N * N * H;
                 const int k = tid % M;
-                const bool keep = !PR || k <= Keep<N>::KMAX;
+                const bool keep = !PO || k <= Keep<N>::KMAX;
                 if (keep) {
                     const cpx<T> w = tw[k];
                     const int r0 = tid / M;
@@ -770,7 +771,7 @@ struct XPass {
                 const cpx<T> zm = conj(row[X::px(k == 0 ? 0 : M - k)]);
                 const cpx<T> a = T(0.5) * (zk + zm);
                 const cpx<T> b = (T(0.5) * (zk - zm)) * tw[k];
-                if (!PR || k <= Keep<N>::KMAX || k == M) out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
+                if (!PO || k <= Keep<N>::KMAX || k == M) out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
             }
         } else {
             for (int e = tid; e < FO * RB * M; e += NTX) {
@@ -781,14 +782,15 @@ struct XPass {
     }
 };
 
-template <int N, class T, class Op, bool PR = false>
+template <int N, class T, class Op, bool PR = false, bool PO = PR>
 __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long long in_fs,
                                            cpx<T>* __restrict__ out0, long long out_fs, int blk, cpx<T>* lds,
                                            const cpx<T>* __restrict__ tw, XRed<T>& red)
 {
-    using XP = XPass<N, T, Op, PR>;
+    using XP = XPass<N, T, Op, PR, PO>;
     using X = typename XP::X;
-    static_assert(!PR || (Op::IN_SPEC && Op::OUT_SPEC && DNSBOX_X_PRELOAD), "pruned x pass: spectral in and out");
+    static_assert(!PR || (Op::IN_SPEC && DNSBOX_X_PRELOAD), "pruned x-pass input: spectral");
+    static_assert(!PO || Op::OUT_SPEC, "pruned x-pass output: spectral");
     if constexpr (Op::IN_SPEC && DNSBOX_X_PRELOAD) {
         cpx<T> z[XP::QZ];
         XP::load_pre(in0, in_fs, blk, tw, z);
@@ -833,7 +835,7 @@ __device__ __forceinline__ void block_max_atomic(XRed<T>& red, unsigned long lon
     }
 }
 
-template <int N, class T, class Op, bool PR = false>
+template <int N, class T, class Op, bool PR = false, bool PO = PR>
 __global__ void __launch_bounds__((Cfg<N, T>::NTX))
     k_xpass(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
             long long out_fs, const cpx<T>* __restrict__ tw_g, unsigned long long* red_bits,
@@ -847,7 +849,7 @@ __global__ void __launch_bounds__((Cfg<N, T>::NTX))
     XRed<T> red;
 #pragma unroll
     for (int j = 0; j < 4; ++j) red.m[j] = Op::NRED == 4 && j == 3 ? T(-1e300) : T(0);   // -1e300: -inf in FP32
-    xpass_tile<N, T, Op, PR>(in0, in_fs, out0, out_fs, blockIdx.x, lds, tw, red);
+    xpass_tile<N, T, Op, PR, PO>(in0, in_fs, out0, out_fs, blockIdx.x, lds, tw, red);
     if constexpr (Op::NRED > 0) {
         if (red_bits) block_max_atomic<Op::NRED>(red, red_bits, signed_last != 0, wsm);
     }

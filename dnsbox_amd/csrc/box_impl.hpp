@@ -78,6 +78,10 @@ class BoxBase {
     bool prune = true;       // run the step's passes pruned to the kept cube when dealiased
     bool pruned() const { return prune && dealiased; }
     void* pred_out_ = nullptr;   // where the IF-Heun step stores u* (null: internal buffer)
+    // the caller guarantees that the states the step / RHS adjoints linearise about have no
+    // energy outside the kept cube (dnsbox_set_vjp_cube); their first passes then read the cube only
+    bool vjp_cube = false;
+    bool last_ext_pruned = false;   // the last step of a caller's buffer ran the pruned passes
     int scratch_fields = 6;
     void* adjbuf = nullptr;  // 9 fields: A, Mu, u~ (adjoint passes), lazily
     double* band2 = nullptr; // band-forcing scalars of the VJP
@@ -558,9 +562,18 @@ class BoxT final : public BoxBase {
     C* adjMu() const { return static_cast<C*>(adjbuf) + 3 * fs; }
     C* adjUt() const { return static_cast<C*>(adjbuf) + 6 * fs; }
 
-    // VJP of N at u with cotangent lam_scale * lam; epilogue MODE of k_zfwd_vjp
+    // VJP of N at u with cotangent lam_scale * lam; epilogue MODE of k_zfwd_vjp.  With vjp_cube
+    // set (u confined to the kept cube, as the caller guarantees) the passes up to the x pass's
+    // loads read the kept cube only (the cotangent enters masked); the products, and so the rest,
+    // fill the whole spectrum.
     template <int MODE>
     void vjp_passes(const C* u, const C* lam, double lam_scale, VjpArgs<T> va)
+    {
+        if (prune && vjp_cube) vjp_passes_t<MODE, true>(u, lam, lam_scale, va);
+        else vjp_passes_t<MODE, false>(u, lam, lam_scale, va);
+    }
+    template <int MODE, bool PR>
+    void vjp_passes_t(const C* u, const C* lam, double lam_scale, VjpArgs<T> va)
     {
         const double sm = 1.0 / (double(N) * N * N);
         if (cfg.ph.bandlim) {
@@ -570,13 +583,14 @@ class BoxT final : public BoxBase {
         }
         Ptrs<T, 6> in{{lam, lam + fs, lam + 2 * fs, u, u + fs, u + 2 * fs}};
         ProVjpIn<T> pro{T(sm), T(lam_scale), N, cfg.ph};
-        hipLaunchKernelGGL((k_zinv_p<N, T, ProVjpIn<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in, X(), fs,
-                           twc(), pro);
+        hipLaunchKernelGGL((k_zinv_p<N, T, ProVjpIn<T>, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream, in,
+                           X(), fs, twc(), pro);
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 9), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<N, T, +1, PR>), dim3(ytiles<N, T, PR>(), 9), dim3(YNT), 0, stream, X(), fs,
+                           twc());
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_xpass<N, T, OpVjp<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream, X(),
-                           fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+        hipLaunchKernelGGL((k_xpass<N, T, OpVjp<T>, PR, false>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream,
+                           X(), fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
@@ -726,6 +740,7 @@ class BoxT final : public BoxBase {
         const size_t sb3 = 3 * size_t(fs) * sizeof(CS);
         try {
             dealiased = prune && buffer_dealiased(static_cast<const CS*>(st));
+            last_ext_pruned = dealiased;
             state = st;
             if (traj_pred && dealiased) DNSBOX_HIP(hipMemsetAsync(traj_pred, 0, nsteps * sb3, stream));
             for (int i = 0; i < nsteps; ++i) {

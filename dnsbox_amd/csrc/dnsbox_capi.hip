@@ -537,6 +537,21 @@ int dnsbox_set_pruning(dnsbox_ctx* c, int enable)
     return 0;
 }
 
+int dnsbox_set_vjp_cube(dnsbox_ctx* c, int confined)
+{
+    if (int e = check_ctx(c)) return e;
+    c->box->vjp_cube = confined != 0;
+    return 0;
+}
+
+int dnsbox_get_last_step_pruned(dnsbox_ctx* c, int* pruned)
+{
+    if (int e = check_ctx(c)) return e;
+    if (!pruned) return fail(DNSBOX_E_ARG, "null pointer");
+    *pruned = c->box->last_ext_pruned ? 1 : 0;
+    return 0;
+}
+
 int dnsbox_get_pruning(dnsbox_ctx* c, int* active)
 {
     if (int e = check_ctx(c)) return e;
@@ -763,6 +778,20 @@ int dnsbox_group_step_vjp_pred_dev(dnsbox_group* g, const void* const* u, const 
         for (auto& b : g->g->boxes) b->x6_valid = false;
         return 0;
     });
+}
+
+int dnsbox_group_set_vjp_cube(dnsbox_group* g, int confined)
+{
+    if (!g) return fail(DNSBOX_E_ARG, "null group");
+    g->g->vjp_cube = confined != 0;
+    return 0;
+}
+
+int dnsbox_group_get_last_step_pruned(dnsbox_group* g, int* pruned)
+{
+    if (!g || !pruned) return fail(DNSBOX_E_ARG, "null argument");
+    *pruned = g->g->last_ext_pruned ? 1 : 0;
+    return 0;
 }
 
 int dnsbox_group_step_vjp_dev(dnsbox_group* g, const void* const* u, const void* const* lam, void* const* lam_out,

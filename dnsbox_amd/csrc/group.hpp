@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restr
 }
 
 // x pass of Op in place on the scratch; OpCross can track the CFL maxima
-template <class T, class Op, bool PR = false>
+template <class T, class Op, bool PR = false, bool PO = PR>
 __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __restrict__ gb, int nbox, int track)
 {
     __shared__ cpx<T> lds[GroupLds<T>::template X<Op>];
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __rest
     XRed<T> red;
 #pragma unroll
     for (int j = 0; j < 4; ++j) red.m[j] = T(0);
-#define DNSBOX_G_X(NN) xpass_tile<NN, T, Op, PR>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
+#define DNSBOX_G_X(NN) xpass_tile<NN, T, Op, PR, PO>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_X)
 #undef DNSBOX_G_X
     if constexpr (Op::NRED == 3) {
@@ -225,8 +225,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_predonly(const GBox<T>
 }
 
 // z-inverse of the VJP prologue: STAGE 1 at u* with cotangent dt/2 lam, STAGE 2 at u
-// with cotangent Mu
-template <class T, int STAGE>
+// with cotangent Mu; PR: the kept columns only, inputs read at kept kz (states in the cube)
+template <class T, int STAGE, bool PR = false>
 __global__ void __launch_bounds__(kGroupThreads) k_g_zinv_vjp(const GBox<T>* __restrict__ gb, int nbox)
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
@@ -234,10 +234,10 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv_vjp(const GBox<T>* __r
     // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
     // all XCDs, which balances boxes of uneven cost
     const int gt = blockIdx.x;
-    const int b = find_box(gb, nbox, gt, 0);
+    const int b = find_box(gb, nbox, gt, PR ? 3 : 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    const int tile = gt - bx.zt0;
+    const int tile = gt - (PR ? bx.zt0p : bx.zt0);
     const cpx<T>* l = STAGE == 1 ? bx.lam : bx.Mu;
     const cpx<T>* p = STAGE == 1 ? bx.pred : bx.u;
     const cpx<T>* in[6] = {l, l + bx.fs, l + 2 * bx.fs, p, p + bx.fs, p + 2 * bx.fs};
@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv_vjp(const GBox<T>* __r
     for (int f = 0; f < 9; ++f) out[f] = bx.X + f * bx.fs;
     const double sm = 1.0 / (double(bx.n) * bx.n * bx.n);
     const ProVjpIn<T> pro{T(sm), T(STAGE == 1 ? 0.5 * bx.dt : 1.0), bx.n, bx.ph};
-#define DNSBOX_G_ZV(NN) zinv_tile<NN, T, ProVjpIn<T>>(in, out, tile, lds, tw, pro)
+#define DNSBOX_G_ZV(NN) zinv_tile<NN, T, ProVjpIn<T>, T, PR>(in, out, tile, lds, tw, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZV)
 #undef DNSBOX_G_ZV
 }
@@ -364,6 +364,8 @@ class GroupBase {
     virtual ~GroupBase() = default;
     std::vector<std::unique_ptr<BoxBase>> boxes;
     hipStream_t stream = nullptr;
+    bool vjp_cube = false;            // the caller guarantees the VJPs' states lie in the kept cubes
+    bool last_ext_pruned = false;     // the last steps of caller buffers ran the pruned passes
     virtual void step(int nsteps) = 0;
     virtual void step_ext(void* const* states, int nsteps) = 0;
     virtual void step_vjp(const void* const* u, const void* const* lam, void* const* out, const double* dt) = 0;
@@ -606,6 +608,7 @@ class Group final : public GroupBase {
         const GBox<T>* d = descriptors(h);
         x6_valid = false;
         const bool pr = buffers_dealiased(d);
+        last_ext_pruned = pr;
         if (pr) launch_zinv<true>(stream, d, 0);
         else launch_zinv<false>(stream, d, 0);
         for (int i = 0; i < nsteps; ++i) one_step(stream, d, pr);
@@ -622,6 +625,7 @@ class Group final : public GroupBase {
         x6_valid = false;
         const GBox<T>* d0 = descriptors(h);
         const bool pr = buffers_dealiased(d0);
+        last_ext_pruned = pr;
         if (pr) {
             // the pruned predictor stores u* at the kept modes only: the rest of each slot is zero
             for (size_t i = 0; i < h.size(); ++i)
@@ -693,13 +697,23 @@ class Group final : public GroupBase {
         vjp_passes<VJP_S2>(stream, d);
     }
 
+    // the VJP passes; with vjp_cube (every box's linearisation state confined to its kept cube,
+    // as the caller guarantees) those up to the x pass's loads read the kept cube only
     template <int MODE>
     void vjp_passes(hipStream_t s, const GBox<T>* d)
     {
+        bool pr = vjp_cube;
+        for (auto& b : boxes) pr = pr && b->prune;
+        if (pr) vjp_passes_t<MODE, true>(s, d);
+        else vjp_passes_t<MODE, false>(s, d);
+    }
+    template <int MODE, bool PR>
+    void vjp_passes_t(hipStream_t s, const GBox<T>* d)
+    {
         const int nb = int(h_gb.size());
-        hipLaunchKernelGGL((k_g_zinv_vjp<T, MODE>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
-        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 9), dim3(kGroupThreads), 0, s, d, nb);
-        hipLaunchKernelGGL((k_g_xpass<T, OpVjp<T>>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_zinv_vjp<T, MODE, PR>), dim3(PR ? ztp : zt), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(PR ? ytp : yt, 9), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpVjp<T>, PR, false>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
         hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
         hipLaunchKernelGGL((k_g_zfwd_vjp<T, MODE>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
         DNSBOX_LAUNCH_CHECK();

@@ -157,6 +157,16 @@ int dnsbox_pass_bytes(dnsbox_ctx* ctx, double* bytes);
 int dnsbox_set_pruning(dnsbox_ctx* ctx, int enable);
 int dnsbox_get_pruning(dnsbox_ctx* ctx, int* active);
 
+/* Step and RHS adjoints about states confined to the 2/3-rule cube.  set_vjp_cube(ctx, 1) is the
+   caller's guarantee that the states the *_vjp_* entry points linearise about (u, and u* where
+   given) have no energy outside the kept cube; their passes up to the x pass then read the cube
+   only (the cotangent enters masked), and the results agree with the full passes to rounding.
+   A state outside the cube with the flag set gives a wrong adjoint: set it only for states made
+   by pruned steps, which get_last_step_pruned reports for the last dnsbox_step_dev(_record)
+   call (the device check of its buffer).  Default off.  No reference counterpart. */
+int dnsbox_set_vjp_cube(dnsbox_ctx* ctx, int confined);
+int dnsbox_get_last_step_pruned(dnsbox_ctx* ctx, int* pruned);
+
 /* device-pointer entry points (internal layout, context precision) */
 int dnsbox_state_ptr(dnsbox_ctx* ctx, void** dev_state);   /* the context's state buffer (ends
                                                               pruning until the next dealias) */
@@ -221,6 +231,10 @@ int dnsbox_group_step_dev_record(dnsbox_group* g, void* const* states, int nstep
    counterpart. */
 int dnsbox_group_step_vjp_pred_dev(dnsbox_group* g, const void* const* u, const void* const* ustar,
                                    const void* const* lam, void* const* lam_out, const double* dt);
+/* the group forms of dnsbox_set_vjp_cube / dnsbox_get_last_step_pruned (every box's states;
+   the last dnsbox_group_step_dev(_record) call).  No reference counterpart. */
+int dnsbox_group_set_vjp_cube(dnsbox_group* g, int confined);
+int dnsbox_group_get_last_step_pruned(dnsbox_group* g, int* pruned);
 
 #ifdef __cplusplus
 }

@@ -231,3 +231,38 @@ def test_overlapped_backward_equals_one_stream_loop(gpu_lib, n, T, k):
         err = rel_l2(host(got), host(lam))
         print(f"overlapped backward n={n} T={T} k={k}: rel_l2 vs one-stream loop {err:.3e}", flush=True)
         assert err <= 1e-14
+
+
+@pytest.mark.parametrize("n,prec", [(32, 64), (64, 64), (48, 32)])
+def test_vjp_cube_matches_full_passes(gpu_lib, n, prec):
+    """With set_vjp_cube (linearisation states confined to the kept cube) the RHS and step
+    adjoints read only the cube in their first passes and agree with the full passes to rounding;
+    last_step_pruned reports the device check of the last step on a caller's buffer."""
+    import torch
+    o = Oracle(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4)
+    dt = 0.005
+    dtype = torch.complex128 if prec == 64 else torch.complex64
+    tol = 1e-13 if prec == 64 else 1e-6
+    with make(n, precision=prec, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        b.rhs_int_fact(dt)
+        u = dev(o.init_solenoidal(5, 0, 4.0, 0.5)).to(dtype)
+        b.step_dev(u, 2)
+        assert b.last_step_pruned()
+        lam = dev(rand_spec(o, 6)).to(dtype)
+        outs = {}
+        for cube in (False, True):
+            b.set_vjp_cube(cube)
+            r = torch.empty_like(u)
+            st = torch.empty_like(u)
+            b.rhs_vjp_dev(u, lam, r)
+            b.step_vjp_dev(u, lam, st, dt)
+            torch.cuda.synchronize()
+            outs[cube] = (host(r), host(st))
+        b.set_vjp_cube(False)
+        e_rhs = rel_l2(outs[True][0], outs[False][0])
+        e_step = rel_l2(outs[True][1], outs[False][1])
+        print(f"vjp cube n={n} prec={prec}: RHS-VJP rel {e_rhs:.2e}, step-VJP rel {e_step:.2e}", flush=True)
+        assert e_rhs <= tol and e_step <= tol
+        w = dev(rand_spec(o, 7)).to(dtype)   # energy on every mode
+        b.step_dev(w, 1)
+        assert not b.last_step_pruned()

@@ -199,3 +199,30 @@ def test_group_overlapped_backward_equals_one_stream_loop(gpu_lib):
     worst = max(rel_l2(host(a), host(b)) for a, b in zip(got, lams))
     print(f"group overlapped backward: worst rel_l2 vs one-stream loop {worst:.3e}", flush=True)
     assert worst <= 1e-14
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_group_vjp_cube_matches_full_passes(gpu_lib, prec):
+    """Group step adjoints with set_vjp_cube (states made by pruned steps) agree with the full
+    passes to rounding."""
+    import torch
+    from dnsbox_amd import BoxGroup
+    tol = 1e-13 if prec == 64 else 1e-6
+    us = fields(prec, 20)
+    lams = rand_like(us, 21)
+    with BoxGroup([kw(n, nu, prec) for n, nu in zip(SIZES, NUS)]) as g:
+        for b in g.boxes:
+            b.rhs_int_fact(DT)
+        g.step_dev(us, 2)
+        assert g.last_step_pruned()
+        res = {}
+        for cube in (False, True):
+            g.set_vjp_cube(cube)
+            outs = [torch.empty_like(u) for u in us]
+            g.step_vjp_dev(us, lams, outs, [DT] * len(us))
+            torch.cuda.synchronize()
+            res[cube] = [host(o) for o in outs]
+        g.set_vjp_cube(False)
+    worst = max(rel_l2(a, b) for a, b in zip(res[True], res[False]))
+    print(f"group vjp cube prec={prec}: worst rel {worst:.2e}", flush=True)
+    assert worst <= tol
