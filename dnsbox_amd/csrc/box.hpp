@@ -1127,6 +1127,18 @@ __device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long
     }
 }
 
+// end of step bookkeeping: t += dt, dt_prev = dt, clear the CFL maxima
+__device__ __forceinline__ void step_end(BoxScalars* sc)
+{
+    sc->t += sc->dt;
+    sc->dt_prev = sc->dt;
+    sc->maxu[0] = sc->maxu[1] = sc->maxu[2] = 0ull;
+    sc->nstep += 1;
+}
+
+// The corrector pass also ends the step (block 0, first thread): no block of that pass reads
+// what step_end writes (the corrector reads dt, which step_end only reads), and the next
+// reader of those scalars is a later kernel.
 template <int N, class T, int MODE, class S = T, bool PR = false>
 __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw_g, EpiArgs<T, S> ea,
@@ -1137,18 +1149,15 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     zfwd_zinv_tile<N, T, MODE, S, PR>(X, fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea, pro);
+    if constexpr (MODE == EPI_CORR) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) step_end(ea.sc);
+    }
 }
 
-// end of step bookkeeping: t += dt, dt_prev = dt, clear the CFL maxima
 template <int D = 0>
 __global__ void k_step_end(BoxScalars* sc)
 {
-    if (threadIdx.x == 0) {
-        sc->t += sc->dt;
-        sc->dt_prev = sc->dt;
-        sc->maxu[0] = sc->maxu[1] = sc->maxu[2] = 0ull;
-        sc->nstep += 1;
-    }
+    if (threadIdx.x == 0) step_end(sc);
 }
 
 // ===========================================================================

@@ -341,10 +341,8 @@ class BoxT final : public BoxBase {
         hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream,
                            X(), fs, twc(), eb, pro);
         DNSBOX_LAUNCH_CHECK();
-        mark(5);
+        mark(5);   // the corrector pass also ended the step (t, dt_prev, CFL maxima)
         eband(St());
-        hipLaunchKernelGGL(k_step_end<0>, dim3(1), dim3(64), 0, stream, sc);
-        DNSBOX_LAUNCH_CHECK();
         mark(-1);
         x6_valid = true;
         x6_pruned = PR;

@@ -195,6 +195,11 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
 #define DNSBOX_G_Z(NN) zfwd_zinv_tile<NN, T, MODE, T, PR>(bx.X, bx.fs, tile, lds, tw, ea, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_Z)
 #undef DNSBOX_G_Z
+    // the corrector also ends each box's step (its first tile; no tile of this pass reads the
+    // scalars step_end writes)
+    if constexpr (MODE == EPI_CORR) {
+        if (tile == 0 && threadIdx.x == 0) step_end(bx.sc);
+    }
 }
 
 // u* = E (u + dt N(u)) of the step adjoint, into pred
@@ -304,18 +309,6 @@ __global__ void __launch_bounds__(256) k_g_copy_state(const GBox<T>* __restrict_
     const long long n = 3 * bx.fs;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         bx.tout[i] = bx.st[i];
-}
-
-template <class T>
-__global__ void k_g_step_end(const GBox<T>* __restrict__ gb, int nbox)
-{
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbox) return;
-    BoxScalars* sc = gb[b].sc;
-    sc->t += sc->dt;
-    sc->dt_prev = sc->dt;
-    sc->maxu[0] = sc->maxu[1] = sc->maxu[2] = 0ull;
-    sc->nstep += 1;
 }
 
 inline int group_ztiles(int n, int prec)
@@ -514,7 +507,6 @@ class Group final : public GroupBase {
         hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>, PR>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
         hipLaunchKernelGGL((k_g_ycol<T, -1, PR>), dim3(y, 3), dim3(kGroupThreads), 0, s, d, nb);
         hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_CORR, PR>), dim3(z), dim3(kGroupThreads), 0, s, d, nb);
-        hipLaunchKernelGGL((k_g_step_end<T>), dim3((nb + 63) / 64), dim3(64), 0, s, d, nb);
         DNSBOX_LAUNCH_CHECK();
     }
     void one_step(hipStream_t s, const GBox<T>* d, bool pr)
