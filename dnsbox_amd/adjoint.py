@@ -41,9 +41,22 @@ class _StepMap(torch.autograd.Function):
         u = u0.detach().clone().contiguous()
         ckpts = []
         cube = True
-        for t in range(0, T, k):   # k steps per call: the step pipeline runs through a segment
+        ctx.last = None
+        starts = list(range(0, T, k))
+        for t in starts:   # k steps per call: the step pipeline runs through a segment
             ckpts.append(u.clone())
-            box.step_dev(u, min(k, T - t))
+            m = min(k, T - t)
+            if t == starts[-1] and m > 1 and ctx.needs_input_grad[0]:
+                # the last segment, which the backward starts from, is recorded now rather than
+                # recomputed then: its first m - 1 states and predicted states, then its last step
+                shape = (max(k - 1, 1),) + tuple(u.shape)
+                ctx.last = (torch.empty(shape, dtype=u.dtype, device=u.device),
+                            torch.empty(shape, dtype=u.dtype, device=u.device))
+                box.step_dev_record(u, m - 1, *ctx.last)
+                cube = cube and box.last_step_pruned()
+                box.step_dev(u, 1)
+            else:
+                box.step_dev(u, m)
             cube = cube and box.last_step_pruned()
         ctx.box, ctx.T, ctx.dt, ctx.k = box, T, dt, k
         ctx.ckpts = ckpts
@@ -64,6 +77,8 @@ class _StepMap(torch.autograd.Function):
         # from the other: at small N neither fills the GPU, so the two overlap.
         bufs = [(torch.empty(shape, dtype=lam.dtype, device=lam.device),
                  torch.empty(shape, dtype=lam.dtype, device=lam.device)) for _ in range(min(2, nseg))]
+        if ctx.last is not None:   # the last segment was recorded by the forward
+            bufs[(nseg - 1) % 2] = ctx.last
         rec = box.twin()
         rec.rhs_int_fact(dt)
         main = torch.cuda.current_stream(lam.device)
@@ -86,7 +101,11 @@ class _StepMap(torch.autograd.Function):
                 recorded[seg] = torch.cuda.Event()
                 recorded[seg].record(side)
 
-        recompute(nseg - 1)
+        if ctx.last is not None:
+            recorded[nseg - 1] = torch.cuda.Event()
+            recorded[nseg - 1].record(main)
+        else:
+            recompute(nseg - 1)
         box.set_vjp_cube(ctx.cube)   # states made by pruned steps: the adjoints read the cube only
         try:
             for seg in reversed(range(nseg)):
@@ -124,11 +143,23 @@ class _GroupStepMap(torch.autograd.Function):
         us = [u.detach().clone().contiguous() for u in u0s]
         ckpts = []
         cube = True
-        for t in range(0, T, k):   # k steps per call: the step pipeline runs through a segment
+        ctx.last = None
+        starts = list(range(0, T, k))
+        for t in starts:   # k steps per call: the step pipeline runs through a segment
             c = [torch.empty_like(u) for u in us]
             _copy_all(c, us)
             ckpts.append(c)
-            group.step_dev(us, min(k, T - t))
+            m = min(k, T - t)
+            if t == starts[-1] and m > 1 and any(ctx.needs_input_grad[4:]):
+                # the last segment is recorded now (as in _StepMap.forward)
+                w = max(k - 1, 1)
+                ctx.last = [(torch.empty((w,) + tuple(u.shape), dtype=u.dtype, device=u.device),
+                             torch.empty((w,) + tuple(u.shape), dtype=u.dtype, device=u.device)) for u in us]
+                group.step_dev_record(us, m - 1, [a for a, _ in ctx.last], [b for _, b in ctx.last])
+                cube = cube and group.last_step_pruned()
+                group.step_dev(us, 1)
+            else:
+                group.step_dev(us, m)
             cube = cube and group.last_step_pruned()
         ctx.group, ctx.T, ctx.dts, ctx.k, ctx.ckpts = group, T, list(dts), k, ckpts
         ctx.cube = cube
@@ -150,6 +181,8 @@ class _GroupStepMap(torch.autograd.Function):
         bufs = [[(torch.empty((m,) + tuple(l.shape), dtype=l.dtype, device=l.device),
                   torch.empty((m,) + tuple(l.shape), dtype=l.dtype, device=l.device)) for l in lams]
                 for _ in range(min(2, nseg))]
+        if ctx.last is not None:   # the last segment was recorded by the forward
+            bufs[(nseg - 1) % 2] = ctx.last
         rec = group.twin()
         for b, dt in zip(rec.boxes, dts):
             b.rhs_int_fact(dt)
@@ -173,7 +206,11 @@ class _GroupStepMap(torch.autograd.Function):
                 recorded[seg] = torch.cuda.Event()
                 recorded[seg].record(side)
 
-        recompute(nseg - 1)
+        if ctx.last is not None:
+            recorded[nseg - 1] = torch.cuda.Event()
+            recorded[nseg - 1].record(main)
+        else:
+            recompute(nseg - 1)
         group.set_vjp_cube(ctx.cube)
         try:
             for seg in reversed(range(nseg)):
