@@ -1465,4 +1465,50 @@ __global__ void k_convert(TO* __restrict__ dst, const TI* __restrict__ src, long
         dst[i] = TO(src[i]);
 }
 
+// One spectral field between the reference layout uhat(Nz, Ny, Nx/2+1) (element (ix, iy, iz) at
+// (ix N + iy) N + iz, FP64 complex) and the device layout (at (iz N + iy) H + ix, type D):
+// TO_DEV reads the reference layout.  A block transposes a 32 x 32 (ix, iz) tile of one iy
+// plane through LDS, so that both sides are read and written along their fastest index.
+// Grid (ceil(N / 32), ceil(H / 32), N), 256 threads.
+template <class D, bool TO_DEV>
+__global__ void __launch_bounds__(256) k_ref_layout(const void* __restrict__ src, void* __restrict__ dst, int n)
+{
+    __shared__ cpx<double> tile[32][33];
+    const int h = n / 2 + 1;
+    const int iz0 = blockIdx.x * 32, ix0 = blockIdx.y * 32, iy = blockIdx.z;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    // tile[a][b] holds (ix, iz) = (ix0 + a, iz0 + b)
+    if constexpr (TO_DEV) {
+        const cpx<double>* r = static_cast<const cpx<double>*>(src);
+        for (int j = 0; j < 32; j += 8) {
+            const int ix = ix0 + ty + j, iz = iz0 + tx;
+            if (ix < h && iz < n) tile[ty + j][tx] = r[((long long)ix * n + iy) * n + iz];
+        }
+        __syncthreads();
+        cpx<D>* d = static_cast<cpx<D>*>(dst);
+        for (int j = 0; j < 32; j += 8) {
+            const int iz = iz0 + ty + j, ix = ix0 + tx;
+            if (ix < h && iz < n) {
+                const cpx<double> v = tile[tx][ty + j];
+                d[((long long)iz * n + iy) * h + ix] = cpx<D>{D(v.x), D(v.y)};
+            }
+        }
+    } else {
+        const cpx<D>* d = static_cast<const cpx<D>*>(src);
+        for (int j = 0; j < 32; j += 8) {
+            const int iz = iz0 + ty + j, ix = ix0 + tx;
+            if (ix < h && iz < n) {
+                const cpx<D> v = d[((long long)iz * n + iy) * h + ix];
+                tile[tx][ty + j] = cpx<double>{double(v.x), double(v.y)};
+            }
+        }
+        __syncthreads();
+        cpx<double>* r = static_cast<cpx<double>*>(dst);
+        for (int j = 0; j < 32; j += 8) {
+            const int ix = ix0 + ty + j, iz = iz0 + tx;
+            if (ix < h && iz < n) r[((long long)ix * n + iy) * n + iz] = tile[ty + j][tx];
+        }
+    }
+}
+
 }  // namespace dnsbox

@@ -118,6 +118,12 @@ class BoxBase {
     // FP64 device array -> transform type / state type, and back
     virtual void convert_in(void* dst, const double* src, long long n, bool state_type) = 0;
     virtual void convert_out(double* dst, const void* src, long long n, bool state_type) = 0;
+    // one spectral field: the reference layout (FP64 complex, device memory) -> dst in the device
+    // layout (state type or transform type), and back
+    virtual void from_ref_layout(void* dst, const double* src, bool state_type) = 0;
+    virtual void to_ref_layout(double* dst, const void* src, bool state_type) = 0;
+    // the state has no energy outside the kept cube (checked on the device; synchronises)
+    virtual bool state_confined() = 0;
     // allocate at least nf scratch fields (and the adjoint buffers) now
     virtual void reserve(int nf, bool adj) = 0;
 };
@@ -856,6 +862,22 @@ class BoxT final : public BoxBase {
         DNSBOX_HIP(hipFree(tmp));
         for (int p = 0; p < kNPass; ++p) ms[p] = cnt[p] ? sum[p] / cnt[p] : 0.0;
     }
+
+    void from_ref_layout(void* dst, const double* src, bool state_type) override
+    {
+        const dim3 grid((N + 31) / 32, (G::H + 31) / 32, N);
+        if (state_type) hipLaunchKernelGGL((k_ref_layout<S, true>), grid, dim3(256), 0, stream, src, dst, N);
+        else hipLaunchKernelGGL((k_ref_layout<T, true>), grid, dim3(256), 0, stream, src, dst, N);
+        DNSBOX_LAUNCH_CHECK();
+    }
+    void to_ref_layout(double* dst, const void* src, bool state_type) override
+    {
+        const dim3 grid((N + 31) / 32, (G::H + 31) / 32, N);
+        if (state_type) hipLaunchKernelGGL((k_ref_layout<S, false>), grid, dim3(256), 0, stream, src, dst, N);
+        else hipLaunchKernelGGL((k_ref_layout<T, false>), grid, dim3(256), 0, stream, src, dst, N);
+        DNSBOX_LAUNCH_CHECK();
+    }
+    bool state_confined() override { return buffer_dealiased(St()); }
 
     void convert_in(void* dst, const double* src, long long n, bool state_type) override
     {

@@ -75,52 +75,33 @@ struct DevBuf {
     }
 };
 
-// reference layout [ix][iy][iz] (complex) -> device layout [iz][iy][ix]
-void ref_to_dev_layout(const double* src, std::vector<double>& dst, int n)
+// 3 host spectral fields in the reference layout uhat(Nz, Ny, Nx/2+1) (complex(8)) -> dev, in the
+// device layout and the state or transform type: a contiguous copy of each field to the GPU,
+// then the layout transpose there (k_ref_layout)
+void upload_ref3(dnsbox_ctx* c, void* dev, const double* const* src, long long dstride_bytes, bool state_type)
 {
-    const int h = n / 2 + 1;
-    dst.resize(size_t(2) * n * n * h);
-    for (int ix = 0; ix < h; ++ix)
-        for (int iy = 0; iy < n; ++iy)
-            for (int iz = 0; iz < n; ++iz) {
-                const size_t s = (size_t(ix) * n + iy) * n + iz;
-                const size_t d = (size_t(iz) * n + iy) * h + ix;
-                dst[2 * d] = src[2 * s];
-                dst[2 * d + 1] = src[2 * s + 1];
-            }
+    BoxBase* b = c->box;
+    const size_t per = size_t(b->fs) * 2 * sizeof(double);
+    DevBuf tmp(per);
+    for (int f = 0; f < 3; ++f) {
+        DNSBOX_HIP(hipMemcpyAsync(tmp.p, src[f], per, hipMemcpyHostToDevice, b->stream));
+        b->from_ref_layout(static_cast<char*>(dev) + f * dstride_bytes, static_cast<const double*>(tmp.p),
+                           state_type);
+        DNSBOX_HIP(hipStreamSynchronize(b->stream));   // tmp holds the next field next
+    }
 }
 
-// true when the 3 device-layout fields hold no energy outside the kept cube of the 2/3 rule
-// (|kval| <= (n-1)/3 on every axis; the step's pruned passes rely on it, box.hpp Keep)
-bool zero_outside_kept_cube(const std::vector<double>* h, int n)
+// the reverse of upload_ref3
+void download_ref3(dnsbox_ctx* c, const void* dev, double* const* dst, long long sstride_bytes, bool state_type)
 {
-    const int hh = n / 2 + 1, kmax = (n - 1) / 3;
-    auto kept = [n, kmax](int i) {
-        const int k = i < n / 2 ? i : (i == n / 2 ? 0 : i - n);
-        return (k < 0 ? -k : k) <= kmax;
-    };
-    for (int iz = 0; iz < n; ++iz)
-        for (int iy = 0; iy < n; ++iy)
-            for (int ix = 0; ix < hh; ++ix) {
-                if (kept(iz) && kept(iy) && kept(ix)) continue;
-                const size_t d = (size_t(iz) * n + iy) * hh + ix;
-                for (int f = 0; f < 3; ++f)
-                    if (h[f][2 * d] != 0.0 || h[f][2 * d + 1] != 0.0) return false;
-            }
-    return true;
-}
-
-void dev_to_ref_layout(const std::vector<double>& src, double* dst, int n)
-{
-    const int h = n / 2 + 1;
-    for (int ix = 0; ix < h; ++ix)
-        for (int iy = 0; iy < n; ++iy)
-            for (int iz = 0; iz < n; ++iz) {
-                const size_t d = (size_t(ix) * n + iy) * n + iz;
-                const size_t s = (size_t(iz) * n + iy) * h + ix;
-                dst[2 * d] = src[2 * s];
-                dst[2 * d + 1] = src[2 * s + 1];
-            }
+    BoxBase* b = c->box;
+    const size_t per = size_t(b->fs) * 2 * sizeof(double);
+    DevBuf tmp(per);
+    for (int f = 0; f < 3; ++f) {
+        b->to_ref_layout(static_cast<double*>(tmp.p), static_cast<const char*>(dev) + f * sstride_bytes, state_type);
+        DNSBOX_HIP(hipMemcpyAsync(dst[f], tmp.p, per, hipMemcpyDeviceToHost, b->stream));
+        DNSBOX_HIP(hipStreamSynchronize(b->stream));
+    }
 }
 
 // copy 3 host FP64 arrays (already in device layout, nper reals each) into dev (context precision)
@@ -275,13 +256,10 @@ int dnsbox_set_state(dnsbox_ctx* c, const double* u, const double* v, const doub
     invalidate(c);
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null state array");
     return guarded([&]() -> int {
-        const int n = c->p.n;
-        std::vector<double> h[3];
-        const double* src[3] = {u, v, w};
-        for (int f = 0; f < 3; ++f) ref_to_dev_layout(src[f], h[f], n);
         BoxBase* b = c->box;
-        upload3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->sbytes, true);
-        b->dealiased = zero_outside_kept_cube(h, n);
+        const double* src[3] = {u, v, w};
+        upload_ref3(c, b->state, src, b->fs * (long long)b->sbytes, true);
+        b->dealiased = b->state_confined();   // the pruned passes rely on it
         b->ab2_started = false;
         return 0;
     });
@@ -293,10 +271,8 @@ int dnsbox_get_state(dnsbox_ctx* c, double* u, double* v, double* w)
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null state array");
     return guarded([&]() -> int {
         BoxBase* b = c->box;
-        std::vector<double> h[3];
-        download3(c, b->state, h, 2 * b->fs, b->fs * (long long)b->sbytes, true);
         double* dst[3] = {u, v, w};
-        for (int f = 0; f < 3; ++f) dev_to_ref_layout(h[f], dst[f], c->p.n);
+        download_ref3(c, b->state, dst, b->fs * (long long)b->sbytes, true);
         return 0;
     });
 }
@@ -307,10 +283,8 @@ int dnsbox_get_nonlin(dnsbox_ctx* c, double* u, double* v, double* w)
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
     return guarded([&]() -> int {
         BoxBase* b = c->box;
-        std::vector<double> h[3];
-        download3(c, b->nonlin, h, 2 * b->fs, b->fs * (long long)b->cbytes, false);
         double* dst[3] = {u, v, w};
-        for (int f = 0; f < 3; ++f) dev_to_ref_layout(h[f], dst[f], c->p.n);
+        download_ref3(c, b->nonlin, dst, b->fs * (long long)b->cbytes, false);
         return 0;
     });
 }
