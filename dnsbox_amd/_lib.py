@@ -29,7 +29,8 @@ class Params(C.Structure):
         ("cfl_max", C.c_double), ("dt_max", C.c_double), ("dt", C.c_double),
         ("n", C.c_int), ("precision", C.c_int), ("kolmo_k", C.c_int), ("bandlim", C.c_int),
         ("spherical", C.c_int), ("parity_mode", C.c_int), ("adaptive_dt", C.c_int),
-        ("state_precision", C.c_int),
+        ("state_precision", C.c_int), ("alpha_x", C.c_double), ("alpha_y", C.c_double),
+        ("alpha_z", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
     ]
 
 
@@ -78,6 +79,8 @@ SIGNATURES = {
     "dnsbox_rhs_vjp_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "dnsbox_step_vjp_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
     "dnsbox_rhs_adj_dev": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "dnsbox_adj_descent_step_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_int]),
+    "dnsbox_adj_descent_step": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "dnsbox_group_create": (C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(_vp)]),
     "dnsbox_group_destroy": (C.c_int, [_vp]),
     "dnsbox_group_size": (C.c_int, [_vp]),

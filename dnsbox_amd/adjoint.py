@@ -34,9 +34,19 @@ def hermitian_weight(n, device):
     return w
 
 
+def _require_fixed_dt(boxes):
+    """The step adjoints differentiate fixed-dt steps: a box whose CFL controller would pick the
+    step size itself (adaptive_dt) cannot be stepped by the T-step map."""
+    for b in boxes:
+        if b.params.adaptive_dt:
+            raise ValueError("the T-step map needs boxes created with adaptive_dt=0 (fixed dt): with the "
+                             "CFL controller the forward steps would not be the steps the adjoint differentiates")
+
+
 class _StepMap(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, box, T, dt, k):
+        _require_fixed_dt([box])
         box.rhs_int_fact(dt)
         u = u0.detach().clone().contiguous()
         ckpts = []
@@ -138,6 +148,7 @@ class _GroupStepMap(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, group, T, dts, k, *u0s):
+        _require_fixed_dt(group.boxes)
         for b, dt in zip(group.boxes, dts):
             b.rhs_int_fact(dt)
         us = [u.detach().clone().contiguous() for u in u0s]

@@ -49,7 +49,8 @@ def default_params(**kw) -> Params:
 
 
 class Box:
-    """One periodic box [0, 2 pi)^3 on the GPU."""
+    """One periodic box on the GPU: [0, 2 pi / alpha_x) x [0, 2 pi / alpha_y) x [0, 2 pi / alpha_z)
+    (alpha_* keyword arguments, default 1), N^3 points."""
 
     def __init__(self, n: int = 32, precision: int = 64, **params):
         self.lib = load()
@@ -329,6 +330,16 @@ class Box:
         self._on_torch_stream()
         check(self.lib.dnsbox_rhs_adj_dev(self._h, self._check_tensor(u), self._check_tensor(out),
                                           lin_sign))
+
+    def adj_descent_step_dev(self, u, dtau: float, nsteps: int = 1, lin_sign: int = 1):
+        """The adjoint-descent loop body of dnsboxAdj.f90:76-126, nsteps times, in place on a
+        device tensor: Heun in fictitious time on rhsAdj, then state_project."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_adj_descent_step_dev(self._h, self._check_tensor(u), dtau, nsteps, lin_sign))
+
+    def adj_descent_step(self, dtau: float, nsteps: int = 1, lin_sign: int = 1):
+        """adj_descent_step_dev on the box's own state; its clock advances by dtau per step."""
+        check(self.lib.dnsbox_adj_descent_step(self._h, dtau, nsteps, lin_sign))
 
 
 class BoxGroup:

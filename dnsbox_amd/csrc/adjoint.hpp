@@ -34,7 +34,8 @@ __device__ __forceinline__ double herm_w(int ix, int n) { return (ix == 0 || ix 
 // here; parity = 0 removes the Nyquist-index modes as everywhere else)
 __device__ __forceinline__ bool masked_gt(const ModeK& m, int n, const BoxPhys& ph)
 {
-    if (double(abs(m.kx)) > ph.mask_thr || double(abs(m.ky)) > ph.mask_thr || double(abs(m.kz)) > ph.mask_thr)
+    if (fabs(double(m.kx) * ph.a[0]) > ph.thr[0] || fabs(double(m.ky) * ph.a[1]) > ph.thr[1] ||
+        fabs(double(m.kz) * ph.a[2]) > ph.thr[2])
         return true;
     if (!ph.parity && (m.ix == n / 2 || m.iy == n / 2 || m.iz == n / 2)) return true;
     return false;
@@ -52,10 +53,10 @@ struct ProVjpIn {  // (lambda, u) -> (lam_scale (-P M lambda) / W, s u, s i k x 
     BoxPhys ph;
     __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
     {
-        const T kx = T(m.kx), ky = T(m.ky), kz = T(m.kz);
+        const T kx = T(double(m.kx) * ph.a[0]), ky = T(double(m.ky) * ph.a[1]), kz = T(double(m.kz) * ph.a[2]);
         if (o < 3) {
             if (masked(m.ix, m.iy, m.iz, m.kx, m.ky, m.kz, n, ph)) return cpx<T>{T(0), T(0)};
-            const T inv = T(1) / T(m.kx * m.kx + m.ky * m.ky + m.kz * m.kz);
+            const T inv = T(1) / T(kk2(m.kx, m.ky, m.kz, ph));
             const cpx<T> kl = inv * (kx * in[0] + ky * in[1] + kz * in[2]);
             const T k = o == 0 ? kx : (o == 1 ? ky : kz);
             const T f = lam_scale / T(herm_w(m.ix, n));
@@ -77,12 +78,12 @@ template <class T>
 struct ProGrad12 {  // u -> (s u_i, s i k_j u_i):  outputs 3 + 3 i + j = d_j u_i
     static constexpr int FI = 3, FO = 12;
     T s;
+    KScale<T> k;
     __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
     {
         if (o < 3) return s * in[o];
         const int i = (o - 3) / 3, j = (o - 3) % 3;
-        const T k = j == 0 ? T(m.kx) : (j == 1 ? T(m.ky) : T(m.kz));
-        return s * mul_i<1>(k * in[i]);
+        return s * mul_i<1>(k.d(m, j) * in[i]);
     }
 };
 
@@ -90,12 +91,12 @@ template <class T>
 struct ProAdj12 {  // (u~, u) -> (s u_i, s i k_j u~_i)
     static constexpr int FI = 6, FO = 12;
     T s;
+    KScale<T> k;
     __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
     {
         if (o < 3) return s * in[3 + o];
         const int i = (o - 3) / 3, j = (o - 3) % 3;
-        const T k = j == 0 ? T(m.kx) : (j == 1 ? T(m.ky) : T(m.kz));
-        return s * mul_i<1>(k * in[i]);
+        return s * mul_i<1>(k.d(m, j) * in[i]);
     }
 };
 
@@ -202,6 +203,7 @@ __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long
     const int ix = col % H, iy = col / H;
     const int kx = kval(ix, N), ky = kval(iy, N);
     const BoxPhys& ph = va.ph;
+    const T kxs = T(double(kx) * ph.a[0]), kys = T(double(ky) * ph.a[1]);
     const T s = T(1.0 / (double(N) * N * N));
     const T w = T(herm_w(ix, N));
     double cband = 0.0, fband = 0.0;
@@ -213,16 +215,17 @@ __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long
     for (int q = 0; q < RM; ++q) {
         const int iz = CF::out_pos(g, q);
         const int kz = kval(iz, N);
+        const T kzs = T(double(kz) * ph.a[2]);
+        const double k2d = kk2(kx, ky, kz, ph);
         const long long idx = (long long)iz * ZS + col;
         // g = s W (Ubar + i k x Wbar)
         const cpx<T> a0 = O[3][q], a1 = O[4][q], a2 = O[5][q];
         cpx<T> gg[3];
-        gg[0] = (s * w) * (O[0][q] + mul_i<1>(T(ky) * a2 - T(kz) * a1));
-        gg[1] = (s * w) * (O[1][q] + mul_i<1>(T(kz) * a0 - T(kx) * a2));
-        gg[2] = (s * w) * (O[2][q] + mul_i<1>(T(kx) * a1 - T(ky) * a0));
+        gg[0] = (s * w) * (O[0][q] + mul_i<1>(kys * a2 - kzs * a1));
+        gg[1] = (s * w) * (O[1][q] + mul_i<1>(kzs * a0 - kxs * a2));
+        gg[2] = (s * w) * (O[2][q] + mul_i<1>(kxs * a1 - kys * a0));
         if (ph.bandlim) {
-            const int k2 = kx * kx + ky * ky + kz * kz;
-            if (sqrt(double(k2)) <= ph.kcut) {
+            if (sqrt(k2d) <= ph.kcut) {
                 const bool mk = masked(ix, iy, iz, kx, ky, kz, N, ph);
                 const T we = T(kx == 0 ? 0.5 : 1.0);
 #pragma unroll
@@ -237,7 +240,6 @@ __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long
 #pragma unroll
             for (int f = 0; f < 3; ++f) va.out[f * va.fs + idx] = gg[f];
         } else if constexpr (MODE == VJP_S1) {
-            const double k2d = double(kx * kx + ky * ky + kz * kz);
             const T e = T(exp((-ph.nu * k2d + ph.Q) * va.dt));
             const T hdt = T(0.5 * va.dt), tdt = T(va.dt);
 #pragma unroll
@@ -271,18 +273,17 @@ __global__ void __launch_bounds__(256)
                        BoxPhys ph, double lam_scale, double sm2, double* out)
 {
     constexpr int H = N / 2 + 1;
-    int K = int(ph.kcut);
-    if (K > N / 2 - 1) K = N / 2 - 1;
-    const int nxc = K + 2, nyc = 2 * K + 2;
-    const int tot = nxc * nyc * nyc;
+    const int KX = band_k<N>(ph.kcut, ph.a[0]), KY = band_k<N>(ph.kcut, ph.a[1]), KZ = band_k<N>(ph.kcut, ph.a[2]);
+    const int nxc = KX + 2, nyc = 2 * KY + 2, nzc = 2 * KZ + 2;
+    const int tot = nxc * nyc * nzc;
     double se = 0.0, sl = 0.0;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int a = e / (nyc * nyc), b = (e / nyc) % nyc, c = e % nyc;
-        const int ix = a <= K ? a : N / 2;
-        const int iy = b <= K ? b : (b < 2 * K + 1 ? N - (b - K) : N / 2);
-        const int iz = c <= K ? c : (c < 2 * K + 1 ? N - (c - K) : N / 2);
+        const int a = e / (nyc * nzc), b = (e / nzc) % nyc, c = e % nzc;
+        const int ix = a <= KX ? a : N / 2;
+        const int iy = band_idx<N>(b, KY);
+        const int iz = band_idx<N>(c, KZ);
         const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
-        if (sqrt(double(kx * kx + ky * ky + kz * kz)) > ph.kcut) continue;
+        if (sqrt(kk2(kx, ky, kz, ph)) > ph.kcut) continue;
         const long long idx = ((long long)iz * N + iy) * H + ix;
         const bool mk = masked(ix, iy, iz, kx, ky, kz, N, ph);
         double en = 0.0, lu = 0.0;
@@ -352,21 +353,22 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
         const int kz = kval(iz, N);
         const long long idx = (long long)iz * ZS + col;
         const ModeK m{kx, ky, kz, ix, iy, iz};
-        const int k2 = kx * kx + ky * ky + kz * kz;
+        const double k2 = kk2(kx, ky, kz, ph);
+        const T kxs = T(double(kx) * ph.a[0]), kys = T(double(ky) * ph.a[1]), kzs = T(double(kz) * ph.a[2]);
         cpx<T> nl[3];
         if (masked_gt(m, N, ph)) {
 #pragma unroll
             for (int f = 0; f < 3; ++f) nl[f] = cpx<T>{T(0), T(0)};
         } else {
             cpx<T> pk = {T(0), T(0)};
-            if (k2 > 0) pk = (T(1) / T(k2)) * (T(kx) * O[0][q] + T(ky) * O[1][q] + T(kz) * O[2][q]);
-            nl[0] = T(kx) * pk - O[0][q];
-            nl[1] = T(ky) * pk - O[1][q];
-            nl[2] = T(kz) * pk - O[2][q];
+            if (k2 > 0) pk = (T(1) / T(k2)) * (kxs * O[0][q] + kys * O[1][q] + kzs * O[2][q]);
+            nl[0] = kxs * pk - O[0][q];
+            nl[1] = kys * pk - O[1][q];
+            nl[2] = kzs * pk - O[2][q];
         }
-        const double L = -ph.nu * double(k2) + ph.Q;
+        const double L = -ph.nu * k2 + ph.Q;
         if constexpr (STAGE == 1) {
-            const T hf = T(1.0 / (1.0 + double(k2)));
+            const T hf = T(1.0 / (1.0 + k2));
 #pragma unroll
             for (int f = 0; f < 3; ++f)
                 aa.ut[f * aa.fs + idx] = hf * (T(L) * aa.u[f * aa.fs + idx] + nl[f]);

@@ -60,17 +60,21 @@ struct BoxScalars {
     double t;           // simulated time
     double eband;       // band energy of the RHS input (rhs.f90:137-168)
     double courant;     // Courant number of the last controller evaluation
-    double pad0;
+    double h_prev;      // size of the last step taken, t_n - t_{n-1} (IF-AB2 history spacing)
     unsigned long long maxu[3];   // max |u_i| bits (double, >= 0) from the cross-product pass
     unsigned long long divmax;    // ordered bits of the signed max of div u
     long long nstep;
     double pad1[5];
 };
 
+// Box lengths: the wavenumber of index i along axis d is alpha_d kval(i) (variables.f90:226-262,
+// alpha_d = 2 pi / L_d); kernels take the integer kval and scale it by a[d] in FP64.
 struct BoxPhys {
     double nu, Q, Pin, kcut, kolmo_amp;
-    double mask_thr;    // (N/2)(2/3): cubic 2/3 rule threshold  (rhs.f90:80-85)
-    double sph_thr2;    // ((N/2) 0.5)^2: spherical threshold     (rhs.f90:75-78)
+    double thr[3];      // (N/2)(2 alpha_d/3): cubic 2/3 rule thresholds      (rhs.f90:80-85)
+    double sph_thr2;    // ((N/2) 0.5 alpha_x)^2: spherical rule of the RHS   (rhs.f90:75-78)
+    double sph_dthr2;   // ((N/2)(2 alpha_x/3))^2: spherical rule of the dealias (state.f90:186-188)
+    double a[3];        // alpha_x, alpha_y, alpha_z (their single-precision values, parameters.f90:18-20)
     double cmin, cmax, dtmax;
     int kolmo_k, bandlim, spherical, parity, adaptive, pad;
 };
@@ -80,15 +84,21 @@ __host__ __device__ __forceinline__ constexpr int kval(int i, int n)
     return i < n / 2 ? i : (i == n / 2 ? 0 : i - n);
 }
 
+// |k|^2 of the table wavenumbers (kx, ky, kz) scaled by the box's alphas, in FP64
+__device__ __forceinline__ double kk2(int kx, int ky, int kz, const BoxPhys& ph)
+{
+    const double x = double(kx) * ph.a[0], y = double(ky) * ph.a[1], z = double(kz) * ph.a[2];
+    return x * x + y * y + z * z;
+}
+
 // rhs.f90:74-88: cubic ".ge." 2/3 rule on the table wavenumbers (Nyquist k = 0),
 // spherical option, k = 0; parity = 0 also removes the Nyquist-index modes.
 __device__ __forceinline__ bool masked(int ix, int iy, int iz, int kx, int ky, int kz, int n,
                                        const BoxPhys& ph)
 {
-    const double k2 = double(kx * kx + ky * ky + kz * kz);
-    if (ph.spherical && k2 >= ph.sph_thr2) return true;
-    if (double(abs(kx)) >= ph.mask_thr || double(abs(ky)) >= ph.mask_thr ||
-        double(abs(kz)) >= ph.mask_thr)
+    if (ph.spherical && kk2(kx, ky, kz, ph) >= ph.sph_thr2) return true;
+    if (fabs(double(kx) * ph.a[0]) >= ph.thr[0] || fabs(double(ky) * ph.a[1]) >= ph.thr[1] ||
+        fabs(double(kz) * ph.a[2]) >= ph.thr[2])
         return true;
     if (kx == 0 && ky == 0 && kz == 0) return true;
     if (!ph.parity && (ix == n / 2 || iy == n / 2 || iz == n / 2)) return true;
@@ -281,21 +291,33 @@ struct ModeK {
 };
 
 // Prologue functors: FI input fields -> FO output fields, mode by mode.
+// the scaled wavenumbers of a mode in the transform type
+template <class T>
+struct KScale {
+    double a[3];
+    __device__ __forceinline__ T x(const ModeK& m) const { return T(double(m.kx) * a[0]); }
+    __device__ __forceinline__ T y(const ModeK& m) const { return T(double(m.ky) * a[1]); }
+    __device__ __forceinline__ T z(const ModeK& m) const { return T(double(m.kz) * a[2]); }
+    __device__ __forceinline__ T d(const ModeK& m, int j) const { return j == 0 ? x(m) : (j == 1 ? y(m) : z(m)); }
+    __host__ __device__ static KScale of(const BoxPhys& ph) { return KScale{{ph.a[0], ph.a[1], ph.a[2]}}; }
+};
+
 template <class T>
 struct ProCurl {  // (u, v, w) -> s (u, v, w, i k x u)      state_derivatives + state_vorticity
     static constexpr int FI = 3, FO = 6;
     T s;
+    KScale<T> k;
     __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
     {
-        const int kx = m.kx, ky = m.ky, kz = m.kz;
+        const T kx = k.x(m), ky = k.y(m), kz = k.z(m);
         const cpx<T> u = in[0], v = in[1], w = in[2];
         switch (o) {
             case 0: return s * u;
             case 1: return s * v;
             case 2: return s * w;
-            case 3: return s * mul_i<1>(T(ky) * w - T(kz) * v);  // wy - vz
-            case 4: return s * mul_i<1>(T(kz) * u - T(kx) * w);  // uz - wx
-            default: return s * mul_i<1>(T(kx) * v - T(ky) * u); // vx - uy
+            case 3: return s * mul_i<1>(ky * w - kz * v);  // wy - vz
+            case 4: return s * mul_i<1>(kz * u - kx * w);  // uz - wx
+            default: return s * mul_i<1>(kx * v - ky * u); // vx - uy
         }
     }
 };
@@ -303,11 +325,11 @@ template <class T>
 struct ProVelDiv {  // (u, v, w) -> s (u, v, w, i k . u)       io_Courant + state_divergence
     static constexpr int FI = 3, FO = 4;
     T s;
+    KScale<T> k;
     __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
     {
-        const int kx = m.kx, ky = m.ky, kz = m.kz;
         if (o < 3) return s * in[o];
-        return s * mul_i<1>(T(kx) * in[0] + T(ky) * in[1] + T(kz) * in[2]);
+        return s * mul_i<1>(k.x(m) * in[0] + k.y(m) * in[1] + k.z(m) * in[2]);
     }
 };
 template <class T>
@@ -885,17 +907,22 @@ __device__ __forceinline__ double cfl_dt(double c, double dt, const BoxPhys& ph)
     return dt;
 }
 
+// io.f90:664-676: sum_d max|u_d| dt / (L_d / N), L_d = 2 pi / alpha_d, from the maxima of the
+// current RHS's x pass
+template <int N>
+__device__ __forceinline__ double courant_of(const BoxScalars* sc, const BoxPhys& ph, double dt)
+{
+    const double dx = 6.283185307179586476925286766559 / double(N);
+    return (__longlong_as_double((long long)sc->maxu[0]) * ph.a[0] +
+            __longlong_as_double((long long)sc->maxu[1]) * ph.a[1] +
+            __longlong_as_double((long long)sc->maxu[2]) * ph.a[2]) * dt / dx;
+}
+
 template <int N>
 __device__ __forceinline__ double step_dt(BoxScalars* sc, const BoxPhys& ph)
 {
     double dt = sc->dt_prev;
-    if (ph.adaptive) {
-        const double dx = 6.283185307179586476925286766559 / double(N);
-        const double c = (__longlong_as_double((long long)sc->maxu[0]) +
-                          __longlong_as_double((long long)sc->maxu[1]) +
-                          __longlong_as_double((long long)sc->maxu[2])) * dt / dx;
-        dt = cfl_dt(c, dt, ph);
-    }
+    if (ph.adaptive) dt = cfl_dt(courant_of<N>(sc, ph, dt), dt, ph);
     return dt;
 }
 
@@ -940,9 +967,11 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
     const int ix = col % H, iy = col / H;
     const int kx = kval(ix, N), ky = kval(iy, N);
     const BoxPhys& ph = ea.ph;
-    double dt = 0.0;
-    if constexpr (MODE == EPI_PRED) dt = step_dt<N>(ea.sc, ph);
-    if constexpr (MODE == EPI_CORR || MODE == EPI_AB2) dt = ea.sc->dt;
+    const T kxs = T(double(kx) * ph.a[0]), kys = T(double(ky) * ph.a[1]);
+    double dt = 0.0, hp = 0.0;
+    if constexpr (MODE == EPI_PRED || MODE == EPI_AB2) dt = step_dt<N>(ea.sc, ph);
+    if constexpr (MODE == EPI_AB2) hp = ea.sc->h_prev;
+    if constexpr (MODE == EPI_CORR) dt = ea.sc->dt;
     if constexpr (MODE == EPI_PREDONLY) dt = ea.dt_fixed;
     const double n3 = double(N) * double(N) * double(N);
     double fband = 0.0;
@@ -952,6 +981,8 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
     for (int q = 0; q < RM; ++q) {
         const int iz = CF::out_pos(g, q);
         const int kz = kval(iz, N);
+        const T kzs = T(double(kz) * ph.a[2]);
+        const double k2d = kk2(kx, ky, kz, ph);
         const long long idx = (long long)iz * ZS + col;
         cpx<T> nl[3];
         if constexpr (MODE == EPI_STORE) {
@@ -966,14 +997,13 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
                 for (int f = 0; f < 3; ++f) nl[f] = cpx<T>{T(0), T(0)};
             } else {
                 // phat = i (k . n) / |k|^2 ;  N = -n - (i k) phat = -n + k (k . n)/|k|^2
-                const int k2 = kx * kx + ky * ky + kz * kz;
-                const T inv = T(1) / T(k2);
-                const cpx<T> kn = T(kx) * O[0][q] + T(ky) * O[1][q] + T(kz) * O[2][q];
+                const T inv = T(1) / T(k2d);
+                const cpx<T> kn = kxs * O[0][q] + kys * O[1][q] + kzs * O[2][q];
                 const cpx<T> pk = inv * kn;
-                nl[0] = T(kx) * pk - O[0][q];
-                nl[1] = T(ky) * pk - O[1][q];
-                nl[2] = T(kz) * pk - O[2][q];
-                if (ph.bandlim && sqrt(double(k2)) <= ph.kcut) {  // rhs.f90:109-118
+                nl[0] = kxs * pk - O[0][q];
+                nl[1] = kys * pk - O[1][q];
+                nl[2] = kzs * pk - O[2][q];
+                if (ph.bandlim && sqrt(k2d) <= ph.kcut) {  // rhs.f90:109-118
 #pragma unroll
                     for (int f = 0; f < 3; ++f) {
                         const cpx<T> ui = ea.rhs_in_s ? cvt<T>(static_cast<const cpx<S>*>(ea.rhs_in)[f * ea.fs + idx])
@@ -994,7 +1024,6 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
             }
         } else {
             // rhs.f90:8-16 integrating factor, exponent in FP64; the update in the state type S
-            const double k2d = double(kx * kx + ky * ky + kz * kz);
             const double e64 = exp((-ph.nu * k2d + ph.Q) * dt);
             const S e = S(e64), sdt = S(dt);
             const bool live = valid && (!PR || Keep<N>::kept(iz));
@@ -1027,14 +1056,18 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
                     if (live) ea.state[f * ea.fs + idx] = un;
                 }
             } else if constexpr (MODE == EPI_AB2) {
-                const S e2 = S(e64 * e64);
+                // variable-step IF-AB2 (dnsbox.f90:143-193 with consistent weights), h = dt of this
+                // step, h' = t_n - t_{n-1}:
+                //   u_{n+1} = E_h u_n + h (1 + h/(2h')) E_h N_n - (h^2/(2h')) E_{h+h'} N_{n-1}
+                const double r = dt / (2.0 * hp);
+                const S e2 = S(exp((-ph.nu * k2d + ph.Q) * (dt + hp)));
+                const S b0 = S(dt * (1.0 + r)) * e, b1 = S(dt * r) * e2;
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     if (!valid) continue;
                     const cpx<S> u = ea.state[f * ea.fs + idx];
                     const cpx<S> no = cvt<S>(ea.nold[f * ea.fs + idx]);
-                    ea.state[f * ea.fs + idx] =
-                        e * u + sdt * (S(1.5) * e * cvt<S>(nl[f]) - S(0.5) * e2 * no);
+                    ea.state[f * ea.fs + idx] = e * u + b0 * cvt<S>(nl[f]) - b1 * no;
                     ea.nold[f * ea.fs + idx] = nl[f];
                 }
             }
@@ -1042,16 +1075,13 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
     }
 }
 
+// the step size of the step being taken (and the Courant number it came from), written by one
+// thread of the pass whose epilogue chose it; no block of that pass reads what it writes
 template <int N>
 __device__ __forceinline__ void pred_scalars(const BoxScalars* /*unused*/, BoxScalars* sc, const BoxPhys& ph)
 {
     const double dt = step_dt<N>(sc, ph);
-    if (ph.adaptive) {
-        const double dx = 6.283185307179586476925286766559 / double(N);
-        sc->courant = (__longlong_as_double((long long)sc->maxu[0]) +
-                       __longlong_as_double((long long)sc->maxu[1]) +
-                       __longlong_as_double((long long)sc->maxu[2])) * sc->dt_prev / dx;
-    }
+    if (ph.adaptive) sc->courant = courant_of<N>(sc, ph, sc->dt_prev);
     sc->dt = dt;
 }
 
@@ -1065,7 +1095,7 @@ __device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long l
     cpx<T> O[3][G::ZRM];
     zfwd_regs<N, T>(in0, in_fs, col, valid, lds, tw, O);
     zfwd_epilogue<N, T, MODE, false, false, S>(O, col, valid, ea, O);
-    if constexpr (MODE == EPI_PRED) {
+    if constexpr (MODE == EPI_PRED || MODE == EPI_AB2) {
         if (tile == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
     }
 }
@@ -1127,11 +1157,12 @@ __device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long
     }
 }
 
-// end of step bookkeeping: t += dt, dt_prev = dt, clear the CFL maxima
+// end of step bookkeeping: t += dt, dt_prev = h_prev = dt, clear the CFL maxima
 __device__ __forceinline__ void step_end(BoxScalars* sc)
 {
     sc->t += sc->dt;
     sc->dt_prev = sc->dt;
+    sc->h_prev = sc->dt;
     sc->maxu[0] = sc->maxu[1] = sc->maxu[2] = 0ull;
     sc->nstep += 1;
 }
@@ -1166,59 +1197,100 @@ __global__ void k_step_end(BoxScalars* sc)
 // ===========================================================================
 enum { DG_EKIN = 0, DG_DISP, DG_EZERO, DG_EONE, DG_ESQRT2, DG_EBAND, DG_NSCAL };
 
+// dynamic LDS of k_spec_reduce: per wave (4 per block) a sum and a compensation per shell
+inline size_t spec_reduce_lds(int nspec) { return size_t(2 * 4 * nspec) * sizeof(double); }
+
+// Shell spectrum and scalar sums of a state, independent of scheduling.  Wave w of the grid takes
+// the rows (iz, iy) = w, w + W, ... (W waves in the grid), 64 modes of a row at a time.  Along a
+// row the shell index is non-decreasing in ix < N/2 (|k| grows with kx), so the lanes of one shell
+// form one run: a segmented sum over the lanes (fixed order) gives each run's total, and the run's
+// last lane adds it to the wave's own bin with Kahan compensation; the Nyquist column (ix = N/2,
+// table kx = 0) is added in a second round.  Each block combines its waves' bins in wave order and
+// k_spec_finish sums the blocks in block order.  Scalars: per-lane Kahan sums, reduced in a fixed
+// order.  256 threads; dynamic LDS spec_reduce_lds(nspec).
 template <int N, class T>
 __global__ void __launch_bounds__(256)
     k_spec_reduce(const cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, double deltak,
                   int nspec, double* __restrict__ partial /* [gridDim.x][DG_NSCAL + nspec] */)
 {
     constexpr int H = N / 2 + 1;
-    constexpr long long TOT = (long long)N * N * H;
-    extern __shared__ double bins[];  // nspec
-    __shared__ double wsm[DG_NSCAL][4];
-    for (int i = threadIdx.x; i < nspec; i += blockDim.x) bins[i] = 0.0;
-    __syncthreads();
-    // per-thread compensated sums
+    constexpr int NW = 4;             // waves per block
+    extern __shared__ double bins[];  // [NW][2][nspec]: sums, compensations
+    __shared__ double wsm[DG_NSCAL][NW];
+    const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+    double* bs = bins + wid * 2 * nspec;
+    double* bc = bs + nspec;
+    for (int i = lane; i < nspec; i += 64) bs[i] = bc[i] = 0.0;
+    __builtin_amdgcn_wave_barrier();
     double s[DG_NSCAL], cmp[DG_NSCAL];
 #pragma unroll
     for (int j = 0; j < DG_NSCAL; ++j) s[j] = cmp[j] = 0.0;
     const double sqrt2 = 1.4142135623730951;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < TOT;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int ix = int(idx % H);
-        const int iy = int((idx / H) % N);
-        const int iz = int(idx / ((long long)H * N));
-        const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
-        double e = 0.0;
+    const int nwave = gridDim.x * NW;
+    for (int row = blockIdx.x * NW + wid; row < N * N; row += nwave) {
+        const int iy = row % N, iz = row / N;
+        const int ky = kval(iy, N), kz = kval(iz, N);
+        const cpx<T>* base = u0 + (long long)row * H;
+        for (int c0 = 0; c0 < H; c0 += 64) {
+            const int ix = c0 + lane;
+            const bool valid = ix < H;
+            int bin = -1;   // -1: no shell of the spectrum
+            double e = 0.0;
+            if (valid) {
+                const int kx = kval(ix, N);
 #pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            const cpx<T> a = u0[f * fs + idx];
-            e += double(a.x) * double(a.x) + double(a.y) * double(a.y);
-        }
-        const double k2 = double(kx * kx + ky * ky + kz * kz);
-        const double absk = sqrt(k2);
-        const double wx = (kx == 0) ? 0.5 : 1.0;
-        double v[DG_NSCAL];
-        v[DG_EKIN] = wx * e;
-        v[DG_DISP] = 2.0 * wx * k2 * e;
-        v[DG_EZERO] = absk == 0.0 ? e : 0.0;
-        v[DG_EONE] = absk == 1.0 ? e : 0.0;
-        v[DG_ESQRT2] = (absk - sqrt2 < 1.0e-9) ? e : 0.0;  // state.f90:383 as written
-        v[DG_EBAND] = (absk <= ph.kcut) ? wx * e : 0.0;
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<T> a = base[f * fs + ix];
+                    e += double(a.x) * double(a.x) + double(a.y) * double(a.y);
+                }
+                const double k2 = kk2(kx, ky, kz, ph);
+                const double absk = sqrt(k2);
+                const double wx = (kx == 0) ? 0.5 : 1.0;
+                double v[DG_NSCAL];
+                v[DG_EKIN] = wx * e;
+                v[DG_DISP] = 2.0 * wx * k2 * e;
+                v[DG_EZERO] = absk == 0.0 ? e : 0.0;
+                v[DG_EONE] = absk == 1.0 ? e : 0.0;
+                v[DG_ESQRT2] = (absk - sqrt2 < 1.0e-9) ? e : 0.0;  // state.f90:383 as written
+                v[DG_EBAND] = (absk <= ph.kcut) ? wx * e : 0.0;
 #pragma unroll
-        for (int j = 0; j < DG_NSCAL; ++j) {  // Kahan
-            const double y = v[j] - cmp[j];
-            const double t = s[j] + y;
-            cmp[j] = (t - s[j]) - y;
-            s[j] = t;
+                for (int j = 0; j < DG_NSCAL; ++j) {  // Kahan
+                    const double y = v[j] - cmp[j];
+                    const double t = s[j] + y;
+                    cmp[j] = (t - s[j]) - y;
+                    s[j] = t;
+                }
+                // state.f90:360-365: nearest shell, the single-precision real(int(.)) test as written
+                const int iq = int(absk / deltak);
+                const int nk = (absk < double(float(iq)) + 0.5 * deltak) ? iq : iq + 1;
+                if (nk < nspec && nk != 0) bin = nk - 1;
+            }
+#pragma unroll
+            for (int round = 0; round < 2; ++round) {
+                const bool mine = valid && ((ix == N / 2) == (round == 1));
+                // run key: the bin for this round's lanes, a value of its own for every other lane
+                const int key = mine ? bin : -2 - lane;
+                double v = mine ? e : 0.0;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {   // segmented inclusive sum over equal keys
+                    const double vu = __shfl_up(v, d, 64);
+                    const int ku = __shfl_up(key, d, 64);
+                    if (lane >= d && ku == key) v += vu;
+                }
+                const int kn = __shfl_down(key, 1, 64);
+                if (mine && bin >= 0 && (lane == 63 || kn != key)) {   // the last lane of a run
+                    const double y = v - bc[bin];
+                    const double t = bs[bin] + y;
+                    bc[bin] = (t - bs[bin]) - y;
+                    bs[bin] = t;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
         }
-        const int iq = int(absk / deltak);
-        const int nk = (absk < double(float(iq)) + 0.5 * deltak) ? iq : iq + 1;
-        if (nk < nspec && nk != 0) atomicAdd(&bins[nk - 1], e);
     }
-    const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
 #pragma unroll
     for (int j = 0; j < DG_NSCAL; ++j) {
-        const double v = wave_sum(s[j]);
+        const double v = wave_sum(s[j] - cmp[j]);
         if (lane == 0) wsm[j][wid] = v;
     }
     __syncthreads();
@@ -1227,7 +1299,22 @@ __global__ void __launch_bounds__(256)
         const int j = threadIdx.x;
         out[j] = ((wsm[j][0] + wsm[j][1]) + wsm[j][2]) + wsm[j][3];
     }
-    for (int i = threadIdx.x; i < nspec; i += blockDim.x) out[DG_NSCAL + i] = bins[i];
+    for (int i = threadIdx.x; i < nspec; i += blockDim.x) {
+        double acc = 0.0, c = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const double* wb = bins + w * 2 * nspec;
+            const double terms[2] = {wb[i], -wb[nspec + i]};
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const double y = terms[q] - c;
+                const double t = acc + y;
+                c = (t - acc) - y;
+                acc = t;
+            }
+        }
+        out[DG_NSCAL + i] = acc;
+    }
 }
 
 // sums the per-block partials in a fixed order (Kahan), scales, writes [DG_NSCAL + nspec]
@@ -1257,24 +1344,38 @@ __global__ void k_spec_finish(const double* __restrict__ partial, int nblk, int 
 // table wavenumber decides membership, so with the reference k-table the
 // Nyquist index (k = 0) counts as low: per axis the candidate indices are
 // |k| <= K plus the Nyquist index.
+// candidate indices of the band |k| <= kcut along one axis: |kval| <= K (and the Nyquist index)
+template <int N>
+__device__ __forceinline__ int band_k(double kcut, double a)
+{
+    int K = int(kcut / a) + 1;   // |kval| a <= kcut needs |kval| <= kcut / a; one more for rounding
+    return K > N / 2 - 1 ? N / 2 - 1 : K;
+}
+template <int N>
+__device__ __forceinline__ int band_idx(int b, int K)
+{
+    return b <= K ? b : (b < 2 * K + 1 ? N - (b - K) : N / 2);
+}
+
 template <int N, class T>
 __global__ void __launch_bounds__(256)
-    k_eband(const cpx<T>* __restrict__ u0, long long fs, double kcut, double sm2, BoxScalars* sc)
+    k_eband(const cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, double sm2, BoxScalars* sc)
 {
     constexpr int H = N / 2 + 1;
-    int K = int(kcut);
-    if (K > N / 2 - 1) K = N / 2 - 1;
-    const int nxc = K + 2;          // kx indices 0..K, N/2
-    const int nyc = 2 * K + 2;      // ky indices 0..K, N-K..N-1, N/2
-    const int tot = nxc * nyc * nyc;
+    const double kcut = ph.kcut;
+    const int KX = band_k<N>(kcut, ph.a[0]), KY = band_k<N>(kcut, ph.a[1]), KZ = band_k<N>(kcut, ph.a[2]);
+    const int nxc = KX + 2;          // kx indices 0..KX, N/2
+    const int nyc = 2 * KY + 2;      // ky indices 0..KY, N-KY..N-1, N/2
+    const int nzc = 2 * KZ + 2;
+    const int tot = nxc * nyc * nzc;
     double s = 0.0;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int a = e / (nyc * nyc), b = (e / nyc) % nyc, c = e % nyc;
-        const int ix = a <= K ? a : N / 2;
-        const int iy = b <= K ? b : (b < 2 * K + 1 ? N - (b - K) : N / 2);
-        const int iz = c <= K ? c : (c < 2 * K + 1 ? N - (c - K) : N / 2);
+        const int a = e / (nyc * nzc), b = (e / nzc) % nyc, c = e % nzc;
+        const int ix = a <= KX ? a : N / 2;
+        const int iy = band_idx<N>(b, KY);
+        const int iz = band_idx<N>(c, KZ);
         const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
-        const double absk = sqrt(double(kx * kx + ky * ky + kz * kz));
+        const double absk = sqrt(kk2(kx, ky, kz, ph));
         if (absk > kcut) continue;
         const long long idx = ((long long)iz * N + iy) * H + ix;
         double en = 0.0;
@@ -1348,10 +1449,10 @@ __global__ void __launch_bounds__(256)
             ar[c] = cos(phs);
             ai[c] = sin(phs);
         }
-        const double k2 = double(kx) * kx + double(ky) * ky + double(kz) * kz;
-        const double kdr = (kx * ar[0] + ky * ar[1] + kz * ar[2]) / k2;
-        const double kdi = (kx * ai[0] + ky * ai[1] + kz * ai[2]) / k2;
-        const double kk[3] = {double(kx), double(ky), double(kz)};
+        const double kk[3] = {double(kx) * ph.a[0], double(ky) * ph.a[1], double(kz) * ph.a[2]};
+        const double k2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+        const double kdr = (kk[0] * ar[0] + kk[1] * ar[1] + kk[2] * ar[2]) / k2;
+        const double kdi = (kk[0] * ai[0] + kk[1] * ai[1] + kk[2] * ai[2]) / k2;
         double na2 = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -1436,25 +1537,54 @@ __global__ void __launch_bounds__(256)
     const int iz = int(idx / ((long long)H * N));
     const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
     cpx<T> u = u0[idx], v = u0[fs + idx], w = u0[2 * fs + idx];
+    const double rx = double(kx) * ph.a[0], ry = double(ky) * ph.a[1], rz = double(kz) * ph.a[2];
+    const double k2 = rx * rx + ry * ry + rz * rz;
     if (do_project && !(kx == 0 && ky == 0 && kz == 0)) {
-        const T inv = T(1) / T(kx * kx + ky * ky + kz * kz);
-        const cpx<T> kd = inv * (T(kx) * u + T(ky) * v + T(kz) * w);
-        u = u - T(kx) * kd;
-        v = v - T(ky) * kd;
-        w = w - T(kz) * kd;
+        const T inv = T(1) / T(k2);
+        const cpx<T> kd = inv * (T(rx) * u + T(ry) * v + T(rz) * w);
+        u = u - T(rx) * kd;
+        v = v - T(ry) * kd;
+        w = w - T(rz) * kd;
     }
     if (do_dealias) {
-        // state.f90:184-197: spherical uses (N/3)^2 here
-        const double thr = ph.mask_thr;
-        const double k2 = double(kx * kx + ky * ky + kz * kz);
-        bool m = (ph.spherical && k2 >= thr * thr) || double(abs(kx)) >= thr ||
-                 double(abs(ky)) >= thr || double(abs(kz)) >= thr || (kx == 0 && ky == 0 && kz == 0) ||
+        // state.f90:184-197: spherical uses ((N/2)(2 alpha_x/3))^2 here
+        bool m = (ph.spherical && k2 >= ph.sph_dthr2) || fabs(rx) >= ph.thr[0] || fabs(ry) >= ph.thr[1] ||
+                 fabs(rz) >= ph.thr[2] || (kx == 0 && ky == 0 && kz == 0) ||
                  (!ph.parity && (ix == N / 2 || iy == N / 2 || iz == N / 2));
         if (m) u = v = w = cpx<T>{T(0), T(0)};
     }
     u0[idx] = u;
     u0[fs + idx] = v;
     u0[2 * fs + idx] = w;
+}
+
+// elementwise helpers of the adjoint-descent step (n complex entries)
+// y = a + (h b) 0.5 when HALF, else y = a + h b
+template <class T, bool HALF>
+__global__ void __launch_bounds__(256)
+    k_axpy(cpx<T>* __restrict__ y, const cpx<T>* __restrict__ a, const cpx<T>* __restrict__ b, double h, long long n)
+{
+    const T hh = T(h);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        cpx<T> d = hh * b[i];
+        if (HALF) d = T(0.5) * d;
+        y[i] = a[i] + d;
+    }
+}
+
+// zero n complex entries (a kernel, not a memset: it may be captured into a graph)
+template <class T>
+__global__ void __launch_bounds__(256) k_fill_zero(cpx<T>* __restrict__ y, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = cpx<T>{T(0), T(0)};
+}
+
+// the clock of a box advanced by h (fictitious time of the adjoint descent)
+template <int D = 0>
+__global__ void k_add_time(BoxScalars* sc, double h)
+{
+    if (threadIdx.x == 0) sc->t += h;
 }
 
 template <class TO, class TI>

@@ -43,6 +43,45 @@ struct BoxConfig {
 
 constexpr int kNPass = 6;  // passes reported by time_passes (DNSBOX_PASS_* order)
 
+// twiddle table tw[j] = e^{-2 pi i j / N} in the transform type, from long-double angles
+template <int N, class T>
+void host_twiddles(cpx<T>* h)
+{
+    for (int j = 0; j < N; ++j) {
+        long double a = 6.283185307179586476925286766559005768L * (long double)j / (long double)N;
+        h[j].x = T(std::cos(a));
+        h[j].y = T(-std::sin(a));
+    }
+}
+
+// rhs_nonlinear (rhs.f90:18-135) on device buffers of the transform type: out <- N(in), with the
+// transform scratch X (6 fields), the twiddles tw and the scalars sc (band energy) given.  The
+// five passes of one RHS: z-inverse + curl, y-inverse (6), x cross product, y-forward (3),
+// z-forward with the mask / projection / forcing epilogue.
+template <int N, class T>
+void launch_rhs(hipStream_t s, const BoxPhys& ph, const cpx<T>* in, cpx<T>* out, cpx<T>* X, const cpx<T>* tw,
+                BoxScalars* sc)
+{
+    using G = Geo<N, T>;
+    constexpr long long fs = G::FS;
+    const double sm = 1.0 / (double(N) * N * N);
+    if (ph.bandlim) hipLaunchKernelGGL((k_eband<N, T>), dim3(1), dim3(256), 0, s, in, fs, ph, sm * sm, sc);
+    hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>, T, false>), dim3(G::ZTILES), dim3(G::ZNT), 0, s, in, fs, X, fs, tw,
+                       ProCurl<T>{T(sm), KScale<T>::of(ph)});
+    hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(G::YNT), 0, s, X, fs, tw);
+    hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, s, X, fs, X, fs, tw,
+                       (unsigned long long*)nullptr, 0);
+    hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(G::YNT), 0, s, X, fs, tw);
+    EpiArgs<T, T> ea{};
+    ea.out = out;
+    ea.rhs_in = in;
+    ea.rhs_in_s = 0;
+    ea.fs = fs;
+    ea.sc = sc;
+    ea.ph = ph;
+    hipLaunchKernelGGL((k_zfwd<N, T, EPI_RHS, T>), dim3(G::ZTILES), dim3(G::ZNT), 0, s, X, fs, tw, ea);
+}
+
 // Per-box device buffers and the operations the C-ABI exposes.
 class BoxBase {
    public:
@@ -76,7 +115,11 @@ class BoxBase {
     // state, after init_random, state_dealias and every step; set_state checks the field
     bool dealiased = true;
     bool prune = true;       // run the step's passes pruned to the kept cube when dealiased
-    bool pruned() const { return prune && dealiased; }
+    // the mask removes every mode outside the kept cube (false only for box lengths whose 2/3-rule
+    // threshold falls at an index of the kept range's edge); pruning needs it
+    bool prunable = true;
+    bool may_prune() const { return prune && prunable; }
+    bool pruned() const { return may_prune() && dealiased; }
     void* pred_out_ = nullptr;   // where the IF-Heun step stores u* (null: internal buffer)
     // the caller guarantees that the states the step / RHS adjoints linearise about have no
     // energy outside the kept cube (dnsbox_set_vjp_cube); their first passes then read the cube only
@@ -84,6 +127,7 @@ class BoxBase {
     bool last_ext_pruned = false;   // the last step of a caller's buffer ran the pruned passes
     int scratch_fields = 6;
     void* adjbuf = nullptr;  // 9 fields: A, Mu, u~ (adjoint passes), lazily
+    void* descbuf = nullptr; // 9 fields: G(u), u*, G(u*) of the adjoint-descent step, lazily
     double* band2 = nullptr; // band-forcing scalars of the VJP
     unsigned int* chk = nullptr;   // flag of k_outside_cube
 
@@ -109,6 +153,9 @@ class BoxBase {
     virtual void rhs_vjp(const void* u, const void* lam, void* out) = 0;
     virtual void step_vjp(const void* u, const void* lam, void* lam_out, double dt) = 0;
     virtual void rhs_adj(const void* u, void* out, double lin_sign) = 0;
+    // the adjoint-descent loop of dnsboxAdj.f90:76-126 on a device state (null: the box's state):
+    // nsteps Heun steps of size dtau in fictitious time on rhsAdj, each followed by state_project
+    virtual void adj_descent(void* u, double dtau, int nsteps, double lin_sign) = 0;
     virtual void step_ext(void* st, int nsteps) = 0;
     // step_ext that also stores the state after step j + 1 in traj + j * (3 state fields) and,
     // when traj_pred is set, the predicted state u* of step j + 1 in traj_pred + j * 3 fields
@@ -149,10 +196,17 @@ class BoxT final : public BoxBase {
         sbytes = sizeof(CS);
         rbytes = sizeof(T);
         uniform = UNIFORM;
-        const double ax = (double(N) / 2.0) * (2.0 * 1.0 / 3.0);
-        if (cfg.ph.spherical) nspec = int(ax / cfg.deltak) - 1;
-        else nspec = int(std::sqrt(3.0 * ax * ax) / cfg.deltak) - 1;
+        // variables.f90:112-123: Nspec from the 2/3-rule extent of each axis
+        const double* al = cfg.ph.a;
+        const double ex = (double(N) / 2.0) * (2.0 * al[0] / 3.0), ey = (double(N) / 2.0) * (2.0 * al[1] / 3.0),
+                     ez = (double(N) / 2.0) * (2.0 * al[2] / 3.0);
+        if (cfg.ph.spherical) nspec = int(ex / cfg.deltak) - 1;
+        else nspec = int(std::sqrt(ex * ex + ey * ey + ez * ez) / cfg.deltak) - 1;
         if (nspec < 1) nspec = 1;
+        // the pruned passes skip |kval| > KMAX; that is exact when the mask removes index KMAX + 1
+        // on every axis (|kval| alpha >= (N/2)(2 alpha/3), evaluated as the kernels do)
+        for (int d = 0; d < 3; ++d)
+            if (!(std::fabs(double(Keep<N>::KMAX + 1) * al[d]) >= cfg.ph.thr[d])) prunable = false;
         const size_t fb = size_t(fs) * sizeof(C);
         try {
             DNSBOX_HIP(hipMalloc(&state, 3 * size_t(fs) * sizeof(CS)));
@@ -170,11 +224,7 @@ class BoxT final : public BoxBase {
             DNSBOX_HIP(hipMemset(pred, 0, 3 * fb));
             DNSBOX_HIP(hipMemset(nonlin, 0, 3 * fb));
             std::vector<C> h(N);
-            for (int j = 0; j < N; ++j) {
-                long double a = 6.283185307179586476925286766559005768L * (long double)j / (long double)N;
-                h[j].x = T(std::cos(a));
-                h[j].y = T(-std::sin(a));
-            }
+            host_twiddles<N, T>(h.data());
             DNSBOX_HIP(hipMemcpy(tw, h.data(), N * sizeof(C), hipMemcpyHostToDevice));
             BoxScalars s{};
             s.dt = s.dt_prev = cfg.dt;
@@ -194,10 +244,11 @@ class BoxT final : public BoxBase {
             (void)hipStreamDestroy(gstream);
             gstream = nullptr;
         }
-        void* ps[] = {state, pred, scratch, nonlin, nold, tw, sc, red_partial, red_out, red_bits, adjbuf, band2, chk};
+        void* ps[] = {state, pred, scratch, nonlin, nold, tw, sc, red_partial, red_out, red_bits, adjbuf, band2, chk,
+                      descbuf};
         for (void* p : ps)
             if (p) (void)hipFree(p);
-        state = pred = scratch = nonlin = nold = tw = adjbuf = nullptr;
+        state = pred = scratch = nonlin = nold = tw = adjbuf = descbuf = nullptr;
         band2 = nullptr;
         sc = nullptr;
         red_partial = red_out = nullptr;
@@ -206,6 +257,7 @@ class BoxT final : public BoxBase {
     }
 
     const C* twc() const { return static_cast<const C*>(tw); }
+    ProCurl<T> curl() const { return ProCurl<T>{inv_n3(), KScale<T>::of(cfg.ph)}; }
     C* X() const { return static_cast<C*>(scratch); }
     CS* St() const { return static_cast<CS*>(state); }
     C* Pr() const { return static_cast<C*>(pred); }
@@ -232,7 +284,7 @@ class BoxT final : public BoxBase {
     {
         if (!cfg.ph.bandlim) return;
         const double sm = 1.0 / (double(N) * N * N);
-        hipLaunchKernelGGL((k_eband<N, E>), dim3(1), dim3(256), 0, stream, u, fs, cfg.ph.kcut, sm * sm, sc);
+        hipLaunchKernelGGL((k_eband<N, E>), dim3(1), dim3(256), 0, stream, u, fs, cfg.ph, sm * sm, sc);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -240,7 +292,7 @@ class BoxT final : public BoxBase {
     void zinv_curl(const cpx<E>* in)
     {
         hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>, E, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream, in,
-                           fs, X(), fs, twc(), ProCurl<T>{inv_n3()});
+                           fs, X(), fs, twc(), curl());
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -295,17 +347,12 @@ class BoxT final : public BoxBase {
         rhs_passes<EPI_RHS>(St(), ea, false);
     }
 
-    void rhs(const void* in_, void* out) override
+    void rhs(const void* in, void* out) override
     {
         require_uniform("rhs on device buffers");
         x6_valid = false;
-        const C* in = static_cast<const C*>(in_);
-        eband(in);
-        EpiArgs<T, S> ea = epi();
-        ea.out = static_cast<C*>(out);
-        ea.rhs_in = in;
-        ea.rhs_in_s = 0;
-        rhs_passes<EPI_RHS>(in, ea, false);
+        launch_rhs<N, T>(stream, cfg.ph, static_cast<const C*>(in), static_cast<C*>(out), X(), twc(), sc);
+        DNSBOX_LAUNCH_CHECK();
     }
 
     // IF-Heun step (dnsboxheun.f90:84-126) as a pipeline: the forward z pass of each
@@ -317,10 +364,24 @@ class BoxT final : public BoxBase {
         else heun_step_t<false>();
     }
 
+    // the internal predicted-state buffer holds no value outside the kept cube (the pruned
+    // predictor writes only the kept modes, and the band energy of u* reads the band's modes)
+    bool pred_clean = true;
+    void clear_pred()
+    {
+        hipLaunchKernelGGL((k_fill_zero<T>), dim3(1024), dim3(256), 0, stream, Pr(), 3 * fs);
+        DNSBOX_LAUNCH_CHECK();
+        pred_clean = true;
+    }
+
     template <bool PR>
     void heun_step_t()
     {
-        const ProCurl<T> pro{inv_n3()};
+        const ProCurl<T> pro = curl();
+        if (cfg.ph.bandlim && !pred_out_) {
+            if (PR && !pred_clean) clear_pred();
+            if (!PR) pred_clean = false;
+        }
         if (!x6_valid || x6_pruned != PR) {
             eband(St());
             zinv_curl<S, PR>(St());
@@ -364,6 +425,7 @@ class BoxT final : public BoxBase {
     hipStream_t gstream = nullptr;
     hipGraphExec_t step_graph = nullptr;
     const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool graph_pr = false;  // the captured steps run the pruned passes
     int graph_enabled = -1;
     int graph_steps = 16;   // steps per captured graph (one hipGraphLaunch replays them all)
 
@@ -381,7 +443,7 @@ class BoxT final : public BoxBase {
             const char* k = std::getenv("DNSBOX_GRAPH_STEPS");
             if (k && std::atoi(k) > 0) graph_steps = std::atoi(k);
         }
-        if (nsteps > 0 && !x6_valid) {
+        if (nsteps > 0 && (!x6_valid || x6_pruned != pruned())) {
             heun_step();
             --nsteps;
         }
@@ -391,7 +453,7 @@ class BoxT final : public BoxBase {
             return;
         }
         const void* key[4] = {state, scratch, pred, sc};
-        if (step_graph && std::memcmp(key, graph_key, sizeof(key)) != 0) drop_graph();
+        if (step_graph && (std::memcmp(key, graph_key, sizeof(key)) != 0 || graph_pr != pruned())) drop_graph();
         if (!gstream) DNSBOX_HIP(hipStreamCreate(&gstream));
         hipEvent_t ev;
         DNSBOX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -418,6 +480,7 @@ class BoxT final : public BoxBase {
             (void)hipGraphDestroy(g);
             DNSBOX_HIP(ie);
             std::memcpy(graph_key, key, sizeof(key));
+            graph_pr = pruned();
         }
         for (int i = 0; i + graph_steps <= nsteps; i += graph_steps) DNSBOX_HIP(hipGraphLaunch(step_graph, gstream));
         DNSBOX_HIP(hipEventRecord(ev, gstream));
@@ -439,6 +502,7 @@ class BoxT final : public BoxBase {
             ea.rhs_in = St();
             ea.rhs_in_s = 1;
             ea.nold = static_cast<C*>(nold);
+            pred_clean = false;
             rhs_passes<EPI_PRED>(St(), ea, true);
             eband(Pr());
             EpiArgs<T, S> eb = epi();
@@ -464,7 +528,7 @@ class BoxT final : public BoxBase {
     {
         const double sm = 1.0 / (double(N) * N * N);
         const int nslot = DG_NSCAL + nspec;
-        hipLaunchKernelGGL((k_spec_reduce<N, S>), dim3(red_blocks), dim3(256), nspec * sizeof(double), stream,
+        hipLaunchKernelGGL((k_spec_reduce<N, S>), dim3(red_blocks), dim3(256), spec_reduce_lds(nspec), stream,
                            St(), fs, cfg.ph, cfg.deltak, nspec, red_partial);
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_spec_finish<0>, dim3(1), dim3(256), 0, stream, red_partial, red_blocks, nslot,
@@ -477,7 +541,7 @@ class BoxT final : public BoxBase {
         x6_valid = false;
         DNSBOX_HIP(hipMemsetAsync(red_bits, 0, 4 * sizeof(unsigned long long), stream));
         hipLaunchKernelGGL((k_zinv<N, T, ProVelDiv<T>, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, St(), fs, X(),
-                           fs, twc(), ProVelDiv<T>{inv_n3()});
+                           fs, twc(), ProVelDiv<T>{inv_n3(), KScale<T>::of(cfg.ph)});
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 4), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
@@ -502,7 +566,7 @@ class BoxT final : public BoxBase {
     void project(int dealias, int proj) override
     {
         x6_valid = false;
-        if (dealias) dealiased = true;   // state_dealias zeroes every mode outside the kept cube
+        if (dealias) dealiased = prunable;   // state_dealias zeroes every mode outside the kept cube
         hipLaunchKernelGGL((k_project_dealias<N, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
                            cfg.ph, dealias, proj);
         DNSBOX_LAUNCH_CHECK();
@@ -573,7 +637,7 @@ class BoxT final : public BoxBase {
     template <int MODE>
     void vjp_passes(const C* u, const C* lam, double lam_scale, VjpArgs<T> va)
     {
-        if (prune && vjp_cube) vjp_passes_t<MODE, true>(u, lam, lam_scale, va);
+        if (may_prune() && vjp_cube) vjp_passes_t<MODE, true>(u, lam, lam_scale, va);
         else vjp_passes_t<MODE, false>(u, lam, lam_scale, va);
     }
     template <int MODE, bool PR>
@@ -637,6 +701,7 @@ class BoxT final : public BoxBase {
         ea.rhs_in = u;
         ea.rhs_in_s = 0;
         ea.dt_fixed = dt;
+        pred_clean = false;
         hipLaunchKernelGGL((k_zfwd<N, T, EPI_PREDONLY, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
                            ea);
         DNSBOX_LAUNCH_CHECK();
@@ -689,7 +754,7 @@ class BoxT final : public BoxBase {
         aa.ph = cfg.ph;
         Ptrs<T, 3> in1{{u, u + fs, u + 2 * fs}};
         hipLaunchKernelGGL((k_zinv_p<N, T, ProGrad12<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in1, X(), fs,
-                           twc(), ProGrad12<T>{s});
+                           twc(), ProGrad12<T>{s, KScale<T>::of(cfg.ph)});
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 12), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
@@ -703,7 +768,7 @@ class BoxT final : public BoxBase {
         const C* ut = adjUt();
         Ptrs<T, 6> in2{{ut, ut + fs, ut + 2 * fs, u, u + fs, u + 2 * fs}};
         hipLaunchKernelGGL((k_zinv_p<N, T, ProAdj12<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in2, X(), fs,
-                           twc(), ProAdj12<T>{s});
+                           twc(), ProAdj12<T>{s, KScale<T>::of(cfg.ph)});
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 12), dim3(YNT), 0, stream, X(), fs, twc());
         DNSBOX_LAUNCH_CHECK();
@@ -714,6 +779,42 @@ class BoxT final : public BoxBase {
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_zfwd_adj<N, T, 2>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), aa);
         DNSBOX_LAUNCH_CHECK();
+    }
+
+    // dnsboxAdj.f90:76-126: G = rhsAdj(u); u* = u + dtau G; u <- u + (dtau G) / 2 + (dtau G(u*)) / 2;
+    // state_project (:117).  On the box's state the clock advances by dtau per step (time(n+1) =
+    // time(n) + dt) and the step pipeline restarts unpruned (rhsAdj's ".gt." rule keeps the
+    // |k| = N/3 modes that the forward mask removes).
+    void adj_descent(void* u_, double dtau, int nsteps, double lin_sign) override
+    {
+        require_uniform("the adjoint-descent step");
+        if (nsteps <= 0) return;
+        const bool own = u_ == nullptr;
+        C* u = own ? reinterpret_cast<C*>(St()) : static_cast<C*>(u_);
+        if (!descbuf) DNSBOX_HIP(hipMalloc(&descbuf, 9 * size_t(fs) * sizeof(C)));
+        C* g0 = static_cast<C*>(descbuf);
+        C* us = g0 + 3 * fs;
+        C* g1 = us + 3 * fs;
+        const long long n3 = 3 * fs;
+        const dim3 eg((n3 + 255) / 256 < 2048 ? (n3 + 255) / 256 : 2048);
+        for (int i = 0; i < nsteps; ++i) {
+            rhs_adj(u, g0, lin_sign);
+            hipLaunchKernelGGL((k_axpy<T, false>), eg, dim3(256), 0, stream, us, u, g0, dtau, n3);
+            DNSBOX_LAUNCH_CHECK();
+            rhs_adj(us, g1, lin_sign);
+            hipLaunchKernelGGL((k_axpy<T, true>), eg, dim3(256), 0, stream, u, u, g0, dtau, n3);
+            hipLaunchKernelGGL((k_axpy<T, true>), eg, dim3(256), 0, stream, u, u, g1, dtau, n3);
+            hipLaunchKernelGGL((k_project_dealias<N, T>), dim3((fs + 255) / 256), dim3(256), 0, stream, u, fs,
+                               cfg.ph, 0, 1);
+            DNSBOX_LAUNCH_CHECK();
+        }
+        if (own) {
+            hipLaunchKernelGGL(k_add_time<0>, dim3(1), dim3(64), 0, stream, sc, dtau * nsteps);
+            DNSBOX_LAUNCH_CHECK();
+            dealiased = false;
+            ab2_started = false;
+        }
+        x6_valid = false;
     }
 
     // IF-Heun steps on a caller-owned state buffer
@@ -743,10 +844,14 @@ class BoxT final : public BoxBase {
         x6_valid = false;
         const size_t sb3 = 3 * size_t(fs) * sizeof(CS);
         try {
-            dealiased = prune && buffer_dealiased(static_cast<const CS*>(st));
+            dealiased = may_prune() && buffer_dealiased(static_cast<const CS*>(st));
             last_ext_pruned = dealiased;
             state = st;
-            if (traj_pred && dealiased) DNSBOX_HIP(hipMemsetAsync(traj_pred, 0, nsteps * sb3, stream));
+            if (traj_pred && dealiased) {   // the pruned predictor stores u* at the kept modes only
+                hipLaunchKernelGGL((k_fill_zero<S>), dim3(1024), dim3(256), 0, stream, static_cast<CS*>(traj_pred),
+                                   (long long)nsteps * 3 * fs);
+                DNSBOX_LAUNCH_CHECK();
+            }
             for (int i = 0; i < nsteps; ++i) {
                 pred_out_ = traj_pred ? static_cast<char*>(traj_pred) + i * sb3 : nullptr;
                 heun_step();

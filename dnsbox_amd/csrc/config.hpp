@@ -20,9 +20,14 @@ inline BoxConfig make_config(const dnsbox_params* p)
     ph.Pin = p->Pin;
     ph.kcut = p->kcut;
     ph.kolmo_amp = p->kolmo_amp;
-    ph.mask_thr = (double(p->n) / 2.0) * (2.0 * 1.0 / 3.0);
-    const double ts = (double(p->n) / 2.0) * 0.5 * 1.0;
+    // alpha_d = 2 pi / L_d, as the reference's real(kind=4) parameters hold it (parameters.f90:18-20)
+    const double al[3] = {p->alpha_x, p->alpha_y, p->alpha_z};
+    for (int d = 0; d < 3; ++d) ph.a[d] = al[d] > 0.0 ? double(float(al[d])) : 1.0;
+    // rhs.f90:74-88 / state.f90:184-197 thresholds, evaluated as the reference writes them
+    for (int d = 0; d < 3; ++d) ph.thr[d] = (double(p->n) / 2.0) * (2.0 * ph.a[d] / 3.0);
+    const double ts = (double(p->n) / 2.0) * 0.5 * ph.a[0];
     ph.sph_thr2 = ts * ts;
+    ph.sph_dthr2 = ph.thr[0] * ph.thr[0];
     ph.cmin = p->cfl_min;
     ph.cmax = p->cfl_max;
     ph.dtmax = p->dt_max;

@@ -163,10 +163,23 @@ int check_params(const dnsbox_params* p)
     if (!(p->dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
     if (p->adaptive_dt && !(p->cfl_max >= p->cfl_min && p->cfl_min > 0.0 && p->dt_max > 0.0))
         return fail(DNSBOX_E_ARG, "adaptive dt needs 0 < cfl_min <= cfl_max and dt_max > 0");
+    const double al[3] = {p->alpha_x, p->alpha_y, p->alpha_z};
+    for (double a : al)
+        if (!(a >= 0.0) || !(a < 1e30)) return fail(DNSBOX_E_ARG, "alpha_x/y/z must be >= 0 (0 = 1) and finite");
+    const int g[3] = {p->nx, p->ny, p->nz};
+    for (int m : g)
+        if (m != 0 && m != p->n)
+            return fail(DNSBOX_E_ARG, "nx, ny, nz must be 0 or n: this build transforms cubic grids only (n = " +
+                                          std::to_string(p->n) + ", nx, ny, nz = " + std::to_string(p->nx) + ", " +
+                                          std::to_string(p->ny) + ", " + std::to_string(p->nz) + ")");
     return 0;
 }
 
 }  // namespace
+
+namespace dnsbox {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}
 
 extern "C" {
 
@@ -192,6 +205,7 @@ int dnsbox_params_default(dnsbox_params* p)
     p->spherical = 0;    // :50
     p->parity_mode = 1;
     p->adaptive_dt = 1;  // tStepFix = .false. (:37)
+    p->alpha_x = p->alpha_y = p->alpha_z = 1.0;   // :18-20
     return 0;
 }
 
@@ -381,8 +395,6 @@ int dnsbox_step(dnsbox_ctx* c, int nsteps, int stepper)
     if (nsteps < 0) return fail(DNSBOX_E_ARG, "nsteps < 0");
     if (stepper < 1 || stepper > 3)
         return fail(DNSBOX_E_ARG, "stepper must be 1 (IF-Heun), 2 (IF-AB2) or 3 (IF-Heun + projection)");
-    if (stepper == 2 && c->p.adaptive_dt)
-        return fail(DNSBOX_E_STATE, "IF-AB2 runs at constant dt; create the box with adaptive_dt = 0");
     if (c->group) c->group->g->invalidate();
     return guarded([&]() -> int {
         BoxBase* b = c->box;
@@ -455,10 +467,11 @@ int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
         std::memcpy(&divmax, &ob, sizeof(double));
         const double dx = 6.283185307179586476925286766559 / double(c->p.n);
         const double dt = s.dt_prev;
+        const double* al = b->cfg.ph.a;   // io.f90:664-676: dx_d = L_d / N, L_d = 2 pi / alpha_d
         diag[DNSBOX_D_EKIN] = red[dnsbox::DG_EKIN];
         diag[DNSBOX_D_DISP] = red[dnsbox::DG_DISP];
         diag[DNSBOX_D_PROD] = 2.0 * c->p.Q * red[dnsbox::DG_EKIN];
-        diag[DNSBOX_D_COURANT] = (mx[0] + mx[1] + mx[2]) * dt / dx;
+        diag[DNSBOX_D_COURANT] = (mx[0] * al[0] + mx[1] * al[1] + mx[2] * al[2]) * dt / dx;
         diag[DNSBOX_D_DIVMAX] = divmax;
         diag[DNSBOX_D_EZERO] = red[dnsbox::DG_EZERO];
         diag[DNSBOX_D_EONE] = red[dnsbox::DG_EONE];
@@ -614,6 +627,32 @@ int dnsbox_rhs_adj_dev(dnsbox_ctx* c, const void* u, void* out, int lin_sign)
     if (lin_sign != 1 && lin_sign != -1) return fail(DNSBOX_E_ARG, "lin_sign must be +1 or -1");
     return guarded([&]() -> int {
         c->box->rhs_adj(u, out, double(lin_sign));
+        return 0;
+    });
+}
+
+int dnsbox_adj_descent_step_dev(dnsbox_ctx* c, void* u, double dtau, int nsteps, int lin_sign)
+{
+    if (int e = check_ctx(c)) return e;
+    invalidate(c);
+    if (!u || nsteps < 0) return fail(DNSBOX_E_ARG, "null device pointer or nsteps < 0");
+    if (!(dtau > 0.0)) return fail(DNSBOX_E_ARG, "dtau must be > 0");
+    if (lin_sign != 1 && lin_sign != -1) return fail(DNSBOX_E_ARG, "lin_sign must be +1 or -1");
+    return guarded([&]() -> int {
+        c->box->adj_descent(u, dtau, nsteps, double(lin_sign));
+        return 0;
+    });
+}
+
+int dnsbox_adj_descent_step(dnsbox_ctx* c, double dtau, int nsteps, int lin_sign)
+{
+    if (int e = check_ctx(c)) return e;
+    invalidate(c);
+    if (nsteps < 0) return fail(DNSBOX_E_ARG, "nsteps < 0");
+    if (!(dtau > 0.0)) return fail(DNSBOX_E_ARG, "dtau must be > 0");
+    if (lin_sign != 1 && lin_sign != -1) return fail(DNSBOX_E_ARG, "lin_sign must be +1 or -1");
+    return guarded([&]() -> int {
+        c->box->adj_descent(nullptr, dtau, nsteps, double(lin_sign));
         return 0;
     });
 }

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restr
     cpx<T>* out[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) out[f] = bx.X + f * bx.fs;
-    const ProCurl<T> pro{T(1.0 / (double(bx.n) * bx.n * bx.n))};
+    const ProCurl<T> pro{T(1.0 / (double(bx.n) * bx.n * bx.n)), KScale<T>::of(bx.ph)};
 #define DNSBOX_G_ZINV(NN) zinv_tile<NN, T, ProCurl<T>, T, PR>(in, out, tile, lds, tw, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZINV)
 #undef DNSBOX_G_ZINV
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
     ea.fs = bx.fs;
     ea.sc = bx.sc;
     ea.ph = bx.ph;
-    const ProCurl<T> pro{T(1.0 / (double(bx.n) * bx.n * bx.n))};
+    const ProCurl<T> pro{T(1.0 / (double(bx.n) * bx.n * bx.n)), KScale<T>::of(bx.ph)};
 #define DNSBOX_G_Z(NN) zfwd_zinv_tile<NN, T, MODE, T, PR>(bx.X, bx.fs, tile, lds, tw, ea, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_Z)
 #undef DNSBOX_G_Z
@@ -576,7 +576,7 @@ class Group final : public GroupBase {
     bool buffers_dealiased(const GBox<T>* d)
     {
         for (auto& b : boxes)
-            if (!b->prune) return false;
+            if (!b->may_prune()) return false;
         unsigned int* flag = boxes[0]->chk;
         const int nb = int(h_gb.size());
         DNSBOX_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), stream));
@@ -621,9 +621,11 @@ class Group final : public GroupBase {
         if (pr) {
             // the pruned predictor stores u* at the kept modes only: the rest of each slot is zero
             for (size_t i = 0; i < h.size(); ++i)
-                if (traj_pred)
-                    DNSBOX_HIP(hipMemsetAsync(traj_pred[i], 0, size_t(nsteps) * 3 * size_t(h[i].fs) * sizeof(C),
-                                              stream));
+                if (traj_pred) {
+                    hipLaunchKernelGGL((k_fill_zero<T>), dim3(256), dim3(256), 0, stream, static_cast<C*>(traj_pred[i]),
+                                       (long long)nsteps * 3 * h[i].fs);
+                    DNSBOX_LAUNCH_CHECK();
+                }
             launch_zinv<true>(stream, d0, 0);
         } else {
             launch_zinv<false>(stream, d0, 0);
@@ -695,7 +697,7 @@ class Group final : public GroupBase {
     void vjp_passes(hipStream_t s, const GBox<T>* d)
     {
         bool pr = vjp_cube;
-        for (auto& b : boxes) pr = pr && b->prune;
+        for (auto& b : boxes) pr = pr && b->may_prune();
         if (pr) vjp_passes_t<MODE, true>(s, d);
         else vjp_passes_t<MODE, false>(s, d);
     }

@@ -23,6 +23,8 @@ module dnsbox_gpu
         real(c_double) :: cfl_min, cfl_max, dt_max, dt
         integer(c_int) :: n, precision, kolmo_k, bandlim, spherical, parity_mode
         integer(c_int) :: adaptive_dt, state_precision
+        real(c_double) :: alpha_x, alpha_y, alpha_z
+        integer(c_int) :: nx, ny, nz
     end type dnsbox_params
 
     interface

@@ -75,6 +75,12 @@ typedef struct dnsbox_params {
                            the mask, variables.f90:229,245,259); 0: Nyquist modes removed */
     int adaptive_dt;    /* 1: CFL controller (tStepFix = .false., parameters.f90:37) */
     int state_precision;/* 0 = precision; 64 with precision 32: FP64 state, FP32 transforms */
+    double alpha_x;     /* 2 pi / Lx (0 = 1: the 2 pi box)   parameters.f90:18; held in single  */
+    double alpha_y;     /* 2 pi / Ly                          parameters.f90:19; precision, as  */
+    double alpha_z;     /* 2 pi / Lz                          parameters.f90:20; the reference's */
+                        /* real(kind=4) parameters are (wavenumbers, 2/3 rule, Nspec, Courant dx) */
+    int nx, ny, nz;     /* grid (0 = n)                       parameters.f90:12-14; cubic grids:
+                           a value other than 0 or n is rejected with DNSBOX_E_ARG */
 } dnsbox_params;
 
 typedef struct dnsbox_ctx dnsbox_ctx;
@@ -114,9 +120,11 @@ int dnsbox_int_fact(dnsbox_ctx* ctx, double dt);
 int dnsbox_rhs_nonlinear(dnsbox_ctx* ctx);
 int dnsbox_get_nonlin(dnsbox_ctx* ctx, double* nuhat, double* nvhat, double* nwhat);
 /* the time loop body of dnsboxheun.f90:81-157: stepper 1 = IF-Heun predictor-corrector
-   (dnsboxheun.f90:84-126), 2 = IF-AB2 (dnsbox.f90:143-193, consistent weights), 3 = IF-Heun
-   with the state projected and dealiased after every step (dnsboxband.f90:91-156), with the
-   CFL controller set_time_step (dnsboxheun.f90:227-246) when adaptive_dt */
+   (dnsboxheun.f90:84-126), 2 = variable-step IF-AB2 (dnsbox.f90:87-216 with consistent weights:
+   u_{n+1} = E_h u_n + h (1 + h/2h') E_h N_n - (h^2/2h') E_{h+h'} N_{n-1}, h' = t_n - t_{n-1};
+   the first step after a state change is IF-Heun, dnsbox.f90:87-141), 3 = IF-Heun with the
+   state projected and dealiased after every step (dnsboxband.f90:91-156); each with the CFL
+   controller set_time_step (dnsboxheun.f90:227-246) when adaptive_dt */
 int dnsbox_step(dnsbox_ctx* ctx, int nsteps, int stepper);
 /* io_save_stats (io.f90:628-661) + state_spectrum (state.f90:350-411): diag[DNSBOX_NDIAG],
    espec[Nspec] (may be NULL) */
@@ -196,6 +204,14 @@ int dnsbox_step_vjp_pred_dev(dnsbox_ctx* ctx, const void* u, const void* ustar, 
    lin_sign = +1 reproduces the reference's L u~ - P n_adj; lin_sign = -1 gives the descent
    direction -DF^+(u~) = -L u~ - P n_adj of J = 1/2 ||F(u)||^2_{H^-1}. */
 int dnsbox_rhs_adj_dev(dnsbox_ctx* ctx, const void* u, void* out, int lin_sign);
+/* the adjoint-descent loop body of dnsboxAdj.f90:76-126, nsteps times, in place on a device state:
+   G = rhsAdj(u) (:79), u* = u + dtau G (:83-101), u <- u + (dtau G)/2 + (dtau rhsAdj(u*))/2
+   (:87-113), state_project (:117).  lin_sign as in dnsbox_rhs_adj_dev (+1: the reference's rhsAdj;
+   -1: the descent direction). */
+int dnsbox_adj_descent_step_dev(dnsbox_ctx* ctx, void* u, double dtau, int nsteps, int lin_sign);
+/* the same on the context's state; its clock advances by dtau per step (time(n+1) = time(n) + dt,
+   dnsboxAdj.f90:125) */
+int dnsbox_adj_descent_step(dnsbox_ctx* ctx, double dtau, int nsteps, int lin_sign);
 
 /* Groups: many independent boxes (N in {16, 32, 48, 64, 128}, one precision, each with its
    own parameters) advanced together, one launch per pass for all of them, the step captured

@@ -44,13 +44,25 @@ void orc_ktable(int n, double* k)
     }
 }
 
+/* alpha of axis d (parameters.f90:18-20: real(kind=4), so its single-precision value; 0 = 1) */
+static double alpha_of(const orc_params* p, int d)
+{
+    const double a = d == 0 ? p->alpha_x : (d == 1 ? p->alpha_y : p->alpha_z);
+    return a > 0.0 ? (double)(float)a : 1.0;
+}
+
+/* 2/3-rule threshold of axis d, (N_d/2)(2 alpha_d/3) (rhs.f90:80-85, state.f90:191-196) */
+static double thr_of(const orc_params* p, int d)
+{
+    const int n = d == 0 ? p->nx : (d == 1 ? p->ny : p->nz);
+    return ((double)n / 2.0) * (2.0 * alpha_of(p, d) / 3.0);
+}
+
 /* variables.f90:112-123 */
 int orc_nspec(const orc_params* p)
 {
     const double deltak = p->deltak > 0 ? p->deltak : 1.0;
-    const double ax = ((double)p->nx / 2.0) * (2.0 * 1.0 / 3.0);
-    const double ay = ((double)p->ny / 2.0) * (2.0 * 1.0 / 3.0);
-    const double az = ((double)p->nz / 2.0) * (2.0 * 1.0 / 3.0);
+    const double ax = thr_of(p, 0), ay = thr_of(p, 1), az = thr_of(p, 2);
     if (p->spherical) return (int)(ax / deltak) - 1;
     return (int)(sqrt(ax * ax + ay * ay + az * az) / deltak) - 1;
 }
@@ -74,13 +86,11 @@ static inline size_t nphys_elems(const orc_params* p) { return (size_t)p->nx * p
 int orc_masked(const orc_params* p, int ix, int iy, int iz)
 {
     const int nx = p->nx, ny = p->ny, nz = p->nz;
-    double rkx = (ix < nx / 2) ? ix : (ix == nx / 2 ? 0.0 : ix - nx);
-    double rky = (iy < ny / 2) ? iy : (iy == ny / 2 ? 0.0 : iy - ny);
-    double rkz = (iz < nz / 2) ? iz : (iz == nz / 2 ? 0.0 : iz - nz);
-    const double tx = ((double)nx / 2.0) * (2.0 * 1.0 / 3.0);
-    const double ty = ((double)ny / 2.0) * (2.0 * 1.0 / 3.0);
-    const double tz = ((double)nz / 2.0) * (2.0 * 1.0 / 3.0);
-    const double ts = ((double)nx / 2.0) * 0.5 * 1.0;
+    double rkx = ((ix < nx / 2) ? ix : (ix == nx / 2 ? 0.0 : ix - nx)) * alpha_of(p, 0);
+    double rky = ((iy < ny / 2) ? iy : (iy == ny / 2 ? 0.0 : iy - ny)) * alpha_of(p, 1);
+    double rkz = ((iz < nz / 2) ? iz : (iz == nz / 2 ? 0.0 : iz - nz)) * alpha_of(p, 2);
+    const double tx = thr_of(p, 0), ty = thr_of(p, 1), tz = thr_of(p, 2);
+    const double ts = ((double)nx / 2.0) * 0.5 * alpha_of(p, 0);
     if (p->spherical && (rkx * rkx + rky * rky + rkz * rkz >= ts * ts)) return 1;
     if (fabs(rkx) >= tx || fabs(rky) >= ty || fabs(rkz) >= tz) return 1;
     if (rkx == 0.0 && rky == 0.0 && rkz == 0.0) return 1;
@@ -257,6 +267,10 @@ static ktabs ktabs_make(const orc_params* p)
     orc_ktable(p->nx, t.kx);
     orc_ktable(p->ny, t.ky);
     orc_ktable(p->nz, t.kz);
+    /* variables.f90:226-262: k = i (i-1) alpha */
+    for (int i = 0; i < p->nx; ++i) t.kx[i] *= alpha_of(p, 0);
+    for (int i = 0; i < p->ny; ++i) t.ky[i] *= alpha_of(p, 1);
+    for (int i = 0; i < p->nz; ++i) t.kz[i] *= alpha_of(p, 2);
     return t;
 }
 static void ktabs_free(ktabs* t) { free(t->kx); free(t->ky); free(t->kz); }
@@ -451,6 +465,72 @@ void orc_ab2(const orc_params* p, double* uh_, double* vh_, double* wh_, int nst
     free(itf);
 }
 
+/* One variable-step IF-AB2 step of size h after a step of size hp (the history N_{n-1} in nold,
+ * replaced by N_n).  dnsbox.f90:143-193 integrates v = e^{-Lt} u with AB2 weights written as
+ * 0.5 (2 t_n + dt - 2 t_{n-1}) and 0.5 (t_n - dt - t_{n-1}); the consistent variable-step
+ * weights of the same scheme are h (1 + h/2h') and -h^2/2h':
+ *   u_{n+1} = E_h u_n + h (1 + h/2h') E_h N_n - (h^2/2h') E_{h+h'} N_{n-1},  E_s = e^{(-nu|k|^2+Q) s} */
+static void ab2_step_var(const orc_params* p, cplx* f[3], cplx* nold, double h, double hp)
+{
+    const size_t ms = nspec_elems(p);
+    cplx* nl = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    double* e = (double*)malloc(sizeof(double) * ms);
+    double* e2 = (double*)malloc(sizeof(double) * ms);
+    orc_rhs_nonlinear(p, (const double*)f[0], (const double*)f[1], (const double*)f[2], (double*)nl,
+                      (double*)(nl + ms), (double*)(nl + 2 * ms));
+    orc_int_fact(p, h, e);
+    orc_int_fact(p, h + hp, e2);
+    const double r = h / (2.0 * hp);
+    for (int c = 0; c < 3; ++c)
+        for (size_t m = 0; m < ms; ++m) {
+            f[c][m] = e[m] * f[c][m] + h * (1.0 + r) * e[m] * nl[c * ms + m] - h * r * e2[m] * nold[c * ms + m];
+            nold[c * ms + m] = nl[c * ms + m];
+        }
+    free(e2);
+    free(e);
+    free(nl);
+}
+
+/* Variable-step IF-AB2 over the step sizes dts[0..nsteps-1]: the first step is IF-Heun
+ * (dnsbox.f90:87-141, keeping N(u_0) as the history), then ab2_step_var with h = dts[n],
+ * h' = dts[n-1]. */
+void orc_ab2_var(const orc_params* p, double* uh_, double* vh_, double* wh_, int nsteps, const double* dts)
+{
+    if (nsteps <= 0) return;
+    const size_t ms = nspec_elems(p);
+    cplx* f[3] = {(cplx*)uh_, (cplx*)vh_, (cplx*)wh_};
+    cplx* nold = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    orc_rhs_nonlinear(p, uh_, vh_, wh_, (double*)nold, (double*)(nold + ms), (double*)(nold + 2 * ms));
+    orc_heun(p, uh_, vh_, wh_, 1, dts[0]);
+    for (int n = 1; n < nsteps; ++n) ab2_step_var(p, f, nold, dts[n], dts[n - 1]);
+    free(nold);
+}
+
+/* orc_ab2_var with the step sizes from the CFL controller applied before every step on the
+ * current field (as orc_heun_adaptive does): h_n = set_time_step(Courant(u_n, h_{n-1})). */
+void orc_ab2_adaptive(const orc_params* p, double* uh_, double* vh_, double* wh_, int nsteps, double* dt_inout,
+                      double cmin, double cmax, double dtmax, double* t_out)
+{
+    double dt = *dt_inout, t = t_out ? *t_out : 0.0;
+    if (nsteps <= 0) return;
+    const size_t ms = nspec_elems(p);
+    cplx* f[3] = {(cplx*)uh_, (cplx*)vh_, (cplx*)wh_};
+    cplx* nold = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    dt = orc_cfl_dt(orc_courant(p, uh_, vh_, wh_, dt), dt, cmin, cmax, dtmax);
+    orc_rhs_nonlinear(p, uh_, vh_, wh_, (double*)nold, (double*)(nold + ms), (double*)(nold + 2 * ms));
+    orc_heun(p, uh_, vh_, wh_, 1, dt);
+    t += dt;
+    for (int n = 1; n < nsteps; ++n) {
+        const double h = orc_cfl_dt(orc_courant(p, uh_, vh_, wh_, dt), dt, cmin, cmax, dtmax);
+        ab2_step_var(p, f, nold, h, dt);
+        t += h;
+        dt = h;
+    }
+    free(nold);
+    *dt_inout = dt;
+    if (t_out) *t_out = t;
+}
+
 /* state.f90:120-179 state_project (eps = 0 in the forward build) */
 void orc_project(const orc_params* p, double* uh_, double* vh_, double* wh_)
 {
@@ -476,9 +556,7 @@ void orc_dealias(const orc_params* p, double* uh_, double* vh_, double* wh_)
 {
     cplx *uh = (cplx*)uh_, *vh = (cplx*)vh_, *wh = (cplx*)wh_;
     ktabs t = ktabs_make(p);
-    const double tx = ((double)p->nx / 2.0) * (2.0 * 1.0 / 3.0);
-    const double ty = ((double)p->ny / 2.0) * (2.0 * 1.0 / 3.0);
-    const double tz = ((double)p->nz / 2.0) * (2.0 * 1.0 / 3.0);
+    const double tx = thr_of(p, 0), ty = thr_of(p, 1), tz = thr_of(p, 2);
     for (int ix = 0; ix < nxh(p); ++ix)
         for (int iy = 0; iy < p->ny; ++iy)
             for (int iz = 0; iz < p->nz; ++iz) {
@@ -509,7 +587,7 @@ double orc_courant(const orc_params* p, const double* uh_, const double* vh_, co
         orc_c2r(p, in[d], u);
         double mx = 0.0;
         for (size_t i = 0; i < mp; ++i) mx = fmax(mx, fabs(u[i] * sm));
-        c += mx * dt / (2.0 * M_PI / (double)nn[d]);
+        c += mx * dt / ((2.0 * M_PI / alpha_of(p, d)) / (double)nn[d]);   /* dt / (L_d / N_d) */
     }
     free(u);
     return c;
@@ -659,11 +737,12 @@ void orc_init_solenoidal(const orc_params* p, uint64_t seed, int box, double k0,
                     double ph = 2.0 * M_PI * ((double)(h >> 11) * (1.0 / 9007199254740992.0));
                     a[c] = cos(ph) + sin(ph) * I;
                 }
-                const double k2 = (double)kx * kx + (double)ky * ky + (double)kz * kz;
-                cplx kd = (kx * a[0] + ky * a[1] + kz * a[2]) / k2;
-                a[0] -= kx * kd;
-                a[1] -= ky * kd;
-                a[2] -= kz * kd;
+                const double kpx = kx * alpha_of(p, 0), kpy = ky * alpha_of(p, 1), kpz = kz * alpha_of(p, 2);
+                const double k2 = kpx * kpx + kpy * kpy + kpz * kpz;
+                cplx kd = (kpx * a[0] + kpy * a[1] + kpz * a[2]) / k2;
+                a[0] -= kpx * kd;
+                a[1] -= kpy * kd;
+                a[2] -= kpz * kd;
                 double na = sqrt(creal(conj(a[0]) * a[0] + conj(a[1]) * a[1] + conj(a[2]) * a[2]));
                 if (na < 1e-300) continue;
                 double kk = sqrt(k2);
@@ -697,9 +776,9 @@ void orc_init_shapiro(const orc_params* p, double t, double* u, double* v, doubl
     for (int k = 0; k < p->nz; ++k)
         for (int j = 0; j < p->ny; ++j)
             for (int i = 0; i < p->nx; ++i) {
-                double x = (double)(i - p->nx / 2) * (2.0 * M_PI) / (double)p->nx;
-                double y = (double)(j - p->ny / 2) * (2.0 * M_PI) / (double)p->ny;
-                double z = (double)(k - p->nz / 2) * (2.0 * M_PI) / (double)p->nz;
+                double x = (double)(i - p->nx / 2) * (2.0 * M_PI / alpha_of(p, 0)) / (double)p->nx;
+                double y = (double)(j - p->ny / 2) * (2.0 * M_PI / alpha_of(p, 1)) / (double)p->ny;
+                double z = (double)(k - p->nz / 2) * (2.0 * M_PI / alpha_of(p, 2)) / (double)p->nz;
                 size_t m = pidx(p, i, j, k);
                 u[m] = -0.5 * (factor * cos(x) * sin(y) * sin(z) + sin(x) * cos(y) * cos(z)) * e;
                 v[m] = 0.5 * (factor * sin(x) * cos(y) * sin(z) - cos(x) * sin(y) * cos(z)) * e;
@@ -896,9 +975,7 @@ static void grads_of(const orc_params* p, const ktabs* t, const cplx* fh, cplx* 
 
 static int masked_gt(const orc_params* p, const ktabs* t, int ix, int iy, int iz)
 {
-    const double tx = ((double)p->nx / 2.0) * (2.0 * 1.0 / 3.0);
-    const double ty = ((double)p->ny / 2.0) * (2.0 * 1.0 / 3.0);
-    const double tz = ((double)p->nz / 2.0) * (2.0 * 1.0 / 3.0);
+    const double tx = thr_of(p, 0), ty = thr_of(p, 1), tz = thr_of(p, 2);
     if (!p->parity_mode && (ix == p->nx / 2 || iy == p->ny / 2 || iz == p->nz / 2)) return 1;
     return fabs(t->kx[ix]) > tx || fabs(t->ky[iy]) > ty || fabs(t->kz[iz]) > tz;
 }
@@ -924,6 +1001,14 @@ static void project_gt(const orc_params* p, const ktabs* t, cplx* n[3])
 
 void orc_rhs_adj(const orc_params* p, const double* uh_, const double* vh_, const double* wh_,
                  double* au_, double* av_, double* aw_)
+{
+    orc_rhs_adj_sign(p, uh_, vh_, wh_, 1.0, au_, av_, aw_);
+}
+
+/* orc_rhs_adj with the linear term multiplied by lin_sign: +1 is rhsAdj.f90:356-381; -1 turns
+ * L u~ into -L u~, the descent direction -DF^+(u~) of J = 1/2 ||F(u)||^2_{H^-1} */
+void orc_rhs_adj_sign(const orc_params* p, const double* uh_, const double* vh_, const double* wh_,
+                      double lin_sign, double* au_, double* av_, double* aw_)
 {
     const cplx* u[3] = {(const cplx*)uh_, (const cplx*)vh_, (const cplx*)wh_};
     cplx* out[3] = {(cplx*)au_, (cplx*)av_, (cplx*)aw_};
@@ -976,8 +1061,35 @@ void orc_rhs_adj(const orc_params* p, const double* uh_, const double* vh_, cons
                 size_t m = sidx(p, ix, iy, iz);
                 const cplx kx = t.kx[ix] * I, ky = t.ky[iy] * I, kz = t.kz[iz] * I;
                 const cplx L = p->nu * (kx * kx + ky * ky + kz * kz) + p->Q;
-                for (int c = 0; c < 3; ++c) out[c][m] = L * uti[c * ms + m] + n[c][m];
+                for (int c = 0; c < 3; ++c) out[c][m] = lin_sign * L * uti[c * ms + m] + n[c][m];
             }
     free(uti); free(nl); free(tr); free(ut); free(d); free(tmp);
     ktabs_free(&t);
+}
+
+/* dnsboxAdj.f90:76-126, nsteps times: Heun in fictitious time on rhsAdj, then state_project:
+ *   G0 = rhsAdj(u) (:79); u* = u + dtau G0 (:83-101); u1 = u + (dtau G0) 0.5 (:87-113);
+ *   u = u1 + (dtau rhsAdj(u*)) 0.5 (:105-113); state_project (:117) */
+void orc_adj_descent(const orc_params* p, double* uh_, double* vh_, double* wh_, double dtau, int nsteps,
+                     double lin_sign)
+{
+    const size_t ms = nspec_elems(p);
+    cplx* f[3] = {(cplx*)uh_, (cplx*)vh_, (cplx*)wh_};
+    cplx* g = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    cplx* us = (cplx*)malloc(sizeof(cplx) * ms * 3);
+    for (int n = 0; n < nsteps; ++n) {
+        orc_rhs_adj_sign(p, uh_, vh_, wh_, lin_sign, (double*)g, (double*)(g + ms), (double*)(g + 2 * ms));
+        for (int c = 0; c < 3; ++c)
+            for (size_t m = 0; m < ms; ++m) {
+                us[c * ms + m] = f[c][m] + dtau * g[c * ms + m];
+                f[c][m] = f[c][m] + dtau * g[c * ms + m] * 0.5;
+            }
+        orc_rhs_adj_sign(p, (double*)us, (double*)(us + ms), (double*)(us + 2 * ms), lin_sign, (double*)g,
+                         (double*)(g + ms), (double*)(g + 2 * ms));
+        for (int c = 0; c < 3; ++c)
+            for (size_t m = 0; m < ms; ++m) f[c][m] = f[c][m] + dtau * g[c * ms + m] * 0.5;
+        orc_project(p, uh_, vh_, wh_);
+    }
+    free(us);
+    free(g);
 }

@@ -36,6 +36,9 @@ typedef struct {
     int parity_mode;   /* 1: reference k-table (Nyquist k=0, Nyquist modes kept by the mask);
                           0: Nyquist-index modes removed by the mask as well           */
     int nthreads;      /* OpenMP threads for the transforms (0 = runtime default) */
+    double alpha_x, alpha_y, alpha_z;  /* 2 pi / L per axis (0 = 1), used at their single-precision
+                                          values as the reference's real(kind=4) parameters
+                                          (parameters.f90:18-20)                             */
 } orc_params;
 
 /* diagnostics vector layout of orc_diagnostics */
@@ -54,6 +57,9 @@ void   orc_rhs_nonlinear(const orc_params* p, const double* uh, const double* vh
 void   orc_int_fact(const orc_params* p, double dt, double* intfact);
 void   orc_heun(const orc_params* p, double* uh, double* vh, double* wh, int nsteps, double dt);
 void   orc_ab2(const orc_params* p, double* uh, double* vh, double* wh, int nsteps, double dt);
+void   orc_ab2_var(const orc_params* p, double* uh, double* vh, double* wh, int nsteps, const double* dts);
+void   orc_ab2_adaptive(const orc_params* p, double* uh, double* vh, double* wh, int nsteps,
+                        double* dt_inout, double cmin, double cmax, double dtmax, double* t_out);
 void   orc_project(const orc_params* p, double* uh, double* vh, double* wh);
 void   orc_dealias(const orc_params* p, double* uh, double* vh, double* wh);
 void   orc_diagnostics(const orc_params* p, const double* uh, const double* vh, const double* wh,
@@ -73,6 +79,10 @@ uint64_t orc_hash(uint64_t seed, uint64_t box, uint64_t idx);
 /* adjoint pieces */
 void   orc_rhs_adj(const orc_params* p, const double* uh, const double* vh, const double* wh,
                    double* au, double* av, double* aw);
+void   orc_rhs_adj_sign(const orc_params* p, const double* uh, const double* vh, const double* wh,
+                        double lin_sign, double* au, double* av, double* aw);
+void   orc_adj_descent(const orc_params* p, double* uh, double* vh, double* wh, double dtau, int nsteps,
+                       double lin_sign);
 void   orc_rhs_vjp(const orc_params* p, const double* uh, const double* vh, const double* wh,
                    const double* lu, const double* lv, const double* lw,
                    double* gu, double* gv, double* gw);

@@ -28,7 +28,8 @@ class OrcParams(C.Structure):
         ("kolmo_amp", C.c_double), ("deltak", C.c_double),
         ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("kolmo_k", C.c_int),
         ("bandlim", C.c_int), ("spherical", C.c_int), ("parity_mode", C.c_int),
-        ("nthreads", C.c_int),
+        ("nthreads", C.c_int), ("alpha_x", C.c_double), ("alpha_y", C.c_double),
+        ("alpha_z", C.c_double),
     ]
 
 
@@ -59,6 +60,9 @@ def lib():
             "orc_int_fact": (None, [P, C.c_double, d]),
             "orc_heun": (None, [P, d, d, d, C.c_int, C.c_double]),
             "orc_ab2": (None, [P, d, d, d, C.c_int, C.c_double]),
+            "orc_ab2_var": (None, [P, d, d, d, C.c_int, d]),
+            "orc_ab2_adaptive": (None, [P, d, d, d, C.c_int, d, C.c_double, C.c_double,
+                                        C.c_double, d]),
             "orc_project": (None, [P, d, d, d]),
             "orc_dealias": (None, [P, d, d, d]),
             "orc_diagnostics": (None, [P, d, d, d, C.c_double, d, d]),
@@ -74,6 +78,8 @@ def lib():
             "orc_rhs_vjp": (None, [P, d, d, d, d, d, d, d, d, d]),
             "orc_heun_vjp": (None, [P, d, d, d, C.c_double, d, d, d]),
             "orc_rhs_adj": (None, [P, d, d, d, d, d, d]),
+            "orc_rhs_adj_sign": (None, [P, d, d, d, C.c_double, d, d, d]),
+            "orc_adj_descent": (None, [P, d, d, d, C.c_double, C.c_int, C.c_double]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -92,12 +98,14 @@ class Oracle:
 
     def __init__(self, n, nu=0.05, Q=0.0, Pin=0.0, kcut=0.0, kolmo_amp=0.0, kolmo_k=0,
                  bandlim=False, spherical=False, parity_mode=True, deltak=1.0, nthreads=0,
-                 ny=None, nz=None):
+                 ny=None, nz=None, alpha=(1.0, 1.0, 1.0)):
         self.nx = n
         self.ny = ny or n
         self.nz = nz or n
+        self.alpha = tuple(float(a) for a in alpha)
         self.p = OrcParams(nu, Q, Pin, kcut, kolmo_amp, deltak, self.nx, self.ny, self.nz,
-                           kolmo_k, int(bandlim), int(spherical), int(parity_mode), nthreads)
+                           kolmo_k, int(bandlim), int(spherical), int(parity_mode), nthreads,
+                           *self.alpha)
         self.L = lib()
 
     # shapes
@@ -169,6 +177,20 @@ class Oracle:
         self.L.orc_ab2(C.byref(self.p), *[_p(a) for a in f], nsteps, dt)
         return f
 
+    def ab2_var(self, f, dts):
+        f = [np.array(a, np.complex128, order="C") for a in f]
+        d = np.ascontiguousarray(dts, np.float64)
+        self.L.orc_ab2_var(C.byref(self.p), *[_p(a) for a in f], len(d), _p(d))
+        return f
+
+    def ab2_adaptive(self, f, nsteps, dt, cmin, cmax, dtmax, t=0.0):
+        f = [np.array(a, np.complex128, order="C") for a in f]
+        dtv = np.array([dt])
+        tv = np.array([t])
+        self.L.orc_ab2_adaptive(C.byref(self.p), *[_p(a) for a in f], nsteps, _p(dtv), cmin,
+                                cmax, dtmax, _p(tv))
+        return f, float(dtv[0]), float(tv[0])
+
     def heun_adaptive(self, f, nsteps, dt, cmin, cmax, dtmax, t=0.0):
         f = [np.array(a, np.complex128, order="C") for a in f]
         dtv = np.array([dt])
@@ -234,6 +256,18 @@ class Oracle:
         out = self.zeros_spec()
         self.L.orc_rhs_adj(C.byref(self.p), *[_p(a) for a in f], *[_p(a) for a in out])
         return out
+
+    def rhs_adj_sign(self, f, lin_sign):
+        f = [np.ascontiguousarray(a, np.complex128) for a in f]
+        out = self.zeros_spec()
+        self.L.orc_rhs_adj_sign(C.byref(self.p), *[_p(a) for a in f], float(lin_sign),
+                                *[_p(a) for a in out])
+        return out
+
+    def adj_descent(self, f, dtau, nsteps, lin_sign=1):
+        f = [np.array(a, np.complex128, order="C") for a in f]
+        self.L.orc_adj_descent(C.byref(self.p), *[_p(a) for a in f], dtau, nsteps, float(lin_sign))
+        return f
 
     def hermitian_weights(self):
         w = np.full(self.spec_shape, 2.0)

@@ -41,8 +41,41 @@ def test_python_binding_covers_header():
 
 def test_params_struct_layout():
     from dnsbox_amd._lib import Params
-    # 10 doubles then 8 ints, natural alignment, no padding
-    assert C.sizeof(Params) == 10 * 8 + 8 * 4
+    # 10 doubles, 8 ints, 3 doubles (alpha_x/y/z), 3 ints (nx, ny, nz), padded to 8 bytes
+    assert C.sizeof(Params) == 10 * 8 + 8 * 4 + 3 * 8 + 3 * 4 + 4
+    assert Params.alpha_x.offset == 112 and Params.nx.offset == 136
+
+
+def test_non_cubic_grids_and_bad_lengths_are_rejected():
+    """A grid other than n^3 and a negative box length fail with DNSBOX_E_ARG before any device
+    work (parameters.f90:12-20 allow both; this build transforms cubic grids)."""
+    from dnsbox_amd import _lib
+    lib = _lib.load()
+    p = _lib.Params()
+    assert lib.dnsbox_params_default(C.byref(p)) == 0
+    assert (p.alpha_x, p.alpha_y, p.alpha_z, p.nx, p.ny, p.nz) == (1.0, 1.0, 1.0, 0, 0, 0)
+    h = C.c_void_p()
+    p.n, p.nz = 32, 64
+    assert lib.dnsbox_create(C.byref(p), C.byref(h)) == -1
+    assert b"cubic grids only" in lib.dnsbox_last_error()
+    p.nz, p.alpha_y = 0, -1.0
+    assert lib.dnsbox_create(C.byref(p), C.byref(h)) == -1
+    assert b"alpha" in lib.dnsbox_last_error()
+
+
+def test_step_map_refuses_adaptive_boxes():
+    """The step adjoints differentiate fixed-dt steps; a box with the CFL controller on (the
+    reference default, tStepFix = .false.) is refused before any step is taken."""
+    import types
+
+    torch = pytest.importorskip("torch")
+    from dnsbox_amd.adjoint import group_step_map, step_map
+    box = types.SimpleNamespace(params=types.SimpleNamespace(adaptive_dt=1))
+    with pytest.raises(ValueError, match="adaptive_dt=0"):
+        step_map(box, torch.zeros(3, 4, 4, 3, dtype=torch.complex128), 2, 0.01, 1)
+    grp = types.SimpleNamespace(boxes=[types.SimpleNamespace(params=types.SimpleNamespace(adaptive_dt=0)), box])
+    with pytest.raises(ValueError, match="adaptive_dt=0"):
+        group_step_map(grp, [torch.zeros(3, 4, 4, 3, dtype=torch.complex128)] * 2, 2, [0.01, 0.01], 1)
 
 
 def test_errors_without_device_do_not_abort():
