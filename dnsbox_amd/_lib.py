@@ -81,6 +81,7 @@ SIGNATURES = {
     "dnsbox_rhs_adj_dev": (C.c_int, [_vp, _vp, _vp, C.c_int]),
     "dnsbox_adj_descent_step_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_int]),
     "dnsbox_adj_descent_step": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
+    "dnsbox_rhs_dev_noalloc": (C.c_int, [C.POINTER(Params), _vp, _vp, _vp]),
     "dnsbox_group_create": (C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(_vp)]),
     "dnsbox_group_destroy": (C.c_int, [_vp]),
     "dnsbox_group_size": (C.c_int, [_vp]),

@@ -213,6 +213,16 @@ int dnsbox_adj_descent_step_dev(dnsbox_ctx* ctx, void* u, double dtau, int nstep
    dnsboxAdj.f90:125) */
 int dnsbox_adj_descent_step(dnsbox_ctx* ctx, double dtau, int nsteps, int lin_sign);
 
+/* rhs_nonlinear on device buffers (dnsbox_rhs_dev's computation, out <- N(in), internal layout,
+   precision = state_precision) with no context and no device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+   serving loops, harnesses that forbid hipMalloc).  n must be 16, 32, 48 or 64: its transform
+   scratch, twiddles and band-energy scalar are one static device array of 12.98 MB per GPU
+   (6 FP64 fields of 64^3), kept until the process ends and shared by every call on that GPU,
+   so calls that overlap on one GPU race.  Larger n fail with DNSBOX_E_ARG.  Runs on hip_stream
+   (NULL: the null stream); the caller synchronises. */
+int dnsbox_rhs_dev_noalloc(const dnsbox_params* p, const void* in, void* out, void* hip_stream);
+
 /* Groups: many independent boxes (N in {16, 32, 48, 64, 128}, one precision, each with its
    own parameters) advanced together, one launch per pass for all of them, the step captured
    in a hipGraph.  Batched counterpart of dnsbox_step for adjoint-descent guesses.  No
