@@ -24,7 +24,8 @@ Printed as one JSON line:
   large_box              the same precision form at 512^3 (nu = 0.002): FP32-form and FP64
                          steps/s and the FP32 form's error vs FP64 after 1000 fixed-dt steps
   adjoint                64^3 FP64 checkpointed T-step gradients per second (T = 200,
-                         dt = 0.005, checkpoint every 20) and one finite-difference check
+                         dt = 0.005, checkpoint every 20), one finite-difference check, and
+                         50 adjoint-descent iterations with the FP64 line search (J0 -> J50)
   batched                48 boxes (16 x 32^3, 16 x 48^3, 16 x 64^3, FP32, own nu and k0):
                          box-steps/s forward (hipGraph step) and forward + adjoint
                          (checkpointed gradient of the T = 100 map)
@@ -197,12 +198,12 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def adjoint_rate(reps=3):
+def adjoint_rate(reps=3, descent_iters=50):
     """64^3 FP64 residual gradients per second (T = 200, dt = 0.005, checkpoint every 20)
     and a central-difference check of one gradient along a seeded solenoidal direction."""
     import torch
     from dnsbox_amd import Box
-    from dnsbox_amd.adjoint import residual
+    from dnsbox_amd.adjoint import adjoint_descent, residual
     n, T, dt, k = 64, 200, 0.005, 20
     with Box(n=n, precision=64, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
         b.init_random(0, 0, 4.0, 0.5)
@@ -230,30 +231,33 @@ def adjoint_rate(reps=3):
             jm = float(residual(b, u0 - eps * v, T, dt, k))
         fd = (jp - jm) / (2 * eps)
         an = float((g.conj() * v).real.sum())
+        # the configured descent: 50 iterations with the FP64 backtracking line search
+        t0 = time.perf_counter()
+        _, hist = adjoint_descent(b, u0, T, dt, k=k, iters=descent_iters)
+        torch.cuda.synchronize()
+        t_desc = time.perf_counter() - t0
     return {"grads_per_s": 1.0 / secs, "ms_per_grad": secs * 1e3, "n": n, "T": T, "dt": dt,
             "checkpoint_every": k, "dtype": "f64", "J": float(J.detach()),
             "fd_rel_err": abs(fd - an) / abs(fd),
+            "descent": {"iterations": len(hist) - 1, "J0": hist[0], "J_final": hist[-1],
+                        "monotone": all(b_ <= a_ for a_, b_ in zip(hist, hist[1:])), "seconds": t_desc},
             "workload": "64^3 FP64, nu=0.02, f=(sin 4y,0,0), J=1/2<|Phi_T(u)-u|^2>"}
 
 
 def batched_rates(steps=500, T=100, k=20, reps=3, warm=2):
     """The 48-box group: forward box-steps/s (hipGraph-replayed step of the boxes' own states)
     and forward + adjoint box-steps/s (checkpointed gradient of the T-step map)."""
-    import numpy as np
     import torch
     from dnsbox_amd import BoxGroup
     from dnsbox_amd.adjoint import group_residual_grads
-    rng = np.random.default_rng(2024)
-    sizes = [32] * 16 + [48] * 16 + [64] * 16
-    nus = rng.uniform(0.01, 0.05, len(sizes))
-    k0s = rng.uniform(2.0, 6.0, len(sizes))
+    from dnsbox_amd.workloads import batched48, init_batched48
     dt = 0.005
-    params = [dict(n=n, precision=32, nu=float(nu), kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0,
-                   dt=dt) for n, nu in zip(sizes, nus)]
+    params, k0s = batched48(32, dt=dt)
+    sizes = [p["n"] for p in params]
     with BoxGroup(params) as g:
+        init_batched48(g.boxes, k0s)
         us = []
-        for i, b in enumerate(g.boxes):
-            b.init_random(100, i, float(k0s[i]), 0.5)
+        for b in g.boxes:
             b.rhs_int_fact(dt)
             us.append(b.state_tensor())
         g.step(10)

@@ -365,7 +365,7 @@ def adjoint_descent(box, u0, T, dt, k=20, iters=50, step0=1.0, c1=1e-4, shrink=0
         gm = g.to(torch.complex128) / w * float(n) ** 6
         d = -project_masked(gm, n).to(u.dtype)
         slope = float((g.to(torch.complex128).conj() * d.to(torch.complex128)).real.sum())
-        J0 = float(J)
+        J0 = float(J.detach())
         hist.append(J0)
         if callback:
             callback(it, J0)

@@ -1,0 +1,131 @@
+"""The configured workloads at their stated sizes, on the GPU.
+
+* The 48-box batched workload (16 x 32^3 + 16 x 48^3 + 16 x 64^3, FP32, each box its own
+  nu ~ U[0.01, 0.05] and k0 ~ U[2, 6]; dnsbox_amd.workloads.batched48, the group bench.py times):
+  100 group steps equal 100 single-box steps of every box; the batched T = 100 gradient equals
+  the single-box gradient for one box of each size; in the same group at FP64 the gradient
+  matches central differences for one box of each size.
+* The 64^3 FP64 differentiation workload: 50 adjoint-descent iterations of
+  J = 1/2 <|Phi_T(u) - u|^2> (T = 200, dt = 0.005, checkpoints every 20) with the FP64 line
+  search lower J at every iteration.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+T_GRAD, K_GRAD = 100, 20
+ONE_PER_SIZE = (0, 16, 32)   # a 32^3, a 48^3 and a 64^3 box
+
+
+def rel_l2(a, b):
+    num = sum(np.linalg.norm((x - y).ravel()) ** 2 for x, y in zip(a, b))
+    return float(np.sqrt(num / sum(np.linalg.norm(y.ravel()) ** 2 for y in b)))
+
+
+def trel(a, b):
+    import torch
+    return float(torch.linalg.vector_norm((a - b).to(torch.complex128)) /
+                 torch.linalg.vector_norm(b.to(torch.complex128)))
+
+
+def test_batched48_group_steps_match_single_boxes(gpu_lib):
+    from dnsbox_amd import Box, BoxGroup
+    from dnsbox_amd.workloads import batched48, init_batched48
+    params, k0s = batched48(32)
+    assert len(params) == 48
+    with BoxGroup(params) as g:
+        init_batched48(g.boxes, k0s)
+        for b in g.boxes:
+            b.rhs_int_fact(b.params.dt)
+        g.step(100)
+        g.sync()
+        got = [b.get_state() for b in g.boxes]
+        times = [b.time()[0] for b in g.boxes]
+    # the group's kernels and the single box's are separate instantiations of the same tile
+    # functions: FP32 results agree to a few ulp (the compiled FMA contractions may differ), so
+    # the bar after 100 steps is 2e-7, about 3 FP32 ulp
+    worst = {}
+    for i, p in enumerate(params):
+        with Box(**p) as b:
+            b.init_random(100, i, k0s[i], 0.5)
+            b.rhs_int_fact(p["dt"])
+            b.step(100)
+            ref = b.get_state()
+            t = b.time()[0]
+        err = rel_l2(got[i], ref)
+        worst[p["n"]] = max(worst.get(p["n"], 0.0), err)
+        assert abs(times[i] - t) < 1e-12
+        assert err < 2e-7, (i, p["n"], err)
+    print(f"48-box group vs single boxes after 100 steps: worst rel L2 per size {worst}", flush=True)
+
+
+def _grads(group, params, k0s, dtype):
+    import torch
+    from dnsbox_amd.adjoint import group_residual_grads
+    from dnsbox_amd.workloads import init_batched48
+    init_batched48(group.boxes, k0s)
+    us = [b.state_tensor().to(dtype) for b in group.boxes]
+    J, gs = group_residual_grads(group, us, T_GRAD, [p["dt"] for p in params], K_GRAD)
+    torch.cuda.synchronize()
+    return us, J, gs
+
+
+def test_batched48_gradient_matches_single_boxes(gpu_lib):
+    import torch
+    from dnsbox_amd import Box, BoxGroup
+    from dnsbox_amd.adjoint import residual
+    from dnsbox_amd.workloads import batched48
+    params, k0s = batched48(32)
+    with BoxGroup(params) as g:
+        us, J, gs = _grads(g, params, k0s, torch.complex64)
+    for i in ONE_PER_SIZE:
+        p = params[i]
+        with Box(**p) as b:
+            x = us[i].clone().requires_grad_(True)
+            Ji = residual(b, x, T_GRAD, p["dt"], K_GRAD)
+            (gi,) = torch.autograd.grad(Ji, x)
+        eg, ej = trel(gs[i], gi), abs(float(J[i]) - Ji.item()) / abs(Ji.item())
+        print(f"box {i} n={p['n']} FP32: J={float(J[i]):.8e} (single {Ji.item():.8e}) grad rel {eg:.2e}", flush=True)
+        assert ej < 1e-6 and eg < 1e-5
+
+
+def test_batched48_fp64_gradient_matches_central_differences(gpu_lib):
+    import torch
+    from dnsbox_amd import BoxGroup
+    from dnsbox_amd.adjoint import group_residuals
+    from dnsbox_amd.workloads import batched48
+    params, k0s = batched48(64)
+    with BoxGroup(params) as g:
+        us, J, gs = _grads(g, params, k0s, torch.complex128)
+        # directions: seeded solenoidal fields of the same boxes, one box perturbed per size
+        for b, k0 in zip(g.boxes, k0s):
+            b.init_random(7, 0, k0, 0.5)
+        vs = [0.1 * b.state_tensor() for b in g.boxes]
+        eps = 1e-6
+        dts = [p["dt"] for p in params]
+        with torch.no_grad():
+            plus = [u + eps * v if i in ONE_PER_SIZE else u for i, (u, v) in enumerate(zip(us, vs))]
+            minus = [u - eps * v if i in ONE_PER_SIZE else u for i, (u, v) in enumerate(zip(us, vs))]
+            jp = group_residuals(g, plus, T_GRAD, dts, K_GRAD).tolist()
+            jm = group_residuals(g, minus, T_GRAD, dts, K_GRAD).tolist()
+    for i in ONE_PER_SIZE:
+        fd = (jp[i] - jm[i]) / (2 * eps)
+        an = float((gs[i].conj() * vs[i]).real.sum())
+        rel = abs(fd - an) / abs(fd)
+        print(f"box {i} n={params[i]['n']} FP64: fd={fd:.12e} adjoint={an:.12e} rel {rel:.2e}", flush=True)
+        assert rel < 1e-6
+
+
+def test_64_fp64_adjoint_descent_50_iterations(gpu_lib):
+    from dnsbox_amd import Box
+    from dnsbox_amd.adjoint import adjoint_descent
+    with Box(n=64, precision=64, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=0.005) as b:
+        b.init_random(0, 0, 4.0, 0.5)
+        u0 = b.state_tensor()
+        u, hist = adjoint_descent(b, u0, 200, 0.005, k=20, iters=50)
+    print(f"64^3 FP64 adjoint descent: J0 = {hist[0]:.6e} -> J50 = {hist[-1]:.6e} "
+          f"({len(hist) - 1} iterations)", flush=True)
+    assert len(hist) == 51
+    assert all(b_ <= a_ for a_, b_ in zip(hist, hist[1:]))
+    assert hist[-1] < hist[0]
