@@ -158,6 +158,25 @@ class Box:
         check(self.lib.dnsbox_get_state_physical(self._h, *[_dp(a) for a in out]))
         return out
 
+    def save_state(self, path):
+        """io_save_state (io.f90:100-278): the physical velocity to an HDF5 state file (datasets
+        u, v, w, FP64, Fortran shape (Nx, Ny, Nz))."""
+        from . import state_io
+        state_io.write_state(path, *self.get_state_physical())
+
+    def load_state(self, path, dealias: bool = True, project: bool = True):
+        """io_load_state (io.f90:455-626) followed, as the reference program does
+        (dnsboxheun.f90:48-55), by state_u2uhat, state_dealias and state_project."""
+        from . import state_io
+        dims = state_io.state_dims(path)
+        if dims != (self.n, self.n, self.n):
+            raise ValueError(f"{path} holds a {dims} grid, the box is {self.n}^3")
+        self.set_state_physical(*state_io.read_state(path))
+        if dealias:
+            self.state_dealias()
+        if project:
+            self.state_project()
+
     def init_random(self, seed: int, box: int = 0, k0: float = 4.0, energy: float = 0.5):
         check(self.lib.dnsbox_init_random(self._h, seed, box, k0, energy))
 

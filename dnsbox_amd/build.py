@@ -39,6 +39,32 @@ def _compile(src: str) -> str:
     return obj
 
 
+HDF5_ROOT = os.environ.get("DNSBOX_HDF5_ROOT", "/opt/conda")
+H5_LIB = os.path.join(LIBDIR, "libdnsbox_h5.so")
+
+
+def build_h5(verbose: bool = False):
+    """libdnsbox_h5.so (state files, host code) against the HDF5 C library, when one is present."""
+    inc, libdir = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(libdir, "libhdf5.so"))):
+        if verbose:
+            print(f"no HDF5 C library under {HDF5_ROOT}: libdnsbox_h5.so not built", file=sys.stderr)
+        return None
+    tmp = H5_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + inc, "-I" + os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "state_h5.cpp"), "-o", tmp, "-L" + libdir, "-lhdf5", "-Wl,-rpath," + libdir,
+           # the C++ runtime linked in: the HDF5 directory's own libstdc++ must not be the one a
+           # process loads (torch's libraries need a newer one)
+           "-static-libstdc++", "-static-libgcc"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for state_h5.cpp:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, H5_LIB)
+    if verbose:
+        print("built", H5_LIB, file=sys.stderr)
+    return H5_LIB
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -52,6 +78,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB, file=sys.stderr)
+    build_h5(verbose)
     return LIB
 
 

@@ -7,10 +7,10 @@ reproduces the driver around the hot path (dnsboxheun.f90:18-189):
     time, Ekin, Disp, 2 Q Ekin, Courant, divMax, EZero, Eband as 8e20.12), specNNNN.dat
     (io_save_spectrum, io.f90:680-702: kSpec, Espec as 2e20.12, appended), in the
     reference's formats so that python/analysis.py reads a run made here;
-  * the state at t = 0 and every iSaveRate1 steps.  The reference writes HDF5
-    (stateNNNN.h5, datasets u, v, w); h5py is not available here, so the state is written as
-    stateNNNN.npz with the same datasets (physical, FP64, (Nz, Ny, Nx) = Fortran (Nx, Ny, Nz))
-    and the time;
+  * the state at t = 0 and every iSaveRate1 steps as stateNNNN.h5 (io_save_state,
+    io.f90:100-278: datasets u, v, w, the physical velocity, FP64, Fortran (Nx, Ny, Nz)), through
+    dnsbox_amd.state_io; a run can start from such a file (initial_state, as io_load_state at
+    dnsboxheun.f90:48);
   * the RUNNING sentinel: created at the start, the run stops cleanly when it is deleted;
   * the step-size policy: with tstep_fix the run aborts when the Courant number exceeds
     courant_max (saving the final spectrum and state); otherwise set_time_step
@@ -59,7 +59,9 @@ def write_info(path, box, isave1, isave2, dt, courant, initrand=True):
     lines = [f"Nx,y,z = {n:4d}{n:4d}{n:4d}",
              f"iSaveRate1 = {isave1:6d}",
              f"iSaveRate2 = {isave2:6d}",
-             "alpha_x,y,z = " + fortran_f(1.0) * 3,
+             "alpha_x,y,z = " + "".join(fortran_f(float(np.float32(a or 1.0)))
+                                        for a in (getattr(p, "alpha_x", 1.0), getattr(p, "alpha_y", 1.0),
+                                                  getattr(p, "alpha_z", 1.0))),
              "dt = " + fortran_f(dt),
              "nu = " + fortran_f(p.nu)]
     if p.bandlim:
@@ -103,9 +105,7 @@ class Run:
                 f.write(fortran_e(nk * dk) + fortran_e(e) + "\n")
 
     def save_state(self):
-        u, v, w = self.box.get_state_physical()
-        t, _ = self.box.time()
-        np.savez(self.path(f"state{self.save_count1:04d}.npz"), u=u, v=v, w=w, time=t)
+        self.box.save_state(self.path(f"state{self.save_count1:04d}.h5"))
         self.save_count1 += 1
 
     def close(self):
@@ -122,10 +122,14 @@ def set_time_step(box, courant, cmin, cmax, tstep_max):
 
 
 def run(box, outdir, nt, isave1=1000, isave2=10, tstep_fix=True, courant_min=0.15, courant_max=0.2,
-        tstep_max=0.01, policy="reference", running_file=True, initrand=True, log=None):
-    """The time loop of dnsboxheun.f90 on `box` (its state and dt already set); returns the
-    number of steps taken and the reason the run stopped."""
+        tstep_max=0.01, policy="reference", running_file=True, initrand=True, log=None, initial_state=None):
+    """The time loop of dnsboxheun.f90 on `box` (its state and dt already set, or the state read
+    from the state file `initial_state`, then dealiased and projected as dnsboxheun.f90:48-55
+    does); returns the number of steps taken and the reason the run stopped."""
     log = log or (lambda *a: None)
+    if initial_state is not None:
+        box.load_state(initial_state)
+        initrand = False
     r = Run(box, outdir)
     _, dt = box.time()
     # dnsboxheun.f90:18-60: io_saveInfo before the first statistics (Courant not yet

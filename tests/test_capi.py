@@ -9,15 +9,29 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    names = []
-    for hdr in sorted(os.listdir(os.path.join(ROOT, "include"))):
-        if not hdr.endswith(".h"):
-            continue
-        text = open(os.path.join(ROOT, "include", hdr)).read()
-        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dnsbox_\w+)\s*\(", text, flags=re.M)
+# header -> the library that exports its functions
+HEADERS = {"dnsbox.h": "libdnsbox_hip.so", "dnsbox_io.h": "libdnsbox_h5.so"}
+
+
+def declared_functions(header="dnsbox.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dnsbox_\w+)\s*\(", text, flags=re.M)
     return sorted(set(names))
+
+
+def test_every_header_maps_to_a_library():
+    assert sorted(h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h")) == sorted(HEADERS)
+
+
+def test_state_file_library_exports_its_header():
+    path = os.path.join(ROOT, "dnsbox_amd", "lib", HEADERS["dnsbox_io.h"])
+    if not os.path.exists(path):
+        pytest.skip("libdnsbox_h5.so was not built (no HDF5 C library on this host)")
+    from dnsbox_amd import state_io
+    lib = state_io.load()
+    names = declared_functions("dnsbox_io.h")
+    assert len(names) == 4 and not [n for n in names if not hasattr(lib, n)]
 
 
 def test_header_declares_entry_points():

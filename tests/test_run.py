@@ -60,13 +60,27 @@ def test_driver_files_match_the_oracle(gpu_lib, tmp_path):
     # saves at n = 0, 20, 40 and, as in the reference, once more when the run ends at n = 40
     specs = sorted(p.name for p in tmp_path.glob("spec*.dat"))
     assert specs == ["spec0000.dat", "spec0001.dat", "spec0002.dat", "spec0003.dat"]
-    assert len(list(tmp_path.glob("state*.npz"))) == 4
+    assert len(list(tmp_path.glob("state*.h5"))) == 4
     s2 = np.loadtxt(tmp_path / "spec0002.dat")
-    _, osp = o.diagnostics(o.heun(f0, 40, dt), dt)
+    f40 = o.heun(f0, 40, dt)
+    _, osp = o.diagnostics(f40, dt)
     assert np.allclose(s2[:, 1], osp, rtol=1e-10, atol=1e-14)
     assert np.array_equal(np.loadtxt(tmp_path / "spec0003.dat"), s2)
-    st = np.load(tmp_path / "state0002.npz")
-    assert st["u"].shape == (n, n, n) and abs(float(st["time"]) - 0.2) < 1e-12
+    # the state file holds the physical velocity of the oracle's state at t = 0.2
+    from dnsbox_amd.state_io import read_state
+    st = read_state(tmp_path / "state0002.h5")
+    ref = [o.c2r(a) / n ** 3 for a in f40]
+    assert st[0].shape == (n, n, n)
+    assert max(np.abs(a - b).max() for a, b in zip(st, ref)) < 1e-10 * max(np.abs(b).max() for b in ref)
+    # a run started from it (io_load_state, dnsboxheun.f90:48) continues the same trajectory
+    with Box(n=n, precision=64, nu=nu, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        b.rhs_int_fact(dt)
+        run(b, str(tmp_path / "resume"), 10, isave1=10, isave2=10, tstep_fix=True, courant_max=10.0,
+            initial_state=str(tmp_path / "state0002.h5"))
+        got = b.get_state()
+    want = o.heun(o.project(o.dealias([o.r2c(a) for a in st])), 10, dt)
+    err = np.sqrt(sum(np.sum(np.abs(g - w_) ** 2) for g, w_ in zip(got, want)) / sum(np.sum(np.abs(w_) ** 2) for w_ in want))
+    assert err < 1e-10, err
     assert (tmp_path / "info.dat").read_text().splitlines()[5].split()[2] == "0.050000000"
 
 
