@@ -16,6 +16,7 @@
 
 #include "adjoint.hpp"
 #include "box.hpp"
+#include "xpass_reg.hpp"
 
 namespace dnsbox {
 
@@ -42,6 +43,12 @@ struct BoxConfig {
 };
 
 constexpr int kNPass = 6;  // passes reported by time_passes (DNSBOX_PASS_* order)
+
+#ifndef DNSBOX_LEGACY_XPASS
+#define DNSBOX_LEGACY_XPASS 0
+#endif
+// 1: the cross-product x pass uses the LDS-staged kernel (k_xpass) at every size
+constexpr bool kLegacyXPass = DNSBOX_LEGACY_XPASS != 0;
 
 // twiddle table tw[j] = e^{-2 pi i j / N} in the transform type, from long-double angles
 template <int N, class T>
@@ -316,8 +323,13 @@ class BoxT final : public BoxBase {
                            twc());
         DNSBOX_LAUNCH_CHECK();
         mark(1);
-        hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>, PR>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
-                           stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
+        if constexpr (XReg<N, T>::enabled && !kLegacyXPass) {
+            hipLaunchKernelGGL((xreg::k_xcross_r<N, T, PR>), dim3(N * N / xreg::kLinesPerBlock), dim3(xreg::kThreads),
+                               0, stream, X(), X(), fs, twc(), track_cfl ? sc->maxu : nullptr);
+        } else {
+            hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>, PR>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
+                               stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
+        }
         DNSBOX_LAUNCH_CHECK();
         mark(2);
         hipLaunchKernelGGL((k_ycol<N, T, -1, PR>), dim3(ytiles<N, T, PR>(), 3), dim3(YNT), 0, stream, X(), fs,
