@@ -236,8 +236,12 @@ def adjoint_rate(reps=3, descent_iters=50):
         _, hist = adjoint_descent(b, u0, T, dt, k=k, iters=descent_iters)
         torch.cuda.synchronize()
         t_desc = time.perf_counter() - t0
+        from dnsbox_amd.adjoint import _segment_length
+        seg = _segment_length(T, k, u0.numel() * u0.element_size(), u0.device, "auto")
+    backward = (f"trajectory recorded by the forward ({2 * (T - 1) * u0.numel() * u0.element_size() / 2**30:.2f} GiB)"
+                if seg == T else f"checkpoint every {k} steps, segments recomputed on a second stream")
     return {"grads_per_s": 1.0 / secs, "ms_per_grad": secs * 1e3, "n": n, "T": T, "dt": dt,
-            "checkpoint_every": k, "dtype": "f64", "J": float(J.detach()),
+            "checkpoint_every": k, "backward": backward, "dtype": "f64", "J": float(J.detach()),
             "fd_rel_err": abs(fd - an) / abs(fd),
             "descent": {"iterations": len(hist) - 1, "J0": hist[0], "J_final": hist[-1],
                         "monotone": all(b_ <= a_ for a_, b_ in zip(hist, hist[1:])), "seconds": t_desc},

@@ -1,11 +1,14 @@
 """Differentiable T-step map and adjoint descent on periodic-orbit residuals.
 
 ``step_map(box, u0, T, dt, k)`` returns u_T = Phi_T(u0) for T fixed-dt IF-Heun steps
-(dnsboxheun.f90:84-126) as a ``torch.autograd.Function``.  The forward keeps a
-checkpoint every k steps; the backward recomputes each segment from its checkpoint
-and runs the hand-written step-adjoint kernels (``dnsbox_step_vjp_dev``) backwards
-through it.  Gradients follow torch's convention for complex tensors
-(dL/dRe + i dL/dIm), i.e. the Euclidean adjoint in the stored coordinates.
+(dnsboxheun.f90:84-126) as a ``torch.autograd.Function``.  When a gradient is needed the
+forward records the trajectory (every state and every step's predicted state) in device
+memory if its 2 (T - 1) states fit in a quarter of the free device memory (record="auto"),
+and the backward runs the hand-written step-adjoint kernels (``dnsbox_step_vjp_dev``)
+backwards through it with no recomputation; otherwise (or with record="checkpoint") the
+forward keeps a checkpoint every k steps and the backward recomputes each segment from its
+checkpoint.  Gradients follow torch's convention for complex tensors (dL/dRe + i dL/dIm),
+i.e. the Euclidean adjoint in the stored coordinates.
 
 ``residual`` is J(u) = 1/2 <|Phi_T(u) - u|^2> (the mean-square physical residual, in
 FP64), ``adjoint_descent`` minimises it with an FP64 backtracking line search along
@@ -45,10 +48,12 @@ def _require_fixed_dt(boxes):
 
 class _StepMap(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0, box, T, dt, k):
+    def forward(ctx, u0, box, T, dt, k, record):
         _require_fixed_dt([box])
         box.rhs_int_fact(dt)
         u = u0.detach().clone().contiguous()
+        if ctx.needs_input_grad[0]:
+            k = _segment_length(T, k, u.numel() * u.element_size(), u.device, record)
         ckpts = []
         cube = True
         ctx.last = None
@@ -136,7 +141,27 @@ class _StepMap(torch.autograd.Function):
                     recompute(seg - 1)
         finally:
             box.set_vjp_cube(False)
-        return lam, None, None, None, None
+        return lam, None, None, None, None, None
+
+
+RECORD_FRACTION = 0.25   # share of the free device memory a recorded trajectory may take
+
+
+def _segment_length(T, k, state_bytes, device, record):
+    """Steps per segment of the forward: T (the whole trajectory recorded, no recomputation in
+    the backward) for record="all", or for "auto" when the 2 (T - 1) recorded states of
+    state_bytes each fit in RECORD_FRACTION of the free device memory; else the checkpoint
+    interval k."""
+    if record not in ("auto", "all", "checkpoint"):
+        raise ValueError(f"record must be 'auto', 'all' or 'checkpoint', not {record!r}")
+    if k < 1:
+        raise ValueError("the checkpoint interval k must be >= 1")
+    if record == "checkpoint" or T <= 1:
+        return k
+    if record == "all":
+        return T
+    free, _ = torch.cuda.mem_get_info(device)
+    return T if 2 * (T - 1) * state_bytes <= RECORD_FRACTION * free else k
 
 
 def _copy_all(dst, src):
@@ -147,11 +172,13 @@ class _GroupStepMap(torch.autograd.Function):
     """Phi_T of every box of a BoxGroup at once (one launch per pass for all boxes)."""
 
     @staticmethod
-    def forward(ctx, group, T, dts, k, *u0s):
+    def forward(ctx, group, T, dts, k, record, *u0s):
         _require_fixed_dt(group.boxes)
         for b, dt in zip(group.boxes, dts):
             b.rhs_int_fact(dt)
         us = [u.detach().clone().contiguous() for u in u0s]
+        if any(ctx.needs_input_grad[5:]):
+            k = _segment_length(T, k, sum(u.numel() * u.element_size() for u in us), us[0].device, record)
         ckpts = []
         cube = True
         ctx.last = None
@@ -161,7 +188,7 @@ class _GroupStepMap(torch.autograd.Function):
             _copy_all(c, us)
             ckpts.append(c)
             m = min(k, T - t)
-            if t == starts[-1] and m > 1 and any(ctx.needs_input_grad[4:]):
+            if t == starts[-1] and m > 1 and any(ctx.needs_input_grad[5:]):
                 # the last segment is recorded now (as in _StepMap.forward)
                 w = max(k - 1, 1)
                 ctx.last = [(torch.empty((w,) + tuple(u.shape), dtype=u.dtype, device=u.device),
@@ -241,31 +268,32 @@ class _GroupStepMap(torch.autograd.Function):
                     recompute(seg - 1)
         finally:
             group.set_vjp_cube(False)
-        return (None, None, None, None, *lams)
+        return (None, None, None, None, None, *lams)
 
 
-def group_step_map(group, u0s, T: int, dts, k: int = 20):
-    """[Phi_T(u0_i)] for every box i of a BoxGroup, differentiable, checkpointed every k."""
-    return _GroupStepMap.apply(group, T, list(dts), k, *u0s)
+def group_step_map(group, u0s, T: int, dts, k: int = 20, record: str = "auto"):
+    """[Phi_T(u0_i)] for every box i of a BoxGroup, differentiable; the trajectory is recorded
+    or checkpointed every k as in step_map."""
+    return _GroupStepMap.apply(group, T, list(dts), k, record, *u0s)
 
 
-def group_residuals(group, u0s, T, dts, k=20):
+def group_residuals(group, u0s, T, dts, k=20, record="auto"):
     """[J_i] = [1/2 <|Phi_T(u0_i) - u0_i|^2>] for every box (FP64 scalars, one tensor)."""
-    uTs = group_step_map(group, u0s, T, dts, k)
+    uTs = group_step_map(group, u0s, T, dts, k, record)
     return torch.stack([0.5 * mean_square(uT - u0, b.n)
                         for uT, u0, b in zip(uTs, u0s, group.boxes)])
 
 
-def group_residual_grads(group, u0s, T, dts, k=20):
+def group_residual_grads(group, u0s, T, dts, k=20, record="auto"):
     """(J, [dJ_i/du0_i]): the residuals and their Euclidean gradients, all boxes at once."""
     xs = [u.detach().clone().requires_grad_(True) for u in u0s]
-    J = group_residuals(group, xs, T, dts, k)
+    J = group_residuals(group, xs, T, dts, k, record)
     gs = torch.autograd.grad(J.sum(), xs)
     return J.detach(), list(gs)
 
 
 def group_adjoint_descent(group, u0s, T, dts, k=20, iters=10, step0=1.0, c1=1e-4, shrink=0.5,
-                          max_backtrack=20, callback=None):
+                          max_backtrack=20, callback=None, record="auto"):
     """adjoint_descent for every box of a group at once: one batched gradient per iteration,
     an FP64 Armijo backtracking line search per box (each box keeps its own step length;
     trial residuals of all boxes are evaluated in one batched forward).  Returns (us, hist)
@@ -276,7 +304,7 @@ def group_adjoint_descent(group, u0s, T, dts, k=20, iters=10, step0=1.0, c1=1e-4
     alpha = [step0] * nb
     hist = []
     for it in range(iters):
-        J, gs = group_residual_grads(group, us, T, dts, k)
+        J, gs = group_residual_grads(group, us, T, dts, k, record)
         J0 = J.tolist()
         hist.append(J0)
         if callback:
@@ -295,7 +323,7 @@ def group_adjoint_descent(group, u0s, T, dts, k=20, iters=10, step0=1.0, c1=1e-4
             for i in pending:
                 trial[i] = us[i] + alpha[i] * ds[i]
             with torch.no_grad():
-                Jt = group_residuals(group, trial, T, dts, k).tolist()
+                Jt = group_residuals(group, trial, T, dts, k, record).tolist()
             still = []
             for i in pending:
                 if Jt[i] <= J0[i] + c1 * alpha[i] * slopes[i]:
@@ -307,13 +335,15 @@ def group_adjoint_descent(group, u0s, T, dts, k=20, iters=10, step0=1.0, c1=1e-4
                     still.append(i)
             pending = still
     with torch.no_grad():
-        hist.append(group_residuals(group, us, T, dts, k).tolist())
+        hist.append(group_residuals(group, us, T, dts, k, record).tolist())
     return us, hist
 
 
-def step_map(box, u0, T: int, dt: float, k: int = 20):
-    """Phi_T(u0): T fixed-dt IF-Heun steps, differentiable, checkpointed every k steps."""
-    return _StepMap.apply(u0, box, T, dt, k)
+def step_map(box, u0, T: int, dt: float, k: int = 20, record: str = "auto"):
+    """Phi_T(u0): T fixed-dt IF-Heun steps, differentiable.  record="auto" records the whole
+    trajectory for the backward when it fits in RECORD_FRACTION of the free device memory and
+    checkpoints every k steps otherwise; "all" always records, "checkpoint" always checkpoints."""
+    return _StepMap.apply(u0, box, T, dt, k, record)
 
 
 def mean_square(r, n):
@@ -323,9 +353,9 @@ def mean_square(r, n):
     return (w * (rr.real ** 2 + rr.imag ** 2)).sum() / float(n) ** 6
 
 
-def residual(box, u0, T, dt, k=20):
+def residual(box, u0, T, dt, k=20, record="auto"):
     """J(u0) = 1/2 <|Phi_T(u0) - u0|^2>."""
-    uT = step_map(box, u0, T, dt, k)
+    uT = step_map(box, u0, T, dt, k, record)
     return 0.5 * mean_square(uT - u0, box.n)
 
 
@@ -347,7 +377,7 @@ def project_masked(g, n, parity=True):
 
 
 def adjoint_descent(box, u0, T, dt, k=20, iters=50, step0=1.0, c1=1e-4, shrink=0.5,
-                    max_backtrack=20, callback=None):
+                    max_backtrack=20, callback=None, record="auto"):
     """Minimise J(u) = 1/2 <|Phi_T(u) - u|^2> from u0 by gradient descent in the L2 metric
     (W^-1 times the Euclidean gradient, projected onto retained solenoidal fields) with an
     Armijo backtracking line search in FP64.  Returns (u, history of J)."""
@@ -358,7 +388,7 @@ def adjoint_descent(box, u0, T, dt, k=20, iters=50, step0=1.0, c1=1e-4, shrink=0
     alpha = step0
     for it in range(iters):
         x = u.clone().requires_grad_(True)
-        J = residual(box, x, T, dt, k)
+        J = residual(box, x, T, dt, k, record)
         (g,) = torch.autograd.grad(J, x)
         # J = 1/2 sum W |r|^2 / N^6 in the metric <a, b>_M = sum W Re(conj a b) / N^6, whose
         # gradient is N^6 W^-1 times the Euclidean one; descend along its projection
@@ -373,7 +403,7 @@ def adjoint_descent(box, u0, T, dt, k=20, iters=50, step0=1.0, c1=1e-4, shrink=0
         for _ in range(max_backtrack):
             trial = u + alpha * d
             with torch.no_grad():
-                Jt = float(residual(box, trial, T, dt, k))
+                Jt = float(residual(box, trial, T, dt, k, record))
             if Jt <= J0 + c1 * alpha * slope:
                 u = trial
                 accepted = True
@@ -383,5 +413,5 @@ def adjoint_descent(box, u0, T, dt, k=20, iters=50, step0=1.0, c1=1e-4, shrink=0
         if not accepted:
             break
     with torch.no_grad():
-        hist.append(float(residual(box, u, T, dt, k)))
+        hist.append(float(residual(box, u, T, dt, k, record)))
     return u, hist

@@ -257,8 +257,9 @@ class Box:
 
     def set_vjp_cube(self, confined: bool):
         """Guarantee (True) that the states the step / RHS adjoints linearise about lie in the kept
-        cube, so that their first passes read the cube only; a state outside it then gives a wrong
-        adjoint.  Default False."""
+        cube, so that their first passes read the cube only.  The guarantee is checked on the
+        device: withdrawing it (False) raises DnsboxError if a state outside the cube was
+        differentiated in between (those adjoints are wrong).  Default False."""
         check(self.lib.dnsbox_set_vjp_cube(self._h, int(bool(confined))))
 
     def state_ptr(self) -> int:

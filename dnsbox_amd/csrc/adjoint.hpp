@@ -117,6 +117,41 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     zinv_tile<N, T, Pro, T, PR>(in.p, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
 }
 
+// k_zinv_p over the kept columns (PR) in its first `tiles` blocks; the blocks after them check
+// the three fields at chk (field stride chk_fs) and set *viol when one holds a nonzero mode
+// outside the kept cube.  The cube-confined VJP runs its first pass with this guard on its
+// linearisation state; the flag is read back when the caller withdraws the guarantee.
+template <int N, class T, class Pro>
+__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+    k_zinv_p_chk(Ptrs<T, Pro::FI> in, cpx<T>* __restrict__ out0, long long out_fs,
+                 const cpx<T>* __restrict__ tw_g, Pro pro, const cpx<T>* __restrict__ chk, long long chk_fs,
+                 unsigned int* viol, int tiles)
+{
+    using G = Geo<N, T>;
+    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> stw_[N];
+    const int b = blockIdx.x;
+    if (b >= tiles) {   // a check block (the whole block returns here)
+        const long long nthr = (long long)(gridDim.x - tiles) * blockDim.x;
+        const bool nz = outside_cube_nonzero<N, T>(chk, chk_fs, (long long)(b - tiles) * blockDim.x + threadIdx.x, nthr);
+        if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(viol, 1u);
+        return;
+    }
+    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    cpx<T>* out[Pro::FO];
+#pragma unroll
+    for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
+    zinv_tile<N, T, Pro, T, true>(in.p, out, xcd_contiguous(b, tiles), lds, tw, pro);
+}
+
+// blocks of the confinement check fused into the first pass of a cube-confined VJP
+template <int N>
+constexpr int vjp_check_blocks()
+{
+    constexpr long long tot = (long long)N * N * (N / 2 + 1);
+    return tot / 256 + 1 < 256 ? int(tot / 256 + 1) : 256;
+}
+
 // ---------------------------------------------------------------------------
 // physical-space ops of the x pass
 // ---------------------------------------------------------------------------

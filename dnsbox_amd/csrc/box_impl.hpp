@@ -136,7 +136,25 @@ class BoxBase {
     void* adjbuf = nullptr;  // 9 fields: A, Mu, u~ (adjoint passes), lazily
     void* descbuf = nullptr; // 9 fields: G(u), u*, G(u*) of the adjoint-descent step, lazily
     double* band2 = nullptr; // band-forcing scalars of the VJP
-    unsigned int* chk = nullptr;   // flag of k_outside_cube
+    unsigned int* chk = nullptr;   // [0]: flag of k_outside_cube; [1]: cube-confined VJP guard
+
+    // dnsbox_set_vjp_cube: setting the guarantee clears the device flag that the cube-confined
+    // VJPs raise for a linearisation state with energy outside the kept cube; withdrawing it
+    // reads the flag back (synchronises the stream) and returns whether such a state was seen
+    bool set_vjp_cube(bool on)
+    {
+        if (on) {
+            if (!vjp_cube) DNSBOX_HIP(hipMemsetAsync(chk + 1, 0, sizeof(unsigned int), stream));
+            vjp_cube = true;
+            return false;
+        }
+        if (!vjp_cube) return false;
+        vjp_cube = false;
+        unsigned int h = 0;
+        DNSBOX_HIP(hipMemcpyAsync(&h, chk + 1, sizeof(h), hipMemcpyDeviceToHost, stream));
+        DNSBOX_HIP(hipStreamSynchronize(stream));
+        return h != 0;
+    }
 
     virtual void rhs_state() = 0;                 // nonlin <- N(state)
     virtual void rhs(const void* in, void* out) = 0;
@@ -226,7 +244,7 @@ class BoxT final : public BoxBase {
             DNSBOX_HIP(hipMalloc(&red_partial, size_t(red_blocks) * (DG_NSCAL + nspec) * sizeof(double)));
             DNSBOX_HIP(hipMalloc(&red_out, (DG_NSCAL + nspec) * sizeof(double)));
             DNSBOX_HIP(hipMalloc(&red_bits, 4 * sizeof(unsigned long long)));
-            DNSBOX_HIP(hipMalloc(&chk, sizeof(unsigned int)));
+            DNSBOX_HIP(hipMalloc(&chk, 2 * sizeof(unsigned int)));
             DNSBOX_HIP(hipMemset(state, 0, 3 * size_t(fs) * sizeof(CS)));
             DNSBOX_HIP(hipMemset(pred, 0, 3 * fb));
             DNSBOX_HIP(hipMemset(nonlin, 0, 3 * fb));
@@ -645,7 +663,8 @@ class BoxT final : public BoxBase {
     // VJP of N at u with cotangent lam_scale * lam; epilogue MODE of k_zfwd_vjp.  With vjp_cube
     // set (u confined to the kept cube, as the caller guarantees) the passes up to the x pass's
     // loads read the kept cube only (the cotangent enters masked); the products, and so the rest,
-    // fill the whole spectrum.
+    // fill the whole spectrum.  The first pass then also checks u outside the cube on otherwise
+    // idle blocks (k_zinv_p_chk), raising the flag read back by set_vjp_cube(false).
     template <int MODE>
     void vjp_passes(const C* u, const C* lam, double lam_scale, VjpArgs<T> va)
     {
@@ -663,8 +682,13 @@ class BoxT final : public BoxBase {
         }
         Ptrs<T, 6> in{{lam, lam + fs, lam + 2 * fs, u, u + fs, u + 2 * fs}};
         ProVjpIn<T> pro{T(sm), T(lam_scale), N, cfg.ph};
-        hipLaunchKernelGGL((k_zinv_p<N, T, ProVjpIn<T>, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream, in,
-                           X(), fs, twc(), pro);
+        if constexpr (PR) {
+            hipLaunchKernelGGL((k_zinv_p_chk<N, T, ProVjpIn<T>>), dim3(ztiles<N, T, true>() + vjp_check_blocks<N>()),
+                               dim3(ZNT), 0, stream, in, X(), fs, twc(), pro, u, fs, chk + 1, ztiles<N, T, true>());
+        } else {
+            hipLaunchKernelGGL((k_zinv_p<N, T, ProVjpIn<T>>), dim3(ztiles<N, T, false>()), dim3(ZNT), 0, stream, in,
+                               X(), fs, twc(), pro);
+        }
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<N, T, +1, PR>), dim3(ytiles<N, T, PR>(), 9), dim3(YNT), 0, stream, X(), fs,
                            twc());

@@ -527,8 +527,12 @@ int dnsbox_set_pruning(dnsbox_ctx* c, int enable)
 int dnsbox_set_vjp_cube(dnsbox_ctx* c, int confined)
 {
     if (int e = check_ctx(c)) return e;
-    c->box->vjp_cube = confined != 0;
-    return 0;
+    return guarded([&]() -> int {
+        if (c->box->set_vjp_cube(confined != 0))
+            return fail(DNSBOX_E_STATE, "a state differentiated under dnsbox_set_vjp_cube(ctx, 1) had energy outside "
+                                        "the kept cube: the adjoints since the guarantee was set are wrong");
+        return 0;
+    });
 }
 
 int dnsbox_get_last_step_pruned(dnsbox_ctx* c, int* pruned)
@@ -796,8 +800,12 @@ int dnsbox_group_step_vjp_pred_dev(dnsbox_group* g, const void* const* u, const 
 int dnsbox_group_set_vjp_cube(dnsbox_group* g, int confined)
 {
     if (!g) return fail(DNSBOX_E_ARG, "null group");
-    g->g->vjp_cube = confined != 0;
-    return 0;
+    return guarded([&]() -> int {
+        if (g->g->set_vjp_cube(confined != 0))
+            return fail(DNSBOX_E_STATE, "a state differentiated under dnsbox_group_set_vjp_cube(g, 1) had energy "
+                                        "outside its kept cube: the adjoints since the guarantee was set are wrong");
+        return 0;
+    });
 }
 
 int dnsbox_group_get_last_step_pruned(dnsbox_group* g, int* pruned)

@@ -230,15 +230,31 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_predonly(const GBox<T>
 }
 
 // z-inverse of the VJP prologue: STAGE 1 at u* with cotangent dt/2 lam, STAGE 2 at u
-// with cotangent Mu; PR: the kept columns only, inputs read at kept kz (states in the cube)
+// with cotangent Mu; PR: the kept columns only, inputs read at kept kz (states in the cube),
+// and kGroupCheck blocks per box after the `tiles` tile blocks check the box's state outside
+// its cube (*viol |= 1 on a nonzero mode there)
+constexpr int kGroupCheck = 16;
 template <class T, int STAGE, bool PR = false>
-__global__ void __launch_bounds__(kGroupThreads) k_g_zinv_vjp(const GBox<T>* __restrict__ gb, int nbox)
+__global__ void __launch_bounds__(kGroupThreads)
+    k_g_zinv_vjp(const GBox<T>* __restrict__ gb, int nbox, int tiles, unsigned int* viol)
 {
     __shared__ cpx<T> lds[GroupLds<T>::Z];
     __shared__ cpx<T> stw[128];
     // no XCD remapping across a group: round-robin dispatch spreads every box's tiles over
     // all XCDs, which balances boxes of uneven cost
     const int gt = blockIdx.x;
+    if (PR && gt >= tiles) {   // a check block (the whole block returns here)
+        const int cb = gt - tiles, b = cb / kGroupCheck, part = cb % kGroupCheck;
+        const GBox<T>& bc = gb[b];
+        const cpx<T>* p = STAGE == 1 ? bc.pred : bc.u;
+        bool nz = false;
+#define DNSBOX_G_OC(NN) nz = outside_cube_nonzero<NN, T>(p, bc.fs, part * (long long)blockDim.x + threadIdx.x, \
+                                                         (long long)kGroupCheck * blockDim.x)
+        DNSBOX_GROUP_SWITCH(bc.n, DNSBOX_G_OC)
+#undef DNSBOX_G_OC
+        if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(viol, 1u);
+        return;
+    }
     const int b = find_box(gb, nbox, gt, PR ? 3 : 0);
     const GBox<T> bx = gb[b];
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
@@ -358,6 +374,22 @@ class GroupBase {
     std::vector<std::unique_ptr<BoxBase>> boxes;
     hipStream_t stream = nullptr;
     bool vjp_cube = false;            // the caller guarantees the VJPs' states lie in the kept cubes
+    // the group form of BoxBase::set_vjp_cube (the flag is the first box's, on the group stream)
+    bool set_vjp_cube(bool on)
+    {
+        unsigned int* viol = boxes[0]->chk + 1;
+        if (on) {
+            if (!vjp_cube) DNSBOX_HIP(hipMemsetAsync(viol, 0, sizeof(unsigned int), stream));
+            vjp_cube = true;
+            return false;
+        }
+        if (!vjp_cube) return false;
+        vjp_cube = false;
+        unsigned int h = 0;
+        DNSBOX_HIP(hipMemcpyAsync(&h, viol, sizeof(h), hipMemcpyDeviceToHost, stream));
+        DNSBOX_HIP(hipStreamSynchronize(stream));
+        return h != 0;
+    }
     bool last_ext_pruned = false;     // the last steps of caller buffers ran the pruned passes
     virtual void step(int nsteps) = 0;
     virtual void step_ext(void* const* states, int nsteps) = 0;
@@ -692,7 +724,8 @@ class Group final : public GroupBase {
     }
 
     // the VJP passes; with vjp_cube (every box's linearisation state confined to its kept cube,
-    // as the caller guarantees) those up to the x pass's loads read the kept cube only
+    // as the caller guarantees) those up to the x pass's loads read the kept cube only, and the
+    // first pass checks the states outside their cubes on kGroupCheck extra blocks per box
     template <int MODE>
     void vjp_passes(hipStream_t s, const GBox<T>* d)
     {
@@ -705,7 +738,8 @@ class Group final : public GroupBase {
     void vjp_passes_t(hipStream_t s, const GBox<T>* d)
     {
         const int nb = int(h_gb.size());
-        hipLaunchKernelGGL((k_g_zinv_vjp<T, MODE, PR>), dim3(PR ? ztp : zt), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zinv_vjp<T, MODE, PR>), dim3(PR ? ztp + nb * kGroupCheck : zt), dim3(kGroupThreads), 0,
+                           s, d, nb, PR ? ztp : zt, boxes[0]->chk + 1);
         hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(PR ? ytp : yt, 9), dim3(kGroupThreads), 0, s, d, nb);
         hipLaunchKernelGGL((k_g_xpass<T, OpVjp<T>, PR, false>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
         hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);

@@ -169,9 +169,13 @@ int dnsbox_get_pruning(dnsbox_ctx* ctx, int* active);
    caller's guarantee that the states the *_vjp_* entry points linearise about (u, and u* where
    given) have no energy outside the kept cube; their passes up to the x pass then read the cube
    only (the cotangent enters masked), and the results agree with the full passes to rounding.
-   A state outside the cube with the flag set gives a wrong adjoint: set it only for states made
-   by pruned steps, which get_last_step_pruned reports for the last dnsbox_step_dev(_record)
-   call (the device check of its buffer).  Default off.  No reference counterpart. */
+   The guarantee is checked on the device: the first pass of every such VJP also scans its
+   state outside the cube (on blocks the pruned pass leaves idle) and raises a flag, and
+   set_vjp_cube(ctx, 0) reads the flag back (synchronising the context's stream) and returns
+   DNSBOX_E_STATE if a state outside the cube was differentiated since the guarantee was set:
+   the adjoints computed in between are then wrong.  States made by pruned steps, which
+   get_last_step_pruned reports for the last dnsbox_step_dev(_record) call, satisfy it.
+   Default off.  No reference counterpart. */
 int dnsbox_set_vjp_cube(dnsbox_ctx* ctx, int confined);
 int dnsbox_get_last_step_pruned(dnsbox_ctx* ctx, int* pruned);
 

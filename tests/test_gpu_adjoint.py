@@ -193,10 +193,11 @@ def test_step_dev_prunes_only_dealiased_buffers(gpu_lib):
         assert torch.equal(u, traj[3])
 
 
-@pytest.mark.parametrize("n,T,k", [(32, 20, 5), (64, 200, 20)])
-def test_tstep_gradient_vs_finite_differences(gpu_lib, n, T, k):
+@pytest.mark.parametrize("n,T,k,record", [(32, 20, 5, "checkpoint"), (64, 200, 20, "auto")])
+def test_tstep_gradient_vs_finite_differences(gpu_lib, n, T, k, record):
     """d/de J(u + e v) by central differences (e = 1e-6) along seeded solenoidal directions
-    vs <grad J, v> from the checkpointed adjoint; tolerance 1e-6 relative."""
+    vs <grad J, v> from the adjoint (checkpointed, or with the trajectory recorded, which "auto"
+    picks at 64^3, T = 200); tolerance 1e-6 relative."""
     import torch
     from dnsbox_amd.adjoint import residual
     dt = 0.005
@@ -206,7 +207,7 @@ def test_tstep_gradient_vs_finite_differences(gpu_lib, n, T, k):
     with make(n, nu=nu, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
         u0 = dev(o.init_solenoidal(11, 0, 4.0, 0.5))
         x = u0.clone().requires_grad_(True)
-        J = residual(b, x, T, dt, k)
+        J = residual(b, x, T, dt, k, record)
         (g,) = torch.autograd.grad(J, x)
         worst = 0.0
         for d in range(ndir):
@@ -237,7 +238,7 @@ def test_overlapped_backward_equals_one_stream_loop(gpu_lib, n, T, k):
         u0 = dev(o.init_solenoidal(3, 0, 4.0, 0.5))
         g = dev(rand_spec(o, 4))
         x = u0.clone().requires_grad_(True)
-        (got,) = torch.autograd.grad(step_map(b, x, T, dt, k), x, g)
+        (got,) = torch.autograd.grad(step_map(b, x, T, dt, k, record="checkpoint"), x, g)
         # the same backward, one stream, no twin
         b.rhs_int_fact(dt)
         ckpts, u = [], u0.clone()
@@ -259,6 +260,66 @@ def test_overlapped_backward_equals_one_stream_loop(gpu_lib, n, T, k):
         err = rel_l2(host(got), host(lam))
         print(f"overlapped backward n={n} T={T} k={k}: rel_l2 vs one-stream loop {err:.3e}", flush=True)
         assert err <= 1e-14
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_recorded_trajectory_backward_equals_checkpointed(gpu_lib, n):
+    """The backward through the trajectory recorded by the forward (record="all": no segment is
+    recomputed) gives the checkpointed backward's gradient to rounding (the checkpointed path
+    restarts the step pipeline at every segment, with an unfused z-inverse from the stored state)."""
+    import torch
+    from dnsbox_amd.adjoint import step_map
+    dt, T, k = 0.005, 45, 10
+    o = Oracle(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4)
+    with make(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        u0 = dev(o.init_solenoidal(8, 0, 4.0, 0.5))
+        g = dev(rand_spec(o, 9))
+        grads = {}
+        for record in ("all", "checkpoint"):
+            x = u0.clone().requires_grad_(True)
+            (grads[record],) = torch.autograd.grad(step_map(b, x, T, dt, k, record=record), x, g)
+        torch.cuda.synchronize()
+    err = rel_l2(host(grads["all"]), host(grads["checkpoint"]))
+    print(f"recorded vs checkpointed backward n={n}: rel_l2 {err:.3e}", flush=True)
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_vjp_cube_guard_reports_a_state_outside_the_cube(gpu_lib, n):
+    """set_vjp_cube(True) is checked on the device: differentiating a state with energy outside
+    the kept cube under it (as u, or as a supplied u*) makes set_vjp_cube(False) raise; confined
+    states pass, and a new guarantee starts from a cleared flag."""
+    import torch
+    from dnsbox_amd._lib import DnsboxError
+    o = Oracle(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4)
+    dt = 0.005
+    with make(n, nu=0.02, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt) as b:
+        b.rhs_int_fact(dt)
+        u = dev(o.init_solenoidal(5, 0, 4.0, 0.5))
+        ustar = u.clone()
+        b.step_dev(u, 1)
+        b.step_dev(ustar, 1)
+        assert b.last_step_pruned()
+        lam = dev(rand_spec(o, 6))
+        w = dev(rand_spec(o, 7))   # energy on every mode
+        out = torch.empty_like(u)
+        b.set_vjp_cube(True)
+        b.step_vjp_dev(u, lam, out, dt)
+        b.step_vjp_pred_dev(u, ustar, lam, out, dt)
+        b.rhs_vjp_dev(u, lam, out)
+        b.set_vjp_cube(False)      # confined states: no error
+        for bad in ("u", "ustar"):
+            b.set_vjp_cube(True)
+            if bad == "u":
+                b.rhs_vjp_dev(w, lam, out)
+            else:
+                b.step_vjp_pred_dev(u, w, lam, out, dt)
+            with pytest.raises(DnsboxError, match="outside"):
+                b.set_vjp_cube(False)
+        b.set_vjp_cube(True)
+        b.step_vjp_dev(u, lam, out, dt)
+        b.set_vjp_cube(False)
+    print(f"vjp cube guard n={n}: confined states pass, u or u* outside the cube raise", flush=True)
 
 
 @pytest.mark.parametrize("n,prec", [(32, 64), (64, 64), (48, 32)])

@@ -177,7 +177,7 @@ def test_group_overlapped_backward_equals_one_stream_loop(gpu_lib):
     with BoxGroup([kw(n, nu, 64) for n, nu in zip(SIZES, NUS)]) as g:
         dts = [DT] * len(us)
         xs = [u.clone().requires_grad_(True) for u in us]
-        got = torch.autograd.grad(group_step_map(g, xs, T, dts, k), xs, gs)
+        got = torch.autograd.grad(group_step_map(g, xs, T, dts, k, record="checkpoint"), xs, gs)
         for b in g.boxes:
             b.rhs_int_fact(DT)
         ckpts, cur = [], [u.clone() for u in us]
@@ -226,3 +226,29 @@ def test_group_vjp_cube_matches_full_passes(gpu_lib, prec):
     worst = max(rel_l2(a, b) for a, b in zip(res[True], res[False]))
     print(f"group vjp cube prec={prec}: worst rel {worst:.2e}", flush=True)
     assert worst <= tol
+
+
+def test_group_vjp_cube_guard_reports_a_state_outside_the_cube(gpu_lib):
+    """The group guarantee is checked on the device like the single box's: one box's state with
+    energy outside its kept cube makes set_vjp_cube(False) raise; confined states pass."""
+    import torch
+    from dnsbox_amd import BoxGroup
+    from dnsbox_amd._lib import DnsboxError
+    us = fields(64, 30)
+    lams = rand_like(us, 31)
+    with BoxGroup([kw(n, nu, 64) for n, nu in zip(SIZES, NUS)]) as g:
+        for b in g.boxes:
+            b.rhs_int_fact(DT)
+        g.step_dev(us, 1)
+        assert g.last_step_pruned()
+        outs = [torch.empty_like(u) for u in us]
+        g.set_vjp_cube(True)
+        g.step_vjp_dev(us, lams, outs, [DT] * len(us))
+        g.set_vjp_cube(False)
+        bad = list(us)
+        bad[2] = rand_like([us[2]], 32)[0]   # box 2 (64^3): energy on every mode
+        g.set_vjp_cube(True)
+        g.step_vjp_dev(bad, lams, outs, [DT] * len(us))
+        with pytest.raises(DnsboxError, match="outside"):
+            g.set_vjp_cube(False)
+    print("group vjp cube guard: confined states pass, a box outside its cube raises", flush=True)
