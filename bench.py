@@ -431,8 +431,9 @@ def main():
                      "frac": kd["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom, workload),
                      "kernel": dom, "alg_bytes_per_launch": kd["alg_bytes"], "ms_per_launch": kd["ms"],
                      "copy_ceiling_gbs": copy_gbs,
-                     "note": ("the x pass is bound by VALU issue and LDS (about 56 M wave64 VALU and 7.0 M "
-                              "LDS instructions per launch at 256^3 FP32), not by HBM"
+                     "note": ("the x pass is bound by VALU issue and latency at 3 waves per SIMD (about 40 M "
+                              "wave64 VALU, i.e. 66 us of issue, and 4.3 M LDS instructions per launch at "
+                              "256^3 FP32), not by HBM"
                               if dom == "xcross" else None)},
         "kernels": {q: {"ms": round(v["ms"], 5), "gbs": round(v["gbs"], 1)} for q, v in kernels.items()},
         "large_box": large,
