@@ -109,6 +109,14 @@ module dnsbox_gpu
             type(c_ptr), value :: ctx
             integer(c_int), value :: nsteps, stepper
         end function
+        ! dnsboxAdj.f90:76-126: nsteps Heun steps in fictitious time dtau on rhsAdj, then state_project
+        integer(c_int) function dnsbox_adj_descent_step(ctx, dtau, nsteps, lin_sign) &
+                bind(C, name="dnsbox_adj_descent_step")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: dtau
+            integer(c_int), value :: nsteps, lin_sign
+        end function
         integer(c_int) function dnsbox_diagnostics(ctx, diag, espec) bind(C, name="dnsbox_diagnostics")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx
