@@ -391,11 +391,14 @@ def main():
     # ---- CPU baseline (oracle, restatement of the reference structure) ----
     cpu = None
     if args.cpu_baseline:
-        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        host_cpus = os.cpu_count() or 1
+        usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else host_cpus
+        threads = min(args.cpu_threads, usable)
         rate, secs_cpu, cpu_steps = cpu_baseline(n, threads)
         cpu = {"value": rate, "unit": "time-steps/s", "cores": threads, "kind": "port",
                "sample": f"{cpu_steps} IF-Heun steps of the FP64 CPU oracle at {n}^3 (15 transforms/RHS, "
-                         f"OpenMP), {secs_cpu:.1f} s"}
+                         f"OpenMP), {secs_cpu:.1f} s, on {threads} threads of a host with {host_cpus} "
+                         f"logical CPUs ({usable} usable by this process)"}
         log(f"cpu {rate:.4f} steps/s on {threads} threads")
 
     kd = kernels[dom]

@@ -342,7 +342,7 @@ class BoxT final : public BoxBase {
         DNSBOX_LAUNCH_CHECK();
         mark(1);
         if constexpr (XReg<N, T>::enabled && !kLegacyXPass) {
-            hipLaunchKernelGGL((xreg::k_xcross_r<N, T, PR>), dim3(N * N / xreg::kLinesPerBlock), dim3(xreg::kThreads),
+            hipLaunchKernelGGL((xreg::k_xcross_r<N, T, PR>), dim3(N * N / xreg::Shape<N>::LPB), dim3(xreg::kThreads),
                                0, stream, X(), X(), fs, twc(), track_cfl ? sc->maxu : nullptr);
         } else {
             hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>, PR>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,

@@ -168,6 +168,35 @@ def test_fp32_256_matches_oracle(gpu_lib, state_precision):
     assert err < 1e-6
 
 
+def test_fp32_512_matches_fp64_box(gpu_lib):
+    """512^3: the FP32 transforms (register-resident x pass with 32 lanes per x-line, pruned
+    passes) against the FP64 box (its x pass the LDS Stockham rows) on the same seeded field: the
+    RHS, then two fixed-dt IF-Heun steps with the FP64 state.  The FP64 passes are pinned against
+    the oracle at the sizes the oracle finishes in seconds."""
+    n, dt = 512, 0.001
+    kw = dict(nu=0.002, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=dt)
+    with make(n, precision=64, **kw) as b:
+        b.init_random(3, 0, 4.0, 0.5)
+        f = b.get_state()
+        ref_rhs = b.rhs_nonlinear()
+        b.rhs_int_fact(dt)
+        b.step(2)
+        ref = b.get_state()
+    with make(n, precision=32, state_precision=64, **kw) as b:
+        b.set_state(*f)
+        got = b.rhs_nonlinear()
+        err = rel_l2(got, ref_rhs)
+        print(f"fp32 512 rhs vs fp64 box rel_l2={err:.3e}", flush=True)
+        assert err < 1e-5
+        b.set_state(*f)
+        b.rhs_int_fact(dt)
+        b.step(2)
+        got = b.get_state()
+    err = rel_l2(got, ref)
+    print(f"fp32 512 (state 64) heun x2 vs fp64 box rel_l2={err:.3e}", flush=True)
+    assert err < 1e-6
+
+
 @pytest.mark.parametrize("n,prec,sp,band", [(16, 64, 0, 0), (32, 64, 0, 0), (32, 64, 0, 1), (48, 32, 0, 0),
                                              (64, 64, 0, 0), (128, 32, 64, 0), (256, 32, 64, 0), (256, 64, 0, 0),
                                              (512, 32, 64, 0)])
