@@ -16,6 +16,7 @@
 
 #include "adjoint.hpp"
 #include "box.hpp"
+#include "plane.hpp"
 #include "xpass_reg.hpp"
 
 namespace dnsbox {
@@ -49,6 +50,22 @@ constexpr int kNPass = 6;  // passes reported by time_passes (DNSBOX_PASS_* orde
 #endif
 // 1: the cross-product x pass uses the LDS-staged kernel (k_xpass) at every size
 constexpr bool kLegacyXPass = DNSBOX_LEGACY_XPASS != 0;
+
+// the fused plane pass (plane.hpp) for the sizes whose z-plane fits the LDS; DNSBOX_PLANE=0 in the
+// environment runs the three separate passes instead (read once per process)
+inline bool plane_enabled()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("DNSBOX_PLANE");
+        return !(e && std::strcmp(e, "0") == 0);
+    }();
+    return on;
+}
+template <int N, class T, class Op>
+inline bool use_plane()
+{
+    return PlaneGeo<N, T, Op>::used && plane_enabled();
+}
 
 // twiddle table tw[j] = e^{-2 pi i j / N} in the transform type, from long-double angles
 template <int N, class T>
@@ -337,6 +354,15 @@ class BoxT final : public BoxBase {
     template <bool PR = false>
     void rhs_middle(bool track_cfl)
     {
+        if constexpr (PlaneGeo<N, T, OpCross<T>>::used) {
+            if (plane_enabled()) {
+                hipLaunchKernelGGL((k_plane<N, T, OpCross<T>, PR, PR>), dim3(N), dim3(kPlaneThreads), 0, stream, X(),
+                                   fs, twc(), track_cfl ? sc->maxu : nullptr);
+                DNSBOX_LAUNCH_CHECK();
+                mark(2);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_ycol<N, T, +1, PR>), dim3(ytiles<N, T, PR>(), 6), dim3(YNT), 0, stream, X(), fs,
                            twc());
         DNSBOX_LAUNCH_CHECK();
@@ -690,14 +716,22 @@ class BoxT final : public BoxBase {
                                X(), fs, twc(), pro);
         }
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1, PR>), dim3(ytiles<N, T, PR>(), 9), dim3(YNT), 0, stream, X(), fs,
-                           twc());
-        DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_xpass<N, T, OpVjp<T>, PR, false>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream,
-                           X(), fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
-        DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
-        DNSBOX_LAUNCH_CHECK();
+        if (use_plane<N, T, OpVjp<T>>()) {
+            if constexpr (PlaneGeo<N, T, OpVjp<T>>::used) {
+                hipLaunchKernelGGL((k_plane<N, T, OpVjp<T>, PR, false>), dim3(N), dim3(kPlaneThreads), 0, stream, X(),
+                                   fs, twc(), (unsigned long long*)nullptr);
+                DNSBOX_LAUNCH_CHECK();
+            }
+        } else {
+            hipLaunchKernelGGL((k_ycol<N, T, +1, PR>), dim3(ytiles<N, T, PR>(), 9), dim3(YNT), 0, stream, X(), fs,
+                               twc());
+            DNSBOX_LAUNCH_CHECK();
+            hipLaunchKernelGGL((k_xpass<N, T, OpVjp<T>, PR, false>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
+                               stream, X(), fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+            DNSBOX_LAUNCH_CHECK();
+            hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
+            DNSBOX_LAUNCH_CHECK();
+        }
         va.u = u;
         va.lamN = lam;
         va.lam_scale = lam_scale;
@@ -925,6 +959,10 @@ class BoxT final : public BoxBase {
             b[3] = 3.0 * n * kx * (n + ky) * cb;                           // all y read, ky in Ky written
             b[4] = 9.0 * zc * n * cb + 6.0 * zc * ky * sb + pr;            // 3 + 6 fields all z, state over Kz
             b[5] = 9.0 * zc * n * cb + 6.0 * zc * ky * sb;
+            if (use_plane<N, T, OpCross<T>>()) {   // one launch for the three middle passes
+                b[1] = b[3] = 0.0;
+                b[2] = plane_bytes<N, T, OpCross<T>, true, true>();
+            }
         } else {
             const double f = double(G::FS);
             b[0] = f * (6.0 * cb + 3.0 * sb);
@@ -933,6 +971,10 @@ class BoxT final : public BoxBase {
             b[3] = f * 6.0 * cb;
             b[4] = f * (9.0 * cb + 6.0 * sb) + (cfg.ph.bandlim ? f * 3.0 * cb : 0.0);
             b[5] = f * (9.0 * cb + 6.0 * sb);
+            if (use_plane<N, T, OpCross<T>>()) {
+                b[1] = b[3] = 0.0;
+                b[2] = plane_bytes<N, T, OpCross<T>, false, false>();
+            }
         }
     }
 
