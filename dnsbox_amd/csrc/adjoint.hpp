@@ -108,7 +108,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
              const cpx<T>* __restrict__ tw_g, Pro pro)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> lds[G::ZLDS];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     cpx<T>* out[Pro::FO];
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
                  unsigned int* viol, int tiles)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> lds[G::ZLDS];
     __shared__ cpx<T> stw_[N];
     const int b = blockIdx.x;
     if (b >= tiles) {   // a check block (the whole block returns here)
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     k_zfwd_vjp(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, VjpArgs<T> va)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> lds[G::ZLDS];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     zfwd_vjp_tile<N, T, MODE>(X, xfs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, va);
@@ -370,7 +370,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
     using CF = typename G::CFZ_F;
     constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
     constexpr long long ZS = (long long)N * H;
-    __shared__ cpx<T> lds[N * W];
+    __shared__ cpx<T> lds[G::ZLDS];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     const int col = xcd_contiguous(blockIdx.x, gridDim.x) * W + threadIdx.x % W;

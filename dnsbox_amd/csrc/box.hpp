@@ -261,6 +261,15 @@ template <int N, class T>
 struct Cfg : CfgDefault<N, T> {};
 #endif
 
+// Column transforms of the z passes taken ZIL at a time, stage by stage (ColFFT::run_multi),
+// where the z passes are bound by the latency of their barrier chains rather than by
+// bandwidth: the 64^3 FP64 box (the adjoint workload; fewer z tiles than CUs).  ZIL LDS tiles
+// per block.
+template <int N, class T>
+struct ZInterleave {
+    static constexpr int value = (N == 64 && std::is_same<T, double>::value) ? 6 : 1;
+};
+
 template <int N, class T>
 struct Geo {
     using C = Cfg<N, T>;
@@ -270,6 +279,8 @@ struct Geo {
     static constexpr long long FS = (long long)N * N * H; // elements per spectral field
     static constexpr int ZNT = CFZ_I::NT;               // threads of a z pass
     static constexpr int ZRM = CFZ_I::RM;
+    static constexpr int ZIL = ZInterleave<N, T>::value;
+    static constexpr int ZLDS = N * C::ZW * ZIL;        // LDS (complex elements) of a z pass
     static constexpr int YNT = ColFFT<typename C::YP, C::YW, C::YW, +1, T>::NT;
     static constexpr int ZCOLS = N * H;                 // z-pass columns (ky, kx)
     static constexpr int YCOLS = N * H;                 // y-pass columns (kz, kx)
@@ -342,6 +353,42 @@ struct ProScale {  // (u, v, w) -> s (u, v, w)               state_uhat2u
     }
 };
 
+// outputs OB .. OB + K - 1 of zinv_from_regs transformed together (K = min(ZIL, FO - OB)), then
+// the next group
+template <int N, class T, class Pro, int OB>
+__device__ __forceinline__ void zinv_groups(const cpx<T> (&U)[Pro::FI][Geo<N, T>::ZRM],
+                                            cpx<T>* const* __restrict__ out, int col, bool valid, cpx<T>* lds,
+                                            const cpx<T>* __restrict__ tw, const Pro& pro, int c, int g, int kx,
+                                            int ky, int ix, int iy)
+{
+    using G = Geo<N, T>;
+    using CF = typename G::CFZ_I;
+    constexpr int RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)N * H;
+    if constexpr (OB < Pro::FO) {
+        constexpr int K = (Pro::FO - OB) < G::ZIL ? (Pro::FO - OB) : G::ZIL;
+        cpx<T> a[K][RM];
+#pragma unroll
+        for (int o = 0; o < K; ++o)
+#pragma unroll
+            for (int q = 0; q < RM; ++q) {
+                cpx<T> v[Pro::FI];
+#pragma unroll
+                for (int f = 0; f < Pro::FI; ++f) v[f] = U[f][q];
+                const int iz = CF::in_pos(g, q);
+                a[o][q] = pro(OB + o, v, ModeK{kx, ky, kval(iz, N), ix, iy, iz});
+            }
+        CF::template run_multi<K>(a, lds, c, g, tw, N);
+        if (valid) {
+#pragma unroll
+            for (int o = 0; o < K; ++o)
+#pragma unroll
+                for (int q = 0; q < RM; ++q) out[OB + o][CF::out_pos(g, q) * ZS + col] = a[o][q];
+        }
+        zinv_groups<N, T, Pro, OB + K>(U, out, col, valid, lds, tw, pro, c, g, kx, ky, ix, iy);
+    }
+}
+
 // z-inverse of the FO prologue outputs, starting from the FI inputs held in
 // registers at the stage-0 positions of the z plan: U[f][q] = in_f(kz = in_pos(g, q)).
 template <int N, class T, class Pro>
@@ -357,6 +404,10 @@ __device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<N,
     const int c = tid % W, g = tid / W;
     const int ix = col % H, iy = col / H;
     const int kx = kval(ix, N), ky = kval(iy, N);
+    if constexpr (G::ZIL > 1) {
+        zinv_groups<N, T, Pro, 0>(U, out, col, valid, lds, tw, pro, c, g, kx, ky, ix, iy);
+        return;
+    }
 #pragma unroll
     for (int o = 0; o < Pro::FO; ++o) {
         cpx<T> a[RM];
@@ -430,7 +481,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
            long long out_fs, const cpx<T>* __restrict__ tw_g, Pro pro)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> lds[G::ZLDS];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     const cpx<TI>* in[Pro::FI];
@@ -926,6 +977,19 @@ __device__ __forceinline__ double step_dt(BoxScalars* sc, const BoxPhys& ph)
     return dt;
 }
 
+// fields FB .. FB + K - 1 of zfwd_regs transformed together (K = min(ZIL, F - FB)), then the next group
+template <int N, class T, int F, int FB>
+__device__ __forceinline__ void zfwd_groups(cpx<T>* lds, int c, int g, const cpx<T>* __restrict__ tw,
+                                            cpx<T> (&O)[F][Geo<N, T>::ZRM])
+{
+    using G = Geo<N, T>;
+    if constexpr (FB < F) {
+        constexpr int K = (F - FB) < G::ZIL ? (F - FB) : G::ZIL;
+        G::CFZ_F::template run_multi<K>(O + FB, lds, c, g, tw, N);
+        zfwd_groups<N, T, F, FB + K>(lds, c, g, tw, O);
+    }
+}
+
 // forward z FFT of F fields of a column tile; O[f][q] holds mode kz = out_pos(g, q)
 template <int N, class T, int F>
 __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long long in_fs, int col,
@@ -945,8 +1009,12 @@ __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long l
 #pragma unroll
         for (int q = 0; q < RM; ++q) O[f][q] = valid ? src[CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
     }
+    if constexpr (G::ZIL > 1) {
+        zfwd_groups<N, T, F, 0>(lds, c, g, tw, O);
+    } else {
 #pragma unroll
-    for (int f = 0; f < F; ++f) CF::run(O[f], lds, c, g, tw, N);
+        for (int f = 0; f < F; ++f) CF::run(O[f], lds, c, g, tw, N);
+    }
 }
 
 // Stepper epilogue on the forward-transformed n_hat = FFT((omega x u)/N^3) of a column
@@ -1106,7 +1174,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
            EpiArgs<T, S> ea)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> lds[G::ZLDS];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     zfwd_tile<N, T, MODE, S>(in0, in_fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea);
@@ -1176,7 +1244,7 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
                 ProCurl<T> pro)
 {
     using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::ZW];
+    __shared__ cpx<T> lds[G::ZLDS];
     __shared__ cpx<T> stw_[N];
     const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
     zfwd_zinv_tile<N, T, MODE, S, PR>(X, fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea, pro);

@@ -321,6 +321,62 @@ struct ColFFT {
     {
         stage<0>(a, lds, c, g, tw, twn);
     }
+
+    // F independent column transforms taken stage by stage together (one pair of barriers per
+    // stage for all F instead of one per transform): the same arithmetic per transform as run(),
+    // with the exchanges of transform f in the LDS tile lds + f L LDW
+    template <int F, int t>
+    __device__ __forceinline__ static void stage_multi(cpx<T> (*a)[RM], cpx<T>* lds, int c, int g,
+                                                       const cpx<T>* __restrict__ tw, int twn)
+    {
+        if constexpr (t < P::S) {
+            constexpr int R = P::radix(t);
+            constexpr int p = P::pre(t);
+            constexpr int BPT = RM / R;
+            if constexpr (t > 0) {
+#pragma unroll
+                for (int f = 0; f < F; ++f)
+#pragma unroll
+                    for (int j = 0; j < BPT; ++j) {
+                        const int i = g + j * G;
+#pragma unroll
+                        for (int r = 0; r < R; ++r) a[f][j * R + r] = lds[f * L * LDW + (i + r * (L / R)) * LDW + c];
+                    }
+                __syncthreads();
+#pragma unroll
+                for (int f = 0; f < F; ++f)
+#pragma unroll
+                    for (int j = 0; j < BPT; ++j) {
+                        const int i = g + j * G;
+                        twiddle<R, DIR, T>(a[f] + j * R, i % p, twn / (p * R), tw);
+                    }
+            }
+#pragma unroll
+            for (int f = 0; f < F; ++f)
+#pragma unroll
+                for (int j = 0; j < BPT; ++j) Dft<R, DIR, T>::run(a[f] + j * R);
+            if constexpr (t + 1 < P::S) {
+#pragma unroll
+                for (int f = 0; f < F; ++f)
+#pragma unroll
+                    for (int j = 0; j < BPT; ++j) {
+                        const int i = g + j * G;
+                        const int k = i % p;
+                        const int base = (i - k) * R + k;
+#pragma unroll
+                        for (int r = 0; r < R; ++r) lds[f * L * LDW + (base + r * p) * LDW + c] = a[f][j * R + r];
+                    }
+                __syncthreads();
+            }
+            stage_multi<F, t + 1>(a, lds, c, g, tw, twn);
+        }
+    }
+    template <int F>
+    __device__ __forceinline__ static void run_multi(cpx<T> (*a)[RM], cpx<T>* lds, int c, int g,
+                                                     const cpx<T>* __restrict__ tw, int twn)
+    {
+        stage_multi<F, 0>(a, lds, c, g, tw, twn);
+    }
 };
 
 // ---------------------------------------------------------------------------

@@ -58,7 +58,7 @@ constexpr bool group_size_ok()
 template <class T>
 struct GroupLds {
     template <int N>
-    static constexpr int z() { return N * Cfg<N, T>::ZW; }
+    static constexpr int z() { return Geo<N, T>::ZLDS; }
     template <int N>
     static constexpr int y() { return N * Cfg<N, T>::YW; }
     template <int N, class Op>
