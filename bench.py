@@ -270,6 +270,12 @@ def batched_rates(steps=500, T=100, k=20, reps=3, warm=2):
         g.step(steps)
         g.sync()
         fwd = len(sizes) * steps / (time.perf_counter() - t0)
+        # algorithmic HBM bytes of one group step: every box's middle passes twice and its two
+        # fused z passes (the pipeline-start z-inverse aside), as each box's next step runs them
+        step_bytes = 0.0
+        for b in g.boxes:
+            pb = b.pass_bytes()
+            step_bytes += 2 * (pb["yinv6"] + pb["xcross"] + pb["yfwd3"]) + pb["zpred_zinv"] + pb["zcorr_zinv"]
         dts = [dt] * len(sizes)
         for _ in range(warm):   # the first gradients create the twin group and the descriptor sets
             group_residual_grads(g, us, T, dts, k)
@@ -279,8 +285,14 @@ def batched_rates(steps=500, T=100, k=20, reps=3, warm=2):
             group_residual_grads(g, us, T, dts, k)
         torch.cuda.synchronize()
         secs = (time.perf_counter() - t0) / reps
+    group_steps_per_s = fwd / len(sizes)
+    achieved = step_bytes * group_steps_per_s / 1e9
     return {"boxes": len(sizes), "sizes": "16x32^3 + 16x48^3 + 16x64^3", "dtype": "f32",
-            "forward_box_steps_per_s": fwd, "fwd_adj_box_steps_per_s": len(sizes) * T / secs,
+            "forward_box_steps_per_s": fwd,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_group_step": step_bytes,
+                         "note": "whole group step (all passes of the 48 boxes), algorithmic bytes / step time"},
+            "fwd_adj_box_steps_per_s": len(sizes) * T / secs,
             "ms_per_batched_gradient": secs * 1e3, "T": T, "checkpoint_every": k, "dt": dt}
 
 

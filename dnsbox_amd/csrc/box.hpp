@@ -807,7 +807,7 @@ struct XPass {
             RowFFT<typename Cfg<N, T>::RP, FO * RB, HP, NTX, -1, T, X::XPD>::run(lds, tid, tw, N);
             // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]), k <= M
             if constexpr (FIXED_K && (FO * RB * M) % NTX == 0) {
-                // k < M in the fixed-k layout, then k = M (one entry per line) by the first FO RB threads
+                // k < M in the fixed-k layout, then k = M (one entry per line and field)
                 constexpr long long FS = (long long)N * N * H;
                 const int k = tid % M;
                 const bool keep = !PO || k <= Keep<N>::KMAX;
@@ -826,8 +826,8 @@ struct XPass {
                         dst[((FPQ * q) / RB) * FS + ((FPQ * q) % RB) * H] = a + mul_i<-1>(b);
                     }
                 }
-                if (tid < FO * RB) {
-                    const int f = tid / RB, r = tid % RB;
+                for (int i = tid; i < FO * RB; i += NTX) {   // FO RB may exceed the block (16^3 FP32 VJP)
+                    const int f = i / RB, r = i % RB;
                     const cpx<T> z0 = lds[(f * RB + r) * HP + X::px(0)];
                     const cpx<T> zm = conj(z0);
                     const cpx<T> a = T(0.5) * (z0 + zm);

@@ -10,7 +10,6 @@
 // captured once in a hipGraph and replayed.
 #pragma once
 
-#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -43,7 +42,6 @@ struct GBox {
     cpx<T>* A;           // E (lam + lam_*)
     cpx<T>* Mu;          // E (dt/2 lam + dt lam_*)
     double dt;
-    int pf, pv;          // the box runs the plane pass (plane.hpp) for the forward cross product / the VJP
 };
 
 constexpr int kGroupThreads = 256;
@@ -72,29 +70,6 @@ struct GroupLds {
                       group_size_ok<64, T>() && group_size_ok<128, T>(),
                   "group sizes must share one block shape");
 };
-
-// LDS of the group plane pass: the largest plane among the sizes that run it
-template <class T, class Op>
-struct GroupPlaneLds {
-    template <int N>
-    static constexpr int l() { return PlaneGeo<N, T, Op>::used ? PlaneGeo<N, T, Op>::LDS : 1; }
-    static constexpr int mx(int a, int b) { return a > b ? a : b; }
-    static constexpr int L = mx(mx(l<16>(), l<32>()), mx(l<48>(), l<64>()));
-};
-
-// whether a box of size n runs the plane pass of Op
-template <class T, class Op>
-inline bool group_plane_ok(int n)
-{
-    if (!plane_enabled()) return false;
-    switch (n) {
-        case 16: return PlaneGeo<16, T, Op>::used;
-        case 32: return PlaneGeo<32, T, Op>::used;
-        case 48: return PlaneGeo<48, T, Op>::used;
-        case 64: return PlaneGeo<64, T, Op>::used;
-        default: return false;
-    }
-}
 
 // box owning global block `blk`, by the first-block offsets selected by `which`
 // (0 z tiles, 1 y tiles, 2 x blocks, 3 pruned z tiles, 4 pruned y tiles)
@@ -157,16 +132,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restr
 #undef DNSBOX_G_ZINV
 }
 
-// skip: 1 the boxes that run the forward plane pass, 2 those that run the VJP plane pass (their
-// blocks return at once), 0 none
-template <class T>
-__device__ __forceinline__ bool plane_skip(const GBox<T>& bx, int skip)
-{
-    return skip == 1 ? bx.pf != 0 : (skip == 2 ? bx.pv != 0 : false);
-}
-
 template <class T, int DIR, bool PR = false>
-__global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restrict__ gb, int nbox, int skip)
+__global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restrict__ gb, int nbox)
 {
     __shared__ cpx<T> lds[GroupLds<T>::Y];
     __shared__ cpx<T> stw[128];
@@ -174,7 +141,6 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restr
     const int field = L / gridDim.x, gt = L % gridDim.x;
     const int b = find_box(gb, nbox, gt, PR ? 4 : 1);
     const GBox<T> bx = gb[b];
-    if (plane_skip(bx, skip)) return;   // the whole block
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
     const int tile = gt - (PR ? bx.yt0p : bx.yt0);
     cpx<T>* data = bx.X + field * bx.fs;
@@ -185,15 +151,13 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restr
 
 // x pass of Op in place on the scratch; OpCross can track the CFL maxima
 template <class T, class Op, bool PR = false, bool PO = PR>
-__global__ void __launch_bounds__(kGroupThreads)
-    k_g_xpass(const GBox<T>* __restrict__ gb, int nbox, int track, int skip)
+__global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __restrict__ gb, int nbox, int track)
 {
     __shared__ cpx<T> lds[GroupLds<T>::template X<Op>];
     __shared__ cpx<T> stw[128];
     __shared__ double wsm[4 * 16];
     const int b = find_box(gb, nbox, blockIdx.x, 2);
     const GBox<T> bx = gb[b];
-    if (plane_skip(bx, skip)) return;   // the whole block
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
     const int blk = blockIdx.x - bx.xb0;
     XRed<T> red;
@@ -202,36 +166,6 @@ __global__ void __launch_bounds__(kGroupThreads)
 #define DNSBOX_G_X(NN) xpass_tile<NN, T, Op, PR, PO>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_X)
 #undef DNSBOX_G_X
-    if constexpr (Op::NRED == 3) {
-        if (track) block_max_atomic<3>(red, bx.sc->maxu, false, wsm);
-    }
-}
-
-// the plane pass (plane.hpp) of Op over the planes listed in tab (box << 8 | plane index, the
-// planes of the largest boxes first so that the shortest blocks fill the tail)
-template <class T, class Op, bool PR, bool PO>
-__global__ void __launch_bounds__(kPlaneThreads)
-    k_g_plane(const GBox<T>* __restrict__ gb, const int* __restrict__ tab, int track)
-{
-    __shared__ cpx<T> lds[GroupPlaneLds<T, Op>::L];
-    __shared__ cpx<T> stw[64];
-    __shared__ double wsm[4 * 16];
-    const int code = tab[blockIdx.x];
-    const GBox<T> bx = gb[code >> 8];
-    const int iz = code & 255;
-    const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
-    XRed<T> red;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) red.m[j] = T(0);
-#define DNSBOX_G_PL(NN)                                                                \
-    if constexpr (PlaneGeo<NN, T, Op>::used) plane_tile<NN, T, Op, PR, PO>(bx.X, bx.fs, iz, lds, tw, red)
-    switch (bx.n) {
-        case 16: DNSBOX_G_PL(16); break;
-        case 32: DNSBOX_G_PL(32); break;
-        case 48: DNSBOX_G_PL(48); break;
-        default: DNSBOX_G_PL(64); break;
-    }
-#undef DNSBOX_G_PL
     if constexpr (Op::NRED == 3) {
         if (track) block_max_atomic<3>(red, bx.sc->maxu, false, wsm);
     }
@@ -475,11 +409,6 @@ class Group final : public GroupBase {
     std::vector<GBox<T>> h_gb;
     int zt = 0, yt = 0, xb = 0;
     int ztp = 0, ytp = 0;             // tiles of the pruned passes
-    // plane pass: the plane tables (box << 8 | plane) of the boxes that run it, forward and VJP
-    int* d_ptab_f = nullptr;
-    int* d_ptab_v = nullptr;
-    int npl_f = 0, npl_v = 0;
-    bool all_pf = true, all_pv = true;
     bool x6_valid = false;
     bool x6_pruned = false;           // the scratch was written by the pruned passes
     int graph_pruned = -1;            // mode the graph was captured in
@@ -507,8 +436,6 @@ class Group final : public GroupBase {
             g.xb0 = xb;
             g.zt0p = ztp;
             g.yt0p = ytp;
-            g.pf = group_plane_ok<T, OpCross<T>>(g.n) ? 1 : 0;
-            g.pv = group_plane_ok<T, OpVjp<T>>(g.n) ? 1 : 0;
             zt += group_ztiles(g.n, prec);
             yt += group_ytiles(g.n, prec);
             xb += group_xblocks(g.n, prec);
@@ -519,30 +446,7 @@ class Group final : public GroupBase {
         }
         DNSBOX_HIP(hipMalloc(&d_gb, h_gb.size() * sizeof(GBox<T>)));
         DNSBOX_HIP(hipMemcpy(d_gb, h_gb.data(), h_gb.size() * sizeof(GBox<T>), hipMemcpyHostToDevice));
-        d_ptab_f = plane_table(true, npl_f, all_pf);
-        d_ptab_v = plane_table(false, npl_v, all_pv);
         DNSBOX_HIP(hipStreamCreate(&gstream));
-    }
-
-    // the planes of the boxes that run the plane pass (forward or VJP), largest boxes first
-    int* plane_table(bool fwd, int& count, bool& all)
-    {
-        std::vector<int> order;
-        all = true;
-        for (size_t i = 0; i < h_gb.size(); ++i) {
-            if (fwd ? h_gb[i].pf : h_gb[i].pv) order.push_back(int(i));
-            else all = false;
-        }
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h_gb[a].n > h_gb[b].n; });
-        std::vector<int> tab;
-        for (int i : order)
-            for (int z = 0; z < h_gb[i].n; ++z) tab.push_back((i << 8) | z);
-        count = int(tab.size());
-        if (tab.empty()) return nullptr;
-        int* d = nullptr;
-        DNSBOX_HIP(hipMalloc(&d, tab.size() * sizeof(int)));
-        DNSBOX_HIP(hipMemcpy(d, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
-        return d;
     }
     ~Group() override
     {
@@ -554,8 +458,6 @@ class Group final : public GroupBase {
         (void)hipDeviceSynchronize();
         for (auto& kv : dcache) (void)hipFree(kv.second);
         if (d_gb) (void)hipFree(d_gb);
-        if (d_ptab_f) (void)hipFree(d_ptab_f);
-        if (d_ptab_v) (void)hipFree(d_ptab_v);
     }
 
     static void fill(GBox<T>& g, const BoxBase& b)
@@ -623,33 +525,19 @@ class Group final : public GroupBase {
         DNSBOX_LAUNCH_CHECK();
     }
 
-    // middle passes of the cross-product RHS of every box: the plane pass for the boxes that run
-    // it, the y-inverse, x and y-forward passes for the others
-    template <bool PR>
-    void middle_fwd(hipStream_t s, const GBox<T>* d, int track)
-    {
-        const int nb = int(h_gb.size());
-        if (npl_f > 0)
-            hipLaunchKernelGGL((k_g_plane<T, OpCross<T>, PR, PR>), dim3(npl_f), dim3(kPlaneThreads), 0, s, d, d_ptab_f,
-                               track);
-        if (!all_pf) {
-            const int y = PR ? ytp : yt;
-            hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(y, 6), dim3(kGroupThreads), 0, s, d, nb, 1);
-            hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>, PR>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, track, 1);
-            hipLaunchKernelGGL((k_g_ycol<T, -1, PR>), dim3(y, 3), dim3(kGroupThreads), 0, s, d, nb, 1);
-        }
-        DNSBOX_LAUNCH_CHECK();
-    }
-
     // one IF-Heun step of every box, its z-inverse + curl already in the scratch
     template <bool PR = false>
     void one_step(hipStream_t s, const GBox<T>* d)
     {
         const int nb = int(h_gb.size());
-        const int z = PR ? ztp : zt;
-        middle_fwd<PR>(s, d, 1);
+        const int z = PR ? ztp : zt, y = PR ? ytp : yt;
+        hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(y, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>, PR>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 1);
+        hipLaunchKernelGGL((k_g_ycol<T, -1, PR>), dim3(y, 3), dim3(kGroupThreads), 0, s, d, nb);
         hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_PRED, PR>), dim3(z), dim3(kGroupThreads), 0, s, d, nb);
-        middle_fwd<PR>(s, d, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(y, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>, PR>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, -1, PR>), dim3(y, 3), dim3(kGroupThreads), 0, s, d, nb);
         hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_CORR, PR>), dim3(z), dim3(kGroupThreads), 0, s, d, nb);
         DNSBOX_LAUNCH_CHECK();
     }
@@ -805,7 +693,9 @@ class Group final : public GroupBase {
         x6_valid = false;
         // u* = E (u + dt N(u))
         launch_zinv(s, d, 1);
-        middle_fwd<false>(s, d, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, +1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpCross<T>>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 3), dim3(kGroupThreads), 0, s, d, nb);
         hipLaunchKernelGGL((k_g_zfwd_predonly<T>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
         DNSBOX_LAUNCH_CHECK();
         // lam_* = J(u*)^T (dt/2 lam); A = E (lam + lam_*), Mu = E (dt/2 lam + dt lam_*)
@@ -850,14 +740,9 @@ class Group final : public GroupBase {
         const int nb = int(h_gb.size());
         hipLaunchKernelGGL((k_g_zinv_vjp<T, MODE, PR>), dim3(PR ? ztp + nb * kGroupCheck : zt), dim3(kGroupThreads), 0,
                            s, d, nb, PR ? ztp : zt, boxes[0]->chk + 1);
-        if (npl_v > 0)
-            hipLaunchKernelGGL((k_g_plane<T, OpVjp<T>, PR, false>), dim3(npl_v), dim3(kPlaneThreads), 0, s, d, d_ptab_v,
-                               0);
-        if (!all_pv) {
-            hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(PR ? ytp : yt, 9), dim3(kGroupThreads), 0, s, d, nb, 2);
-            hipLaunchKernelGGL((k_g_xpass<T, OpVjp<T>, PR, false>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0, 2);
-            hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb, 2);
-        }
+        hipLaunchKernelGGL((k_g_ycol<T, +1, PR>), dim3(PR ? ytp : yt, 9), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_xpass<T, OpVjp<T>, PR, false>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
         hipLaunchKernelGGL((k_g_zfwd_vjp<T, MODE>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
         DNSBOX_LAUNCH_CHECK();
     }
