@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--unpruned-steps", type=int, default=300,
                     help="timed steps of the headline form with unpruned transforms (0 = skip)")
     ap.add_argument("--adjoint", type=int, default=1, help="time adjoint gradients (1/0)")
+    ap.add_argument("--descent-iters", type=int, default=50,
+                    help="adjoint-descent iterations run after the gradient timing (0 = skip)")
     ap.add_argument("--batched", type=int, default=1, help="time the 48-box group (1/0)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (1/0)")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -232,10 +234,12 @@ def adjoint_rate(reps=3, descent_iters=50):
         fd = (jp - jm) / (2 * eps)
         an = float((g.conj() * v).real.sum())
         # the configured descent: 50 iterations with the FP64 backtracking line search
-        t0 = time.perf_counter()
-        _, hist = adjoint_descent(b, u0, T, dt, k=k, iters=descent_iters)
-        torch.cuda.synchronize()
-        t_desc = time.perf_counter() - t0
+        hist, t_desc = [float(J.detach())], 0.0
+        if descent_iters > 0:
+            t0 = time.perf_counter()
+            _, hist = adjoint_descent(b, u0, T, dt, k=k, iters=descent_iters)
+            torch.cuda.synchronize()
+            t_desc = time.perf_counter() - t0
         from dnsbox_amd.adjoint import _segment_length
         seg = _segment_length(T, k, u0.numel() * u0.element_size(), u0.device, "auto")
     backward = (f"trajectory recorded by the forward ({2 * (T - 1) * u0.numel() * u0.element_size() / 2**30:.2f} GiB)"
@@ -392,7 +396,7 @@ def main():
         log(f"{args.large_n}^3 (nu 0.002): fp32 (fp64 state) {large['steps_per_s']:.1f} steps/s, fp64 "
             f"{large['fp64_steps_per_s']:.1f}; vs fp64 {large.get('err_vs_fp64')}")
 
-    adj = adjoint_rate() if args.adjoint else None
+    adj = adjoint_rate(descent_iters=args.descent_iters) if args.adjoint else None
     if adj:
         log(f"adjoint {adj['grads_per_s']:.2f} grads/s, fd rel err {adj['fd_rel_err']:.2e}")
     bat = batched_rates() if args.batched else None
