@@ -17,20 +17,30 @@ reference's equilibrium adjoint descent (dnsboxAdj.f90:76-126 with rhsAdj.f90).
 """
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 
-def _k_tables(n, device):
-    k = torch.arange(n, device=device, dtype=torch.float64)
-    k = torch.where(k < n // 2, k, torch.where(k == n // 2, torch.zeros_like(k), k - n))
-    kz = k.view(n, 1, 1)
-    ky = k.view(1, n, 1)
-    kx = k[: n // 2 + 1].view(1, 1, n // 2 + 1)
+def _alphas(box):
+    """alpha_x, alpha_y, alpha_z of a box as the library holds them (single precision, 0 = 1)."""
+    p = box.params
+    return tuple(float(np.float32(a)) if a > 0 else 1.0 for a in (p.alpha_x, p.alpha_y, p.alpha_z))
+
+
+def _k_tables(box, device):
+    """The scaled table wavenumbers (Nyquist 0) of the internal layout (nz, ny, nx/2+1)."""
+    def table(n, a):
+        k = torch.arange(n, device=device, dtype=torch.float64)
+        return torch.where(k < n // 2, k, torch.where(k == n // 2, torch.zeros_like(k), k - n)) * a
+    ax, ay, az = _alphas(box)
+    kz = table(box.nz, az).view(box.nz, 1, 1)
+    ky = table(box.ny, ay).view(1, box.ny, 1)
+    kx = table(box.nx, ax)[: box.nx // 2 + 1].view(1, 1, box.nx // 2 + 1)
     return kx, ky, kz
 
 
 def hermitian_weight(n, device):
-    """1 on the kx = 0 and kx = N/2 planes, 2 elsewhere (c2r counts those twice)."""
+    """1 on the kx = 0 and kx = NX/2 planes, 2 elsewhere (c2r counts those twice); n = NX."""
     w = torch.full((1, 1, n // 2 + 1), 2.0, dtype=torch.float64, device=device)
     w[..., 0] = 1.0
     w[..., n // 2] = 1.0
@@ -280,7 +290,7 @@ def group_step_map(group, u0s, T: int, dts, k: int = 20, record: str = "auto"):
 def group_residuals(group, u0s, T, dts, k=20, record="auto"):
     """[J_i] = [1/2 <|Phi_T(u0_i) - u0_i|^2>] for every box (FP64 scalars, one tensor)."""
     uTs = group_step_map(group, u0s, T, dts, k, record)
-    return torch.stack([0.5 * mean_square(uT - u0, b.n)
+    return torch.stack([0.5 * mean_square(uT - u0, b)
                         for uT, u0, b in zip(uTs, u0s, group.boxes)])
 
 
@@ -298,7 +308,7 @@ def group_adjoint_descent(group, u0s, T, dts, k=20, iters=10, step0=1.0, c1=1e-4
     an FP64 Armijo backtracking line search per box (each box keeps its own step length;
     trial residuals of all boxes are evaluated in one batched forward).  Returns (us, hist)
     with hist[it] the list of residuals of the boxes."""
-    ws = [hermitian_weight(b.n, u.device) for b, u in zip(group.boxes, u0s)]
+    ws = [hermitian_weight(b.nx, u.device) for b, u in zip(group.boxes, u0s)]
     us = [u.detach().clone() for u in u0s]
     nb = len(us)
     alpha = [step0] * nb
@@ -311,8 +321,8 @@ def group_adjoint_descent(group, u0s, T, dts, k=20, iters=10, step0=1.0, c1=1e-4
             callback(it, J0)
         ds, slopes = [], []
         for b, g, w in zip(group.boxes, gs, ws):
-            gm = g.to(torch.complex128) / w * float(b.n) ** 6
-            d = -project_masked(gm, b.n).to(g.dtype)
+            gm = g.to(torch.complex128) / w * float(b.nx * b.ny * b.nz) ** 2
+            d = -project_masked(gm, b).to(g.dtype)
             ds.append(d)
             slopes.append(float((g.to(torch.complex128).conj() * d.to(torch.complex128)).real.sum()))
         pending = list(range(nb))
@@ -346,29 +356,33 @@ def step_map(box, u0, T: int, dt: float, k: int = 20, record: str = "auto"):
     return _StepMap.apply(u0, box, T, dt, k, record)
 
 
-def mean_square(r, n):
-    """<|r_phys|^2> of a spectral field (3, N, N, N/2+1) in FP64 (Parseval, Hermitian weights)."""
-    w = hermitian_weight(n, r.device)
+def mean_square(r, box):
+    """<|r_phys|^2> of a spectral field (3, nz, ny, nx/2+1) of a box, in FP64 (Parseval, Hermitian
+    weights)."""
+    w = hermitian_weight(box.nx, r.device)
     rr = r.to(torch.complex128)
-    return (w * (rr.real ** 2 + rr.imag ** 2)).sum() / float(n) ** 6
+    return (w * (rr.real ** 2 + rr.imag ** 2)).sum() / float(box.nx * box.ny * box.nz) ** 2
 
 
 def residual(box, u0, T, dt, k=20, record="auto"):
     """J(u0) = 1/2 <|Phi_T(u0) - u0|^2>."""
     uT = step_map(box, u0, T, dt, k, record)
-    return 0.5 * mean_square(uT - u0, box.n)
+    return 0.5 * mean_square(uT - u0, box)
 
 
-def project_masked(g, n, parity=True):
-    """Leray projection and 2/3 mask (state_project + state_dealias, state.f90:120-209)."""
-    kx, ky, kz = _k_tables(n, g.device)
+def project_masked(g, box, parity=True):
+    """Leray projection and cubic 2/3 mask (state_project + state_dealias, state.f90:120-209) of a
+    field of a box: |k_d| >= (N_d/2)(2 alpha_d/3) on the scaled wavenumbers, as the library's mask."""
+    kx, ky, kz = _k_tables(box, g.device)
     k2 = kx ** 2 + ky ** 2 + kz ** 2
-    thr = (n / 2.0) * (2.0 / 3.0)
-    m = (kx.abs() >= thr) | (ky.abs() >= thr) | (kz.abs() >= thr) | (k2 == 0)
+    ax, ay, az = _alphas(box)
+    m = ((kx.abs() >= (box.nx / 2.0) * (2.0 * ax / 3.0)) | (ky.abs() >= (box.ny / 2.0) * (2.0 * ay / 3.0))
+         | (kz.abs() >= (box.nz / 2.0) * (2.0 * az / 3.0)) | (k2 == 0))
     if not parity:
-        idx = torch.arange(n, device=g.device)
-        nyq = idx == n // 2
-        m = m | nyq.view(n, 1, 1) | nyq.view(1, n, 1) | nyq[: n // 2 + 1].view(1, 1, n // 2 + 1)
+        def nyq(n):
+            return torch.arange(n, device=g.device) == n // 2
+        m = (m | nyq(box.nz).view(box.nz, 1, 1) | nyq(box.ny).view(1, box.ny, 1)
+             | nyq(box.nx)[: box.nx // 2 + 1].view(1, 1, box.nx // 2 + 1))
     k2s = torch.where(k2 == 0, torch.ones_like(k2), k2)
     gg = g.to(torch.complex128)
     kd = (kx * gg[0] + ky * gg[1] + kz * gg[2]) / k2s
@@ -381,8 +395,7 @@ def adjoint_descent(box, u0, T, dt, k=20, iters=50, step0=1.0, c1=1e-4, shrink=0
     """Minimise J(u) = 1/2 <|Phi_T(u) - u|^2> from u0 by gradient descent in the L2 metric
     (W^-1 times the Euclidean gradient, projected onto retained solenoidal fields) with an
     Armijo backtracking line search in FP64.  Returns (u, history of J)."""
-    n = box.n
-    w = hermitian_weight(n, u0.device)
+    w = hermitian_weight(box.nx, u0.device)
     u = u0.detach().clone()
     hist = []
     alpha = step0
@@ -390,10 +403,10 @@ def adjoint_descent(box, u0, T, dt, k=20, iters=50, step0=1.0, c1=1e-4, shrink=0
         x = u.clone().requires_grad_(True)
         J = residual(box, x, T, dt, k, record)
         (g,) = torch.autograd.grad(J, x)
-        # J = 1/2 sum W |r|^2 / N^6 in the metric <a, b>_M = sum W Re(conj a b) / N^6, whose
-        # gradient is N^6 W^-1 times the Euclidean one; descend along its projection
-        gm = g.to(torch.complex128) / w * float(n) ** 6
-        d = -project_masked(gm, n).to(u.dtype)
+        # J = 1/2 sum W |r|^2 / M^2 (M = nx ny nz points) in the metric <a, b>_M = sum W Re(conj a b)
+        # / M^2, whose gradient is M^2 W^-1 times the Euclidean one; descend along its projection
+        gm = g.to(torch.complex128) / w * float(box.nx * box.ny * box.nz) ** 2
+        d = -project_masked(gm, box).to(u.dtype)
         slope = float((g.to(torch.complex128).conj() * d.to(torch.complex128)).real.sum())
         J0 = float(J.detach())
         hist.append(J0)

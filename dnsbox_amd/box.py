@@ -38,6 +38,11 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _grid(p: Params):
+    """(nx, ny, nz) of a parameter set (each 0 means n)."""
+    return tuple(int(v) if v > 0 else int(p.n) for v in (p.nx, p.ny, p.nz))
+
+
 def default_params(**kw) -> Params:
     p = Params()
     check(load().dnsbox_params_default(C.byref(p)))
@@ -50,7 +55,8 @@ def default_params(**kw) -> Params:
 
 class Box:
     """One periodic box on the GPU: [0, 2 pi / alpha_x) x [0, 2 pi / alpha_y) x [0, 2 pi / alpha_z)
-    (alpha_* keyword arguments, default 1), N^3 points."""
+    (alpha_* keyword arguments, default 1) on an nx x ny x nz grid (nx, ny, nz keyword arguments,
+    each defaulting to n; a non-cubic grid must be one the library lists, grids.hpp)."""
 
     def __init__(self, n: int = 32, precision: int = 64, **params):
         self.lib = load()
@@ -60,6 +66,7 @@ class Box:
         self._h = h
         self._owned = True
         self.n = n
+        self.nx, self.ny, self.nz = _grid(self.params)
         self.precision = precision
 
     @classmethod
@@ -71,6 +78,7 @@ class Box:
         b._h = handle
         b._owned = False
         b.n = params.n
+        b.nx, b.ny, b.nz = _grid(params)
         b.precision = params.precision
         return b
 
@@ -86,6 +94,7 @@ class Box:
             h = C.c_void_p()
             check(self.lib.dnsbox_create(C.byref(tw.params), C.byref(h)))
             tw._h, tw._owned, tw.n, tw.precision = h, True, self.n, self.precision
+            tw.nx, tw.ny, tw.nz = self.nx, self.ny, self.nz
             self._twin = tw
         tw.set_pruning(getattr(self, "_prune", True))
         return tw
@@ -118,11 +127,16 @@ class Box:
 
     @property
     def spec_shape(self):
-        return (self.n // 2 + 1, self.n, self.n)
+        return (self.nx // 2 + 1, self.ny, self.nz)
 
     @property
     def phys_shape(self):
-        return (self.n, self.n, self.n)
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def dev_shape(self):
+        """Shape of a state tensor in the internal layout: (3, nz, ny, nx/2+1)."""
+        return (3, self.nz, self.ny, self.nx // 2 + 1)
 
     def nspec(self) -> int:
         return int(self.lib.dnsbox_nspec(self._h))
@@ -169,8 +183,8 @@ class Box:
         (dnsboxheun.f90:48-55), by state_u2uhat, state_dealias and state_project."""
         from . import state_io
         dims = state_io.state_dims(path)
-        if dims != (self.n, self.n, self.n):
-            raise ValueError(f"{path} holds a {dims} grid, the box is {self.n}^3")
+        if dims != (self.nx, self.ny, self.nz):
+            raise ValueError(f"{path} holds a {dims} grid, the box is {self.nx} x {self.ny} x {self.nz}")
         self.set_state_physical(*state_io.read_state(path))
         if dealias:
             self.state_dealias()
@@ -267,18 +281,18 @@ class Box:
         check(self.lib.dnsbox_state_ptr(self._h, C.byref(p)))
         return p.value
 
-    # -- device tensors (torch, internal layout (3, N, N, N/2+1), context precision) ----
+    # -- device tensors (torch, internal layout (3, nz, ny, nx/2+1), context precision) ----
     def _check_tensor(self, t):
         import torch
         want = torch.complex128 if self.precision == 64 else torch.complex64
-        shape = (3, self.n, self.n, self.n // 2 + 1)
+        shape = self.dev_shape
         if not (t.is_cuda and t.dtype == want and tuple(t.shape) == shape and t.is_contiguous()):
             raise ValueError(f"expected a contiguous cuda {want} tensor of shape {shape}, got "
                              f"{t.dtype} {tuple(t.shape)} on {t.device}")
         return C.c_void_p(t.data_ptr())
 
     def state_tensor(self):
-        """A device-tensor copy of the state, internal layout (3, N, N, N/2+1)."""
+        """A device-tensor copy of the state, internal layout (3, nz, ny, nx/2+1)."""
         import torch
         f = np.stack([np.transpose(a, (2, 1, 0)) for a in self.get_state()])
         dt = torch.complex128 if self.precision == 64 else torch.complex64
@@ -292,7 +306,7 @@ class Box:
     def empty_state(self):
         import torch
         dt = torch.complex128 if self.precision == 64 else torch.complex64
-        return torch.zeros((3, self.n, self.n, self.n // 2 + 1), dtype=dt, device="cuda")
+        return torch.zeros(self.dev_shape, dtype=dt, device="cuda")
 
     def _on_torch_stream(self):
         import torch
@@ -311,7 +325,7 @@ class Box:
     def _check_traj(self, traj, nsteps):
         import torch
         want = torch.complex128 if self.precision == 64 else torch.complex64
-        shape = (3, self.n, self.n, self.n // 2 + 1)
+        shape = self.dev_shape
         if not (traj.is_cuda and traj.dtype == want and traj.is_contiguous() and tuple(traj.shape[1:]) == shape
                 and traj.shape[0] >= nsteps):
             raise ValueError(f"expected a contiguous cuda {want} tensor of shape (>= {nsteps}, {shape}), "
@@ -321,7 +335,7 @@ class Box:
     def step_dev_record(self, state, nsteps: int, traj, traj_pred=None):
         """step_dev, storing the state after step j + 1 in traj[j] and, if given, that step's
         predicted state u* in traj_pred[j] (contiguous device tensors of shape
-        (>= nsteps, 3, N, N, N/2+1) of the context precision)."""
+        (>= nsteps, 3, nz, ny, nx/2+1) of the context precision)."""
         self._on_torch_stream()
         tp = self._check_traj(traj_pred, nsteps) if traj_pred is not None else None
         check(self.lib.dnsbox_step_dev_record(self._h, self._check_tensor(state), nsteps,

@@ -19,7 +19,8 @@ LIB = os.path.join(LIBDIR, "libdnsbox_hip.so")
 OBJDIR = os.path.join(PKG, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["box_f64.hip", "box_f32.hip", "group.hip", "dnsbox_capi.hip", "noalloc.hip"]
+SOURCES = ["box_f64.hip", "box_f32.hip", "box_grid_f64.hip", "box_grid_f32.hip", "box_grid_f32s64.hip", "group.hip",
+           "dnsbox_capi.hip", "noalloc.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -7,7 +7,7 @@
 //         nbar  = -P M lambda,          ybar = c2r(nbar / W)
 //         ubar  = ybar x w_p,           wbar = u_p x ybar
 //         g     = s W (r2c(ubar) + i k x r2c(wbar))  [+ band terms]
-//     W is the Hermitian weight of a stored entry (1 on the kx = 0 and kx = N/2 planes,
+//     W is the Hermitian weight of a stored entry (1 on the kx = 0 and kx = NX/2 planes,
 //     2 elsewhere) and (i k x) is Hermitian, so it is its own adjoint.  Five passes, as
 //     the forward: z-inverse (9 outputs), y-inverse (9), x (9 -> 6), y-forward (6),
 //     z-forward (6) with the step-adjoint epilogue.
@@ -28,16 +28,16 @@ struct OutPtrs {
     cpx<T>* p[K];
 };
 
-__device__ __forceinline__ double herm_w(int ix, int n) { return (ix == 0 || ix == n / 2) ? 1.0 : 2.0; }
+__device__ __forceinline__ double herm_w(int ix, int nx) { return (ix == 0 || ix == nx / 2) ? 1.0 : 2.0; }
 
 // ".gt." 2/3 rule of rhsAdj.f90:131-136 (the reference has no Nyquist or k = 0 exclusion
 // here; parity = 0 removes the Nyquist-index modes as everywhere else)
-__device__ __forceinline__ bool masked_gt(const ModeK& m, int n, const BoxPhys& ph)
+__device__ __forceinline__ bool masked_gt(const ModeK& m, const BoxPhys& ph)
 {
     if (fabs(double(m.kx) * ph.a[0]) > ph.thr[0] || fabs(double(m.ky) * ph.a[1]) > ph.thr[1] ||
         fabs(double(m.kz) * ph.a[2]) > ph.thr[2])
         return true;
-    if (!ph.parity && (m.ix == n / 2 || m.iy == n / 2 || m.iz == n / 2)) return true;
+    if (!ph.parity && (m.ix == ph.nyq[0] || m.iy == ph.nyq[1] || m.iz == ph.nyq[2])) return true;
     return false;
 }
 
@@ -49,17 +49,17 @@ struct ProVjpIn {  // (lambda, u) -> (lam_scale (-P M lambda) / W, s u, s i k x 
     static constexpr int FI = 6, FO = 9;
     T s;
     T lam_scale;
-    int n;
+    int nx;      // points along x (the Hermitian weight of the kx = NX/2 plane)
     BoxPhys ph;
     __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
     {
         const T kx = T(double(m.kx) * ph.a[0]), ky = T(double(m.ky) * ph.a[1]), kz = T(double(m.kz) * ph.a[2]);
         if (o < 3) {
-            if (masked(m.ix, m.iy, m.iz, m.kx, m.ky, m.kz, n, ph)) return cpx<T>{T(0), T(0)};
+            if (masked(m.ix, m.iy, m.iz, m.kx, m.ky, m.kz, ph)) return cpx<T>{T(0), T(0)};
             const T inv = T(1) / T(kk2(m.kx, m.ky, m.kz, ph));
             const cpx<T> kl = inv * (kx * in[0] + ky * in[1] + kz * in[2]);
             const T k = o == 0 ? kx : (o == 1 ? ky : kz);
-            const T f = lam_scale / T(herm_w(m.ix, n));
+            const T f = lam_scale / T(herm_w(m.ix, nx));
             return f * (k * kl - in[o]);
         }
         const cpx<T> u = in[3], v = in[4], w = in[5];
@@ -102,53 +102,53 @@ struct ProAdj12 {  // (u~, u) -> (s u_i, s i k_j u~_i)
 
 // PR: tiles over the kept (ky, kx) columns, inputs read at kept kz only (inputs confined to the
 // kept cube); every z of those columns is written
-template <int N, class T, class Pro, bool PR = false>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+template <class Dm, class T, class Pro, bool PR = false>
+__global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     k_zinv_p(Ptrs<T, Pro::FI> in, cpx<T>* __restrict__ out0, long long out_fs,
              const cpx<T>* __restrict__ tw_g, Pro pro)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     __shared__ cpx<T> lds[G::ZLDS];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    __shared__ cpx<T> stw_[G::NZ];
+    const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
     cpx<T>* out[Pro::FO];
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
-    zinv_tile<N, T, Pro, T, PR>(in.p, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
+    zinv_tile<Dm, T, Pro, T, PR>(in.p, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
 }
 
 // k_zinv_p over the kept columns (PR) in its first `tiles` blocks; the blocks after them check
 // the three fields at chk (field stride chk_fs) and set *viol when one holds a nonzero mode
 // outside the kept cube.  The cube-confined VJP runs its first pass with this guard on its
 // linearisation state; the flag is read back when the caller withdraws the guarantee.
-template <int N, class T, class Pro>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+template <class Dm, class T, class Pro>
+__global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     k_zinv_p_chk(Ptrs<T, Pro::FI> in, cpx<T>* __restrict__ out0, long long out_fs,
                  const cpx<T>* __restrict__ tw_g, Pro pro, const cpx<T>* __restrict__ chk, long long chk_fs,
                  unsigned int* viol, int tiles)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     __shared__ cpx<T> lds[G::ZLDS];
-    __shared__ cpx<T> stw_[N];
+    __shared__ cpx<T> stw_[G::NZ];
     const int b = blockIdx.x;
     if (b >= tiles) {   // a check block (the whole block returns here)
         const long long nthr = (long long)(gridDim.x - tiles) * blockDim.x;
-        const bool nz = outside_cube_nonzero<N, T>(chk, chk_fs, (long long)(b - tiles) * blockDim.x + threadIdx.x, nthr);
+        const bool nz = outside_cube_nonzero<Dm, T>(chk, chk_fs, (long long)(b - tiles) * blockDim.x + threadIdx.x, nthr);
         if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(viol, 1u);
         return;
     }
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
     cpx<T>* out[Pro::FO];
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
-    zinv_tile<N, T, Pro, T, true>(in.p, out, xcd_contiguous(b, tiles), lds, tw, pro);
+    zinv_tile<Dm, T, Pro, T, true>(in.p, out, xcd_contiguous(b, tiles), lds, tw, pro);
 }
 
 // blocks of the confinement check fused into the first pass of a cube-confined VJP
-template <int N>
+template <class Dm>
 constexpr int vjp_check_blocks()
 {
-    constexpr long long tot = (long long)N * N * (N / 2 + 1);
+    constexpr long long tot = Dm::FS;
     return tot / 256 + 1 < 256 ? int(tot / 256 + 1) : 256;
 }
 
@@ -221,26 +221,26 @@ struct VjpArgs {
     BoxPhys ph;
 };
 
-template <int N, class T, int MODE>
+template <class Dm, class T, int MODE>
 __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long long xfs, int tile, cpx<T>* lds,
                                               const cpx<T>* tw, const VjpArgs<T>& va)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     using CF = typename G::CFZ_F;
-    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
-    constexpr long long ZS = (long long)N * H;
+    constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)G::NY * H;
     const int col = tile * W + threadIdx.x % W;
     const int g = threadIdx.x / W;
     const bool valid = col < G::ZCOLS;
     cpx<T> O[6][RM];
-    zfwd_regs<N, T>(X, xfs, col, valid, lds, tw, O);
+    zfwd_regs<Dm, T>(X, xfs, col, valid, lds, tw, O);
     if (!valid) return;
     const int ix = col % H, iy = col / H;
-    const int kx = kval(ix, N), ky = kval(iy, N);
+    const int kx = kval(ix, G::NX), ky = kval(iy, G::NY);
     const BoxPhys& ph = va.ph;
     const T kxs = T(double(kx) * ph.a[0]), kys = T(double(ky) * ph.a[1]);
-    const T s = T(1.0 / (double(N) * N * N));
-    const T w = T(herm_w(ix, N));
+    const T s = T(1.0 / (double(G::NX) * G::NY * G::NZ));
+    const T w = T(herm_w(ix, G::NX));
     double cband = 0.0, fband = 0.0;
     if (ph.bandlim) {
         cband = ph.Pin / (2.0 * va.band[0]);
@@ -249,7 +249,7 @@ __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long
 #pragma unroll
     for (int q = 0; q < RM; ++q) {
         const int iz = CF::out_pos(g, q);
-        const int kz = kval(iz, N);
+        const int kz = kval(iz, G::NZ);
         const T kzs = T(double(kz) * ph.a[2]);
         const double k2d = kk2(kx, ky, kz, ph);
         const long long idx = (long long)iz * ZS + col;
@@ -261,7 +261,7 @@ __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long
         gg[2] = (s * w) * (O[2][q] + mul_i<1>(kxs * a1 - kys * a0));
         if (ph.bandlim) {
             if (sqrt(k2d) <= ph.kcut) {
-                const bool mk = masked(ix, iy, iz, kx, ky, kz, N, ph);
+                const bool mk = masked(ix, iy, iz, kx, ky, kz, ph);
                 const T we = T(kx == 0 ? 0.5 : 1.0);
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
@@ -290,37 +290,37 @@ __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long
     }
 }
 
-template <int N, class T, int MODE>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+template <class Dm, class T, int MODE>
+__global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     k_zfwd_vjp(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, VjpArgs<T> va)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     __shared__ cpx<T> lds[G::ZLDS];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    zfwd_vjp_tile<N, T, MODE>(X, xfs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, va);
+    __shared__ cpx<T> stw_[G::NZ];
+    const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
+    zfwd_vjp_tile<Dm, T, MODE>(X, xfs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, va);
 }
 
 // [Eband(u), lam_scale * sum_{|k| <= kcut, unmasked} Re(conj(lamN) u)]  (one block)
-template <int N, class T>
+template <class Dm, class T>
 __global__ void __launch_bounds__(256)
     k_band_vjp_scalars(const cpx<T>* __restrict__ u, const cpx<T>* __restrict__ lamN, long long fs,
                        BoxPhys ph, double lam_scale, double sm2, double* out)
 {
-    constexpr int H = N / 2 + 1;
-    const int KX = band_k<N>(ph.kcut, ph.a[0]), KY = band_k<N>(ph.kcut, ph.a[1]), KZ = band_k<N>(ph.kcut, ph.a[2]);
+    constexpr int H = Dm::H, NX = Dm::NX, NY = Dm::NY, NZ = Dm::NZ;
+    const int KX = band_k<NX>(ph.kcut, ph.a[0]), KY = band_k<NY>(ph.kcut, ph.a[1]), KZ = band_k<NZ>(ph.kcut, ph.a[2]);
     const int nxc = KX + 2, nyc = 2 * KY + 2, nzc = 2 * KZ + 2;
     const int tot = nxc * nyc * nzc;
     double se = 0.0, sl = 0.0;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int a = e / (nyc * nzc), b = (e / nzc) % nyc, c = e % nzc;
-        const int ix = a <= KX ? a : N / 2;
-        const int iy = band_idx<N>(b, KY);
-        const int iz = band_idx<N>(c, KZ);
-        const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
+        const int ix = a <= KX ? a : NX / 2;
+        const int iy = band_idx<NY>(b, KY);
+        const int iz = band_idx<NZ>(c, KZ);
+        const int kx = kval(ix, NX), ky = kval(iy, NY), kz = kval(iz, NZ);
         if (sqrt(kk2(kx, ky, kz, ph)) > ph.kcut) continue;
-        const long long idx = ((long long)iz * N + iy) * H + ix;
-        const bool mk = masked(ix, iy, iz, kx, ky, kz, N, ph);
+        const long long idx = ((long long)iz * NY + iy) * H + ix;
+        const bool mk = masked(ix, iy, iz, kx, ky, kz, ph);
         double en = 0.0, lu = 0.0;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
@@ -362,36 +362,36 @@ struct AdjArgs {
     BoxPhys ph;
 };
 
-template <int N, class T, int STAGE>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+template <class Dm, class T, int STAGE>
+__global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     k_zfwd_adj(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, AdjArgs<T> aa)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     using CF = typename G::CFZ_F;
-    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
-    constexpr long long ZS = (long long)N * H;
+    constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)G::NY * H;
     __shared__ cpx<T> lds[G::ZLDS];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    __shared__ cpx<T> stw_[G::NZ];
+    const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
     const int col = xcd_contiguous(blockIdx.x, gridDim.x) * W + threadIdx.x % W;
     const int g = threadIdx.x / W;
     const bool valid = col < G::ZCOLS;
     cpx<T> O[3][RM];
-    zfwd_regs<N, T>(X, xfs, col, valid, lds, tw, O);
+    zfwd_regs<Dm, T>(X, xfs, col, valid, lds, tw, O);
     if (!valid) return;
     const int ix = col % H, iy = col / H;
-    const int kx = kval(ix, N), ky = kval(iy, N);
+    const int kx = kval(ix, G::NX), ky = kval(iy, G::NY);
     const BoxPhys& ph = aa.ph;
 #pragma unroll
     for (int q = 0; q < RM; ++q) {
         const int iz = CF::out_pos(g, q);
-        const int kz = kval(iz, N);
+        const int kz = kval(iz, G::NZ);
         const long long idx = (long long)iz * ZS + col;
         const ModeK m{kx, ky, kz, ix, iy, iz};
         const double k2 = kk2(kx, ky, kz, ph);
         const T kxs = T(double(kx) * ph.a[0]), kys = T(double(ky) * ph.a[1]), kzs = T(double(kz) * ph.a[2]);
         cpx<T> nl[3];
-        if (masked_gt(m, N, ph)) {
+        if (masked_gt(m, ph)) {
 #pragma unroll
             for (int f = 0; f < 3; ++f) nl[f] = cpx<T>{T(0), T(0)};
         } else {

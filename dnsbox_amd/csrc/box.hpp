@@ -8,7 +8,7 @@
 //
 // One RHS (rhs.f90:18-135) is five launches, each one pass over HBM:
 //   zinv  : z-inverse FFT of (u, v, w) with the spectral curl fused in the
-//           prologue -> six half-spectra (u, v, w, wx, wy, wz), scaled by 1/N^3
+//           prologue -> six half-spectra (u, v, w, wx, wy, wz), scaled by 1/(NX NY NZ)
 //   ycol  : y-inverse FFT of the six fields, in place
 //   xpass : per x-line: six c2r -> (omega x u) in registers -> three r2c; the
 //           physical fields never reach HBM; max|u_i| for the CFL number is
@@ -77,6 +77,8 @@ struct BoxPhys {
     double a[3];        // alpha_x, alpha_y, alpha_z (their single-precision values, parameters.f90:18-20)
     double cmin, cmax, dtmax;
     int kolmo_k, bandlim, spherical, parity, adaptive, pad;
+    int nyq[3];         // Nyquist index N_d / 2 of each axis (parity = 0 removes those modes)
+    int pad2;
 };
 
 __host__ __device__ __forceinline__ constexpr int kval(int i, int n)
@@ -93,15 +95,14 @@ __device__ __forceinline__ double kk2(int kx, int ky, int kz, const BoxPhys& ph)
 
 // rhs.f90:74-88: cubic ".ge." 2/3 rule on the table wavenumbers (Nyquist k = 0),
 // spherical option, k = 0; parity = 0 also removes the Nyquist-index modes.
-__device__ __forceinline__ bool masked(int ix, int iy, int iz, int kx, int ky, int kz, int n,
-                                       const BoxPhys& ph)
+__device__ __forceinline__ bool masked(int ix, int iy, int iz, int kx, int ky, int kz, const BoxPhys& ph)
 {
     if (ph.spherical && kk2(kx, ky, kz, ph) >= ph.sph_thr2) return true;
     if (fabs(double(kx) * ph.a[0]) >= ph.thr[0] || fabs(double(ky) * ph.a[1]) >= ph.thr[1] ||
         fabs(double(kz) * ph.a[2]) >= ph.thr[2])
         return true;
     if (kx == 0 && ky == 0 && kz == 0) return true;
-    if (!ph.parity && (ix == n / 2 || iy == n / 2 || iz == n / 2)) return true;
+    if (!ph.parity && (ix == ph.nyq[0] || iy == ph.nyq[1] || iz == ph.nyq[2])) return true;
     return false;
 }
 
@@ -110,7 +111,8 @@ __device__ __forceinline__ bool masked(int ix, int iy, int iz, int kx, int ky, i
 // wavenumber has |k_i| >= N/3 on some axis (the threshold is (N/2)(2/3) <= KMAX + 1).  A state
 // with no energy on those modes keeps none through a step -- the masked nonlinear term is zero
 // there and the integrating factor only scales -- so the passes of the step need only the kept
-// index range of each axis, |kval(i)| <= KMAX (the Nyquist index, table wavenumber 0, is kept).
+// index range of each axis, |kval(i)| <= KMAX (the Nyquist index, table wavenumber 0, is kept);
+// Keep<N> describes one axis of N points.
 // A pruned pass loads zeros in place of the other modes and does not store them; it skips only
 // exact zeros, so its results agree with the unpruned pass to rounding (the two instantiations
 // may contract multiply-adds differently).
@@ -124,8 +126,6 @@ struct Keep {
     static constexpr int H = N / 2 + 1;
     static constexpr int KX = KMAX + 2;         // kx indices 0..KMAX and N/2
     static constexpr int KY = 2 * KMAX + 2;     // ky (kz) indices 0..KMAX, N-KMAX..N-1 and N/2
-    static constexpr int ZCOLS = KY * KX;       // z-pass columns
-    static constexpr int YCOLS = N * KX;        // y-pass columns
     static_assert(KMAX + 1 < N / 2, "kept ranges must not meet the Nyquist index");
     __device__ __forceinline__ static bool kept(int i) { return abs(kval(i, N)) <= KMAX; }
     __device__ __forceinline__ static int ix(int x) { return x <= KMAX ? x : N / 2; }
@@ -133,65 +133,80 @@ struct Keep {
     {
         return y <= KMAX ? y : (y <= 2 * KMAX ? y + N - 2 * KMAX - 1 : N / 2);
     }
+};
+
+// Grid of a box: NX, NY, NZ points along x, y, z (parameters.f90:12-14).  Each pass transforms
+// along one axis: the z passes columns of length NZ over the (ky, kx) columns, the y passes
+// columns of length NY over the (kz, kx) columns, the x pass lines of length NX over (y, z).
+template <int NX_, int NY_, int NZ_>
+struct Dims {
+    static constexpr int NX = NX_, NY = NY_, NZ = NZ_;
+    static constexpr int H = NX / 2 + 1;                      // kx modes of a half-spectrum row
+    static constexpr long long FS = (long long)NZ * NY * H;   // elements of a spectral field
+    static constexpr long long NP = (long long)NX * NY * NZ;  // points of a physical field
+    static constexpr bool CUBIC = NX == NY && NY == NZ;
+    static constexpr int YKCOLS = NZ * Keep<NX>::KX;          // columns of the pruned y passes
     // column iy * H + ix of the full layout is a kept (ky, kx) column of the z passes
     __host__ __device__ __forceinline__ static constexpr bool kept_col(int col)
     {
-        if (col < 0 || col >= N * H) return false;
-        const int ky = kval(col / H, N), x = col % H;
-        return ky <= KMAX && ky >= -KMAX && (x <= KMAX || x == N / 2);
+        if (col < 0 || col >= NY * H) return false;
+        const int ky = kval(col / H, NY), x = col % H;
+        return ky <= Keep<NY>::KMAX && ky >= -Keep<NY>::KMAX && (x <= Keep<NX>::KMAX || x == NX / 2);
     }
 };
+template <int N>
+using Cube = Dims<N, N, N>;
 
 // Tiles of the pruned z passes: the W-column chunks [W j, W j + W) of the full (ky, kx)
 // column index that hold a kept column.  A kept row of the z-plane holds a run of KMAX + 1
 // columns and its Nyquist column (the row's last, just before the next row's run); rows are
-// N/2 + 1 columns long, so their runs start at every offset within a cache line.  Tiles that
+// NX/2 + 1 columns long, so their runs start at every offset within a cache line.  Tiles that
 // follow the chunks keep every row of a tile in one aligned segment of W columns (one 128-byte
 // line at W = 16 FP32 columns) at the price of idle lanes (a fill of 85 % at 256^3, W = 16).
-template <int N, int W>
+template <class Dm, int W>
 constexpr bool chunk_kept(int j)
 {
     bool any = false;
-    for (int c = 0; c < W; ++c) any = any || Keep<N>::kept_col(j * W + c);
+    for (int c = 0; c < W; ++c) any = any || Dm::kept_col(j * W + c);
     return any;
 }
-template <int N, int W>
+template <class Dm, int W>
 constexpr int kept_chunk_count()
 {
     int n = 0;
-    for (int j = 0; j * W < N * (N / 2 + 1); ++j) n += chunk_kept<N, W>(j) ? 1 : 0;
+    for (int j = 0; j * W < Dm::NY * Dm::H; ++j) n += chunk_kept<Dm, W>(j) ? 1 : 0;
     return n;
 }
-template <int N, int W>
+template <class Dm, int W>
 struct KeptChunks {
-    static constexpr int NCH = kept_chunk_count<N, W>();
-    static_assert(N * (N / 2 + 1) / W < 65536, "chunk index fits 16 bits");
+    static constexpr int NCH = kept_chunk_count<Dm, W>();
+    static_assert(Dm::NY * Dm::H / W < 65536, "chunk index fits 16 bits");
     unsigned short j[NCH];
 };
-template <int N, int W>
-constexpr KeptChunks<N, W> make_kept_chunks()
+template <class Dm, int W>
+constexpr KeptChunks<Dm, W> make_kept_chunks()
 {
-    KeptChunks<N, W> t{};
+    KeptChunks<Dm, W> t{};
     int n = 0;
-    for (int j = 0; j * W < N * (N / 2 + 1); ++j)
-        if (chunk_kept<N, W>(j)) t.j[n++] = (unsigned short)j;
+    for (int j = 0; j * W < Dm::NY * Dm::H; ++j)
+        if (chunk_kept<Dm, W>(j)) t.j[n++] = (unsigned short)j;
     return t;
 }
-template <int N, int W>
-__device__ constexpr KeptChunks<N, W> kept_chunks = make_kept_chunks<N, W>();
+template <class Dm, int W>
+__device__ constexpr KeptChunks<Dm, W> kept_chunks = make_kept_chunks<Dm, W>();
 
 // column of lane c in tile `tile` of a z pass (full layout index iy * H + ix) and whether it
 // is one the pass transforms; an idle lane gets its chunk's first column (in range)
-template <int N, int W, bool PR>
+template <class Dm, int W, bool PR>
 __device__ __forceinline__ int ztile_col(int tile, int c, bool& valid)
 {
     if constexpr (PR) {
-        const int base = W * int(kept_chunks<N, W>.j[tile]);
-        valid = Keep<N>::kept_col(base + c);
+        const int base = W * int(kept_chunks<Dm, W>.j[tile]);
+        valid = Dm::kept_col(base + c);
         return valid ? base + c : base;
     } else {
         const int col = tile * W + c;
-        valid = col < N * (N / 2 + 1);
+        valid = col < Dm::NY * Dm::H;
         return col;
     }
 }
@@ -265,32 +280,39 @@ struct Cfg : CfgDefault<N, T> {};
 // where the z passes are bound by the latency of their barrier chains rather than by
 // bandwidth: the 64^3 FP64 box (the adjoint workload; fewer z tiles than CUs).  ZIL LDS tiles
 // per block.
-template <int N, class T>
+template <class Dm, class T>
 struct ZInterleave {
-    static constexpr int value = (N == 64 && std::is_same<T, double>::value) ? 6 : 1;
+    static constexpr int value = (Dm::CUBIC && Dm::NZ == 64 && std::is_same<T, double>::value) ? 6 : 1;
 };
 
-template <int N, class T>
+// Launch geometry of the passes of a box of grid Dm: each pass takes the configuration of its
+// own transform length (Cfg<NZ> for the z passes, Cfg<NY> for the y passes, Cfg<NX> for the x pass)
+template <class Dm, class T>
 struct Geo {
-    using C = Cfg<N, T>;
-    using CFZ_I = ColFFT<typename C::ZP, C::ZW, C::ZW, +1, T>;
-    using CFZ_F = ColFFT<typename C::ZP, C::ZW, C::ZW, -1, T>;
-    static constexpr int H = N / 2 + 1;                 // kx modes
-    static constexpr long long FS = (long long)N * N * H; // elements per spectral field
+    using CZ = Cfg<Dm::NZ, T>;
+    using CY = Cfg<Dm::NY, T>;
+    using CX = Cfg<Dm::NX, T>;
+    using CFZ_I = ColFFT<typename CZ::ZP, CZ::ZW, CZ::ZW, +1, T>;
+    using CFZ_F = ColFFT<typename CZ::ZP, CZ::ZW, CZ::ZW, -1, T>;
+    static constexpr int NX = Dm::NX, NY = Dm::NY, NZ = Dm::NZ;
+    static constexpr int H = Dm::H;                     // kx modes
+    static constexpr long long FS = Dm::FS;             // elements per spectral field
+    static constexpr int ZW = CZ::ZW, YW = CY::YW;      // columns per tile of the z / y passes
     static constexpr int ZNT = CFZ_I::NT;               // threads of a z pass
     static constexpr int ZRM = CFZ_I::RM;
-    static constexpr int ZIL = ZInterleave<N, T>::value;
-    static constexpr int ZLDS = N * C::ZW * ZIL;        // LDS (complex elements) of a z pass
-    static constexpr int YNT = ColFFT<typename C::YP, C::YW, C::YW, +1, T>::NT;
-    static constexpr int ZCOLS = N * H;                 // z-pass columns (ky, kx)
-    static constexpr int YCOLS = N * H;                 // y-pass columns (kz, kx)
-    static constexpr int ZTILES = (ZCOLS + C::ZW - 1) / C::ZW;
-    static constexpr int YTILES = (YCOLS + C::YW - 1) / C::YW;
-    static constexpr int XROWS = N * N;
-    static constexpr int XBLOCKS = XROWS / C::RB;
-    static_assert(XROWS % C::RB == 0, "x-lines per block must divide N^2");
-    static_assert(C::ZP::L == N && C::YP::L == N && C::RP::L == N / 2, "plan lengths");
-    static_assert(C::ZP::radix(0) == C::ZP::radix(C::ZP::S - 1), "z plans must be symmetric");
+    static constexpr int ZIL = ZInterleave<Dm, T>::value;
+    static constexpr int ZLDS = NZ * ZW * ZIL;          // LDS (complex elements) of a z pass
+    static constexpr int YNT = ColFFT<typename CY::YP, YW, YW, +1, T>::NT;
+    static constexpr int NTX = CX::NTX;                 // threads of the x pass
+    static constexpr int ZCOLS = NY * H;                // z-pass columns (ky, kx)
+    static constexpr int YCOLS = NZ * H;                // y-pass columns (kz, kx)
+    static constexpr int ZTILES = (ZCOLS + ZW - 1) / ZW;
+    static constexpr int YTILES = (YCOLS + YW - 1) / YW;
+    static constexpr int XROWS = NY * NZ;
+    static constexpr int XBLOCKS = XROWS / CX::RB;
+    static_assert(XROWS % CX::RB == 0, "x-lines per block must divide NY NZ");
+    static_assert(CZ::ZP::L == NZ && CY::YP::L == NY && CX::RP::L == NX / 2, "plan lengths");
+    static_assert(CZ::ZP::radix(0) == CZ::ZP::radix(CZ::ZP::S - 1), "z plans must be symmetric");
 };
 
 // ===========================================================================
@@ -355,16 +377,16 @@ struct ProScale {  // (u, v, w) -> s (u, v, w)               state_uhat2u
 
 // outputs OB .. OB + K - 1 of zinv_from_regs transformed together (K = min(ZIL, FO - OB)), then
 // the next group
-template <int N, class T, class Pro, int OB>
-__device__ __forceinline__ void zinv_groups(const cpx<T> (&U)[Pro::FI][Geo<N, T>::ZRM],
+template <class Dm, class T, class Pro, int OB>
+__device__ __forceinline__ void zinv_groups(const cpx<T> (&U)[Pro::FI][Geo<Dm, T>::ZRM],
                                             cpx<T>* const* __restrict__ out, int col, bool valid, cpx<T>* lds,
                                             const cpx<T>* __restrict__ tw, const Pro& pro, int c, int g, int kx,
                                             int ky, int ix, int iy)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     using CF = typename G::CFZ_I;
     constexpr int RM = G::ZRM, H = G::H;
-    constexpr long long ZS = (long long)N * H;
+    constexpr long long ZS = (long long)G::NY * H;
     if constexpr (OB < Pro::FO) {
         constexpr int K = (Pro::FO - OB) < G::ZIL ? (Pro::FO - OB) : G::ZIL;
         cpx<T> a[K][RM];
@@ -376,36 +398,36 @@ __device__ __forceinline__ void zinv_groups(const cpx<T> (&U)[Pro::FI][Geo<N, T>
 #pragma unroll
                 for (int f = 0; f < Pro::FI; ++f) v[f] = U[f][q];
                 const int iz = CF::in_pos(g, q);
-                a[o][q] = pro(OB + o, v, ModeK{kx, ky, kval(iz, N), ix, iy, iz});
+                a[o][q] = pro(OB + o, v, ModeK{kx, ky, kval(iz, G::NZ), ix, iy, iz});
             }
-        CF::template run_multi<K>(a, lds, c, g, tw, N);
+        CF::template run_multi<K>(a, lds, c, g, tw, G::NZ);
         if (valid) {
 #pragma unroll
             for (int o = 0; o < K; ++o)
 #pragma unroll
                 for (int q = 0; q < RM; ++q) out[OB + o][CF::out_pos(g, q) * ZS + col] = a[o][q];
         }
-        zinv_groups<N, T, Pro, OB + K>(U, out, col, valid, lds, tw, pro, c, g, kx, ky, ix, iy);
+        zinv_groups<Dm, T, Pro, OB + K>(U, out, col, valid, lds, tw, pro, c, g, kx, ky, ix, iy);
     }
 }
 
 // z-inverse of the FO prologue outputs, starting from the FI inputs held in
 // registers at the stage-0 positions of the z plan: U[f][q] = in_f(kz = in_pos(g, q)).
-template <int N, class T, class Pro>
-__device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<N, T>::ZRM],
+template <class Dm, class T, class Pro>
+__device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<Dm, T>::ZRM],
                                                cpx<T>* const* __restrict__ out, int col, bool valid,
                                                cpx<T>* lds, const cpx<T>* __restrict__ tw, const Pro& pro)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     using CF = typename G::CFZ_I;
-    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
-    constexpr long long ZS = (long long)N * H;  // stride between kz planes
+    constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)G::NY * H;  // stride between kz planes
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
     const int ix = col % H, iy = col / H;
-    const int kx = kval(ix, N), ky = kval(iy, N);
+    const int kx = kval(ix, G::NX), ky = kval(iy, G::NY);
     if constexpr (G::ZIL > 1) {
-        zinv_groups<N, T, Pro, 0>(U, out, col, valid, lds, tw, pro, c, g, kx, ky, ix, iy);
+        zinv_groups<Dm, T, Pro, 0>(U, out, col, valid, lds, tw, pro, c, g, kx, ky, ix, iy);
         return;
     }
 #pragma unroll
@@ -417,9 +439,9 @@ __device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<N,
 #pragma unroll
             for (int f = 0; f < Pro::FI; ++f) v[f] = U[f][q];
             const int iz = CF::in_pos(g, q);
-            a[q] = pro(o, v, ModeK{kx, ky, kval(iz, N), ix, iy, iz});
+            a[q] = pro(o, v, ModeK{kx, ky, kval(iz, G::NZ), ix, iy, iz});
         }
-        CF::run(a, lds, c, g, tw, N);
+        CF::run(a, lds, c, g, tw, G::NZ);
         if (valid) {
 #pragma unroll
             for (int q = 0; q < RM; ++q) out[o][CF::out_pos(g, q) * ZS + col] = a[q];
@@ -428,69 +450,70 @@ __device__ __forceinline__ void zinv_from_regs(const cpx<T> (&U)[Pro::FI][Geo<N,
 }
 
 // PR: pruned to the 2/3-rule cube (Keep): tile columns over Ky x Kx, inputs read over Kz
-template <int N, class T, class Pro, class TI = T, bool PR = false>
+template <class Dm, class T, class Pro, class TI = T, bool PR = false>
 __device__ __forceinline__ void zinv_tile(const cpx<TI>* const* __restrict__ in,
                                           cpx<T>* const* __restrict__ out, int tile,
                                           cpx<T>* lds, const cpx<T>* __restrict__ tw, const Pro& pro)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     using CF = typename G::CFZ_I;
-    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
-    constexpr long long ZS = (long long)N * H;
+    constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)G::NY * H;
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
     bool valid;
-    const int col = ztile_col<N, W, PR>(tile, c, valid);
+    const int col = ztile_col<Dm, W, PR>(tile, c, valid);
     cpx<T> U[Pro::FI][RM];
 #pragma unroll
     for (int f = 0; f < Pro::FI; ++f)
 #pragma unroll
         for (int q = 0; q < RM; ++q) {
             const int iz = CF::in_pos(g, q);
-            U[f][q] = valid && (!PR || Keep<N>::kept(iz)) ? cvt<T>(in[f][iz * ZS + col]) : cpx<T>{T(0), T(0)};
+            U[f][q] = valid && (!PR || Keep<G::NZ>::kept(iz)) ? cvt<T>(in[f][iz * ZS + col]) : cpx<T>{T(0), T(0)};
         }
-    zinv_from_regs<N, T, Pro>(U, out, col, valid, lds, tw, pro);
+    zinv_from_regs<Dm, T, Pro>(U, out, col, valid, lds, tw, pro);
 }
 
 // tiles of the z passes
-template <int N, class T, bool PR>
+template <class Dm, class T, bool PR>
 constexpr int ztiles()
 {
-    if constexpr (PR) return KeptChunks<N, Cfg<N, T>::ZW>::NCH;
-    else return (Geo<N, T>::ZCOLS + Cfg<N, T>::ZW - 1) / Cfg<N, T>::ZW;
+    if constexpr (PR) return KeptChunks<Dm, Geo<Dm, T>::ZW>::NCH;
+    else return Geo<Dm, T>::ZTILES;
 }
 // The pruned y passes tile each z-plane's KX columns separately when that keeps at least 85 %
 // of the lanes busy (256^3, 512^3); otherwise (a tile wider than a plane's kept columns, as at
 // 32^3 and 48^3) the tiles run over the planes' columns back to back.
-template <int N, class T>
+template <class Dm, class T>
 constexpr bool ytiles_by_plane()
 {
-    constexpr int W = Cfg<N, T>::YW, TPP = (Keep<N>::KX + W - 1) / W;
-    return 100 * Keep<N>::KX >= 85 * TPP * W;
+    constexpr int W = Geo<Dm, T>::YW, TPP = (Keep<Dm::NX>::KX + W - 1) / W;
+    return 100 * Keep<Dm::NX>::KX >= 85 * TPP * W;
 }
-template <int N, class T, bool PR>
+template <class Dm, class T, bool PR>
 constexpr int ytiles()
 {
-    if constexpr (PR && ytiles_by_plane<N, T>()) return N * ((Keep<N>::KX + Cfg<N, T>::YW - 1) / Cfg<N, T>::YW);
-    else return ((PR ? Keep<N>::YCOLS : Geo<N, T>::YCOLS) + Cfg<N, T>::YW - 1) / Cfg<N, T>::YW;
+    constexpr int W = Geo<Dm, T>::YW;
+    if constexpr (PR && ytiles_by_plane<Dm, T>()) return Dm::NZ * ((Keep<Dm::NX>::KX + W - 1) / W);
+    else return ((PR ? Dm::YKCOLS : Geo<Dm, T>::YCOLS) + W - 1) / W;
 }
 
-template <int N, class T, class Pro, class TI = T, bool PR = false>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+template <class Dm, class T, class Pro, class TI = T, bool PR = false>
+__global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     k_zinv(const cpx<TI>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
            long long out_fs, const cpx<T>* __restrict__ tw_g, Pro pro)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     __shared__ cpx<T> lds[G::ZLDS];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    __shared__ cpx<T> stw_[G::NZ];
+    const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
     const cpx<TI>* in[Pro::FI];
     cpx<T>* out[Pro::FO];
 #pragma unroll
     for (int f = 0; f < Pro::FI; ++f) in[f] = in0 + f * in_fs;
 #pragma unroll
     for (int f = 0; f < Pro::FO; ++f) out[f] = out0 + f * out_fs;
-    zinv_tile<N, T, Pro, TI, PR>(in, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
+    zinv_tile<Dm, T, Pro, TI, PR>(in, out, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, pro);
 }
 
 // ===========================================================================
@@ -498,59 +521,59 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
 // ===========================================================================
 // PR: pruned (Keep): columns over all iz x Kx (tiled plane by plane where ytiles_by_plane); the
 // inverse (DIR = +1) reads ky in Ky only, the forward (DIR = -1) stores ky in Ky only
-template <int N, class T, int DIR, bool PR = false>
+template <class Dm, class T, int DIR, bool PR = false>
 __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, cpx<T>* lds,
                                           const cpx<T>* __restrict__ tw)
 {
-    using G = Geo<N, T>;
-    using CF = ColFFT<typename G::C::YP, G::C::YW, G::C::YW, DIR, T>;
-    constexpr int W = G::C::YW, RM = CF::RM, H = G::H;
-    constexpr int CW = PR ? Keep<N>::KX : H;   // columns per z-plane
+    using G = Geo<Dm, T>;
+    using CF = ColFFT<typename G::CY::YP, G::YW, G::YW, DIR, T>;
+    constexpr int W = G::YW, RM = CF::RM, H = G::H;
+    constexpr int CW = PR ? Keep<G::NX>::KX : H;   // columns per z-plane
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
     int iz, ix;
     bool valid;
-    if constexpr (PR && ytiles_by_plane<N, T>()) {
+    if constexpr (PR && ytiles_by_plane<Dm, T>()) {
         // tiles within one z-plane (no tile spans two planes' column runs)
         constexpr int TPP = (CW + W - 1) / W;   // tiles per z-plane
         const int x = (tile % TPP) * W + c;
         iz = tile / TPP;
         valid = x < CW;
-        ix = Keep<N>::ix(x);
+        ix = Keep<G::NX>::ix(x);
     } else {
         const int col = tile * W + c;
-        valid = col < (PR ? Keep<N>::YCOLS : G::YCOLS);
-        ix = PR ? Keep<N>::ix(col % CW) : col % H;
+        valid = col < (PR ? Dm::YKCOLS : G::YCOLS);
+        ix = PR ? Keep<G::NX>::ix(col % CW) : col % H;
         iz = col / CW;
     }
-    cpx<T>* base = data + (long long)iz * N * H + ix;
+    cpx<T>* base = data + (long long)iz * G::NY * H + ix;
     cpx<T> a[RM];
 #pragma unroll
     for (int q = 0; q < RM; ++q) {
         const int iy = CF::in_pos(g, q);
-        a[q] = valid && (!PR || DIR < 0 || Keep<N>::kept(iy)) ? base[iy * H] : cpx<T>{T(0), T(0)};
+        a[q] = valid && (!PR || DIR < 0 || Keep<G::NY>::kept(iy)) ? base[iy * H] : cpx<T>{T(0), T(0)};
     }
-    CF::run(a, lds, c, g, tw, N);
+    CF::run(a, lds, c, g, tw, G::NY);
     if (valid) {
 #pragma unroll
         for (int q = 0; q < RM; ++q) {
             const int iy = CF::out_pos(g, q);
-            if (!PR || DIR > 0 || Keep<N>::kept(iy)) base[iy * H] = a[q];
+            if (!PR || DIR > 0 || Keep<G::NY>::kept(iy)) base[iy * H] = a[q];
         }
     }
 }
 
-template <int N, class T, int DIR, bool PR = false>
-__global__ void __launch_bounds__((Geo<N, T>::YNT))
+template <class Dm, class T, int DIR, bool PR = false>
+__global__ void __launch_bounds__((Geo<Dm, T>::YNT))
     k_ycol(cpx<T>* __restrict__ data0, long long fs, const cpx<T>* __restrict__ tw_g)
 {
-    using G = Geo<N, T>;
-    __shared__ cpx<T> lds[N * G::C::YW];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    using G = Geo<Dm, T>;
+    __shared__ cpx<T> lds[G::NY * G::YW];
+    __shared__ cpx<T> stw_[G::NY];
+    const cpx<T>* tw = stage_twiddles<G::NY, T>(tw_g, stw_);
     const int L = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int field = L / gridDim.x, tile = L % gridDim.x;
-    ycol_tile<N, T, DIR, PR>(data0 + field * fs, tile, lds, tw);
+    ycol_tile<Dm, T, DIR, PR>(data0 + field * fs, tile, lds, tw);
 }
 
 // ===========================================================================
@@ -619,9 +642,10 @@ struct XGeo {
 //             and the global stores
 // PR: pruned (Keep): the c2r reads kx in Kx only; PO: the r2c stores kx in Kx only (the forward
 // step prunes both; the step adjoint, whose outputs fill the whole spectrum, only its inputs)
-template <int N, class T, class Op, bool PR = false, bool PO = PR>
+template <class Dm, class T, class Op, bool PR = false, bool PO = PR>
 struct XPass {
-    using X = XGeo<N, T, Op>;
+    static constexpr int NX = Dm::NX;
+    using X = XGeo<NX, T, Op>;
     static constexpr int M = X::M, H = X::H, HP = X::HP, RB = X::RB, NTX = X::NTX;
     static constexpr int FI = Op::FI, FO = Op::FO;
     static constexpr int NL = FI * RB * H;
@@ -690,12 +714,12 @@ struct XPass {
         const long long row0 = (long long)blk * RB;
         cpx<T> A[QZ], B[QZ];
         if constexpr (FIXED_K && NI % NTX == 0) {
-            // spectral inputs are whole half-spectrum fields, N^2 (N/2+1) elements apart
-            constexpr long long FS = (long long)N * N * H;
+            // spectral inputs are whole half-spectrum fields, NZ NY (NX/2+1) elements apart
+            constexpr long long FS = Dm::FS;
             const int k = tid % M;
             const cpx<T>* base = in0 + (row0 + tid / M) * H;
-            const bool ka = !PR || k <= Keep<N>::KMAX;                 // X[k] kept
-            const bool kb = !PR || k == 0 || M - k <= Keep<N>::KMAX;   // X[M - k] kept
+            const bool ka = !PR || k <= Keep<NX>::KMAX;                 // X[k] kept
+            const bool kb = !PR || k == 0 || M - k <= Keep<NX>::KMAX;   // X[M - k] kept
 #pragma unroll
             for (int q = 0; q < QZ; ++q) {
                 const int fr = FPQ * q;   // line index of this q relative to the thread's first line
@@ -714,8 +738,8 @@ struct XPass {
             const int fr = e / M, k = e % M;
             const cpx<T>* row = in0 + (fr / RB) * in_fs + (row0 + fr % RB) * H;
             if constexpr (PR) {   // X[k], k < M, is kept for k <= KMAX; X[M - k] for M - k <= KMAX or k = 0
-                A[q] = k <= Keep<N>::KMAX ? row[k] : cpx<T>{T(0), T(0)};
-                B[q] = (k == 0 || M - k <= Keep<N>::KMAX) ? row[M - k] : cpx<T>{T(0), T(0)};
+                A[q] = k <= Keep<NX>::KMAX ? row[k] : cpx<T>{T(0), T(0)};
+                B[q] = (k == 0 || M - k <= Keep<NX>::KMAX) ? row[M - k] : cpx<T>{T(0), T(0)};
             } else {
                 A[q] = row[k];
                 B[q] = row[M - k];
@@ -749,7 +773,7 @@ struct XPass {
         const int tid = threadIdx.x;
         const long long row0 = (long long)blk * RB;
         if constexpr (Op::IN_SPEC && Z_STAGED) {
-            RowFFT<typename Cfg<N, T>::RP, FI * RB, HP, NTX, +1, T, X::XPD>::run(lds, tid, tw, N);
+            RowFFT<typename Cfg<NX, T>::RP, FI * RB, HP, NTX, +1, T, X::XPD>::run(lds, tid, tw, NX);
         } else if constexpr (Op::IN_SPEC) {
             // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),  k < M;
             // imaginary parts of X[0], X[M] are ignored (c2r convention).
@@ -780,7 +804,7 @@ struct XPass {
                 if (NI % NTX == 0 || e < NI) lds[(e / M) * HP + X::px(e % M)] = z[q];
             }
             __syncthreads();
-            RowFFT<typename Cfg<N, T>::RP, FI * RB, HP, NTX, +1, T, X::XPD>::run(lds, tid, tw, N);
+            RowFFT<typename Cfg<NX, T>::RP, FI * RB, HP, NTX, +1, T, X::XPD>::run(lds, tid, tw, NX);
         }
 
         // ---- pointwise op on the physical samples x[2m] = Re z[m], x[2m+1] = Im z[m] ----
@@ -804,13 +828,13 @@ struct XPass {
         // ---- r2c or real store ----
         if constexpr (FO == 0) {
         } else if constexpr (Op::OUT_SPEC) {
-            RowFFT<typename Cfg<N, T>::RP, FO * RB, HP, NTX, -1, T, X::XPD>::run(lds, tid, tw, N);
+            RowFFT<typename Cfg<NX, T>::RP, FO * RB, HP, NTX, -1, T, X::XPD>::run(lds, tid, tw, NX);
             // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]), k <= M
             if constexpr (FIXED_K && (FO * RB * M) % NTX == 0) {
                 // k < M in the fixed-k layout, then k = M (one entry per line and field)
-                constexpr long long FS = (long long)N * N * H;
+                constexpr long long FS = Dm::FS;
                 const int k = tid % M;
-                const bool keep = !PO || k <= Keep<N>::KMAX;
+                const bool keep = !PO || k <= Keep<NX>::KMAX;
                 if (keep) {
                     const cpx<T> w = tw[k];
                     const int r0 = tid / M;
@@ -844,7 +868,7 @@ struct XPass {
                 const cpx<T> zm = conj(row[X::px(k == 0 ? 0 : M - k)]);
                 const cpx<T> a = T(0.5) * (zk + zm);
                 const cpx<T> b = (T(0.5) * (zk - zm)) * tw[k];
-                if (!PO || k <= Keep<N>::KMAX || k == M) out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
+                if (!PO || k <= Keep<NX>::KMAX || k == M) out0[f * out_fs + row0 * H + rk] = a + mul_i<-1>(b);
             }
         } else {
             for (int e = tid; e < FO * RB * M; e += NTX) {
@@ -855,12 +879,12 @@ struct XPass {
     }
 };
 
-template <int N, class T, class Op, bool PR = false, bool PO = PR>
+template <class Dm, class T, class Op, bool PR = false, bool PO = PR>
 __device__ __forceinline__ void xpass_tile(const cpx<T>* __restrict__ in0, long long in_fs,
                                            cpx<T>* __restrict__ out0, long long out_fs, int blk, cpx<T>* lds,
                                            const cpx<T>* __restrict__ tw, XRed<T>& red)
 {
-    using XP = XPass<N, T, Op, PR, PO>;
+    using XP = XPass<Dm, T, Op, PR, PO>;
     using X = typename XP::X;
     static_assert(!PR || (Op::IN_SPEC && DNSBOX_X_PRELOAD), "pruned x-pass input: spectral");
     static_assert(!PO || Op::OUT_SPEC, "pruned x-pass output: spectral");
@@ -908,21 +932,21 @@ __device__ __forceinline__ void block_max_atomic(XRed<T>& red, unsigned long lon
     }
 }
 
-template <int N, class T, class Op, bool PR = false, bool PO = PR>
-__global__ void __launch_bounds__((Cfg<N, T>::NTX))
+template <class Dm, class T, class Op, bool PR = false, bool PO = PR>
+__global__ void __launch_bounds__((Cfg<Dm::NX, T>::NTX))
     k_xpass(const cpx<T>* __restrict__ in0, long long in_fs, cpx<T>* __restrict__ out0,
             long long out_fs, const cpx<T>* __restrict__ tw_g, unsigned long long* red_bits,
             int signed_last)
 {
-    using X = XGeo<N, T, Op>;
+    using X = XGeo<Dm::NX, T, Op>;
     __shared__ cpx<T> lds[X::LDS];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
+    __shared__ cpx<T> stw_[Dm::NX];
+    const cpx<T>* tw = stage_twiddles<Dm::NX, T>(tw_g, stw_);
     __shared__ double wsm[4 * 16];
     XRed<T> red;
 #pragma unroll
     for (int j = 0; j < 4; ++j) red.m[j] = Op::NRED == 4 && j == 3 ? T(-1e300) : T(0);   // -1e300: -inf in FP32
-    xpass_tile<N, T, Op, PR, PO>(in0, in_fs, out0, out_fs, blockIdx.x, lds, tw, red);
+    xpass_tile<Dm, T, Op, PR, PO>(in0, in_fs, out0, out_fs, blockIdx.x, lds, tw, red);
     if constexpr (Op::NRED > 0) {
         if (red_bits) block_max_atomic<Op::NRED>(red, red_bits, signed_last != 0, wsm);
     }
@@ -958,48 +982,54 @@ __device__ __forceinline__ double cfl_dt(double c, double dt, const BoxPhys& ph)
     return dt;
 }
 
-// io.f90:664-676: sum_d max|u_d| dt / (L_d / N), L_d = 2 pi / alpha_d, from the maxima of the
+// io.f90:664-676: sum_d max|u_d| dt / (L_d / N_d), L_d = 2 pi / alpha_d, from the maxima of the
 // current RHS's x pass
-template <int N>
+template <class Dm>
 __device__ __forceinline__ double courant_of(const BoxScalars* sc, const BoxPhys& ph, double dt)
 {
-    const double dx = 6.283185307179586476925286766559 / double(N);
-    return (__longlong_as_double((long long)sc->maxu[0]) * ph.a[0] +
-            __longlong_as_double((long long)sc->maxu[1]) * ph.a[1] +
-            __longlong_as_double((long long)sc->maxu[2]) * ph.a[2]) * dt / dx;
+    constexpr double tp = 6.283185307179586476925286766559;
+    const double m0 = __longlong_as_double((long long)sc->maxu[0]), m1 = __longlong_as_double((long long)sc->maxu[1]),
+                 m2 = __longlong_as_double((long long)sc->maxu[2]);
+    if constexpr (Dm::CUBIC) {
+        const double dx = tp / double(Dm::NX);
+        return (m0 * ph.a[0] + m1 * ph.a[1] + m2 * ph.a[2]) * dt / dx;
+    } else {
+        return m0 * dt / ((tp / ph.a[0]) / double(Dm::NX)) + m1 * dt / ((tp / ph.a[1]) / double(Dm::NY)) +
+               m2 * dt / ((tp / ph.a[2]) / double(Dm::NZ));
+    }
 }
 
-template <int N>
+template <class Dm>
 __device__ __forceinline__ double step_dt(BoxScalars* sc, const BoxPhys& ph)
 {
     double dt = sc->dt_prev;
-    if (ph.adaptive) dt = cfl_dt(courant_of<N>(sc, ph, dt), dt, ph);
+    if (ph.adaptive) dt = cfl_dt(courant_of<Dm>(sc, ph, dt), dt, ph);
     return dt;
 }
 
 // fields FB .. FB + K - 1 of zfwd_regs transformed together (K = min(ZIL, F - FB)), then the next group
-template <int N, class T, int F, int FB>
+template <class Dm, class T, int F, int FB>
 __device__ __forceinline__ void zfwd_groups(cpx<T>* lds, int c, int g, const cpx<T>* __restrict__ tw,
-                                            cpx<T> (&O)[F][Geo<N, T>::ZRM])
+                                            cpx<T> (&O)[F][Geo<Dm, T>::ZRM])
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     if constexpr (FB < F) {
         constexpr int K = (F - FB) < G::ZIL ? (F - FB) : G::ZIL;
-        G::CFZ_F::template run_multi<K>(O + FB, lds, c, g, tw, N);
-        zfwd_groups<N, T, F, FB + K>(lds, c, g, tw, O);
+        G::CFZ_F::template run_multi<K>(O + FB, lds, c, g, tw, G::NZ);
+        zfwd_groups<Dm, T, F, FB + K>(lds, c, g, tw, O);
     }
 }
 
 // forward z FFT of F fields of a column tile; O[f][q] holds mode kz = out_pos(g, q)
-template <int N, class T, int F>
+template <class Dm, class T, int F>
 __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long long in_fs, int col,
                                           bool valid, cpx<T>* lds, const cpx<T>* __restrict__ tw,
-                                          cpx<T> (&O)[F][Geo<N, T>::ZRM])
+                                          cpx<T> (&O)[F][Geo<Dm, T>::ZRM])
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     using CF = typename G::CFZ_F;
-    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
-    constexpr long long ZS = (long long)N * H;
+    constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)G::NY * H;
     const int tid = threadIdx.x;
     const int c = tid % W, g = tid / W;
     // issue the loads of all F fields before the first transform
@@ -1010,45 +1040,45 @@ __device__ __forceinline__ void zfwd_regs(const cpx<T>* __restrict__ in0, long l
         for (int q = 0; q < RM; ++q) O[f][q] = valid ? src[CF::in_pos(g, q) * ZS + col] : cpx<T>{T(0), T(0)};
     }
     if constexpr (G::ZIL > 1) {
-        zfwd_groups<N, T, F, 0>(lds, c, g, tw, O);
+        zfwd_groups<Dm, T, F, 0>(lds, c, g, tw, O);
     } else {
 #pragma unroll
-        for (int f = 0; f < F; ++f) CF::run(O[f], lds, c, g, tw, N);
+        for (int f = 0; f < F; ++f) CF::run(O[f], lds, c, g, tw, G::NZ);
     }
 }
 
-// Stepper epilogue on the forward-transformed n_hat = FFT((omega x u)/N^3) of a column
+// Stepper epilogue on the forward-transformed n_hat = FFT((omega x u)/(NX NY NZ)) of a column
 // tile: mask, pressure projection, band / Kolmogorov forcing (rhs.f90:67-133), then
 // the update of MODE.  When NEXT is set, U receives the input of the next RHS at the
 // same positions (the predicted state for PRED, the new state for CORR).
 // PR: pruned (Keep): state (and predicted-state) modes outside Kz are neither read nor written
-template <int N, class T, int MODE, bool NEXT, bool PREF = false, class S = T, bool PR = false>
-__device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZRM], int col, bool valid,
-                                              const EpiArgs<T, S>& ea, cpx<T> (&U)[3][Geo<N, T>::ZRM],
-                                              const cpx<S> (*Sp)[Geo<N, T>::ZRM] = nullptr)
+template <class Dm, class T, int MODE, bool NEXT, bool PREF = false, class S = T, bool PR = false>
+__device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<Dm, T>::ZRM], int col, bool valid,
+                                              const EpiArgs<T, S>& ea, cpx<T> (&U)[3][Geo<Dm, T>::ZRM],
+                                              const cpx<S> (*Sp)[Geo<Dm, T>::ZRM] = nullptr)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     using CF = typename G::CFZ_F;
-    constexpr int W = G::C::ZW, RM = G::ZRM, H = G::H;
-    constexpr long long ZS = (long long)N * H;
+    constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)G::NY * H;
     const int g = threadIdx.x / W;
     const int ix = col % H, iy = col / H;
-    const int kx = kval(ix, N), ky = kval(iy, N);
+    const int kx = kval(ix, G::NX), ky = kval(iy, G::NY);
     const BoxPhys& ph = ea.ph;
     const T kxs = T(double(kx) * ph.a[0]), kys = T(double(ky) * ph.a[1]);
     double dt = 0.0, hp = 0.0;
-    if constexpr (MODE == EPI_PRED || MODE == EPI_AB2) dt = step_dt<N>(ea.sc, ph);
+    if constexpr (MODE == EPI_PRED || MODE == EPI_AB2) dt = step_dt<Dm>(ea.sc, ph);
     if constexpr (MODE == EPI_AB2) hp = ea.sc->h_prev;
     if constexpr (MODE == EPI_CORR) dt = ea.sc->dt;
     if constexpr (MODE == EPI_PREDONLY) dt = ea.dt_fixed;
-    const double n3 = double(N) * double(N) * double(N);
+    const double n3 = double(G::NX) * double(G::NY) * double(G::NZ);
     double fband = 0.0;
     if (ph.bandlim && MODE != EPI_STORE) fband = ph.Pin / (2.0 * ea.sc->eband);
 
 #pragma unroll
     for (int q = 0; q < RM; ++q) {
         const int iz = CF::out_pos(g, q);
-        const int kz = kval(iz, N);
+        const int kz = kval(iz, G::NZ);
         const T kzs = T(double(kz) * ph.a[2]);
         const double k2d = kk2(kx, ky, kz, ph);
         const long long idx = (long long)iz * ZS + col;
@@ -1060,7 +1090,7 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
             }
             continue;
         } else {
-            if (!valid || masked(ix, iy, iz, kx, ky, kz, N, ph)) {
+            if (!valid || masked(ix, iy, iz, kx, ky, kz, ph)) {
 #pragma unroll
                 for (int f = 0; f < 3; ++f) nl[f] = cpx<T>{T(0), T(0)};
             } else {
@@ -1081,7 +1111,7 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
                 }
                 if (ph.kolmo_k > 0 && ix == 0 && iz == 0) {  // f = amp sin(kf y) e_x
                     if (iy == ph.kolmo_k) nl[0].y -= T(0.5 * ph.kolmo_amp * n3);
-                    if (iy == N - ph.kolmo_k) nl[0].y += T(0.5 * ph.kolmo_amp * n3);
+                    if (iy == G::NY - ph.kolmo_k) nl[0].y += T(0.5 * ph.kolmo_amp * n3);
                 }
             }
         }
@@ -1094,7 +1124,7 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
             // rhs.f90:8-16 integrating factor, exponent in FP64; the update in the state type S
             const double e64 = exp((-ph.nu * k2d + ph.Q) * dt);
             const S e = S(e64), sdt = S(dt);
-            const bool live = valid && (!PR || Keep<N>::kept(iz));
+            const bool live = valid && (!PR || Keep<G::NZ>::kept(iz));
             if constexpr (MODE == EPI_PRED) {  // dnsboxheun.f90:88-114
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
@@ -1145,39 +1175,39 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<N, T>::ZR
 
 // the step size of the step being taken (and the Courant number it came from), written by one
 // thread of the pass whose epilogue chose it; no block of that pass reads what it writes
-template <int N>
+template <class Dm>
 __device__ __forceinline__ void pred_scalars(const BoxScalars* /*unused*/, BoxScalars* sc, const BoxPhys& ph)
 {
-    const double dt = step_dt<N>(sc, ph);
-    if (ph.adaptive) sc->courant = courant_of<N>(sc, ph, sc->dt_prev);
+    const double dt = step_dt<Dm>(sc, ph);
+    if (ph.adaptive) sc->courant = courant_of<Dm>(sc, ph, sc->dt_prev);
     sc->dt = dt;
 }
 
-template <int N, class T, int MODE, class S = T>
+template <class Dm, class T, int MODE, class S = T>
 __device__ __forceinline__ void zfwd_tile(const cpx<T>* __restrict__ in0, long long in_fs, int tile, cpx<T>* lds,
                                           const cpx<T>* tw, const EpiArgs<T, S>& ea)
 {
-    using G = Geo<N, T>;
-    const int col = tile * G::C::ZW + threadIdx.x % G::C::ZW;
+    using G = Geo<Dm, T>;
+    const int col = tile * G::ZW + threadIdx.x % G::ZW;
     const bool valid = col < G::ZCOLS;
     cpx<T> O[3][G::ZRM];
-    zfwd_regs<N, T>(in0, in_fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, false, false, S>(O, col, valid, ea, O);
+    zfwd_regs<Dm, T>(in0, in_fs, col, valid, lds, tw, O);
+    zfwd_epilogue<Dm, T, MODE, false, false, S>(O, col, valid, ea, O);
     if constexpr (MODE == EPI_PRED || MODE == EPI_AB2) {
-        if (tile == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
+        if (tile == 0 && threadIdx.x == 0) pred_scalars<Dm>(nullptr, ea.sc, ea.ph);
     }
 }
 
-template <int N, class T, int MODE, class S = T>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+template <class Dm, class T, int MODE, class S = T>
+__global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     k_zfwd(const cpx<T>* __restrict__ in0, long long in_fs, const cpx<T>* __restrict__ tw_g,
            EpiArgs<T, S> ea)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     __shared__ cpx<T> lds[G::ZLDS];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    zfwd_tile<N, T, MODE, S>(in0, in_fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea);
+    __shared__ cpx<T> stw_[G::NZ];
+    const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
+    zfwd_tile<Dm, T, MODE, S>(in0, in_fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea);
 }
 
 // Forward z pass + stepper epilogue (PRED or CORR) fused with the z-inverse + curl
@@ -1185,13 +1215,13 @@ __global__ void __launch_bounds__((Geo<N, T>::ZNT))
 // input tile (fields 0..2 of X) and the output tile (fields 0..5 of X) cover the
 // same columns, so the block reads its whole tile before it writes.
 // PR: pruned (Keep): tile columns over Ky x Kx, state modes over Kz
-template <int N, class T, int MODE, class S = T, bool PR = false>
+template <class Dm, class T, int MODE, class S = T, bool PR = false>
 __device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long fs, int tile, cpx<T>* lds,
                                                const cpx<T>* tw, const EpiArgs<T, S>& ea, const ProCurl<T>& pro)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     bool valid;
-    const int col = ztile_col<N, G::C::ZW, PR>(tile, threadIdx.x % G::C::ZW, valid);
+    const int col = ztile_col<Dm, G::ZW, PR>(tile, threadIdx.x % G::ZW, valid);
     cpx<T> O[3][G::ZRM];
 #if DNSBOX_Z_PREFETCH
     // the state at this thread's modes (symmetric z plan: output positions = input positions),
@@ -1199,29 +1229,29 @@ __device__ __forceinline__ void zfwd_zinv_tile(cpx<T>* __restrict__ X, long long
     cpx<S> Sp[3][G::ZRM];
     {
         using CF = typename G::CFZ_F;
-        constexpr long long ZS = (long long)N * G::H;
-        const int g = threadIdx.x / G::C::ZW;
+        constexpr long long ZS = (long long)G::NY * G::H;
+        const int g = threadIdx.x / G::ZW;
 #pragma unroll
         for (int f = 0; f < 3; ++f)
 #pragma unroll
             for (int q = 0; q < G::ZRM; ++q) {
                 const int iz = CF::out_pos(g, q);
-                Sp[f][q] = valid && (!PR || Keep<N>::kept(iz)) ? ea.state[f * ea.fs + iz * ZS + col]
+                Sp[f][q] = valid && (!PR || Keep<G::NZ>::kept(iz)) ? ea.state[f * ea.fs + iz * ZS + col]
                                                                  : cpx<S>{S(0), S(0)};
             }
     }
-    zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, true, true, S, PR>(O, col, valid, ea, O, Sp);
+    zfwd_regs<Dm, T>(X, fs, col, valid, lds, tw, O);
+    zfwd_epilogue<Dm, T, MODE, true, true, S, PR>(O, col, valid, ea, O, Sp);
 #else
-    zfwd_regs<N, T>(X, fs, col, valid, lds, tw, O);
-    zfwd_epilogue<N, T, MODE, true, false, S, PR>(O, col, valid, ea, O);
+    zfwd_regs<Dm, T>(X, fs, col, valid, lds, tw, O);
+    zfwd_epilogue<Dm, T, MODE, true, false, S, PR>(O, col, valid, ea, O);
 #endif
     cpx<T>* out[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) out[f] = X + f * fs;
-    zinv_from_regs<N, T, ProCurl<T>>(O, out, col, valid, lds, tw, pro);
+    zinv_from_regs<Dm, T, ProCurl<T>>(O, out, col, valid, lds, tw, pro);
     if constexpr (MODE == EPI_PRED) {
-        if (tile == 0 && threadIdx.x == 0) pred_scalars<N>(nullptr, ea.sc, ea.ph);
+        if (tile == 0 && threadIdx.x == 0) pred_scalars<Dm>(nullptr, ea.sc, ea.ph);
     }
 }
 
@@ -1238,16 +1268,16 @@ __device__ __forceinline__ void step_end(BoxScalars* sc)
 // The corrector pass also ends the step (block 0, first thread): no block of that pass reads
 // what step_end writes (the corrector reads dt, which step_end only reads), and the next
 // reader of those scalars is a later kernel.
-template <int N, class T, int MODE, class S = T, bool PR = false>
-__global__ void __launch_bounds__((Geo<N, T>::ZNT))
+template <class Dm, class T, int MODE, class S = T, bool PR = false>
+__global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     k_zfwd_zinv(cpx<T>* __restrict__ X, long long fs, const cpx<T>* __restrict__ tw_g, EpiArgs<T, S> ea,
                 ProCurl<T> pro)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
     __shared__ cpx<T> lds[G::ZLDS];
-    __shared__ cpx<T> stw_[N];
-    const cpx<T>* tw = stage_twiddles<N, T>(tw_g, stw_);
-    zfwd_zinv_tile<N, T, MODE, S, PR>(X, fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea, pro);
+    __shared__ cpx<T> stw_[G::NZ];
+    const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
+    zfwd_zinv_tile<Dm, T, MODE, S, PR>(X, fs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, ea, pro);
     if constexpr (MODE == EPI_CORR) {
         if (blockIdx.x == 0 && threadIdx.x == 0) step_end(ea.sc);
     }
@@ -1270,18 +1300,18 @@ inline size_t spec_reduce_lds(int nspec) { return size_t(2 * 4 * nspec) * sizeof
 
 // Shell spectrum and scalar sums of a state, independent of scheduling.  Wave w of the grid takes
 // the rows (iz, iy) = w, w + W, ... (W waves in the grid), 64 modes of a row at a time.  Along a
-// row the shell index is non-decreasing in ix < N/2 (|k| grows with kx), so the lanes of one shell
+// row the shell index is non-decreasing in ix < NX/2 (|k| grows with kx), so the lanes of one shell
 // form one run: a segmented sum over the lanes (fixed order) gives each run's total, and the run's
-// last lane adds it to the wave's own bin with Kahan compensation; the Nyquist column (ix = N/2,
+// last lane adds it to the wave's own bin with Kahan compensation; the Nyquist column (ix = NX/2,
 // table kx = 0) is added in a second round.  Each block combines its waves' bins in wave order and
 // k_spec_finish sums the blocks in block order.  Scalars: per-lane Kahan sums, reduced in a fixed
 // order.  256 threads; dynamic LDS spec_reduce_lds(nspec).
-template <int N, class T>
+template <class Dm, class T>
 __global__ void __launch_bounds__(256)
     k_spec_reduce(const cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, double deltak,
                   int nspec, double* __restrict__ partial /* [gridDim.x][DG_NSCAL + nspec] */)
 {
-    constexpr int H = N / 2 + 1;
+    constexpr int H = Dm::H, NX = Dm::NX, NY = Dm::NY, NZ = Dm::NZ;
     constexpr int NW = 4;             // waves per block
     extern __shared__ double bins[];  // [NW][2][nspec]: sums, compensations
     __shared__ double wsm[DG_NSCAL][NW];
@@ -1295,9 +1325,9 @@ __global__ void __launch_bounds__(256)
     for (int j = 0; j < DG_NSCAL; ++j) s[j] = cmp[j] = 0.0;
     const double sqrt2 = 1.4142135623730951;
     const int nwave = gridDim.x * NW;
-    for (int row = blockIdx.x * NW + wid; row < N * N; row += nwave) {
-        const int iy = row % N, iz = row / N;
-        const int ky = kval(iy, N), kz = kval(iz, N);
+    for (int row = blockIdx.x * NW + wid; row < NY * NZ; row += nwave) {
+        const int iy = row % NY, iz = row / NY;
+        const int ky = kval(iy, NY), kz = kval(iz, NZ);
         const cpx<T>* base = u0 + (long long)row * H;
         for (int c0 = 0; c0 < H; c0 += 64) {
             const int ix = c0 + lane;
@@ -1305,7 +1335,7 @@ __global__ void __launch_bounds__(256)
             int bin = -1;   // -1: no shell of the spectrum
             double e = 0.0;
             if (valid) {
-                const int kx = kval(ix, N);
+                const int kx = kval(ix, NX);
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     const cpx<T> a = base[f * fs + ix];
@@ -1335,7 +1365,7 @@ __global__ void __launch_bounds__(256)
             }
 #pragma unroll
             for (int round = 0; round < 2; ++round) {
-                const bool mine = valid && ((ix == N / 2) == (round == 1));
+                const bool mine = valid && ((ix == NX / 2) == (round == 1));
                 // run key: the bin for this round's lanes, a value of its own for every other lane
                 const int key = mine ? bin : -2 - lane;
                 double v = mine ? e : 0.0;
@@ -1425,27 +1455,27 @@ __device__ __forceinline__ int band_idx(int b, int K)
     return b <= K ? b : (b < 2 * K + 1 ? N - (b - K) : N / 2);
 }
 
-template <int N, class T>
+template <class Dm, class T>
 __global__ void __launch_bounds__(256)
     k_eband(const cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, double sm2, BoxScalars* sc)
 {
-    constexpr int H = N / 2 + 1;
+    constexpr int H = Dm::H, NX = Dm::NX, NY = Dm::NY, NZ = Dm::NZ;
     const double kcut = ph.kcut;
-    const int KX = band_k<N>(kcut, ph.a[0]), KY = band_k<N>(kcut, ph.a[1]), KZ = band_k<N>(kcut, ph.a[2]);
-    const int nxc = KX + 2;          // kx indices 0..KX, N/2
-    const int nyc = 2 * KY + 2;      // ky indices 0..KY, N-KY..N-1, N/2
+    const int KX = band_k<NX>(kcut, ph.a[0]), KY = band_k<NY>(kcut, ph.a[1]), KZ = band_k<NZ>(kcut, ph.a[2]);
+    const int nxc = KX + 2;          // kx indices 0..KX, NX/2
+    const int nyc = 2 * KY + 2;      // ky indices 0..KY, NY-KY..NY-1, NY/2
     const int nzc = 2 * KZ + 2;
     const int tot = nxc * nyc * nzc;
     double s = 0.0;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int a = e / (nyc * nzc), b = (e / nzc) % nyc, c = e % nzc;
-        const int ix = a <= KX ? a : N / 2;
-        const int iy = band_idx<N>(b, KY);
-        const int iz = band_idx<N>(c, KZ);
-        const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
+        const int ix = a <= KX ? a : NX / 2;
+        const int iy = band_idx<NY>(b, KY);
+        const int iz = band_idx<NZ>(c, KZ);
+        const int kx = kval(ix, NX), ky = kval(iy, NY), kz = kval(iz, NZ);
         const double absk = sqrt(kk2(kx, ky, kz, ph));
         if (absk > kcut) continue;
-        const long long idx = ((long long)iz * N + iy) * H + ix;
+        const long long idx = ((long long)iz * NY + iy) * H + ix;
         double en = 0.0;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
@@ -1481,34 +1511,34 @@ __device__ __forceinline__ unsigned long long hash3(unsigned long long seed, uns
     return splitmix64(seed ^ splitmix64(box ^ splitmix64(idx)));
 }
 
-template <int N, class T>
+template <class Dm, class T>
 __global__ void __launch_bounds__(256)
     k_init_solenoidal(cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, unsigned long long seed,
                       int box, double k0)
 {
-    constexpr int H = N / 2 + 1;
-    constexpr long long TOT = (long long)N * N * H;
+    constexpr int H = Dm::H, NX = Dm::NX, NY = Dm::NY, NZ = Dm::NZ;
+    constexpr long long TOT = Dm::FS;
     const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (idx >= TOT) return;
     const int ix = int(idx % H);
-    const int iy = int((idx / H) % N);
-    const int iz = int(idx / ((long long)H * N));
+    const int iy = int((idx / H) % NY);
+    const int iz = int(idx / ((long long)H * NY));
     cpx<T> outv[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
-    bool keep = !(ix == N / 2 || iy == N / 2 || iz == N / 2);
-    int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
-    if (keep && masked(ix, iy, iz, kx, ky, kz, N, ph)) keep = false;
+    bool keep = !(ix == NX / 2 || iy == NY / 2 || iz == NZ / 2);
+    int kx = kval(ix, NX), ky = kval(iy, NY), kz = kval(iz, NZ);
+    if (keep && masked(ix, iy, iz, kx, ky, kz, ph)) keep = false;
     if (keep) {
         bool conjugate = false;
         int cy = iy, cz = iz;
         if (ix == 0 && !(ky > 0 || (ky == 0 && kz > 0))) {
             conjugate = true;
-            cy = (N - iy) % N;
-            cz = (N - iz) % N;
+            cy = (NY - iy) % NY;
+            cz = (NZ - iz) % NZ;
             ky = -ky;
             kz = -kz;
         }
         // canonical index in the oracle's [kx][ky][kz] ordering
-        const unsigned long long cidx = ((unsigned long long)ix * N + cy) * N + cz;
+        const unsigned long long cidx = ((unsigned long long)ix * NY + cy) * NZ + cz;
         double ar[3], ai[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -1560,18 +1590,18 @@ __global__ void k_scale_to_energy(cpx<T>* __restrict__ u0, long long n, const do
 // decides whether a step of a caller's buffer may run the pruned passes
 // whether this thread's share (start, start + stride, ...) of the modes of the 3 fields at u0
 // outside the kept cube holds a nonzero value
-template <int N, class T>
+template <class Dm, class T>
 __device__ __forceinline__ bool outside_cube_nonzero(const cpx<T>* __restrict__ u0, long long fs, long long start,
                                                      long long stride)
 {
-    constexpr int H = N / 2 + 1;
-    constexpr long long TOT = (long long)N * N * H;
+    constexpr int H = Dm::H;
+    constexpr long long TOT = Dm::FS;
     bool nz = false;
     for (long long idx = start; idx < TOT; idx += stride) {
         const int ix = int(idx % H);
-        const int iy = int((idx / H) % N);
-        const int iz = int(idx / ((long long)H * N));
-        if (Keep<N>::kept(ix) && Keep<N>::kept(iy) && Keep<N>::kept(iz)) continue;
+        const int iy = int((idx / H) % Dm::NY);
+        const int iz = int(idx / ((long long)H * Dm::NY));
+        if (Keep<Dm::NX>::kept(ix) && Keep<Dm::NY>::kept(iy) && Keep<Dm::NZ>::kept(iz)) continue;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const cpx<T> a = u0[f * fs + idx];
@@ -1581,29 +1611,29 @@ __device__ __forceinline__ bool outside_cube_nonzero(const cpx<T>* __restrict__ 
     return nz;
 }
 
-template <int N, class T>
+template <class Dm, class T>
 __global__ void __launch_bounds__(256)
     k_outside_cube(const cpx<T>* __restrict__ u0, long long fs, unsigned int* flag)
 {
-    const bool nz = outside_cube_nonzero<N, T>(u0, fs, blockIdx.x * (long long)blockDim.x + threadIdx.x,
+    const bool nz = outside_cube_nonzero<Dm, T>(u0, fs, blockIdx.x * (long long)blockDim.x + threadIdx.x,
                                                (long long)gridDim.x * blockDim.x);
     if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(flag, 1u);
 }
 
 // state.f90:120-179 state_project / state.f90:181-209 state_dealias, in place
-template <int N, class T>
+template <class Dm, class T>
 __global__ void __launch_bounds__(256)
     k_project_dealias(cpx<T>* __restrict__ u0, long long fs, BoxPhys ph, int do_dealias,
                       int do_project)
 {
-    constexpr int H = N / 2 + 1;
-    constexpr long long TOT = (long long)N * N * H;
+    constexpr int H = Dm::H, NX = Dm::NX, NY = Dm::NY, NZ = Dm::NZ;
+    constexpr long long TOT = Dm::FS;
     const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (idx >= TOT) return;
     const int ix = int(idx % H);
-    const int iy = int((idx / H) % N);
-    const int iz = int(idx / ((long long)H * N));
-    const int kx = kval(ix, N), ky = kval(iy, N), kz = kval(iz, N);
+    const int iy = int((idx / H) % NY);
+    const int iz = int(idx / ((long long)H * NY));
+    const int kx = kval(ix, NX), ky = kval(iy, NY), kz = kval(iz, NZ);
     cpx<T> u = u0[idx], v = u0[fs + idx], w = u0[2 * fs + idx];
     const double rx = double(kx) * ph.a[0], ry = double(ky) * ph.a[1], rz = double(kz) * ph.a[2];
     const double k2 = rx * rx + ry * ry + rz * rz;
@@ -1615,10 +1645,10 @@ __global__ void __launch_bounds__(256)
         w = w - T(rz) * kd;
     }
     if (do_dealias) {
-        // state.f90:184-197: spherical uses ((N/2)(2 alpha_x/3))^2 here
+        // state.f90:184-197: spherical uses ((NX/2)(2 alpha_x/3))^2 here
         bool m = (ph.spherical && k2 >= ph.sph_dthr2) || fabs(rx) >= ph.thr[0] || fabs(ry) >= ph.thr[1] ||
                  fabs(rz) >= ph.thr[2] || (kx == 0 && ky == 0 && kz == 0) ||
-                 (!ph.parity && (ix == N / 2 || iy == N / 2 || iz == N / 2));
+                 (!ph.parity && (ix == NX / 2 || iy == NY / 2 || iz == NZ / 2));
         if (m) u = v = w = cpx<T>{T(0), T(0)};
     }
     u0[idx] = u;
@@ -1664,15 +1694,16 @@ __global__ void k_convert(TO* __restrict__ dst, const TI* __restrict__ src, long
 }
 
 // One spectral field between the reference layout uhat(Nz, Ny, Nx/2+1) (element (ix, iy, iz) at
-// (ix N + iy) N + iz, FP64 complex) and the device layout (at (iz N + iy) H + ix, type D):
+// (ix Ny + iy) Nz + iz, FP64 complex) and the device layout (at (iz Ny + iy) H + ix, type D):
 // TO_DEV reads the reference layout.  A block transposes a 32 x 32 (ix, iz) tile of one iy
 // plane through LDS, so that both sides are read and written along their fastest index.
-// Grid (ceil(N / 32), ceil(H / 32), N), 256 threads.
+// Grid (ceil(Nz / 32), ceil(H / 32), Ny), 256 threads.
 template <class D, bool TO_DEV>
-__global__ void __launch_bounds__(256) k_ref_layout(const void* __restrict__ src, void* __restrict__ dst, int n)
+__global__ void __launch_bounds__(256)
+    k_ref_layout(const void* __restrict__ src, void* __restrict__ dst, int nx, int ny, int nz)
 {
     __shared__ cpx<double> tile[32][33];
-    const int h = n / 2 + 1;
+    const int h = nx / 2 + 1;
     const int iz0 = blockIdx.x * 32, ix0 = blockIdx.y * 32, iy = blockIdx.z;
     const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
     // tile[a][b] holds (ix, iz) = (ix0 + a, iz0 + b)
@@ -1680,23 +1711,23 @@ __global__ void __launch_bounds__(256) k_ref_layout(const void* __restrict__ src
         const cpx<double>* r = static_cast<const cpx<double>*>(src);
         for (int j = 0; j < 32; j += 8) {
             const int ix = ix0 + ty + j, iz = iz0 + tx;
-            if (ix < h && iz < n) tile[ty + j][tx] = r[((long long)ix * n + iy) * n + iz];
+            if (ix < h && iz < nz) tile[ty + j][tx] = r[((long long)ix * ny + iy) * nz + iz];
         }
         __syncthreads();
         cpx<D>* d = static_cast<cpx<D>*>(dst);
         for (int j = 0; j < 32; j += 8) {
             const int iz = iz0 + ty + j, ix = ix0 + tx;
-            if (ix < h && iz < n) {
+            if (ix < h && iz < nz) {
                 const cpx<double> v = tile[tx][ty + j];
-                d[((long long)iz * n + iy) * h + ix] = cpx<D>{D(v.x), D(v.y)};
+                d[((long long)iz * ny + iy) * h + ix] = cpx<D>{D(v.x), D(v.y)};
             }
         }
     } else {
         const cpx<D>* d = static_cast<const cpx<D>*>(src);
         for (int j = 0; j < 32; j += 8) {
             const int iz = iz0 + ty + j, ix = ix0 + tx;
-            if (ix < h && iz < n) {
-                const cpx<D> v = d[((long long)iz * n + iy) * h + ix];
+            if (ix < h && iz < nz) {
+                const cpx<D> v = d[((long long)iz * ny + iy) * h + ix];
                 tile[tx][ty + j] = cpx<double>{double(v.x), double(v.y)};
             }
         }
@@ -1704,7 +1735,7 @@ __global__ void __launch_bounds__(256) k_ref_layout(const void* __restrict__ src
         cpx<double>* r = static_cast<cpx<double>*>(dst);
         for (int j = 0; j < 32; j += 8) {
             const int ix = ix0 + ty + j, iz = iz0 + tx;
-            if (ix < h && iz < n) r[((long long)ix * n + iy) * n + iz] = tile[ty + j][tx];
+            if (ix < h && iz < nz) r[((long long)ix * ny + iy) * nz + iz] = tile[ty + j][tx];
         }
     }
 }

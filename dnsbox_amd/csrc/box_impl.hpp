@@ -1,7 +1,7 @@
 // Host-side launchers of one box: owns the device buffers of a box and
 // sequences the kernel passes of box.hpp / adjoint.hpp.  Instantiated per
-// (N, FFT type T, state type S) in box_f32.hip / box_f64.hip; the C-ABI
-// (dnsbox_capi.hip) sees only BoxBase.
+// (grid Dims, FFT type T, state type S): cubic grids in box_f32.hip / box_f64.hip, the
+// non-cubic grids of grids.hpp in box_grid_*.hip; the C-ABI (dnsbox_capi.hip) sees only BoxBase.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,7 +35,8 @@ struct HipError : std::runtime_error {
 #define DNSBOX_LAUNCH_CHECK() DNSBOX_HIP(hipGetLastError())
 
 struct BoxConfig {
-    int n = 32;
+    int n = 32;                  // the cubic size (groups), or the dnsbox_params n of a non-cubic box
+    int nx = 32, ny = 32, nz = 32;   // the grid
     int precision = 64;
     int state_precision = 64;
     BoxPhys ph{};
@@ -61,10 +62,24 @@ inline bool plane_enabled()
     }();
     return on;
 }
-template <int N, class T, class Op>
+// the plane pass runs cubic boxes only
+template <class Dm, class T, class Op>
+constexpr bool plane_used()
+{
+    if constexpr (Dm::CUBIC) return PlaneGeo<Dm::NX, T, Op>::used;
+    else return false;
+}
+template <class Dm, class T, class Op>
 inline bool use_plane()
 {
-    return PlaneGeo<N, T, Op>::used && plane_enabled();
+    return plane_used<Dm, T, Op>() && plane_enabled();
+}
+// the register x pass (xpass_reg.hpp) runs cubic boxes of its sizes
+template <class Dm, class T>
+constexpr bool xreg_used()
+{
+    if constexpr (Dm::CUBIC) return XReg<Dm::NX, T>::enabled && !kLegacyXPass;
+    else return false;
 }
 
 // twiddle table tw[j] = e^{-2 pi i j / N} in the transform type, from long-double angles
@@ -77,25 +92,44 @@ void host_twiddles(cpx<T>* h)
         h[j].y = T(-std::sin(a));
     }
 }
+// the twiddle tables of a box of grid Dm: one table of N entries for a cubic grid, else the
+// tables of the x, y and z lengths back to back (offsets TwOff<Dm>)
+template <class Dm>
+struct TwOff {
+    static constexpr int X = 0;
+    static constexpr int Y = Dm::CUBIC ? 0 : Dm::NX;
+    static constexpr int Z = Dm::CUBIC ? 0 : Dm::NX + Dm::NY;
+    static constexpr int LEN = Dm::CUBIC ? Dm::NX : Dm::NX + Dm::NY + Dm::NZ;
+};
+template <class Dm, class T>
+void host_box_twiddles(cpx<T>* h)
+{
+    host_twiddles<Dm::NX, T>(h + TwOff<Dm>::X);
+    if constexpr (!Dm::CUBIC) {
+        host_twiddles<Dm::NY, T>(h + TwOff<Dm>::Y);
+        host_twiddles<Dm::NZ, T>(h + TwOff<Dm>::Z);
+    }
+}
 
 // rhs_nonlinear (rhs.f90:18-135) on device buffers of the transform type: out <- N(in), with the
 // transform scratch X (6 fields), the twiddles tw and the scalars sc (band energy) given.  The
 // five passes of one RHS: z-inverse + curl, y-inverse (6), x cross product, y-forward (3),
-// z-forward with the mask / projection / forcing epilogue.
-template <int N, class T>
+// z-forward with the mask / projection / forcing epilogue.  tw: the box's twiddle tables (TwOff).
+template <class Dm, class T>
 void launch_rhs(hipStream_t s, const BoxPhys& ph, const cpx<T>* in, cpx<T>* out, cpx<T>* X, const cpx<T>* tw,
                 BoxScalars* sc)
 {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
+    using TO = TwOff<Dm>;
     constexpr long long fs = G::FS;
-    const double sm = 1.0 / (double(N) * N * N);
-    if (ph.bandlim) hipLaunchKernelGGL((k_eband<N, T>), dim3(1), dim3(256), 0, s, in, fs, ph, sm * sm, sc);
-    hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>, T, false>), dim3(G::ZTILES), dim3(G::ZNT), 0, s, in, fs, X, fs, tw,
-                       ProCurl<T>{T(sm), KScale<T>::of(ph)});
-    hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 6), dim3(G::YNT), 0, s, X, fs, tw);
-    hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, s, X, fs, X, fs, tw,
+    const double sm = 1.0 / (double(G::NX) * G::NY * G::NZ);
+    if (ph.bandlim) hipLaunchKernelGGL((k_eband<Dm, T>), dim3(1), dim3(256), 0, s, in, fs, ph, sm * sm, sc);
+    hipLaunchKernelGGL((k_zinv<Dm, T, ProCurl<T>, T, false>), dim3(G::ZTILES), dim3(G::ZNT), 0, s, in, fs, X, fs,
+                       tw + TO::Z, ProCurl<T>{T(sm), KScale<T>::of(ph)});
+    hipLaunchKernelGGL((k_ycol<Dm, T, +1>), dim3(G::YTILES, 6), dim3(G::YNT), 0, s, X, fs, tw + TO::Y);
+    hipLaunchKernelGGL((k_xpass<Dm, T, OpCross<T>>), dim3(G::XBLOCKS), dim3(G::NTX), 0, s, X, fs, X, fs, tw + TO::X,
                        (unsigned long long*)nullptr, 0);
-    hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(G::YNT), 0, s, X, fs, tw);
+    hipLaunchKernelGGL((k_ycol<Dm, T, -1>), dim3(G::YTILES, 3), dim3(G::YNT), 0, s, X, fs, tw + TO::Y);
     EpiArgs<T, T> ea{};
     ea.out = out;
     ea.rhs_in = in;
@@ -103,7 +137,7 @@ void launch_rhs(hipStream_t s, const BoxPhys& ph, const cpx<T>* in, cpx<T>* out,
     ea.fs = fs;
     ea.sc = sc;
     ea.ph = ph;
-    hipLaunchKernelGGL((k_zfwd<N, T, EPI_RHS, T>), dim3(G::ZTILES), dim3(G::ZNT), 0, s, X, fs, tw, ea);
+    hipLaunchKernelGGL((k_zfwd<Dm, T, EPI_RHS, T>), dim3(G::ZTILES), dim3(G::ZNT), 0, s, X, fs, tw + TO::Z, ea);
 }
 
 // Per-box device buffers and the operations the C-ABI exposes.
@@ -220,9 +254,10 @@ class BoxBase {
 BoxBase* make_box_f32(const BoxConfig& c);
 BoxBase* make_box_f64(const BoxConfig& c);
 
-template <int N, class T, class S = T>
+template <class Dm, class T, class S = T>
 class BoxT final : public BoxBase {
-    using G = Geo<N, T>;
+    using G = Geo<Dm, T>;
+    static constexpr int NX = Dm::NX, NY = Dm::NY, NZ = Dm::NZ;
     using C = cpx<T>;
     using CS = cpx<S>;
     static constexpr int ZNT = G::ZNT;
@@ -240,22 +275,23 @@ class BoxT final : public BoxBase {
         uniform = UNIFORM;
         // variables.f90:112-123: Nspec from the 2/3-rule extent of each axis
         const double* al = cfg.ph.a;
-        const double ex = (double(N) / 2.0) * (2.0 * al[0] / 3.0), ey = (double(N) / 2.0) * (2.0 * al[1] / 3.0),
-                     ez = (double(N) / 2.0) * (2.0 * al[2] / 3.0);
+        const double ex = (double(NX) / 2.0) * (2.0 * al[0] / 3.0), ey = (double(NY) / 2.0) * (2.0 * al[1] / 3.0),
+                     ez = (double(NZ) / 2.0) * (2.0 * al[2] / 3.0);
         if (cfg.ph.spherical) nspec = int(ex / cfg.deltak) - 1;
         else nspec = int(std::sqrt(ex * ex + ey * ey + ez * ez) / cfg.deltak) - 1;
         if (nspec < 1) nspec = 1;
         // the pruned passes skip |kval| > KMAX; that is exact when the mask removes index KMAX + 1
-        // on every axis (|kval| alpha >= (N/2)(2 alpha/3), evaluated as the kernels do)
+        // on every axis (|kval| alpha_d >= (N_d/2)(2 alpha_d/3), evaluated as the kernels do)
+        const int kmax[3] = {Keep<NX>::KMAX, Keep<NY>::KMAX, Keep<NZ>::KMAX};
         for (int d = 0; d < 3; ++d)
-            if (!(std::fabs(double(Keep<N>::KMAX + 1) * al[d]) >= cfg.ph.thr[d])) prunable = false;
+            if (!(std::fabs(double(kmax[d] + 1) * al[d]) >= cfg.ph.thr[d])) prunable = false;
         const size_t fb = size_t(fs) * sizeof(C);
         try {
             DNSBOX_HIP(hipMalloc(&state, 3 * size_t(fs) * sizeof(CS)));
             DNSBOX_HIP(hipMalloc(&pred, 3 * fb));
             DNSBOX_HIP(hipMalloc(&scratch, 6 * fb));
             DNSBOX_HIP(hipMalloc(&nonlin, 3 * fb));
-            DNSBOX_HIP(hipMalloc(&tw, N * sizeof(C)));
+            DNSBOX_HIP(hipMalloc(&tw, TwOff<Dm>::LEN * sizeof(C)));
             DNSBOX_HIP(hipMalloc(&sc, sizeof(BoxScalars)));
             red_blocks = 512;
             DNSBOX_HIP(hipMalloc(&red_partial, size_t(red_blocks) * (DG_NSCAL + nspec) * sizeof(double)));
@@ -265,9 +301,9 @@ class BoxT final : public BoxBase {
             DNSBOX_HIP(hipMemset(state, 0, 3 * size_t(fs) * sizeof(CS)));
             DNSBOX_HIP(hipMemset(pred, 0, 3 * fb));
             DNSBOX_HIP(hipMemset(nonlin, 0, 3 * fb));
-            std::vector<C> h(N);
-            host_twiddles<N, T>(h.data());
-            DNSBOX_HIP(hipMemcpy(tw, h.data(), N * sizeof(C), hipMemcpyHostToDevice));
+            std::vector<C> h(TwOff<Dm>::LEN);
+            host_box_twiddles<Dm, T>(h.data());
+            DNSBOX_HIP(hipMemcpy(tw, h.data(), h.size() * sizeof(C), hipMemcpyHostToDevice));
             BoxScalars s{};
             s.dt = s.dt_prev = cfg.dt;
             DNSBOX_HIP(hipMemcpy(sc, &s, sizeof(s), hipMemcpyHostToDevice));
@@ -299,11 +335,14 @@ class BoxT final : public BoxBase {
     }
 
     const C* twc() const { return static_cast<const C*>(tw); }
+    const C* twx() const { return twc() + TwOff<Dm>::X; }   // x-pass twiddles (length NX)
+    const C* twy() const { return twc() + TwOff<Dm>::Y; }   // y passes (NY)
+    const C* twz() const { return twc() + TwOff<Dm>::Z; }   // z passes (NZ)
     ProCurl<T> curl() const { return ProCurl<T>{inv_n3(), KScale<T>::of(cfg.ph)}; }
     C* X() const { return static_cast<C*>(scratch); }
     CS* St() const { return static_cast<CS*>(state); }
     C* Pr() const { return static_cast<C*>(pred); }
-    static T inv_n3() { return T(1.0 / (double(N) * N * N)); }
+    static T inv_n3() { return T(1.0 / (double(NX) * NY * NZ)); }
 
     void require_uniform(const char* what) const
     {
@@ -325,16 +364,16 @@ class BoxT final : public BoxBase {
     void eband(const cpx<E>* u)
     {
         if (!cfg.ph.bandlim) return;
-        const double sm = 1.0 / (double(N) * N * N);
-        hipLaunchKernelGGL((k_eband<N, E>), dim3(1), dim3(256), 0, stream, u, fs, cfg.ph, sm * sm, sc);
+        const double sm = 1.0 / (double(NX) * NY * NZ);
+        hipLaunchKernelGGL((k_eband<Dm, E>), dim3(1), dim3(256), 0, stream, u, fs, cfg.ph, sm * sm, sc);
         DNSBOX_LAUNCH_CHECK();
     }
 
     template <class E, bool PR = false>
     void zinv_curl(const cpx<E>* in)
     {
-        hipLaunchKernelGGL((k_zinv<N, T, ProCurl<T>, E, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream, in,
-                           fs, X(), fs, twc(), curl());
+        hipLaunchKernelGGL((k_zinv<Dm, T, ProCurl<T>, E, PR>), dim3(ztiles<Dm, T, PR>()), dim3(ZNT), 0, stream, in,
+                           fs, X(), fs, twz(), curl());
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -354,30 +393,30 @@ class BoxT final : public BoxBase {
     template <bool PR = false>
     void rhs_middle(bool track_cfl)
     {
-        if constexpr (PlaneGeo<N, T, OpCross<T>>::used) {
+        if constexpr (plane_used<Dm, T, OpCross<T>>()) {
             if (plane_enabled()) {
-                hipLaunchKernelGGL((k_plane<N, T, OpCross<T>, PR, PR>), dim3(N), dim3(kPlaneThreads), 0, stream, X(),
+                hipLaunchKernelGGL((k_plane<NX, T, OpCross<T>, PR, PR>), dim3(NX), dim3(kPlaneThreads), 0, stream, X(),
                                    fs, twc(), track_cfl ? sc->maxu : nullptr);
                 DNSBOX_LAUNCH_CHECK();
                 mark(2);
                 return;
             }
         }
-        hipLaunchKernelGGL((k_ycol<N, T, +1, PR>), dim3(ytiles<N, T, PR>(), 6), dim3(YNT), 0, stream, X(), fs,
-                           twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, +1, PR>), dim3(ytiles<Dm, T, PR>(), 6), dim3(YNT), 0, stream, X(), fs,
+                           twy());
         DNSBOX_LAUNCH_CHECK();
         mark(1);
-        if constexpr (XReg<N, T>::enabled && !kLegacyXPass) {
-            hipLaunchKernelGGL((xreg::k_xcross_r<N, T, PR>), dim3(N * N / xreg::Shape<N>::LPB), dim3(xreg::kThreads),
-                               0, stream, X(), X(), fs, twc(), track_cfl ? sc->maxu : nullptr);
+        if constexpr (xreg_used<Dm, T>()) {
+            hipLaunchKernelGGL((xreg::k_xcross_r<NX, T, PR>), dim3(NY * NZ / xreg::Shape<NX>::LPB),
+                               dim3(xreg::kThreads), 0, stream, X(), X(), fs, twx(), track_cfl ? sc->maxu : nullptr);
         } else {
-            hipLaunchKernelGGL((k_xpass<N, T, OpCross<T>, PR>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
-                               stream, X(), fs, X(), fs, twc(), track_cfl ? sc->maxu : nullptr, 0);
+            hipLaunchKernelGGL((k_xpass<Dm, T, OpCross<T>, PR>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, X(), fs,
+                               X(), fs, twx(), track_cfl ? sc->maxu : nullptr, 0);
         }
         DNSBOX_LAUNCH_CHECK();
         mark(2);
-        hipLaunchKernelGGL((k_ycol<N, T, -1, PR>), dim3(ytiles<N, T, PR>(), 3), dim3(YNT), 0, stream, X(), fs,
-                           twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, -1, PR>), dim3(ytiles<Dm, T, PR>(), 3), dim3(YNT), 0, stream, X(), fs,
+                           twy());
         DNSBOX_LAUNCH_CHECK();
         mark(3);
     }
@@ -388,7 +427,7 @@ class BoxT final : public BoxBase {
     {
         zinv_curl(in);
         rhs_middle(track_cfl);
-        hipLaunchKernelGGL((k_zfwd<N, T, MODE, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), ea);
+        hipLaunchKernelGGL((k_zfwd<Dm, T, MODE, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), ea);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -407,7 +446,7 @@ class BoxT final : public BoxBase {
     {
         require_uniform("rhs on device buffers");
         x6_valid = false;
-        launch_rhs<N, T>(stream, cfg.ph, static_cast<const C*>(in), static_cast<C*>(out), X(), twc(), sc);
+        launch_rhs<Dm, T>(stream, cfg.ph, static_cast<const C*>(in), static_cast<C*>(out), X(), twc(), sc);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -451,8 +490,8 @@ class BoxT final : public BoxBase {
         ea.state = St();
         ea.rhs_in = St();
         ea.rhs_in_s = 1;
-        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_PRED, S, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream,
-                           X(), fs, twc(), ea, pro);
+        hipLaunchKernelGGL((k_zfwd_zinv<Dm, T, EPI_PRED, S, PR>), dim3(ztiles<Dm, T, PR>()), dim3(ZNT), 0, stream,
+                           X(), fs, twz(), ea, pro);
         DNSBOX_LAUNCH_CHECK();
         mark(4);
         eband(pst);
@@ -461,8 +500,8 @@ class BoxT final : public BoxBase {
         eb.state = St();
         eb.rhs_in = pst;
         eb.rhs_in_s = 0;
-        hipLaunchKernelGGL((k_zfwd_zinv<N, T, EPI_CORR, S, PR>), dim3(ztiles<N, T, PR>()), dim3(ZNT), 0, stream,
-                           X(), fs, twc(), eb, pro);
+        hipLaunchKernelGGL((k_zfwd_zinv<Dm, T, EPI_CORR, S, PR>), dim3(ztiles<Dm, T, PR>()), dim3(ZNT), 0, stream,
+                           X(), fs, twz(), eb, pro);
         DNSBOX_LAUNCH_CHECK();
         mark(5);   // the corrector pass also ended the step (t, dt_prev, CFL maxima)
         eband(St());
@@ -582,9 +621,9 @@ class BoxT final : public BoxBase {
 
     void spec_reduce() override
     {
-        const double sm = 1.0 / (double(N) * N * N);
+        const double sm = 1.0 / (double(NX) * NY * NZ);
         const int nslot = DG_NSCAL + nspec;
-        hipLaunchKernelGGL((k_spec_reduce<N, S>), dim3(red_blocks), dim3(256), spec_reduce_lds(nspec), stream,
+        hipLaunchKernelGGL((k_spec_reduce<Dm, S>), dim3(red_blocks), dim3(256), spec_reduce_lds(nspec), stream,
                            St(), fs, cfg.ph, cfg.deltak, nspec, red_partial);
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_spec_finish<0>, dim3(1), dim3(256), 0, stream, red_partial, red_blocks, nslot,
@@ -596,13 +635,13 @@ class BoxT final : public BoxBase {
     {
         x6_valid = false;
         DNSBOX_HIP(hipMemsetAsync(red_bits, 0, 4 * sizeof(unsigned long long), stream));
-        hipLaunchKernelGGL((k_zinv<N, T, ProVelDiv<T>, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, St(), fs, X(),
-                           fs, twc(), ProVelDiv<T>{inv_n3(), KScale<T>::of(cfg.ph)});
+        hipLaunchKernelGGL((k_zinv<Dm, T, ProVelDiv<T>, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, St(), fs, X(),
+                           fs, twz(), ProVelDiv<T>{inv_n3(), KScale<T>::of(cfg.ph)});
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 4), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, +1>), dim3(G::YTILES, 4), dim3(YNT), 0, stream, X(), fs, twy());
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_xpass<N, T, OpVelDivMax<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream,
-                           X(), fs, X(), fs, twc(), red_bits, 1);
+        hipLaunchKernelGGL((k_xpass<Dm, T, OpVelDivMax<T>>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, X(), fs, X(),
+                           fs, twx(), red_bits, 1);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -610,7 +649,7 @@ class BoxT final : public BoxBase {
     {
         x6_valid = false;
         dealiased = true;   // the generator leaves every masked mode empty
-        hipLaunchKernelGGL((k_init_solenoidal<N, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
+        hipLaunchKernelGGL((k_init_solenoidal<Dm, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
                            cfg.ph, seed, box, k0);
         DNSBOX_LAUNCH_CHECK();
         spec_reduce();
@@ -623,7 +662,7 @@ class BoxT final : public BoxBase {
     {
         x6_valid = false;
         if (dealias) dealiased = prunable;   // state_dealias zeroes every mode outside the kept cube
-        hipLaunchKernelGGL((k_project_dealias<N, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
+        hipLaunchKernelGGL((k_project_dealias<Dm, S>), dim3((fs + 255) / 256), dim3(256), 0, stream, St(), fs,
                            cfg.ph, dealias, proj);
         DNSBOX_LAUNCH_CHECK();
     }
@@ -631,15 +670,15 @@ class BoxT final : public BoxBase {
     void to_physical(void* phys3) override
     {
         x6_valid = false;
-        hipLaunchKernelGGL((k_zinv<N, T, ProScale<T>, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, St(), fs, X(),
-                           fs, twc(), ProScale<T>{inv_n3()});
+        hipLaunchKernelGGL((k_zinv<Dm, T, ProScale<T>, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, St(), fs, X(),
+                           fs, twz(), ProScale<T>{inv_n3()});
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, +1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twy());
         DNSBOX_LAUNCH_CHECK();
-        // real output field f at phys3 + f * N^3 reals
-        const long long pf = (long long)N * N * N / 2;  // complex pairs per real field
-        hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, true, false>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
-                           stream, X(), fs, static_cast<C*>(phys3), pf, twc(), (unsigned long long*)nullptr, 0);
+        // real output field f at phys3 + f * NX NY NZ reals
+        const long long pf = Dm::NP / 2;  // complex pairs per real field
+        hipLaunchKernelGGL((k_xpass<Dm, T, OpCopy<T, 3, true, false>>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream,
+                           X(), fs, static_cast<C*>(phys3), pf, twx(), (unsigned long long*)nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -647,16 +686,15 @@ class BoxT final : public BoxBase {
     {
         x6_valid = false;
         dealiased = false;   // a physical field may hold energy on every mode
-        const long long pf = (long long)N * N * N / 2;
-        hipLaunchKernelGGL((k_xpass<N, T, OpCopy<T, 3, false, true>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
-                           stream, static_cast<const C*>(phys3), pf, X(), fs, twc(), (unsigned long long*)nullptr,
-                           0);
+        const long long pf = Dm::NP / 2;
+        hipLaunchKernelGGL((k_xpass<Dm, T, OpCopy<T, 3, false, true>>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream,
+                           static_cast<const C*>(phys3), pf, X(), fs, twx(), (unsigned long long*)nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twy());
         DNSBOX_LAUNCH_CHECK();
         EpiArgs<T, S> ea = epi();
         ea.state = St();
-        hipLaunchKernelGGL((k_zfwd<N, T, EPI_STORE, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
+        hipLaunchKernelGGL((k_zfwd<Dm, T, EPI_STORE, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(),
                            ea);
         DNSBOX_LAUNCH_CHECK();
     }
@@ -700,36 +738,36 @@ class BoxT final : public BoxBase {
     template <int MODE, bool PR>
     void vjp_passes_t(const C* u, const C* lam, double lam_scale, VjpArgs<T> va)
     {
-        const double sm = 1.0 / (double(N) * N * N);
+        const double sm = 1.0 / (double(NX) * NY * NZ);
         if (cfg.ph.bandlim) {
-            hipLaunchKernelGGL((k_band_vjp_scalars<N, T>), dim3(1), dim3(256), 0, stream, u, lam, fs, cfg.ph,
+            hipLaunchKernelGGL((k_band_vjp_scalars<Dm, T>), dim3(1), dim3(256), 0, stream, u, lam, fs, cfg.ph,
                                lam_scale, sm * sm, band2);
             DNSBOX_LAUNCH_CHECK();
         }
         Ptrs<T, 6> in{{lam, lam + fs, lam + 2 * fs, u, u + fs, u + 2 * fs}};
-        ProVjpIn<T> pro{T(sm), T(lam_scale), N, cfg.ph};
+        ProVjpIn<T> pro{T(sm), T(lam_scale), NX, cfg.ph};
         if constexpr (PR) {
-            hipLaunchKernelGGL((k_zinv_p_chk<N, T, ProVjpIn<T>>), dim3(ztiles<N, T, true>() + vjp_check_blocks<N>()),
-                               dim3(ZNT), 0, stream, in, X(), fs, twc(), pro, u, fs, chk + 1, ztiles<N, T, true>());
+            hipLaunchKernelGGL((k_zinv_p_chk<Dm, T, ProVjpIn<T>>), dim3(ztiles<Dm, T, true>() + vjp_check_blocks<Dm>()),
+                               dim3(ZNT), 0, stream, in, X(), fs, twz(), pro, u, fs, chk + 1, ztiles<Dm, T, true>());
         } else {
-            hipLaunchKernelGGL((k_zinv_p<N, T, ProVjpIn<T>>), dim3(ztiles<N, T, false>()), dim3(ZNT), 0, stream, in,
-                               X(), fs, twc(), pro);
+            hipLaunchKernelGGL((k_zinv_p<Dm, T, ProVjpIn<T>>), dim3(ztiles<Dm, T, false>()), dim3(ZNT), 0, stream, in,
+                               X(), fs, twz(), pro);
         }
         DNSBOX_LAUNCH_CHECK();
-        if (use_plane<N, T, OpVjp<T>>()) {
-            if constexpr (PlaneGeo<N, T, OpVjp<T>>::used) {
-                hipLaunchKernelGGL((k_plane<N, T, OpVjp<T>, PR, false>), dim3(N), dim3(kPlaneThreads), 0, stream, X(),
-                                   fs, twc(), (unsigned long long*)nullptr);
+        if (use_plane<Dm, T, OpVjp<T>>()) {
+            if constexpr (plane_used<Dm, T, OpVjp<T>>()) {
+                hipLaunchKernelGGL((k_plane<NX, T, OpVjp<T>, PR, false>), dim3(NX), dim3(kPlaneThreads), 0, stream,
+                                   X(), fs, twc(), (unsigned long long*)nullptr);
                 DNSBOX_LAUNCH_CHECK();
             }
         } else {
-            hipLaunchKernelGGL((k_ycol<N, T, +1, PR>), dim3(ytiles<N, T, PR>(), 9), dim3(YNT), 0, stream, X(), fs,
-                               twc());
+            hipLaunchKernelGGL((k_ycol<Dm, T, +1, PR>), dim3(ytiles<Dm, T, PR>(), 9), dim3(YNT), 0, stream, X(), fs,
+                               twy());
             DNSBOX_LAUNCH_CHECK();
-            hipLaunchKernelGGL((k_xpass<N, T, OpVjp<T>, PR, false>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0,
-                               stream, X(), fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+            hipLaunchKernelGGL((k_xpass<Dm, T, OpVjp<T>, PR, false>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, X(),
+                               fs, X(), fs, twx(), (unsigned long long*)nullptr, 0);
             DNSBOX_LAUNCH_CHECK();
-            hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twc());
+            hipLaunchKernelGGL((k_ycol<Dm, T, -1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twy());
             DNSBOX_LAUNCH_CHECK();
         }
         va.u = u;
@@ -738,7 +776,7 @@ class BoxT final : public BoxBase {
         va.band = band2;
         va.fs = fs;
         va.ph = cfg.ph;
-        hipLaunchKernelGGL((k_zfwd_vjp<N, T, MODE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), va);
+        hipLaunchKernelGGL((k_zfwd_vjp<Dm, T, MODE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), va);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -772,7 +810,7 @@ class BoxT final : public BoxBase {
         ea.rhs_in_s = 0;
         ea.dt_fixed = dt;
         pred_clean = false;
-        hipLaunchKernelGGL((k_zfwd<N, T, EPI_PREDONLY, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(),
+        hipLaunchKernelGGL((k_zfwd<Dm, T, EPI_PREDONLY, S>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(),
                            ea);
         DNSBOX_LAUNCH_CHECK();
         step_vjp_at(u, Pr(), lam_, lam_out, dt);
@@ -823,31 +861,31 @@ class BoxT final : public BoxBase {
         aa.lin_sign = lin_sign;
         aa.ph = cfg.ph;
         Ptrs<T, 3> in1{{u, u + fs, u + 2 * fs}};
-        hipLaunchKernelGGL((k_zinv_p<N, T, ProGrad12<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in1, X(), fs,
-                           twc(), ProGrad12<T>{s, KScale<T>::of(cfg.ph)});
+        hipLaunchKernelGGL((k_zinv_p<Dm, T, ProGrad12<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in1, X(), fs,
+                           twz(), ProGrad12<T>{s, KScale<T>::of(cfg.ph)});
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 12), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, +1>), dim3(G::YTILES, 12), dim3(YNT), 0, stream, X(), fs, twy());
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_xpass<N, T, OpConv<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream, X(),
-                           fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+        hipLaunchKernelGGL((k_xpass<Dm, T, OpConv<T>>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, X(), fs, X(), fs,
+                           twx(), (unsigned long long*)nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twy());
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_zfwd_adj<N, T, 1>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), aa);
+        hipLaunchKernelGGL((k_zfwd_adj<Dm, T, 1>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), aa);
         DNSBOX_LAUNCH_CHECK();
         const C* ut = adjUt();
         Ptrs<T, 6> in2{{ut, ut + fs, ut + 2 * fs, u, u + fs, u + 2 * fs}};
-        hipLaunchKernelGGL((k_zinv_p<N, T, ProAdj12<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in2, X(), fs,
-                           twc(), ProAdj12<T>{s, KScale<T>::of(cfg.ph)});
+        hipLaunchKernelGGL((k_zinv_p<Dm, T, ProAdj12<T>>), dim3(G::ZTILES), dim3(ZNT), 0, stream, in2, X(), fs,
+                           twz(), ProAdj12<T>{s, KScale<T>::of(cfg.ph)});
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, +1>), dim3(G::YTILES, 12), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, +1>), dim3(G::YTILES, 12), dim3(YNT), 0, stream, X(), fs, twy());
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_xpass<N, T, OpAdjN<T>>), dim3(G::XBLOCKS), dim3(Cfg<N, T>::NTX), 0, stream, X(),
-                           fs, X(), fs, twc(), (unsigned long long*)nullptr, 0);
+        hipLaunchKernelGGL((k_xpass<Dm, T, OpAdjN<T>>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, X(), fs, X(), fs,
+                           twx(), (unsigned long long*)nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_ycol<N, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twc());
+        hipLaunchKernelGGL((k_ycol<Dm, T, -1>), dim3(G::YTILES, 3), dim3(YNT), 0, stream, X(), fs, twy());
         DNSBOX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_zfwd_adj<N, T, 2>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twc(), aa);
+        hipLaunchKernelGGL((k_zfwd_adj<Dm, T, 2>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), aa);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -874,7 +912,7 @@ class BoxT final : public BoxBase {
             rhs_adj(us, g1, lin_sign);
             hipLaunchKernelGGL((k_axpy<T, true>), eg, dim3(256), 0, stream, u, u, g0, dtau, n3);
             hipLaunchKernelGGL((k_axpy<T, true>), eg, dim3(256), 0, stream, u, u, g1, dtau, n3);
-            hipLaunchKernelGGL((k_project_dealias<N, T>), dim3((fs + 255) / 256), dim3(256), 0, stream, u, fs,
+            hipLaunchKernelGGL((k_project_dealias<Dm, T>), dim3((fs + 255) / 256), dim3(256), 0, stream, u, fs,
                                cfg.ph, 0, 1);
             DNSBOX_LAUNCH_CHECK();
         }
@@ -892,7 +930,7 @@ class BoxT final : public BoxBase {
     bool buffer_dealiased(const CS* st)
     {
         DNSBOX_HIP(hipMemsetAsync(chk, 0, sizeof(unsigned int), stream));
-        hipLaunchKernelGGL((k_outside_cube<N, S>), dim3((fs + 255) / 256 < 1024 ? (fs + 255) / 256 : 1024), dim3(256),
+        hipLaunchKernelGGL((k_outside_cube<Dm, S>), dim3((fs + 255) / 256 < 1024 ? (fs + 255) / 256 : 1024), dim3(256),
                            0, stream, st, fs, chk);
         DNSBOX_LAUNCH_CHECK();
         unsigned int h = 1;
@@ -949,19 +987,23 @@ class BoxT final : public BoxBase {
 
     void pass_bytes(double* b) const override
     {
-        const double cb = double(sizeof(C)), sb = double(sizeof(CS)), n = double(N);
+        const double cb = double(sizeof(C)), sb = double(sizeof(CS));
+        const double ny = double(NY), nz = double(NZ);
         if (pruned()) {
-            const double kx = Keep<N>::KX, ky = Keep<N>::KY, zc = kx * ky;
-            const double pr = cfg.ph.bandlim ? 3.0 * zc * ky * cb : 0.0;   // predicted state (band forcing)
-            b[0] = 3.0 * zc * ky * sb + 6.0 * zc * n * cb;                 // state over Kz -> 6 fields, all z
-            b[1] = 6.0 * n * kx * (ky + n) * cb;                           // ky in Ky read, all y written
-            b[2] = 9.0 * n * n * kx * cb;                                  // 6 lines read, 3 written, kx in Kx
-            b[3] = 3.0 * n * kx * (n + ky) * cb;                           // all y read, ky in Ky written
-            b[4] = 9.0 * zc * n * cb + 6.0 * zc * ky * sb + pr;            // 3 + 6 fields all z, state over Kz
-            b[5] = 9.0 * zc * n * cb + 6.0 * zc * ky * sb;
-            if (use_plane<N, T, OpCross<T>>()) {   // one launch for the three middle passes
-                b[1] = b[3] = 0.0;
-                b[2] = plane_bytes<N, T, OpCross<T>, true, true>();
+            // kept kx, ky and kz indices; zc kept (ky, kx) columns of the z passes
+            const double kx = Keep<NX>::KX, ky = Keep<NY>::KY, kz = Keep<NZ>::KY, zc = kx * ky;
+            const double pr = cfg.ph.bandlim ? 3.0 * zc * kz * cb : 0.0;   // predicted state (band forcing)
+            b[0] = 3.0 * zc * kz * sb + 6.0 * zc * nz * cb;                // state over Kz -> 6 fields, all z
+            b[1] = 6.0 * nz * kx * (ky + ny) * cb;                         // ky in Ky read, all y written
+            b[2] = 9.0 * ny * nz * kx * cb;                                // 6 lines read, 3 written, kx in Kx
+            b[3] = 3.0 * nz * kx * (ny + ky) * cb;                         // all y read, ky in Ky written
+            b[4] = 9.0 * zc * nz * cb + 6.0 * zc * kz * sb + pr;           // 3 + 6 fields all z, state over Kz
+            b[5] = 9.0 * zc * nz * cb + 6.0 * zc * kz * sb;
+            if constexpr (plane_used<Dm, T, OpCross<T>>()) {
+                if (use_plane<Dm, T, OpCross<T>>()) {   // one launch for the three middle passes
+                    b[1] = b[3] = 0.0;
+                    b[2] = plane_bytes<NX, T, OpCross<T>, true, true>();
+                }
             }
         } else {
             const double f = double(G::FS);
@@ -971,9 +1013,11 @@ class BoxT final : public BoxBase {
             b[3] = f * 6.0 * cb;
             b[4] = f * (9.0 * cb + 6.0 * sb) + (cfg.ph.bandlim ? f * 3.0 * cb : 0.0);
             b[5] = f * (9.0 * cb + 6.0 * sb);
-            if (use_plane<N, T, OpCross<T>>()) {
-                b[1] = b[3] = 0.0;
-                b[2] = plane_bytes<N, T, OpCross<T>, false, false>();
+            if constexpr (plane_used<Dm, T, OpCross<T>>()) {
+                if (use_plane<Dm, T, OpCross<T>>()) {
+                    b[1] = b[3] = 0.0;
+                    b[2] = plane_bytes<NX, T, OpCross<T>, false, false>();
+                }
             }
         }
     }
@@ -1048,16 +1092,16 @@ class BoxT final : public BoxBase {
 
     void from_ref_layout(void* dst, const double* src, bool state_type) override
     {
-        const dim3 grid((N + 31) / 32, (G::H + 31) / 32, N);
-        if (state_type) hipLaunchKernelGGL((k_ref_layout<S, true>), grid, dim3(256), 0, stream, src, dst, N);
-        else hipLaunchKernelGGL((k_ref_layout<T, true>), grid, dim3(256), 0, stream, src, dst, N);
+        const dim3 grid((NZ + 31) / 32, (G::H + 31) / 32, NY);
+        if (state_type) hipLaunchKernelGGL((k_ref_layout<S, true>), grid, dim3(256), 0, stream, src, dst, NX, NY, NZ);
+        else hipLaunchKernelGGL((k_ref_layout<T, true>), grid, dim3(256), 0, stream, src, dst, NX, NY, NZ);
         DNSBOX_LAUNCH_CHECK();
     }
     void to_ref_layout(double* dst, const void* src, bool state_type) override
     {
-        const dim3 grid((N + 31) / 32, (G::H + 31) / 32, N);
-        if (state_type) hipLaunchKernelGGL((k_ref_layout<S, false>), grid, dim3(256), 0, stream, src, dst, N);
-        else hipLaunchKernelGGL((k_ref_layout<T, false>), grid, dim3(256), 0, stream, src, dst, N);
+        const dim3 grid((NZ + 31) / 32, (G::H + 31) / 32, NY);
+        if (state_type) hipLaunchKernelGGL((k_ref_layout<S, false>), grid, dim3(256), 0, stream, src, dst, NX, NY, NZ);
+        else hipLaunchKernelGGL((k_ref_layout<T, false>), grid, dim3(256), 0, stream, src, dst, NX, NY, NZ);
         DNSBOX_LAUNCH_CHECK();
     }
     bool state_confined() override { return buffer_dealiased(St()); }
@@ -1082,20 +1126,31 @@ class BoxT final : public BoxBase {
     }
 };
 
+// a box of one of the non-cubic grids of grids.hpp (box_grid_*.hip); null for another grid
+template <class T, class S>
+BoxBase* make_box_grid(const BoxConfig& c);
+template <>
+BoxBase* make_box_grid<double, double>(const BoxConfig& c);
+template <>
+BoxBase* make_box_grid<float, float>(const BoxConfig& c);
+template <>
+BoxBase* make_box_grid<float, double>(const BoxConfig& c);
+
 template <class T, class S>
 BoxBase* make_box_t(const BoxConfig& c)
 {
+    if (!(c.nx == c.ny && c.ny == c.nz)) return make_box_grid<T, S>(c);
 #ifdef DNSBOX_ONLY_N
-    return c.n == DNSBOX_ONLY_N ? new BoxT<DNSBOX_ONLY_N, T, S>(c) : nullptr;
+    return c.nx == DNSBOX_ONLY_N ? new BoxT<Cube<DNSBOX_ONLY_N>, T, S>(c) : nullptr;
 #endif
-    switch (c.n) {
-        case 16: return new BoxT<16, T, S>(c);
-        case 32: return new BoxT<32, T, S>(c);
-        case 48: return new BoxT<48, T, S>(c);
-        case 64: return new BoxT<64, T, S>(c);
-        case 128: return new BoxT<128, T, S>(c);
-        case 256: return new BoxT<256, T, S>(c);
-        case 512: return new BoxT<512, T, S>(c);
+    switch (c.nx) {
+        case 16: return new BoxT<Cube<16>, T, S>(c);
+        case 32: return new BoxT<Cube<32>, T, S>(c);
+        case 48: return new BoxT<Cube<48>, T, S>(c);
+        case 64: return new BoxT<Cube<64>, T, S>(c);
+        case 128: return new BoxT<Cube<128>, T, S>(c);
+        case 256: return new BoxT<Cube<256>, T, S>(c);
+        case 512: return new BoxT<Cube<512>, T, S>(c);
         default: return nullptr;
     }
 }

@@ -10,6 +10,11 @@ inline BoxConfig make_config(const dnsbox_params* p)
 {
     BoxConfig bc;
     bc.n = p->n;
+    // the grid (parameters.f90:12-14): nx, ny, nz, each 0 for n
+    bc.nx = p->nx > 0 ? p->nx : p->n;
+    bc.ny = p->ny > 0 ? p->ny : p->n;
+    bc.nz = p->nz > 0 ? p->nz : p->n;
+    const int nd[3] = {bc.nx, bc.ny, bc.nz};
     bc.precision = p->precision;
     bc.state_precision = p->state_precision == 64 ? 64 : p->precision;
     bc.deltak = p->deltak > 0 ? p->deltak : 1.0;
@@ -24,8 +29,8 @@ inline BoxConfig make_config(const dnsbox_params* p)
     const double al[3] = {p->alpha_x, p->alpha_y, p->alpha_z};
     for (int d = 0; d < 3; ++d) ph.a[d] = al[d] > 0.0 ? double(float(al[d])) : 1.0;
     // rhs.f90:74-88 / state.f90:184-197 thresholds, evaluated as the reference writes them
-    for (int d = 0; d < 3; ++d) ph.thr[d] = (double(p->n) / 2.0) * (2.0 * ph.a[d] / 3.0);
-    const double ts = (double(p->n) / 2.0) * 0.5 * ph.a[0];
+    for (int d = 0; d < 3; ++d) ph.thr[d] = (double(nd[d]) / 2.0) * (2.0 * ph.a[d] / 3.0);
+    const double ts = (double(bc.nx) / 2.0) * 0.5 * ph.a[0];
     ph.sph_thr2 = ts * ts;
     ph.sph_dthr2 = ph.thr[0] * ph.thr[0];
     ph.cmin = p->cfl_min;
@@ -36,6 +41,7 @@ inline BoxConfig make_config(const dnsbox_params* p)
     ph.spherical = p->spherical;
     ph.parity = p->parity_mode;
     ph.adaptive = p->adaptive_dt;
+    for (int d = 0; d < 3; ++d) ph.nyq[d] = nd[d] / 2;
     return bc;
 }
 

@@ -12,6 +12,7 @@
 #include "../../include/dnsbox.h"
 #include "box_impl.hpp"
 #include "config.hpp"
+#include "grids.hpp"
 #include "group.hpp"
 
 using dnsbox::BoxBase;
@@ -147,18 +148,35 @@ void invalidate(dnsbox_ctx* c)
     if (c->group) c->group->g->invalidate();
 }
 
+// the grid of a parameter set: nx, ny, nz, each 0 for n
+void grid_of(const dnsbox_params* p, int* g)
+{
+    g[0] = p->nx > 0 ? p->nx : p->n;
+    g[1] = p->ny > 0 ? p->ny : p->n;
+    g[2] = p->nz > 0 ? p->nz : p->n;
+}
+
 int check_params(const dnsbox_params* p)
 {
-    if (!supported_n(p->n))
-        return fail(DNSBOX_E_ARG, "unsupported n=" + std::to_string(p->n) +
-                                      " (supported: 16, 32, 48, 64, 128, 256, 512)");
+    if (p->nx < 0 || p->ny < 0 || p->nz < 0) return fail(DNSBOX_E_ARG, "nx, ny, nz must be >= 0 (0 = n)");
+    int g[3];
+    grid_of(p, g);
+    const std::string gs = std::to_string(g[0]) + " x " + std::to_string(g[1]) + " x " + std::to_string(g[2]);
+    if (g[0] == g[1] && g[1] == g[2]) {
+        if (!supported_n(g[0]))
+            return fail(DNSBOX_E_ARG, "unsupported grid " + gs + " (cubic grids: 16, 32, 48, 64, 128, 256, 512)");
+    } else if (!dnsbox::grid_listed(g[0], g[1], g[2])) {
+        return fail(DNSBOX_E_ARG, "grid nx x ny x nz = " + gs +
+                                      " is not built into this library (cubic grids 16 .. 512, and " +
+                                      dnsbox::grid_list_text() + ")");
+    }
     if (p->precision != 32 && p->precision != 64)
         return fail(DNSBOX_E_ARG, "precision must be 32 or 64");
     if (p->state_precision != 0 && p->state_precision != 32 && p->state_precision != 64)
         return fail(DNSBOX_E_ARG, "state_precision must be 0 (= precision), 32 or 64");
     if (p->state_precision == 32 && p->precision == 64)
         return fail(DNSBOX_E_ARG, "an FP64 box keeps an FP64 state");
-    if (p->kolmo_k < 0 || p->kolmo_k >= p->n / 2) return fail(DNSBOX_E_ARG, "kolmo_k out of range");
+    if (p->kolmo_k < 0 || p->kolmo_k >= g[1] / 2) return fail(DNSBOX_E_ARG, "kolmo_k out of range (0 .. ny/2 - 1)");
     if (p->bandlim && !(p->kcut > 0.0)) return fail(DNSBOX_E_ARG, "bandlim needs kcut > 0");
     if (!(p->dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
     if (p->adaptive_dt && !(p->cfl_max >= p->cfl_min && p->cfl_min > 0.0 && p->dt_max > 0.0))
@@ -166,12 +184,6 @@ int check_params(const dnsbox_params* p)
     const double al[3] = {p->alpha_x, p->alpha_y, p->alpha_z};
     for (double a : al)
         if (!(a >= 0.0) || !(a < 1e30)) return fail(DNSBOX_E_ARG, "alpha_x/y/z must be >= 0 (0 = 1) and finite");
-    const int g[3] = {p->nx, p->ny, p->nz};
-    for (int m : g)
-        if (m != 0 && m != p->n)
-            return fail(DNSBOX_E_ARG, "nx, ny, nz must be 0 or n: this build transforms cubic grids only (n = " +
-                                          std::to_string(p->n) + ", nx, ny, nz = " + std::to_string(p->nx) + ", " +
-                                          std::to_string(p->ny) + ", " + std::to_string(p->nz) + ")");
     return 0;
 }
 
@@ -310,7 +322,7 @@ int dnsbox_set_state_physical(dnsbox_ctx* c, const double* u, const double* v, c
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
     return guarded([&]() -> int {
         BoxBase* b = c->box;
-        const long long n3 = (long long)c->p.n * c->p.n * c->p.n;
+        const long long n3 = (long long)b->cfg.nx * b->cfg.ny * b->cfg.nz;
         std::vector<double> h[3];
         const double* src[3] = {u, v, w};
         for (int f = 0; f < 3; ++f) h[f].assign(src[f], src[f] + n3);
@@ -330,7 +342,7 @@ int dnsbox_get_state_physical(dnsbox_ctx* c, double* u, double* v, double* w)
     if (!u || !v || !w) return fail(DNSBOX_E_ARG, "null array");
     return guarded([&]() -> int {
         BoxBase* b = c->box;
-        const long long n3 = (long long)c->p.n * c->p.n * c->p.n;
+        const long long n3 = (long long)b->cfg.nx * b->cfg.ny * b->cfg.nz;
         DevBuf phys(size_t(3 * n3) * b->rbytes);
         b->to_physical(phys.p);
         std::vector<double> h[3];
@@ -465,13 +477,17 @@ int dnsbox_diagnostics(dnsbox_ctx* c, double* diag, double* espec)
         ob = (ob >> 63) ? (ob & 0x7fffffffffffffffull) : ~ob;
         double divmax;
         std::memcpy(&divmax, &ob, sizeof(double));
-        const double dx = 6.283185307179586476925286766559 / double(c->p.n);
+        const double tp = 6.283185307179586476925286766559;
         const double dt = s.dt_prev;
-        const double* al = b->cfg.ph.a;   // io.f90:664-676: dx_d = L_d / N, L_d = 2 pi / alpha_d
+        const double* al = b->cfg.ph.a;   // io.f90:664-676: dx_d = L_d / N_d, L_d = 2 pi / alpha_d
+        const bool cubic = b->cfg.nx == b->cfg.ny && b->cfg.ny == b->cfg.nz;
         diag[DNSBOX_D_EKIN] = red[dnsbox::DG_EKIN];
         diag[DNSBOX_D_DISP] = red[dnsbox::DG_DISP];
         diag[DNSBOX_D_PROD] = 2.0 * c->p.Q * red[dnsbox::DG_EKIN];
-        diag[DNSBOX_D_COURANT] = (mx[0] * al[0] + mx[1] * al[1] + mx[2] * al[2]) * dt / dx;
+        diag[DNSBOX_D_COURANT] = cubic ? (mx[0] * al[0] + mx[1] * al[1] + mx[2] * al[2]) * dt / (tp / double(b->cfg.nx))
+                                       : mx[0] * dt / ((tp / al[0]) / double(b->cfg.nx)) +
+                                             mx[1] * dt / ((tp / al[1]) / double(b->cfg.ny)) +
+                                             mx[2] * dt / ((tp / al[2]) / double(b->cfg.nz));
         diag[DNSBOX_D_DIVMAX] = divmax;
         diag[DNSBOX_D_EZERO] = red[dnsbox::DG_EZERO];
         diag[DNSBOX_D_EONE] = red[dnsbox::DG_EONE];
@@ -671,6 +687,10 @@ int dnsbox_group_create(const dnsbox_params* params, int nbox, dnsbox_group** ou
         if (int e = check_params(p)) return e;
         if (!(p->n == 16 || p->n == 32 || p->n == 48 || p->n == 64 || p->n == 128))
             return fail(DNSBOX_E_ARG, "group boxes need n in {16, 32, 48, 64, 128}");
+        int g[3];
+        grid_of(p, g);
+        if (!(g[0] == p->n && g[1] == p->n && g[2] == p->n))
+            return fail(DNSBOX_E_ARG, "group boxes are cubic (nx, ny, nz 0 or n)");
         if (p->precision != prec || (p->state_precision != 0 && p->state_precision != prec))
             return fail(DNSBOX_E_ARG, "group boxes share one precision (state included)");
         if (p->bandlim) return fail(DNSBOX_E_ARG, "group boxes do not support band-limited forcing");

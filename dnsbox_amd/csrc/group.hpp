@@ -49,14 +49,14 @@ constexpr int kGroupThreads = 256;
 template <int N, class T>
 constexpr bool group_size_ok()
 {
-    return Geo<N, T>::ZNT == kGroupThreads && Geo<N, T>::YNT == kGroupThreads &&
+    return Geo<Cube<N>, T>::ZNT == kGroupThreads && Geo<Cube<N>, T>::YNT == kGroupThreads &&
            Cfg<N, T>::NTX == kGroupThreads;
 }
 
 template <class T>
 struct GroupLds {
     template <int N>
-    static constexpr int z() { return Geo<N, T>::ZLDS; }
+    static constexpr int z() { return Geo<Cube<N>, T>::ZLDS; }
     template <int N>
     static constexpr int y() { return N * Cfg<N, T>::YW; }
     template <int N, class Op>
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zinv(const GBox<T>* __restr
 #pragma unroll
     for (int f = 0; f < 6; ++f) out[f] = bx.X + f * bx.fs;
     const ProCurl<T> pro{T(1.0 / (double(bx.n) * bx.n * bx.n)), KScale<T>::of(bx.ph)};
-#define DNSBOX_G_ZINV(NN) zinv_tile<NN, T, ProCurl<T>, T, PR>(in, out, tile, lds, tw, pro)
+#define DNSBOX_G_ZINV(NN) zinv_tile<Cube<NN>, T, ProCurl<T>, T, PR>(in, out, tile, lds, tw, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZINV)
 #undef DNSBOX_G_ZINV
 }
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restr
     const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
     const int tile = gt - (PR ? bx.yt0p : bx.yt0);
     cpx<T>* data = bx.X + field * bx.fs;
-#define DNSBOX_G_YCOL(NN) ycol_tile<NN, T, DIR, PR>(data, tile, lds, tw)
+#define DNSBOX_G_YCOL(NN) ycol_tile<Cube<NN>, T, DIR, PR>(data, tile, lds, tw)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_YCOL)
 #undef DNSBOX_G_YCOL
 }
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __rest
     XRed<T> red;
 #pragma unroll
     for (int j = 0; j < 4; ++j) red.m[j] = T(0);
-#define DNSBOX_G_X(NN) xpass_tile<NN, T, Op, PR, PO>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
+#define DNSBOX_G_X(NN) xpass_tile<Cube<NN>, T, Op, PR, PO>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_X)
 #undef DNSBOX_G_X
     if constexpr (Op::NRED == 3) {
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
     ea.sc = bx.sc;
     ea.ph = bx.ph;
     const ProCurl<T> pro{T(1.0 / (double(bx.n) * bx.n * bx.n)), KScale<T>::of(bx.ph)};
-#define DNSBOX_G_Z(NN) zfwd_zinv_tile<NN, T, MODE, T, PR>(bx.X, bx.fs, tile, lds, tw, ea, pro)
+#define DNSBOX_G_Z(NN) zfwd_zinv_tile<Cube<NN>, T, MODE, T, PR>(bx.X, bx.fs, tile, lds, tw, ea, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_Z)
 #undef DNSBOX_G_Z
     // the corrector also ends each box's step (its first tile; no tile of this pass reads the
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_predonly(const GBox<T>
     ea.dt_fixed = bx.dt;
     ea.sc = bx.sc;
     ea.ph = bx.ph;
-#define DNSBOX_G_Z(NN) zfwd_tile<NN, T, EPI_PREDONLY, T>(bx.X, bx.fs, tile, lds, tw, ea)
+#define DNSBOX_G_Z(NN) zfwd_tile<Cube<NN>, T, EPI_PREDONLY, T>(bx.X, bx.fs, tile, lds, tw, ea)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_Z)
 #undef DNSBOX_G_Z
 }
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(kGroupThreads)
         const GBox<T>& bc = gb[b];
         const cpx<T>* p = STAGE == 1 ? bc.pred : bc.u;
         bool nz = false;
-#define DNSBOX_G_OC(NN) nz = outside_cube_nonzero<NN, T>(p, bc.fs, part * (long long)blockDim.x + threadIdx.x, \
+#define DNSBOX_G_OC(NN) nz = outside_cube_nonzero<Cube<NN>, T>(p, bc.fs, part * (long long)blockDim.x + threadIdx.x, \
                                                          (long long)kGroupCheck * blockDim.x)
         DNSBOX_GROUP_SWITCH(bc.n, DNSBOX_G_OC)
 #undef DNSBOX_G_OC
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(kGroupThreads)
     for (int f = 0; f < 9; ++f) out[f] = bx.X + f * bx.fs;
     const double sm = 1.0 / (double(bx.n) * bx.n * bx.n);
     const ProVjpIn<T> pro{T(sm), T(STAGE == 1 ? 0.5 * bx.dt : 1.0), bx.n, bx.ph};
-#define DNSBOX_G_ZV(NN) zinv_tile<NN, T, ProVjpIn<T>, T, PR>(in, out, tile, lds, tw, pro)
+#define DNSBOX_G_ZV(NN) zinv_tile<Cube<NN>, T, ProVjpIn<T>, T, PR>(in, out, tile, lds, tw, pro)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZV)
 #undef DNSBOX_G_ZV
 }
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_vjp(const GBox<T>* __r
     va.band = nullptr;  // group boxes have no band forcing
     va.fs = bx.fs;
     va.ph = bx.ph;
-#define DNSBOX_G_ZF(NN) zfwd_vjp_tile<NN, T, MODE>(bx.X, bx.fs, tile, lds, tw, va)
+#define DNSBOX_G_ZF(NN) zfwd_vjp_tile<Cube<NN>, T, MODE>(bx.X, bx.fs, tile, lds, tw, va)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZF)
 #undef DNSBOX_G_ZF
 }
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(256) k_g_outside_cube(const GBox<T>* __restric
 {
     const GBox<T>& bx = gb[blockIdx.y];
     bool nz = false;
-#define DNSBOX_G_OC(NN) nz = outside_cube_nonzero<NN, T>(bx.st, bx.fs, blockIdx.x * (long long)blockDim.x + threadIdx.x, \
+#define DNSBOX_G_OC(NN) nz = outside_cube_nonzero<Cube<NN>, T>(bx.st, bx.fs, blockIdx.x * (long long)blockDim.x + threadIdx.x, \
                                                          (long long)gridDim.x * blockDim.x)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_OC)
 #undef DNSBOX_G_OC
@@ -346,7 +346,7 @@ inline int group_ytiles(int n, int prec)
 inline int group_ztiles_pruned(int n, int prec)
 {
     int t = 0;
-#define DNSBOX_GT(NN) t = prec == 64 ? ztiles<NN, double, true>() : ztiles<NN, float, true>()
+#define DNSBOX_GT(NN) t = prec == 64 ? ztiles<Cube<NN>, double, true>() : ztiles<Cube<NN>, float, true>()
     DNSBOX_GROUP_SWITCH(n, DNSBOX_GT)
 #undef DNSBOX_GT
     return t;
@@ -354,7 +354,7 @@ inline int group_ztiles_pruned(int n, int prec)
 inline int group_ytiles_pruned(int n, int prec)
 {
     int t = 0;
-#define DNSBOX_GT(NN) t = prec == 64 ? ytiles<NN, double, true>() : ytiles<NN, float, true>()
+#define DNSBOX_GT(NN) t = prec == 64 ? ytiles<Cube<NN>, double, true>() : ytiles<Cube<NN>, float, true>()
     DNSBOX_GROUP_SWITCH(n, DNSBOX_GT)
 #undef DNSBOX_GT
     return t;

@@ -66,14 +66,14 @@ template <int N, class T>
 void rhs_noalloc_t(const BoxPhys& ph, const void* in, void* out, unsigned char* base, hipStream_t s)
 {
     static_assert(N <= kMaxN, "box larger than the scratch array");
-    static_assert(6 * Geo<N, T>::FS * sizeof(cpx<T>) <= kFields, "fields fit the scratch array");
+    static_assert(6 * Geo<Cube<N>, T>::FS * sizeof(cpx<T>) <= kFields, "fields fit the scratch array");
     BoxScalars* sc = reinterpret_cast<BoxScalars*>(base);
     cpx<T>* tw = reinterpret_cast<cpx<T>*>(base + kHead);
     cpx<T>* X = reinterpret_cast<cpx<T>*>(base + kHead + kTw);
     TwTable<T> tab{};
     host_twiddles<N, T>(tab.w);
     hipLaunchKernelGGL((k_put_twiddles<T>), dim3(1), dim3(64), 0, s, tw, tab, N);
-    launch_rhs<N, T>(s, ph, static_cast<const cpx<T>*>(in), static_cast<cpx<T>*>(out), X, tw, sc);
+    launch_rhs<Cube<N>, T>(s, ph, static_cast<const cpx<T>*>(in), static_cast<cpx<T>*>(out), X, tw, sc);
     DNSBOX_LAUNCH_CHECK();
 }
 

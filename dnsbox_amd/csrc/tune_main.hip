@@ -36,7 +36,7 @@ dnsbox_params workload(int adaptive)
     return p;
 }
 
-using Box = BoxT<DNSBOX_TUNE_N, DNSBOX_TUNE_T, DNSBOX_TUNE_S>;
+using Box = BoxT<Cube<DNSBOX_TUNE_N>, DNSBOX_TUNE_T, DNSBOX_TUNE_S>;
 
 }  // namespace
 

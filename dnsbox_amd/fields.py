@@ -31,11 +31,11 @@ def grid(n):
     return (np.arange(n) - n // 2) * (2.0 * np.pi / n)
 
 
-def shapiro(n, nu, t=0.0):
-    """state_init_analytic (state.f90:287-307) at time t: (u, v, w), each (n, n, n)."""
+def shapiro(n, nu, t=0.0, ny=None, nz=None):
+    """state_init_analytic (state.f90:287-307) at time t: (u, v, w), each of shape (nz, ny, n) on the
+    2 pi box (ny, nz default n)."""
     f = np.sqrt(3.0)
-    g = grid(n)
-    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    z, y, x = np.meshgrid(grid(nz or n), grid(ny or n), grid(n), indexing="ij")
     e = np.exp(-(f ** 2) * t * nu)
     u = -0.5 * (f * np.cos(x) * np.sin(y) * np.sin(z) + np.sin(x) * np.cos(y) * np.cos(z)) * e
     v = 0.5 * (f * np.sin(x) * np.cos(y) * np.sin(z) - np.cos(x) * np.sin(y) * np.cos(z)) * e
@@ -44,14 +44,14 @@ def shapiro(n, nu, t=0.0):
 
 
 def init_shapiro(box, t=0.0):
-    box.set_state_physical(*shapiro(box.n, box.params.nu, t))
+    box.set_state_physical(*shapiro(box.nx, box.params.nu, t, box.ny, box.nz))
     box.state_dealias()
     box.state_project()
 
 
 def shapiro_error(box, t):
     """state_check_error (state.f90:309-348): sum_i max |u_i - u_i,exact(t)|."""
-    ex = shapiro(box.n, box.params.nu, t)
+    ex = shapiro(box.nx, box.params.nu, t, box.ny, box.nz)
     got = box.get_state_physical()
     return float(sum(np.max(np.abs(a - b)) for a, b in zip(got, ex)))
 
@@ -78,15 +78,15 @@ def yoffe_spectrum(k, c=YOFFE_C):
 
 def init_yoffe(box, seed=0, c=YOFFE_C, reference_w=True):
     """state_init_rand (state.f90:211-285) on the box, seeded."""
-    n = box.n
+    shape = box.phys_shape
     rng = np.random.default_rng(seed)
-    u = rng.random((n, n, n)) - 0.5
-    v = rng.random((n, n, n)) - 0.5
+    u = rng.random(shape) - 0.5
+    v = rng.random(shape) - 0.5
     if reference_w:
-        rng.random((n, n, n))              # the reference draws u a second time ...
-        w = np.full((n, n, n), -0.5)        # ... and shifts an untouched w by -1/2
+        rng.random(shape)                  # the reference draws u a second time ...
+        w = np.full(shape, -0.5)            # ... and shifts an untouched w by -1/2
     else:
-        w = rng.random((n, n, n)) - 0.5
+        w = rng.random(shape) - 0.5
     box.set_state_physical(u, v, w)
     box.state_dealias()
     box.state_project()
@@ -95,10 +95,11 @@ def init_yoffe(box, seed=0, c=YOFFE_C, reference_w=True):
     nspec = len(espec)
     # reference layout uhat(Nz, Ny, Nx/2+1), z fastest: arrays indexed [kx][ky][kz]
     f = box.get_state()
-    k = _ktable(n)
-    kx = k[: n // 2 + 1].reshape(-1, 1, 1)
-    ky = k.reshape(1, -1, 1)
-    kz = k.reshape(1, 1, -1)
+    p = box.params
+    ax, ay, az = (float(np.float32(a)) if a > 0 else 1.0 for a in (p.alpha_x, p.alpha_y, p.alpha_z))
+    kx = (_ktable(box.nx)[: box.nx // 2 + 1] * ax).reshape(-1, 1, 1)
+    ky = (_ktable(box.ny) * ay).reshape(1, -1, 1)
+    kz = (_ktable(box.nz) * az).reshape(1, 1, -1)
     absk = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
     nk = shell_index(absk, deltak)
     keep = (nk > 0) & (nk < nspec)

@@ -55,8 +55,7 @@ def fortran_f(x, w=12, d=9):
 def write_info(path, box, isave1, isave2, dt, courant, initrand=True):
     """io_saveInfo (io.f90:704-735), appended."""
     p = box.params
-    n = box.n
-    lines = [f"Nx,y,z = {n:4d}{n:4d}{n:4d}",
+    lines = [f"Nx,y,z = {box.nx:4d}{box.ny:4d}{box.nz:4d}",
              f"iSaveRate1 = {isave1:6d}",
              f"iSaveRate2 = {isave2:6d}",
              "alpha_x,y,z = " + "".join(fortran_f(float(np.float32(a or 1.0)))

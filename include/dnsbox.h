@@ -66,7 +66,8 @@ typedef struct dnsbox_params {
     double cfl_max;     /* CourantMax                   parameters.f90:34 */
     double dt_max;      /* tStepMax                     parameters.f90:35 */
     double dt;          /* initial time step            variables.f90:9   */
-    int n;              /* Nx = Ny = Nz: 16, 32, 48, 64, 128, 256, 512   parameters.f90:12-14 */
+    int n;              /* grid size of the axes nx, ny, nz leave at 0: 16, 32, 48, 64, 128, 256,
+                           512 (Nx = Ny = Nz = n when all three are 0)  parameters.f90:12-14 */
     int precision;      /* 64: all FP64; 32: FFTs/nonlinear term FP32, IF/CFL/diagnostics FP64 */
     int kolmo_k;        /* Kolmogorov forcing wavenumber (0 = off) */
     int bandlim;        /* band-limited forcing         parameters.f90:52 */
@@ -79,8 +80,12 @@ typedef struct dnsbox_params {
     double alpha_y;     /* 2 pi / Ly                          parameters.f90:19; precision, as  */
     double alpha_z;     /* 2 pi / Lz                          parameters.f90:20; the reference's */
                         /* real(kind=4) parameters are (wavenumbers, 2/3 rule, Nspec, Courant dx) */
-    int nx, ny, nz;     /* grid (0 = n)                       parameters.f90:12-14; cubic grids:
-                           a value other than 0 or n is rejected with DNSBOX_E_ARG */
+    int nx, ny, nz;     /* grid Nx, Ny, Nz (0 = n)            parameters.f90:12-14: a cubic grid of
+                           the sizes above, or one of the non-cubic grids built into the library
+                           (32x48x16, 48x16x32, 32x32x64, 64x64x128, 256x256x128); any other grid
+                           is rejected with DNSBOX_E_ARG.  Groups and dnsbox_rhs_dev_noalloc take
+                           cubic grids only.  Spectral arrays are uhat(Nz, Ny, Nx/2+1), physical
+                           ones u(Nx, Ny, Nz); kolmo_k < Ny/2 */
 } dnsbox_params;
 
 typedef struct dnsbox_ctx dnsbox_ctx;
@@ -102,7 +107,7 @@ int dnsbox_set_state(dnsbox_ctx* ctx, const double* uhat, const double* vhat, co
 int dnsbox_get_state(dnsbox_ctx* ctx, double* uhat, double* vhat, double* what);
 /* state_u2uhat (state.f90:75-81): physical u(Nx,Ny,Nz) -> state (unnormalised r2c) */
 int dnsbox_set_state_physical(dnsbox_ctx* ctx, const double* u, const double* v, const double* w);
-/* state_uhat2u (state.f90:83-96): state -> physical velocity (scaled by 1/N^3) */
+/* state_uhat2u (state.f90:83-96): state -> physical velocity (scaled by 1/(Nx Ny Nz)) */
 int dnsbox_get_state_physical(dnsbox_ctx* ctx, double* u, double* v, double* w);
 
 /* initial fields and field operators */

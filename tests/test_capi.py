@@ -60,21 +60,43 @@ def test_params_struct_layout():
     assert Params.alpha_x.offset == 112 and Params.nx.offset == 136
 
 
-def test_non_cubic_grids_and_bad_lengths_are_rejected():
-    """A grid other than n^3 and a negative box length fail with DNSBOX_E_ARG before any device
-    work (parameters.f90:12-20 allow both; this build transforms cubic grids)."""
+def test_grid_parameters_checked_before_device_work():
+    """The grid (parameters.f90:12-14) and box lengths (:18-20) are checked before any device work:
+    the cubic grids and the listed non-cubic grids pass (here, without a GPU, the call then fails
+    with DNSBOX_E_HIP), any other grid, a negative length or nx, a Kolmogorov wavenumber at or past
+    ny/2 and a non-cubic group member fail with DNSBOX_E_ARG."""
     from dnsbox_amd import _lib
     lib = _lib.load()
     p = _lib.Params()
     assert lib.dnsbox_params_default(C.byref(p)) == 0
     assert (p.alpha_x, p.alpha_y, p.alpha_z, p.nx, p.ny, p.nz) == (1.0, 1.0, 1.0, 0, 0, 0)
     h = C.c_void_p()
-    p.n, p.nz = 32, 64
+    for n, nx, ny, nz in [(32, 0, 0, 64), (32, 0, 48, 16), (48, 0, 16, 32), (64, 0, 0, 128), (256, 0, 0, 128),
+                          (0, 32, 32, 64), (128, 0, 0, 0)]:
+        p.n, p.nx, p.ny, p.nz = n, nx, ny, nz
+        assert lib.dnsbox_create(C.byref(p), C.byref(h)) == -2, (n, nx, ny, nz)   # DNSBOX_E_HIP: no device
+        assert b"no HIP device" in lib.dnsbox_last_error()
+    for n, nx, ny, nz in [(32, 0, 0, 128), (64, 0, 0, 32), (32, 48, 32, 0), (24, 0, 0, 0), (0, 32, 32, 0)]:
+        p.n, p.nx, p.ny, p.nz = n, nx, ny, nz
+        assert lib.dnsbox_create(C.byref(p), C.byref(h)) == -1, (n, nx, ny, nz)
+        msg = lib.dnsbox_last_error()
+        assert b"grid" in msg and (b"32x48x16" in msg or b"cubic grids" in msg), msg
+    p.n, p.nx, p.ny, p.nz = 32, -1, 0, 0
     assert lib.dnsbox_create(C.byref(p), C.byref(h)) == -1
-    assert b"cubic grids only" in lib.dnsbox_last_error()
-    p.nz, p.alpha_y = 0, -1.0
+    p.nx, p.ny, p.nz, p.kolmo_amp, p.kolmo_k = 0, 48, 16, 1.0, 24   # forcing sin(k y) needs k < ny/2
+    assert lib.dnsbox_create(C.byref(p), C.byref(h)) == -1
+    assert b"kolmo_k" in lib.dnsbox_last_error()
+    p.kolmo_k = 23
+    assert lib.dnsbox_create(C.byref(p), C.byref(h)) == -2
+    p.kolmo_amp, p.kolmo_k = 0.0, 0
+    p.ny, p.nz, p.alpha_y = 0, 0, -1.0
     assert lib.dnsbox_create(C.byref(p), C.byref(h)) == -1
     assert b"alpha" in lib.dnsbox_last_error()
+    p.alpha_y, p.nz = 1.0, 64
+    arr = (_lib.Params * 1)(p)
+    g = C.c_void_p()
+    assert lib.dnsbox_group_create(arr, 1, C.byref(g)) == -1
+    assert b"cubic" in lib.dnsbox_last_error()
 
 
 def test_step_map_refuses_adaptive_boxes():
@@ -101,5 +123,5 @@ def test_errors_without_device_do_not_abort():
     p.n = 20
     h = C.c_void_p()
     rc = lib.dnsbox_create(C.byref(p), C.byref(h))
-    assert rc == -1 and b"unsupported n" in lib.dnsbox_last_error()
+    assert rc == -1 and b"unsupported grid 20 x 20 x 20" in lib.dnsbox_last_error()
     assert lib.dnsbox_step(None, 1, 1) == -1

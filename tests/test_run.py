@@ -25,11 +25,11 @@ def test_info_file_reads_like_the_reference(tmp_path):
     from types import SimpleNamespace
     from dnsbox_amd.run import write_info
     p = SimpleNamespace(nu=0.02, Q=0.1, Pin=0.0, kcut=0.0, deltak=1.0, bandlim=0)
-    box = SimpleNamespace(params=p, n=32)
+    box = SimpleNamespace(params=p, n=32, nx=32, ny=48, nz=16)
     path = tmp_path / "info.dat"
     write_info(str(path), box, 1000, 10, 0.005, 0.0)
     lines = path.read_text().splitlines()
-    assert lines[0] == "Nx,y,z =   32  32  32"
+    assert lines[0] == "Nx,y,z =   32  48  16"
     assert [float(x) for x in lines[3].split()[2:5]] == [1.0, 1.0, 1.0]
     assert float(lines[5].split()[2]) == 0.02 and float(lines[6].split()[2]) == 0.1
 
