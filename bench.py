@@ -66,6 +66,7 @@ def parse():
     ap.add_argument("--descent-iters", type=int, default=50,
                     help="adjoint-descent iterations run after the gradient timing (0 = skip)")
     ap.add_argument("--batched", type=int, default=1, help="time the 48-box group (1/0)")
+    ap.add_argument("--grids", type=int, default=1, help="time the non-cubic grids (1/0)")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle (1/0)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--reps", type=int, default=20, help="launches per pass for kernel timing")
@@ -300,6 +301,28 @@ def batched_rates(steps=500, T=100, k=20, reps=3, warm=2):
             "ms_per_batched_gradient": secs * 1e3, "T": T, "checkpoint_every": k, "dt": dt}
 
 
+def grid_rates(steps=200, reps=10):
+    """Steps/s of non-cubic grids (CFL 0.5, nu = 0.005, f = sin 4y): 256 x 256 x 128 in the FP32
+    form with the FP64 state and in FP64, and the 2 pi x 2 pi x 4 pi box at 64 x 64 x 128
+    (alpha_z = 1/2) in FP64, with the per-pass times and algorithmic rates of each step."""
+    from dnsbox_amd import Box
+    out = {}
+    for key, (nx, ny, nz), az, prec, sp, k in (
+            ("256x256x128_fp32_state64", (256, 256, 128), 1.0, 32, 64, steps),
+            ("256x256x128_fp64", (256, 256, 128), 1.0, 64, 0, max(1, steps // 2)),
+            ("64x64x128_fp64_lz_4pi", (64, 64, 128), 0.5, 64, 0, steps)):
+        b = Box(n=nx, nx=nx, ny=ny, nz=nz, alpha_z=az, precision=prec, state_precision=sp, nu=NU,
+                kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=1, cfl_min=0.5, cfl_max=0.5, dt_max=0.01,
+                dt=0.002)
+        b.init_random(0, 0, 4.0, 0.5)
+        rate = k / timed_steps(b, k, 10)
+        kt = kernel_table(b, reps)
+        b.close()
+        out[key] = {"steps_per_s": rate, "point_steps_per_s": rate * nx * ny * nz,
+                    "kernels": {q: {"ms": round(v["ms"], 5), "gbs": round(v["gbs"], 1)} for q, v in kt.items()}}
+    return out
+
+
 def cpu_baseline(n, threads, nsteps=2):
     """`nsteps` IF-Heun steps of the FP64 CPU oracle (reference-structure restatement: 15 3-D
     transforms per RHS) at n^3 on `threads` host threads; about 10 s at 256^3 on 16 threads."""
@@ -400,6 +423,9 @@ def main():
     if adj:
         log(f"adjoint {adj['grads_per_s']:.2f} grads/s, fd rel err {adj['fd_rel_err']:.2e}")
     bat = batched_rates() if args.batched else None
+    grids = grid_rates() if args.grids else None
+    if grids:
+        log("grids " + ", ".join(f"{k} {v['steps_per_s']:.1f} steps/s" for k, v in grids.items()))
     if bat:
         log(f"batched {bat['forward_box_steps_per_s']:.0f} box-steps/s forward, "
             f"{bat['fwd_adj_box_steps_per_s']:.0f} forward+adjoint")
@@ -458,6 +484,7 @@ def main():
         "large_box": large,
         "adjoint": adj,
         "batched": bat,
+        "grids": grids,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
