@@ -241,9 +241,12 @@ def adjoint_rate(reps=3, descent_iters=50):
             _, hist = adjoint_descent(b, u0, T, dt, k=k, iters=descent_iters)
             torch.cuda.synchronize()
             t_desc = time.perf_counter() - t0
-        from dnsbox_amd.adjoint import _segment_length
+        from dnsbox_amd.adjoint import _mixed_fits, _segment_length
         seg = _segment_length(T, k, u0.numel() * u0.element_size(), u0.device, "auto")
-    backward = (f"trajectory recorded by the forward ({2 * (T - 1) * u0.numel() * u0.element_size() / 2**30:.2f} GiB)"
+        mixed = _mixed_fits(b, T - 1, u0.device)
+    gib = 2 * (T - 1) * u0.numel() * u0.element_size() / 2**30
+    backward = (f"trajectory recorded by the forward ({gib:.2f} GiB of states and predicted states"
+                + (f", {3 * gib:.2f} GiB of x-pass inputs read by the step adjoints)" if mixed else ")")
                 if seg == T else f"checkpoint every {k} steps, segments recomputed on a second stream")
     return {"grads_per_s": 1.0 / secs, "ms_per_grad": secs * 1e3, "n": n, "T": T, "dt": dt,
             "checkpoint_every": k, "backward": backward, "dtype": "f64", "J": float(J.detach()),

@@ -73,11 +73,18 @@ class _StepMap(torch.autograd.Function):
             m = min(k, T - t)
             if t == starts[-1] and m > 1 and ctx.needs_input_grad[0]:
                 # the last segment, which the backward starts from, is recorded now rather than
-                # recomputed then: its first m - 1 states and predicted states, then its last step
+                # recomputed then: its first m - 1 states and predicted states (and, memory
+                # permitting, the x-pass inputs of their right-hand sides), then its last step
                 shape = (max(k - 1, 1),) + tuple(u.shape)
-                ctx.last = (torch.empty(shape, dtype=u.dtype, device=u.device),
-                            torch.empty(shape, dtype=u.dtype, device=u.device))
-                box.step_dev_record(u, m - 1, *ctx.last)
+                traj = torch.empty(shape, dtype=u.dtype, device=u.device)
+                preds = torch.empty(shape, dtype=u.dtype, device=u.device)
+                fits = _mixed_fits(box, max(k - 1, 1), u.device)
+                mixed = _mixed_buffer(box, max(k - 1, 1), u.device) if fits else None
+                if mixed is not None:
+                    ctx.last = (traj, preds, mixed, box.step_dev_record_mixed(u, m - 1, traj, preds, mixed))
+                else:
+                    box.step_dev_record(u, m - 1, traj, preds)
+                    ctx.last = (traj, preds, None, False)
                 cube = cube and box.last_step_pruned()
                 box.step_dev(u, 1)
             else:
@@ -97,13 +104,20 @@ class _StepMap(torch.autograd.Function):
         nseg = len(ctx.ckpts)
         shape = (max(k - 1, 1),) + tuple(lam.shape)
         # Segment s's states u_start .. u_{stop-1} and the predicted states of its steps but the
-        # last are recomputed by a twin box (same parameters, own buffers) on a second stream
-        # into one of two buffer pairs, while this box runs the step adjoints of segment s + 1
-        # from the other: at small N neither fills the GPU, so the two overlap.
+        # last (and, memory permitting, the x-pass inputs of their right-hand sides) are
+        # recomputed by a twin box (same parameters, own buffers) on a second stream into one of
+        # two buffer sets, while this box runs the step adjoints of segment s + 1 from the other:
+        # at small N neither fills the GPU, so the two overlap.
+        nbuf = min(2, nseg) - (1 if ctx.last is not None else 0)
+        mix = nseg > 1 and _mixed_fits(box, max(k - 1, 1) * nbuf, lam.device)
         bufs = [(torch.empty(shape, dtype=lam.dtype, device=lam.device),
-                 torch.empty(shape, dtype=lam.dtype, device=lam.device)) for _ in range(min(2, nseg))]
+                 torch.empty(shape, dtype=lam.dtype, device=lam.device),
+                 _mixed_buffer(box, max(k - 1, 1), lam.device) if mix else None)
+                for _ in range(nbuf)]
+        rec_pruned = [False] * nseg       # segment s's recording ran pruned
         if ctx.last is not None:   # the last segment was recorded by the forward
-            bufs[(nseg - 1) % 2] = ctx.last
+            bufs.insert((nseg - 1) % 2, ctx.last[:3])
+            rec_pruned[nseg - 1] = ctx.last[3]
         rec = box.twin()
         rec.rhs_int_fact(dt)
         main = torch.cuda.current_stream(lam.device)
@@ -121,8 +135,12 @@ class _StepMap(torch.autograd.Function):
                 if seg + 2 < nseg:
                     side.wait_event(adjointed[seg + 2])   # its buffers were segment seg + 2's
                 if stop - start > 1:
-                    traj, preds = bufs[seg % 2]
-                    rec.step_dev_record(ctx.ckpts[seg].clone(), stop - start - 1, traj, preds)
+                    traj, preds, mixed = bufs[seg % 2]
+                    if mixed is not None:
+                        rec_pruned[seg] = rec.step_dev_record_mixed(ctx.ckpts[seg].clone(), stop - start - 1, traj,
+                                                                    preds, mixed)
+                    else:
+                        rec.step_dev_record(ctx.ckpts[seg].clone(), stop - start - 1, traj, preds)
                 recorded[seg] = torch.cuda.Event()
                 recorded[seg].record(side)
 
@@ -136,11 +154,16 @@ class _StepMap(torch.autograd.Function):
             for seg in reversed(range(nseg)):
                 main.wait_event(recorded[seg])
                 start, stop = span(seg)
-                traj, preds = bufs[seg % 2]
+                traj, preds, mixed = bufs[seg % 2]
                 states = [ctx.ckpts[seg]] + [traj[j] for j in range(stop - start - 1)]
+                # a pruned recording holds the kept kx only: its inputs serve the cube-confined adjoints
+                use_mixed = mixed is not None and (ctx.cube or not rec_pruned[seg])
                 # the recorded u* of every step but the segment's last, whose VJP recomputes its own
                 for j in reversed(range(stop - start)):
-                    if j < stop - start - 1:
+                    if j < stop - start - 1 and use_mixed:
+                        box.step_vjp_mixed_dev(states[j], preds[j], lam, out, dt, mixed[2 * j], mixed[2 * j + 1],
+                                               rec_pruned[seg])
+                    elif j < stop - start - 1:
                         box.step_vjp_pred_dev(states[j], preds[j], lam, out, dt)
                     else:
                         box.step_vjp_dev(states[j], lam, out, dt)
@@ -155,6 +178,23 @@ class _StepMap(torch.autograd.Function):
 
 
 RECORD_FRACTION = 0.25   # share of the free device memory a recorded trajectory may take
+RECORD_MIXED = True      # record the x-pass inputs of recorded steps for their adjoints
+
+
+def _mixed_fits(box, nsteps, device):
+    """Whether record slots for the x-pass inputs of nsteps steps (two 9-field slots per step) are
+    wanted (RECORD_MIXED) and fit in RECORD_FRACTION of the free device memory."""
+    if not RECORD_MIXED or nsteps < 1:
+        return False
+    need = 2 * nsteps * int(np.prod(box.mixed_slot_shape)) * (16 if box.precision == 64 else 8)
+    free, _ = torch.cuda.mem_get_info(device)
+    return need <= RECORD_FRACTION * free
+
+
+def _mixed_buffer(box, nsteps, device):
+    """Record slots for nsteps steps (step_dev_record_mixed)."""
+    dtype = torch.complex128 if box.precision == 64 else torch.complex64
+    return torch.empty((2 * nsteps,) + tuple(box.mixed_slot_shape), dtype=dtype, device=device)
 
 
 def _segment_length(T, k, state_bytes, device, record):

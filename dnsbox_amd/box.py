@@ -347,6 +347,41 @@ class Box:
         check(self.lib.dnsbox_step_vjp_pred_dev(self._h, self._check_tensor(u), self._check_tensor(ustar),
                                                 self._check_tensor(lam), self._check_tensor(out), dt))
 
+    @property
+    def mixed_slot_shape(self):
+        """Shape of one record slot of step_dev_record_mixed: 9 fields of the internal layout."""
+        return (9,) + self.dev_shape[1:]
+
+    def _check_mixed(self, t, nslots):
+        import torch
+        want = torch.complex128 if self.precision == 64 else torch.complex64
+        shape = self.mixed_slot_shape
+        if not (t.is_cuda and t.dtype == want and t.is_contiguous() and tuple(t.shape[-4:]) == shape
+                and t.numel() >= nslots * int(np.prod(shape))):
+            raise ValueError(f"expected a contiguous cuda {want} tensor of >= {nslots} slots {shape}, "
+                             f"got {t.dtype} {tuple(t.shape)}")
+        return C.c_void_p(t.data_ptr())
+
+    def step_dev_record_mixed(self, state, nsteps: int, traj, traj_pred, mixed):
+        """step_dev_record that also records the x-pass inputs of the two RHS of step j + 1 in fields
+        3..8 of the slots mixed[2 j] and mixed[2 j + 1] (a contiguous tensor of >= 2 nsteps slots of
+        shape mixed_slot_shape); returns whether the steps ran pruned (the slots then hold the kept
+        kx only)."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_step_dev_record_mixed(self._h, self._check_tensor(state), nsteps,
+                                                    self._check_traj(traj, nsteps), self._check_traj(traj_pred, nsteps),
+                                                    self._check_mixed(mixed, 2 * nsteps)))
+        return self.last_step_pruned()
+
+    def step_vjp_mixed_dev(self, u, ustar, lam, out, dt: float, mixed_u, mixed_ustar, pruned_record: bool):
+        """step_vjp_pred_dev reading the x-pass inputs at u and u* from the slots that
+        step_dev_record_mixed wrote for the step (their fields 0..2 are overwritten)."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_step_vjp_mixed_dev(self._h, self._check_tensor(u), self._check_tensor(ustar),
+                                                 self._check_tensor(lam), self._check_tensor(out), dt,
+                                                 self._check_mixed(mixed_u, 1), self._check_mixed(mixed_ustar, 1),
+                                                 int(bool(pruned_record))))
+
     def rhs_vjp_dev(self, u, lam, out):
         """out <- J_N(u)^T lam (Euclidean inner product of the stored coordinates)."""
         self._on_torch_stream()

@@ -75,6 +75,24 @@ struct ProVjpIn {  // (lambda, u) -> (lam_scale (-P M lambda) / W, s u, s i k x 
 };
 
 template <class T>
+struct ProVjpLam {  // lambda -> lam_scale (-P M lambda) / W: the cotangent part of ProVjpIn alone
+    static constexpr int FI = 3, FO = 3;
+    T lam_scale;
+    int nx;
+    BoxPhys ph;
+    __device__ __forceinline__ cpx<T> operator()(int o, const cpx<T>* in, const ModeK& m) const
+    {
+        if (masked(m.ix, m.iy, m.iz, m.kx, m.ky, m.kz, ph)) return cpx<T>{T(0), T(0)};
+        const T kx = T(double(m.kx) * ph.a[0]), ky = T(double(m.ky) * ph.a[1]), kz = T(double(m.kz) * ph.a[2]);
+        const T inv = T(1) / T(kk2(m.kx, m.ky, m.kz, ph));
+        const cpx<T> kl = inv * (kx * in[0] + ky * in[1] + kz * in[2]);
+        const T k = o == 0 ? kx : (o == 1 ? ky : kz);
+        const T f = lam_scale / T(herm_w(m.ix, nx));
+        return f * (k * kl - in[o]);
+    }
+};
+
+template <class T>
 struct ProGrad12 {  // u -> (s u_i, s i k_j u_i):  outputs 3 + 3 i + j = d_j u_i
     static constexpr int FI = 3, FO = 12;
     T s;

@@ -520,10 +520,11 @@ __global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
 // y pass (plain, in place), one field per blockIdx.y
 // ===========================================================================
 // PR: pruned (Keep): columns over all iz x Kx (tiled plane by plane where ytiles_by_plane); the
-// inverse (DIR = +1) reads ky in Ky only, the forward (DIR = -1) stores ky in Ky only
-template <class Dm, class T, int DIR, bool PR = false>
+// inverse (DIR = +1) reads ky in Ky only, the forward (DIR = -1) stores ky in Ky only.
+// OOP: the tile is read from src (same layout) and written to data; otherwise in place.
+template <class Dm, class T, int DIR, bool PR = false, bool OOP = false>
 __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, cpx<T>* lds,
-                                          const cpx<T>* __restrict__ tw)
+                                          const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ src = nullptr)
 {
     using G = Geo<Dm, T>;
     using CF = ColFFT<typename G::CY::YP, G::YW, G::YW, DIR, T>;
@@ -547,11 +548,12 @@ __device__ __forceinline__ void ycol_tile(cpx<T>* __restrict__ data, int tile, c
         iz = col / CW;
     }
     cpx<T>* base = data + (long long)iz * G::NY * H + ix;
+    const cpx<T>* sbase = OOP ? src + (long long)iz * G::NY * H + ix : base;
     cpx<T> a[RM];
 #pragma unroll
     for (int q = 0; q < RM; ++q) {
         const int iy = CF::in_pos(g, q);
-        a[q] = valid && (!PR || DIR < 0 || Keep<G::NY>::kept(iy)) ? base[iy * H] : cpx<T>{T(0), T(0)};
+        a[q] = valid && (!PR || DIR < 0 || Keep<G::NY>::kept(iy)) ? sbase[iy * H] : cpx<T>{T(0), T(0)};
     }
     CF::run(a, lds, c, g, tw, G::NY);
     if (valid) {
@@ -574,6 +576,22 @@ __global__ void __launch_bounds__((Geo<Dm, T>::YNT))
     const int L = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int field = L / gridDim.x, tile = L % gridDim.x;
     ycol_tile<Dm, T, DIR, PR>(data0 + field * fs, tile, lds, tw);
+}
+
+// k_ycol from src0 into dst0 (fields fs apart in both): the y-inverse that also keeps its
+// output apart from the z pass's scratch (the step records it for the step adjoint)
+template <class Dm, class T, int DIR, bool PR = false>
+__global__ void __launch_bounds__((Geo<Dm, T>::YNT))
+    k_ycol_oop(const cpx<T>* __restrict__ src0, cpx<T>* __restrict__ dst0, long long fs,
+               const cpx<T>* __restrict__ tw_g)
+{
+    using G = Geo<Dm, T>;
+    __shared__ cpx<T> lds[G::NY * G::YW];
+    __shared__ cpx<T> stw_[G::NY];
+    const cpx<T>* tw = stage_twiddles<G::NY, T>(tw_g, stw_);
+    const int L = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int field = L / gridDim.x, tile = L % gridDim.x;
+    ycol_tile<Dm, T, DIR, PR, true>(dst0 + field * fs, tile, lds, tw, src0 + field * fs);
 }
 
 // ===========================================================================
@@ -970,6 +988,7 @@ struct EpiArgs {
     double dt_fixed;      // EPI_PREDONLY: step size
     BoxScalars* sc;
     BoxPhys ph;
+    cpx<S>* tout;         // EPI_CORR: also receives the new state (null: not stored)
 };
 
 // dnsboxheun.f90:227-246 set_time_step, triggered as at :153-157
@@ -1151,7 +1170,10 @@ __device__ __forceinline__ void zfwd_epilogue(const cpx<T> (&O)[3][Geo<Dm, T>::Z
                     const cpx<S> u = PREF ? Sp[f][q] : (live ? ea.state[f * ea.fs + idx] : cpx<S>{S(0), S(0)});
                     const cpx<S> un = u + (sdt * S(0.5)) * cvt<S>(nl[f]);
                     if (NEXT) U[f][q] = cvt<T>(un);
-                    if (live) ea.state[f * ea.fs + idx] = un;
+                    if (live) {
+                        ea.state[f * ea.fs + idx] = un;
+                        if (ea.tout) ea.tout[f * ea.fs + idx] = un;
+                    }
                 }
             } else if constexpr (MODE == EPI_AB2) {
                 // variable-step IF-AB2 (dnsbox.f90:143-193 with consistent weights), h = dt of this

@@ -179,6 +179,10 @@ class BoxBase {
     bool may_prune() const { return prune && prunable; }
     bool pruned() const { return may_prune() && dealiased; }
     void* pred_out_ = nullptr;   // where the IF-Heun step stores u* (null: internal buffer)
+    void* traj_out_ = nullptr;   // where the IF-Heun corrector also stores the new state (null: nowhere)
+    // two record slots of 9 fields: the step's two RHS write their x-pass inputs (u, v, w and the
+    // vorticity after the z- and y-inverse transforms) to fields 3..8 of slot 0 and of slot 1
+    void* mixed_out_ = nullptr;
     // the caller guarantees that the states the step / RHS adjoints linearise about have no
     // energy outside the kept cube (dnsbox_set_vjp_cube); their first passes then read the cube only
     bool vjp_cube = false;
@@ -238,6 +242,14 @@ class BoxBase {
     virtual void step_ext_record(void* st, int nsteps, void* traj, void* traj_pred) = 0;
     // step_vjp at u with the step's predicted state u* = E (u + dt N(u)) given
     virtual void step_vjp_pred(const void* u, const void* ustar, const void* lam, void* lam_out, double dt) = 0;
+    // step_ext_record that also records the x-pass inputs of the two RHS of step j + 1 in fields 3..8
+    // of the 9-field slots mixed + 2 j and mixed + 2 j + 1 (transform type, internal layout)
+    virtual void step_ext_record_mixed(void* st, int nsteps, void* traj, void* traj_pred, void* mixed) = 0;
+    // step_vjp_pred with the x-pass inputs of the step's RHS at u and at u* read from the slots
+    // mixed_u, mixed_us that recorded them (their fields 0..2 are overwritten); pruned_rec: the
+    // recording step ran pruned, so the slots hold the kept kx only and vjp_cube must be set
+    virtual void step_vjp_mixed(const void* u, const void* ustar, const void* lam, void* lam_out, double dt,
+                                void* mixed_u, void* mixed_us, bool pruned_rec) = 0;
     // FP64 device array -> transform type / state type, and back
     virtual void convert_in(void* dst, const double* src, long long n, bool state_type) = 0;
     virtual void convert_out(double* dst, const void* src, long long n, bool state_type) = 0;
@@ -389,12 +401,13 @@ class BoxT final : public BoxBase {
         DNSBOX_HIP(hipEventRecord(e, stream));
     }
 
-    // middle passes of an RHS: y-inverse (6), x cross product, y-forward (3)
+    // middle passes of an RHS: y-inverse (6), x cross product, y-forward (3).  rec: a 9-field
+    // record slot whose fields 3..8 receive the y-inverse's output, which the x pass reads there
     template <bool PR = false>
-    void rhs_middle(bool track_cfl)
+    void rhs_middle(bool track_cfl, C* rec = nullptr)
     {
         if constexpr (plane_used<Dm, T, OpCross<T>>()) {
-            if (plane_enabled()) {
+            if (plane_enabled() && !rec) {
                 hipLaunchKernelGGL((k_plane<NX, T, OpCross<T>, PR, PR>), dim3(NX), dim3(kPlaneThreads), 0, stream, X(),
                                    fs, twc(), track_cfl ? sc->maxu : nullptr);
                 DNSBOX_LAUNCH_CHECK();
@@ -402,15 +415,22 @@ class BoxT final : public BoxBase {
                 return;
             }
         }
-        hipLaunchKernelGGL((k_ycol<Dm, T, +1, PR>), dim3(ytiles<Dm, T, PR>(), 6), dim3(YNT), 0, stream, X(), fs,
-                           twy());
+        const C* xin = X();
+        if (rec) {
+            xin = rec + 3 * fs;
+            hipLaunchKernelGGL((k_ycol_oop<Dm, T, +1, PR>), dim3(ytiles<Dm, T, PR>(), 6), dim3(YNT), 0, stream, X(),
+                               rec + 3 * fs, fs, twy());
+        } else {
+            hipLaunchKernelGGL((k_ycol<Dm, T, +1, PR>), dim3(ytiles<Dm, T, PR>(), 6), dim3(YNT), 0, stream, X(), fs,
+                               twy());
+        }
         DNSBOX_LAUNCH_CHECK();
         mark(1);
         if constexpr (xreg_used<Dm, T>()) {
             hipLaunchKernelGGL((xreg::k_xcross_r<NX, T, PR>), dim3(NY * NZ / xreg::Shape<NX>::LPB),
-                               dim3(xreg::kThreads), 0, stream, X(), X(), fs, twx(), track_cfl ? sc->maxu : nullptr);
+                               dim3(xreg::kThreads), 0, stream, xin, X(), fs, twx(), track_cfl ? sc->maxu : nullptr);
         } else {
-            hipLaunchKernelGGL((k_xpass<Dm, T, OpCross<T>, PR>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, X(), fs,
+            hipLaunchKernelGGL((k_xpass<Dm, T, OpCross<T>, PR>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, xin, fs,
                                X(), fs, twx(), track_cfl ? sc->maxu : nullptr, 0);
         }
         DNSBOX_LAUNCH_CHECK();
@@ -482,7 +502,8 @@ class BoxT final : public BoxBase {
             zinv_curl<S, PR>(St());
             mark(0);
         }
-        rhs_middle<PR>(true);
+        C* rec = static_cast<C*>(mixed_out_);
+        rhs_middle<PR>(true, rec);
         EpiArgs<T, S> ea = epi();
         // the predicted state is kept for band forcing, or in the caller's buffer when recorded
         C* pst = pred_out_ ? static_cast<C*>(pred_out_) : Pr();
@@ -495,11 +516,12 @@ class BoxT final : public BoxBase {
         DNSBOX_LAUNCH_CHECK();
         mark(4);
         eband(pst);
-        rhs_middle<PR>(false);
+        rhs_middle<PR>(false, rec ? rec + 9 * fs : nullptr);
         EpiArgs<T, S> eb = epi();
         eb.state = St();
         eb.rhs_in = pst;
         eb.rhs_in_s = 0;
+        eb.tout = static_cast<CS*>(traj_out_);
         hipLaunchKernelGGL((k_zfwd_zinv<Dm, T, EPI_CORR, S, PR>), dim3(ztiles<Dm, T, PR>()), dim3(ZNT), 0, stream,
                            X(), fs, twz(), eb, pro);
         DNSBOX_LAUNCH_CHECK();
@@ -844,6 +866,76 @@ class BoxT final : public BoxBase {
         vjp_passes<VJP_S2>(u, adjMu(), 1.0, v2);
     }
 
+    // the passes of one VJP of N at u whose x-pass inputs of u (fields 3..8 of `slot`) were recorded
+    // by the step that evaluated N(u): only the cotangent is transformed in (fields 0..2 of the slot)
+    template <int MODE, bool PR>
+    void vjp_rec_passes(const C* u, const C* lam, double lam_scale, VjpArgs<T> va, C* slot)
+    {
+        const double sm = 1.0 / (double(NX) * NY * NZ);
+        if (cfg.ph.bandlim) {
+            hipLaunchKernelGGL((k_band_vjp_scalars<Dm, T>), dim3(1), dim3(256), 0, stream, u, lam, fs, cfg.ph,
+                               lam_scale, sm * sm, band2);
+            DNSBOX_LAUNCH_CHECK();
+        }
+        Ptrs<T, 3> in{{lam, lam + fs, lam + 2 * fs}};
+        ProVjpLam<T> pro{T(lam_scale), NX, cfg.ph};
+        if constexpr (PR) {
+            hipLaunchKernelGGL((k_zinv_p_chk<Dm, T, ProVjpLam<T>>), dim3(ztiles<Dm, T, true>() + vjp_check_blocks<Dm>()),
+                               dim3(ZNT), 0, stream, in, slot, fs, twz(), pro, u, fs, chk + 1, ztiles<Dm, T, true>());
+        } else {
+            hipLaunchKernelGGL((k_zinv_p<Dm, T, ProVjpLam<T>>), dim3(ztiles<Dm, T, false>()), dim3(ZNT), 0, stream, in,
+                               slot, fs, twz(), pro);
+        }
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<Dm, T, +1, PR>), dim3(ytiles<Dm, T, PR>(), 3), dim3(YNT), 0, stream, slot, fs, twy());
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_xpass<Dm, T, OpVjp<T>, PR, false>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, slot, fs,
+                           X(), fs, twx(), (unsigned long long*)nullptr, 0);
+        DNSBOX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_ycol<Dm, T, -1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twy());
+        DNSBOX_LAUNCH_CHECK();
+        va.u = u;
+        va.lamN = lam;
+        va.lam_scale = lam_scale;
+        va.band = band2;
+        va.fs = fs;
+        va.ph = cfg.ph;
+        hipLaunchKernelGGL((k_zfwd_vjp<Dm, T, MODE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), va);
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    void step_vjp_mixed(const void* u_, const void* us_, const void* lam_, void* lam_out, double dt, void* mu_,
+                        void* ms_, bool pruned_rec) override
+    {
+        require_uniform("the step adjoint");
+        const bool pr = may_prune() && vjp_cube;
+        if (pruned_rec && !pr)
+            throw std::invalid_argument("a step recorded with pruned passes holds its x-pass inputs at the kept kx "
+                                        "only: its adjoint needs dnsbox_set_vjp_cube(ctx, 1) and pruning enabled");
+        x6_valid = false;
+        ensure_adj();
+        const C* u = static_cast<const C*>(u_);
+        const C* us = static_cast<const C*>(us_);
+        const C* lam = static_cast<const C*>(lam_);
+        // lam_* = J(u*)^T (dt/2 lam);  A = E (lam + lam_*),  Mu = E (dt/2 lam + dt lam_*);  lam_n = A + J(u)^T Mu
+        VjpArgs<T> v1{};
+        v1.lam = lam;
+        v1.A = adjA();
+        v1.Mu = adjMu();
+        v1.dt = dt;
+        VjpArgs<T> v2{};
+        v2.out = static_cast<C*>(lam_out);
+        v2.A = adjA();
+        v2.dt = dt;
+        if (pr) {
+            vjp_rec_passes<VJP_S1, true>(us, lam, 0.5 * dt, v1, static_cast<C*>(ms_));
+            vjp_rec_passes<VJP_S2, true>(u, adjMu(), 1.0, v2, static_cast<C*>(mu_));
+        } else {
+            vjp_rec_passes<VJP_S1, false>(us, lam, 0.5 * dt, v1, static_cast<C*>(ms_));
+            vjp_rec_passes<VJP_S2, false>(u, adjMu(), 1.0, v2, static_cast<C*>(mu_));
+        }
+    }
+
     // rhsAdj.f90:64-381 (phatadj read as phat)
     void rhs_adj(const void* u_, void* out, double lin_sign) override
     {
@@ -940,10 +1032,11 @@ class BoxT final : public BoxBase {
     }
 
     // IF-Heun steps of a caller's buffer, pruned when the buffer allows it (checked on the
-    // device); after step j the state is copied to traj + j * 3 fields when traj is set, and
-    // the step's predicted state is stored at traj_pred + j * 3 fields when that is set (the
-    // pruned step writes only the kept modes, so the slots are cleared first)
-    void step_buffer(void* st, int nsteps, void* traj, void* traj_pred)
+    // device); the state after step j is stored at traj + j * 3 fields when traj is set (by the
+    // corrector pass, beside the state), the step's predicted state at traj_pred + j * 3 fields
+    // when that is set (the pruned step writes only the kept modes, so both are cleared first),
+    // and the x-pass inputs of its two RHS in the record slots mixed + (2 j, 2 j + 1) * 9 fields
+    void step_buffer(void* st, int nsteps, void* traj, void* traj_pred, void* mixed = nullptr)
     {
         require_uniform("stepping a device buffer");
         if (nsteps <= 0) return;
@@ -955,27 +1048,29 @@ class BoxT final : public BoxBase {
             dealiased = may_prune() && buffer_dealiased(static_cast<const CS*>(st));
             last_ext_pruned = dealiased;
             state = st;
-            if (traj_pred && dealiased) {   // the pruned predictor stores u* at the kept modes only
-                hipLaunchKernelGGL((k_fill_zero<S>), dim3(1024), dim3(256), 0, stream, static_cast<CS*>(traj_pred),
+            // the pruned steps store u* and the new state at the kept modes only
+            for (void* buf : {traj_pred, traj}) {
+                if (!buf || !dealiased) continue;
+                hipLaunchKernelGGL((k_fill_zero<S>), dim3(1024), dim3(256), 0, stream, static_cast<CS*>(buf),
                                    (long long)nsteps * 3 * fs);
                 DNSBOX_LAUNCH_CHECK();
             }
+            const size_t slot2 = 18 * size_t(fs) * sizeof(C);   // the two 9-field record slots of a step
             for (int i = 0; i < nsteps; ++i) {
                 pred_out_ = traj_pred ? static_cast<char*>(traj_pred) + i * sb3 : nullptr;
+                traj_out_ = traj ? static_cast<char*>(traj) + i * sb3 : nullptr;
+                mixed_out_ = mixed ? static_cast<char*>(mixed) + i * slot2 : nullptr;
                 heun_step();
-                if (traj)
-                    DNSBOX_HIP(hipMemcpyAsync(static_cast<char*>(traj) + i * sb3, st, sb3, hipMemcpyDeviceToDevice,
-                                              stream));
             }
         } catch (...) {
             state = own;
-            pred_out_ = nullptr;
+            pred_out_ = traj_out_ = mixed_out_ = nullptr;
             dealiased = own_dealiased;
             x6_valid = false;
             throw;
         }
         state = own;
-        pred_out_ = nullptr;
+        pred_out_ = traj_out_ = mixed_out_ = nullptr;
         dealiased = own_dealiased;
         x6_valid = false;
     }
@@ -983,6 +1078,10 @@ class BoxT final : public BoxBase {
     void step_ext_record(void* st, int nsteps, void* traj, void* traj_pred) override
     {
         step_buffer(st, nsteps, traj, traj_pred);
+    }
+    void step_ext_record_mixed(void* st, int nsteps, void* traj, void* traj_pred, void* mixed) override
+    {
+        step_buffer(st, nsteps, traj, traj_pred, mixed);
     }
 
     void pass_bytes(double* b) const override

@@ -601,6 +601,37 @@ int dnsbox_step_dev_record(dnsbox_ctx* c, void* state, int nsteps, void* traj, v
     });
 }
 
+int dnsbox_step_dev_record_mixed(dnsbox_ctx* c, void* state, int nsteps, void* traj, void* traj_pred, void* mixed)
+{
+    if (int e = check_ctx(c)) return e;
+    invalidate(c);
+    if (!state || !traj || !traj_pred || !mixed || nsteps < 0)
+        return fail(DNSBOX_E_ARG, "null buffer or nsteps < 0");
+    if (traj_pred == traj || traj_pred == state || mixed == traj || mixed == traj_pred || mixed == state)
+        return fail(DNSBOX_E_ARG, "state, traj, traj_pred and mixed must be distinct buffers");
+    if (!c->box->uniform) return fail(DNSBOX_E_ARG, "recording needs state_precision = precision");
+    return guarded([&]() -> int {
+        c->box->step_ext_record_mixed(state, nsteps, traj, traj_pred, mixed);
+        return 0;
+    });
+}
+
+int dnsbox_step_vjp_mixed_dev(dnsbox_ctx* c, const void* u, const void* ustar, const void* lam, void* lam_out,
+                              double dt, void* mixed_u, void* mixed_ustar, int pruned_record)
+{
+    if (int e = check_ctx(c)) return e;
+    invalidate(c);
+    if (!u || !ustar || !lam || !lam_out || !mixed_u || !mixed_ustar) return fail(DNSBOX_E_ARG, "null device pointer");
+    if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+    if (lam_out == u || lam_out == lam || lam_out == ustar)
+        return fail(DNSBOX_E_ARG, "lam_out must not alias u, ustar or lam");
+    if (mixed_u == mixed_ustar) return fail(DNSBOX_E_ARG, "the two record slots must differ");
+    return guarded([&]() -> int {
+        c->box->step_vjp_mixed(u, ustar, lam, lam_out, dt, mixed_u, mixed_ustar, pruned_record != 0);
+        return 0;
+    });
+}
+
 int dnsbox_step_vjp_pred_dev(dnsbox_ctx* c, const void* u, const void* ustar, const void* lam, void* lam_out,
                              double dt)
 {

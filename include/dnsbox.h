@@ -192,7 +192,7 @@ int dnsbox_rhs_dev(dnsbox_ctx* ctx, const void* in, void* out);
 /* the time loop of dnsboxheun.f90:81-126 on a caller-owned device state (in place); the passes
    run pruned when a device check finds the buffer empty outside the 2/3-rule cube */
 int dnsbox_step_dev(dnsbox_ctx* ctx, void* state, int nsteps);
-/* dnsbox_step_dev that also copies the state after step j + 1 to traj + j * (3 fields of the
+/* dnsbox_step_dev that also stores the state after step j + 1 at traj + j * (3 fields of the
    internal layout), j < nsteps, and (traj_pred not NULL) stores that step's predicted state
    u* = E (u + dt N(u)) at traj_pred + j * 3 fields: the recomputed trajectory of an adjoint
    checkpoint segment in one call (the step pipeline runs through it).  No reference counterpart. */
@@ -208,6 +208,21 @@ int dnsbox_step_vjp_dev(dnsbox_ctx* ctx, const void* u, const void* lam, void* l
    recorded by dnsbox_step_dev_record), so it is not recomputed.  No reference counterpart. */
 int dnsbox_step_vjp_pred_dev(dnsbox_ctx* ctx, const void* u, const void* ustar, const void* lam, void* lam_out,
                              double dt);
+/* dnsbox_step_dev_record (traj and traj_pred both required) that also records, for step j + 1,
+   the x-pass inputs of its two right-hand sides, N(u_j) and N(u*_j): u, v, w and the vorticity
+   after the z- and y-inverse transforms, six fields of the internal layout each, in fields 3..8
+   of the 9-field slots mixed + (2 j) * 9 fields and mixed + (2 j + 1) * 9 fields (mixed holds
+   18 nsteps fields).  Needs state_precision = precision.  No reference counterpart: the forward
+   half of the backward step of the T-step map (dnsboxheun.f90:84-126). */
+int dnsbox_step_dev_record_mixed(dnsbox_ctx* ctx, void* state, int nsteps, void* traj, void* traj_pred,
+                                 void* mixed);
+/* dnsbox_step_vjp_pred_dev that reads the x-pass inputs at u and at ustar from the record slots
+   mixed_u and mixed_ustar that dnsbox_step_dev_record_mixed wrote for that step, instead of
+   transforming u and ustar again (fields 0..2 of both slots are overwritten).  pruned_record:
+   dnsbox_get_last_step_pruned after the recording; a pruned recording holds the kept kx only and
+   needs dnsbox_set_vjp_cube(ctx, 1), else DNSBOX_E_ARG.  No reference counterpart. */
+int dnsbox_step_vjp_mixed_dev(dnsbox_ctx* ctx, const void* u, const void* ustar, const void* lam, void* lam_out,
+                              double dt, void* mixed_u, void* mixed_ustar, int pruned_record);
 /* rhsAdj (rhsAdj.f90:356-381 via rhsNonlinearAdj :261-354, rhsupptilde :177-213, convective
    rhsNonlinear :64-175), with the never-assigned phatadj (:343-348) read as phat (:335-340).
    lin_sign = +1 reproduces the reference's L u~ - P n_adj; lin_sign = -1 gives the descent
