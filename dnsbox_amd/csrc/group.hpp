@@ -38,7 +38,7 @@ struct GBox {
     cpx<T>* out;
     cpx<T>* pred;        // u* of the step
     cpx<T>* pout;        // forward step: where the IF-Heun predictor stores u* (null: not stored)
-    cpx<T>* tout;        // forward step: where k_g_copy_state copies the new state (null: none)
+    cpx<T>* tout;        // forward step: where the corrector also stores the new state (null: none)
     cpx<T>* A;           // E (lam + lam_*)
     cpx<T>* Mu;          // E (dt/2 lam + dt lam_*)
     double dt;
@@ -185,6 +185,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_zinv(const GBox<T>* __
     const int tile = gt - (PR ? bx.zt0p : bx.zt0);
     EpiArgs<T, T> ea{};
     ea.out = MODE == EPI_PRED ? bx.pout : nullptr;
+    ea.tout = MODE == EPI_CORR ? bx.tout : nullptr;
     ea.state = bx.st;
     ea.rhs_in = bx.st;
     ea.rhs_in_s = 1;
@@ -232,7 +233,10 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_predonly(const GBox<T>
 // z-inverse of the VJP prologue: STAGE 1 at u* with cotangent dt/2 lam, STAGE 2 at u
 // with cotangent Mu; PR: the kept columns only, inputs read at kept kz (states in the cube),
 // and kGroupCheck blocks per box after the `tiles` tile blocks check the box's state outside
-// its cube (*viol |= 1 on a nonzero mode there)
+// its cube (*viol |= 1 on a nonzero mode there).  The cotangent's three outputs (ProVjpLam) and
+// the state's six (ProCurl: s u, s i k x u, the outputs 3..8 of ProVjpIn) are two z-inverses in
+// turn, so that three inputs rather than six are held in registers (256 VGPRs and one wave per
+// SIMD when all six were held).
 constexpr int kGroupCheck = 16;
 template <class T, int STAGE, bool PR = false>
 __global__ void __launch_bounds__(kGroupThreads)
@@ -261,13 +265,20 @@ __global__ void __launch_bounds__(kGroupThreads)
     const int tile = gt - (PR ? bx.zt0p : bx.zt0);
     const cpx<T>* l = STAGE == 1 ? bx.lam : bx.Mu;
     const cpx<T>* p = STAGE == 1 ? bx.pred : bx.u;
-    const cpx<T>* in[6] = {l, l + bx.fs, l + 2 * bx.fs, p, p + bx.fs, p + 2 * bx.fs};
-    cpx<T>* out[9];
+    const cpx<T>* in_l[3] = {l, l + bx.fs, l + 2 * bx.fs};
+    const cpx<T>* in_u[3] = {p, p + bx.fs, p + 2 * bx.fs};
+    cpx<T>* out_l[3];
+    cpx<T>* out_u[6];
 #pragma unroll
-    for (int f = 0; f < 9; ++f) out[f] = bx.X + f * bx.fs;
+    for (int f = 0; f < 3; ++f) out_l[f] = bx.X + f * bx.fs;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) out_u[f] = bx.X + (3 + f) * bx.fs;
     const double sm = 1.0 / (double(bx.n) * bx.n * bx.n);
-    const ProVjpIn<T> pro{T(sm), T(STAGE == 1 ? 0.5 * bx.dt : 1.0), bx.n, bx.ph};
-#define DNSBOX_G_ZV(NN) zinv_tile<Cube<NN>, T, ProVjpIn<T>, T, PR>(in, out, tile, lds, tw, pro)
+    const ProVjpLam<T> pl{T(STAGE == 1 ? 0.5 * bx.dt : 1.0), bx.n, bx.ph};
+    const ProCurl<T> pu{T(sm), KScale<T>::of(bx.ph)};
+#define DNSBOX_G_ZV(NN)                                                             \
+    zinv_tile<Cube<NN>, T, ProVjpLam<T>, T, PR>(in_l, out_l, tile, lds, tw, pl); \
+    zinv_tile<Cube<NN>, T, ProCurl<T>, T, PR>(in_u, out_u, tile, lds, tw, pu)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZV)
 #undef DNSBOX_G_ZV
 }
@@ -314,17 +325,6 @@ __global__ void __launch_bounds__(256) k_g_outside_cube(const GBox<T>* __restric
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_OC)
 #undef DNSBOX_G_OC
     if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(flag, 1u);
-}
-
-// state of box blockIdx.y -> its trajectory slot (boxes without one return at once)
-template <class T>
-__global__ void __launch_bounds__(256) k_g_copy_state(const GBox<T>* __restrict__ gb, int nbox)
-{
-    const GBox<T>& bx = gb[blockIdx.y];
-    if (!bx.tout) return;
-    const long long n = 3 * bx.fs;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        bx.tout[i] = bx.st[i];
 }
 
 inline int group_ztiles(int n, int prec)
@@ -638,8 +638,9 @@ class Group final : public GroupBase {
         for (int i = 0; i < nsteps; ++i) one_step(stream, d, pr);
     }
 
-    // step_ext that also copies box i's state after step j + 1 to traj[i] + j * 3 fields and, when
-    // traj_pred is set, stores that step's predicted state u* at traj_pred[i] + j * 3 fields
+    // step_ext that also stores box i's state after step j + 1 at traj[i] + j * 3 fields (by the
+    // corrector pass) and, when traj_pred is set, that step's predicted state u* at
+    // traj_pred[i] + j * 3 fields
     void step_ext_record(void* const* states, int nsteps, void* const* traj, void* const* traj_pred) override
     {
         if (nsteps <= 0) return;
@@ -651,10 +652,12 @@ class Group final : public GroupBase {
         const bool pr = buffers_dealiased(d0);
         last_ext_pruned = pr;
         if (pr) {
-            // the pruned predictor stores u* at the kept modes only: the rest of each slot is zero
+            // the pruned predictor and corrector store u* and the new state at the kept modes only:
+            // the rest of each slot is zero
             for (size_t i = 0; i < h.size(); ++i)
-                if (traj_pred) {
-                    hipLaunchKernelGGL((k_fill_zero<T>), dim3(256), dim3(256), 0, stream, static_cast<C*>(traj_pred[i]),
+                for (void* const* bufs : {traj_pred, traj}) {
+                    if (!bufs) continue;
+                    hipLaunchKernelGGL((k_fill_zero<T>), dim3(256), dim3(256), 0, stream, static_cast<C*>(bufs[i]),
                                        (long long)nsteps * 3 * h[i].fs);
                     DNSBOX_LAUNCH_CHECK();
                 }
@@ -662,7 +665,6 @@ class Group final : public GroupBase {
         } else {
             launch_zinv<false>(stream, d0, 0);
         }
-        const int nb = int(h.size());
         for (int j = 0; j < nsteps; ++j) {
             for (size_t i = 0; i < h.size(); ++i) {
                 const long long off = (long long)j * 3 * h[i].fs;
@@ -671,8 +673,6 @@ class Group final : public GroupBase {
             }
             const GBox<T>* d = descriptors(h);
             one_step(stream, d, pr);
-            hipLaunchKernelGGL((k_g_copy_state<T>), dim3(64, nb), dim3(256), 0, stream, d, nb);
-            DNSBOX_LAUNCH_CHECK();
         }
     }
 
