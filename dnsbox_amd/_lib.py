@@ -95,6 +95,10 @@ SIGNATURES = {
     "dnsbox_group_step_dev_record": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "dnsbox_group_step_vjp_pred_dev": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
                                                 C.POINTER(_vp), C.POINTER(C.c_double)]),
+    "dnsbox_group_step_dev_record_mixed": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.POINTER(_vp),
+                                                    C.POINTER(_vp)]),
+    "dnsbox_group_step_vjp_mixed_dev": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
+                                                 C.POINTER(_vp), C.POINTER(C.c_double), C.POINTER(_vp), C.c_int]),
     "dnsbox_group_step_vjp_dev": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _dp]),
 }
 

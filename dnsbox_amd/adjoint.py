@@ -197,6 +197,23 @@ def _mixed_buffer(box, nsteps, device):
     return torch.empty((2 * nsteps,) + tuple(box.mixed_slot_shape), dtype=dtype, device=device)
 
 
+def _group_mixed(group, nsteps, device, allocate=True, check=True):
+    """Record slots of nsteps steps for every box of a group (a list of tensors) when RECORD_MIXED
+    is set and they fit in RECORD_FRACTION of the free device memory (check), else None; with
+    allocate=False only whether they fit."""
+    if check:
+        if not RECORD_MIXED or nsteps < 1:
+            return None if allocate else False
+        need = sum(2 * nsteps * int(np.prod(b.mixed_slot_shape)) * (16 if b.precision == 64 else 8)
+                   for b in group.boxes)
+        free, _ = torch.cuda.mem_get_info(device)
+        if need > RECORD_FRACTION * free:
+            return None if allocate else False
+    if not allocate:
+        return True
+    return [_mixed_buffer(b, nsteps, device) for b in group.boxes]
+
+
 def _segment_length(T, k, state_bytes, device, record):
     """Steps per segment of the forward: T (the whole trajectory recorded, no recomputation in
     the backward) for record="all", or for "auto" when the 2 (T - 1) recorded states of
@@ -241,9 +258,15 @@ class _GroupStepMap(torch.autograd.Function):
             if t == starts[-1] and m > 1 and any(ctx.needs_input_grad[5:]):
                 # the last segment is recorded now (as in _StepMap.forward)
                 w = max(k - 1, 1)
-                ctx.last = [(torch.empty((w,) + tuple(u.shape), dtype=u.dtype, device=u.device),
-                             torch.empty((w,) + tuple(u.shape), dtype=u.dtype, device=u.device)) for u in us]
-                group.step_dev_record(us, m - 1, [a for a, _ in ctx.last], [b for _, b in ctx.last])
+                pairs = [(torch.empty((w,) + tuple(u.shape), dtype=u.dtype, device=u.device),
+                          torch.empty((w,) + tuple(u.shape), dtype=u.dtype, device=u.device)) for u in us]
+                mixeds = _group_mixed(group, w, us[0].device)
+                if mixeds is not None:
+                    pr = group.step_dev_record_mixed(us, m - 1, [a for a, _ in pairs], [b for _, b in pairs], mixeds)
+                else:
+                    group.step_dev_record(us, m - 1, [a for a, _ in pairs], [b for _, b in pairs])
+                    pr = False
+                ctx.last = (pairs, mixeds, pr)
                 cube = cube and group.last_step_pruned()
                 group.step_dev(us, 1)
             else:
@@ -266,11 +289,17 @@ class _GroupStepMap(torch.autograd.Function):
         # and predicted states recorded in one call) into one of two buffer sets while this group
         # runs the adjoints of segment s + 1 from the other
         m = max(k - 1, 1)
-        bufs = [[(torch.empty((m,) + tuple(l.shape), dtype=l.dtype, device=l.device),
-                  torch.empty((m,) + tuple(l.shape), dtype=l.dtype, device=l.device)) for l in lams]
-                for _ in range(min(2, nseg))]
+        nbuf = min(2, nseg) - (1 if ctx.last is not None else 0)
+        mix = nseg > 1 and _group_mixed(group, m * nbuf, lams[0].device, allocate=False)
+        # a buffer set: per box (states, predicted states), and the record slots or None
+        bufs = [([(torch.empty((m,) + tuple(l.shape), dtype=l.dtype, device=l.device),
+                   torch.empty((m,) + tuple(l.shape), dtype=l.dtype, device=l.device)) for l in lams],
+                 _group_mixed(group, m, lams[0].device, check=False) if mix else None)
+                for _ in range(nbuf)]
+        rec_pruned = [False] * nseg
         if ctx.last is not None:   # the last segment was recorded by the forward
-            bufs[(nseg - 1) % 2] = ctx.last
+            bufs.insert((nseg - 1) % 2, ctx.last[:2])
+            rec_pruned[nseg - 1] = ctx.last[2]
         rec = group.twin()
         for b, dt in zip(rec.boxes, dts):
             b.rhs_int_fact(dt)
@@ -289,8 +318,12 @@ class _GroupStepMap(torch.autograd.Function):
                     side.wait_event(adjointed[seg + 2])
                 if stop - start > 1:
                     us = [c.clone() for c in ctx.ckpts[seg]]
-                    rec.step_dev_record(us, stop - start - 1, [t for t, _ in bufs[seg % 2]],
-                                        [p for _, p in bufs[seg % 2]])
+                    pairs, mixeds = bufs[seg % 2]
+                    if mixeds is not None:
+                        rec_pruned[seg] = rec.step_dev_record_mixed(us, stop - start - 1, [t for t, _ in pairs],
+                                                                    [p for _, p in pairs], mixeds)
+                    else:
+                        rec.step_dev_record(us, stop - start - 1, [t for t, _ in pairs], [p for _, p in pairs])
                 recorded[seg] = torch.cuda.Event()
                 recorded[seg].record(side)
 
@@ -304,10 +337,14 @@ class _GroupStepMap(torch.autograd.Function):
             for seg in reversed(range(nseg)):
                 main.wait_event(recorded[seg])
                 start, stop = span(seg)
-                buf = bufs[seg % 2]
+                buf, mixeds = bufs[seg % 2]
+                use_mixed = mixeds is not None and (ctx.cube or not rec_pruned[seg])
                 for j in reversed(range(stop - start)):
                     states = ctx.ckpts[seg] if j == 0 else [t[j - 1] for t, _ in buf]
-                    if j < stop - start - 1:
+                    if j < stop - start - 1 and use_mixed:
+                        group.step_vjp_mixed_dev(states, [p[j] for _, p in buf], lams, outs, dts,
+                                                 [mx[2 * j:2 * j + 2] for mx in mixeds], rec_pruned[seg])
+                    elif j < stop - start - 1:
                         group.step_vjp_pred_dev(states, [p[j] for _, p in buf], lams, outs, dts)
                     else:
                         group.step_vjp_dev(states, lams, outs, dts)

@@ -519,6 +519,34 @@ class BoxGroup:
         check(self.lib.dnsbox_group_step_dev_record(self._h, self._ptrs(states), nsteps,
                                                     self._traj_ptrs(trajs, nsteps), tp))
 
+    def _mixed_ptrs(self, mixeds, nslots):
+        if len(mixeds) != len(self.boxes):
+            raise ValueError(f"expected {len(self.boxes)} tensors, got {len(mixeds)}")
+        arr = (C.c_void_p * len(mixeds))()
+        for i, (b, t) in enumerate(zip(self.boxes, mixeds)):
+            arr[i] = b._check_mixed(t, nslots).value
+        return arr
+
+    def step_dev_record_mixed(self, states, nsteps: int, trajs, traj_preds, mixeds):
+        """step_dev_record that also records the x-pass inputs of box i's step j + 1 in the slots
+        mixeds[i][2 j] and mixeds[i][2 j + 1] (Box.step_dev_record_mixed per box); returns whether
+        the steps ran pruned."""
+        self._on_torch_stream()
+        check(self.lib.dnsbox_group_step_dev_record_mixed(self._h, self._ptrs(states), nsteps,
+                                                          self._traj_ptrs(trajs, nsteps),
+                                                          self._traj_ptrs(traj_preds, nsteps),
+                                                          self._mixed_ptrs(mixeds, 2 * nsteps)))
+        return self.last_step_pruned()
+
+    def step_vjp_mixed_dev(self, us, ustars, lams, outs, dts, mixeds, pruned_record: bool):
+        """step_vjp_pred_dev reading the recorded x-pass inputs: mixeds[i] is box i's slot pair of
+        the step (two slots, the RHS at u then at u*)."""
+        self._on_torch_stream()
+        d = (C.c_double * len(self.boxes))(*[float(x) for x in dts])
+        check(self.lib.dnsbox_group_step_vjp_mixed_dev(self._h, self._ptrs(us), self._ptrs(ustars), self._ptrs(lams),
+                                                       self._ptrs(outs), d, self._mixed_ptrs(mixeds, 2),
+                                                       int(bool(pruned_record))))
+
     def step_vjp_pred_dev(self, us, ustars, lams, outs, dts):
         """step_vjp_dev with each box's predicted state u* supplied (from step_dev_record)."""
         self._on_torch_stream()

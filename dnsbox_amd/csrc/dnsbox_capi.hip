@@ -848,6 +848,44 @@ int dnsbox_group_step_vjp_pred_dev(dnsbox_group* g, const void* const* u, const 
     });
 }
 
+int dnsbox_group_step_dev_record_mixed(dnsbox_group* g, void* const* states, int nsteps, void* const* traj,
+                                       void* const* traj_pred, void* const* mixed)
+{
+    if (!g || !states || !traj || !traj_pred || !mixed) return fail(DNSBOX_E_ARG, "null argument");
+    if (nsteps < 0) return fail(DNSBOX_E_ARG, "nsteps < 0");
+    for (size_t i = 0; i < g->views.size(); ++i) {
+        if (!states[i] || !traj[i] || !traj_pred[i] || !mixed[i])
+            return fail(DNSBOX_E_ARG, "null state, trajectory or record pointer");
+        if (traj_pred[i] == traj[i] || traj_pred[i] == states[i] || mixed[i] == traj[i] || mixed[i] == traj_pred[i] ||
+            mixed[i] == states[i])
+            return fail(DNSBOX_E_ARG, "states, traj, traj_pred and mixed of a box must be distinct buffers");
+    }
+    return guarded([&]() -> int {
+        g->g->step_ext_record_mixed(states, nsteps, traj, traj_pred, mixed);
+        for (auto& b : g->g->boxes) b->x6_valid = false;
+        return 0;
+    });
+}
+
+int dnsbox_group_step_vjp_mixed_dev(dnsbox_group* g, const void* const* u, const void* const* ustar,
+                                    const void* const* lam, void* const* lam_out, const double* dt,
+                                    void* const* mixed, int pruned_record)
+{
+    if (!g || !u || !ustar || !lam || !lam_out || !dt || !mixed) return fail(DNSBOX_E_ARG, "null argument");
+    for (size_t i = 0; i < g->views.size(); ++i) {
+        if (!u[i] || !ustar[i] || !lam[i] || !lam_out[i] || !mixed[i])
+            return fail(DNSBOX_E_ARG, "null device pointer");
+        if (!(dt[i] > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+        if (lam_out[i] == u[i] || lam_out[i] == lam[i] || lam_out[i] == ustar[i])
+            return fail(DNSBOX_E_ARG, "lam_out must not alias u, ustar or lam");
+    }
+    return guarded([&]() -> int {
+        g->g->step_vjp_mixed(u, ustar, lam, lam_out, dt, mixed, pruned_record != 0);
+        for (auto& b : g->g->boxes) b->x6_valid = false;
+        return 0;
+    });
+}
+
 int dnsbox_group_set_vjp_cube(dnsbox_group* g, int confined)
 {
     if (!g) return fail(DNSBOX_E_ARG, "null group");

@@ -41,6 +41,10 @@ struct GBox {
     cpx<T>* tout;        // forward step: where the corrector also stores the new state (null: none)
     cpx<T>* A;           // E (lam + lam_*)
     cpx<T>* Mu;          // E (dt/2 lam + dt lam_*)
+    // the step's two record slots of 9 fields (slot r at mix + 9 r fields): the x-pass inputs of the
+    // RHS at u (r = 0) and at u* (r = 1) in fields 3..8, written by a recording step and read by the
+    // step adjoint, which transforms the cotangent into fields 0..2 (null: no recording)
+    cpx<T>* mix;
     double dt;
 };
 
@@ -149,6 +153,29 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_ycol(const GBox<T>* __restr
 #undef DNSBOX_G_YCOL
 }
 
+// the y pass on the record slots: SRC 1 reads the scratch and writes fields 3.. of slot `slot`
+// (the recording step's y-inverse); SRC 2 runs in place on fields 0.. of slot `slot` (the step
+// adjoint's cotangent)
+template <class T, int DIR, bool PR, int SRC>
+__global__ void __launch_bounds__(kGroupThreads) k_g_ycol_rec(const GBox<T>* __restrict__ gb, int nbox, int slot)
+{
+    __shared__ cpx<T> lds[GroupLds<T>::Y];
+    __shared__ cpx<T> stw[128];
+    const int L = xcd_contiguous(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int field = L / gridDim.x, gt = L % gridDim.x;
+    const int b = find_box(gb, nbox, gt, PR ? 4 : 1);
+    const GBox<T> bx = gb[b];
+    const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
+    const int tile = gt - (PR ? bx.yt0p : bx.yt0);
+    cpx<T>* rec = bx.mix + (9LL * slot + (SRC == 1 ? 3 : 0) + field) * bx.fs;
+    const cpx<T>* src = bx.X + field * bx.fs;
+#define DNSBOX_G_YCOL(NN)                                                                   \
+    if constexpr (SRC == 1) ycol_tile<Cube<NN>, T, DIR, PR, true>(rec, tile, lds, tw, src); \
+    else ycol_tile<Cube<NN>, T, DIR, PR>(rec, tile, lds, tw)
+    DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_YCOL)
+#undef DNSBOX_G_YCOL
+}
+
 // x pass of Op in place on the scratch; OpCross can track the CFL maxima
 template <class T, class Op, bool PR = false, bool PO = PR>
 __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __restrict__ gb, int nbox, int track)
@@ -164,6 +191,31 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_xpass(const GBox<T>* __rest
 #pragma unroll
     for (int j = 0; j < 4; ++j) red.m[j] = T(0);
 #define DNSBOX_G_X(NN) xpass_tile<Cube<NN>, T, Op, PR, PO>(bx.X, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
+    DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_X)
+#undef DNSBOX_G_X
+    if constexpr (Op::NRED == 3) {
+        if (track) block_max_atomic<3>(red, bx.sc->maxu, false, wsm);
+    }
+}
+
+// k_g_xpass reading its inputs from record slot `slot` (SRC 1: the six x-pass inputs in fields
+// 3..8, as the recording step; SRC 2: all nine fields, as the step adjoint) into the scratch
+template <class T, class Op, bool PR, bool PO, int SRC>
+__global__ void __launch_bounds__(kGroupThreads)
+    k_g_xpass_rec(const GBox<T>* __restrict__ gb, int nbox, int track, int slot)
+{
+    __shared__ cpx<T> lds[GroupLds<T>::template X<Op>];
+    __shared__ cpx<T> stw[128];
+    __shared__ double wsm[4 * 16];
+    const int b = find_box(gb, nbox, blockIdx.x, 2);
+    const GBox<T> bx = gb[b];
+    const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
+    const int blk = blockIdx.x - bx.xb0;
+    const cpx<T>* in = bx.mix + (9LL * slot + (SRC == 1 ? 3 : 0)) * bx.fs;
+    XRed<T> red;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) red.m[j] = T(0);
+#define DNSBOX_G_X(NN) xpass_tile<Cube<NN>, T, Op, PR, PO>(in, bx.fs, bx.X, bx.fs, blk, lds, tw, red)
     DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_X)
 #undef DNSBOX_G_X
     if constexpr (Op::NRED == 3) {
@@ -283,6 +335,42 @@ __global__ void __launch_bounds__(kGroupThreads)
 #undef DNSBOX_G_ZV
 }
 
+// the step adjoint's first pass with recorded x-pass inputs: the cotangent's z-inverse
+// (ProVjpLam) into fields 0..2 of the record slot of the stage's RHS (STAGE 1: at u*, slot 1;
+// STAGE 2: at u, slot 0); PR: the kept columns, with the kGroupCheck check blocks of k_g_zinv_vjp
+template <class T, int STAGE, bool PR = false>
+__global__ void __launch_bounds__(kGroupThreads)
+    k_g_zinv_vjp_rec(const GBox<T>* __restrict__ gb, int nbox, int tiles, unsigned int* viol)
+{
+    __shared__ cpx<T> lds[GroupLds<T>::Z];
+    __shared__ cpx<T> stw[128];
+    const int gt = blockIdx.x;
+    if (PR && gt >= tiles) {   // a check block (the whole block returns here)
+        const int cb = gt - tiles, b = cb / kGroupCheck, part = cb % kGroupCheck;
+        const GBox<T>& bc = gb[b];
+        const cpx<T>* p = STAGE == 1 ? bc.pred : bc.u;
+        bool nz = false;
+#define DNSBOX_G_OC(NN) nz = outside_cube_nonzero<Cube<NN>, T>(p, bc.fs, part * (long long)blockDim.x + threadIdx.x, \
+                                                         (long long)kGroupCheck * blockDim.x)
+        DNSBOX_GROUP_SWITCH(bc.n, DNSBOX_G_OC)
+#undef DNSBOX_G_OC
+        if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(viol, 1u);
+        return;
+    }
+    const int b = find_box(gb, nbox, gt, PR ? 3 : 0);
+    const GBox<T> bx = gb[b];
+    const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
+    const int tile = gt - (PR ? bx.zt0p : bx.zt0);
+    const cpx<T>* l = STAGE == 1 ? bx.lam : bx.Mu;
+    const cpx<T>* in[3] = {l, l + bx.fs, l + 2 * bx.fs};
+    cpx<T>* rec = bx.mix + (STAGE == 1 ? 9 : 0) * bx.fs;
+    cpx<T>* out[3] = {rec, rec + bx.fs, rec + 2 * bx.fs};
+    const ProVjpLam<T> pl{T(STAGE == 1 ? 0.5 * bx.dt : 1.0), bx.n, bx.ph};
+#define DNSBOX_G_ZV(NN) zinv_tile<Cube<NN>, T, ProVjpLam<T>, T, PR>(in, out, tile, lds, tw, pl)
+    DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZV)
+#undef DNSBOX_G_ZV
+}
+
 // z-forward of the VJP with the step-adjoint epilogue (VJP_S1 / VJP_S2)
 template <class T, int MODE>
 __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_vjp(const GBox<T>* __restrict__ gb, int nbox)
@@ -397,6 +485,15 @@ class GroupBase {
     virtual void step_ext_record(void* const* states, int nsteps, void* const* traj, void* const* traj_pred) = 0;
     virtual void step_vjp_pred(const void* const* u, const void* const* ustar, const void* const* lam,
                                void* const* out, const double* dt) = 0;
+    // step_ext_record that also records the x-pass inputs of the two RHS of each step: box i's
+    // step j + 1 in the two 9-field slots at mixed[i] + 18 j fields (fields 3..8 of each)
+    virtual void step_ext_record_mixed(void* const* states, int nsteps, void* const* traj, void* const* traj_pred,
+                                       void* const* mixed) = 0;
+    // step_vjp_pred reading the x-pass inputs of each box's step from its slot pair mixed[i] (18
+    // fields, as recorded; fields 0..2 of both slots are overwritten); pruned_rec: the recording ran
+    // pruned (the slots hold the kept kx only; needs vjp_cube)
+    virtual void step_vjp_mixed(const void* const* u, const void* const* ustar, const void* const* lam,
+                                void* const* out, const double* dt, void* const* mixed, bool pruned_rec) = 0;
     virtual void invalidate() = 0;
 };
 
@@ -546,6 +643,23 @@ class Group final : public GroupBase {
         if (pr) one_step<true>(s, d);
         else one_step<false>(s, d);
     }
+    // one_step whose y-inverses write the x-pass inputs into the boxes' record slots (GBox::mix),
+    // from which the x passes read them
+    template <bool PR>
+    void one_step_rec(hipStream_t s, const GBox<T>* d)
+    {
+        const int nb = int(h_gb.size());
+        const int z = PR ? ztp : zt, y = PR ? ytp : yt;
+        hipLaunchKernelGGL((k_g_ycol_rec<T, +1, PR, 1>), dim3(y, 6), dim3(kGroupThreads), 0, s, d, nb, 0);
+        hipLaunchKernelGGL((k_g_xpass_rec<T, OpCross<T>, PR, PR, 1>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 1, 0);
+        hipLaunchKernelGGL((k_g_ycol<T, -1, PR>), dim3(y, 3), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_PRED, PR>), dim3(z), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_ycol_rec<T, +1, PR, 1>), dim3(y, 6), dim3(kGroupThreads), 0, s, d, nb, 1);
+        hipLaunchKernelGGL((k_g_xpass_rec<T, OpCross<T>, PR, PR, 1>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0, 1);
+        hipLaunchKernelGGL((k_g_ycol<T, -1, PR>), dim3(y, 3), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_zinv<T, EPI_CORR, PR>), dim3(z), dim3(kGroupThreads), 0, s, d, nb);
+        DNSBOX_LAUNCH_CHECK();
+    }
 
     // the boxes' own states run the pruned passes when every box's state allows it
     bool pruned() const
@@ -643,6 +757,15 @@ class Group final : public GroupBase {
     // traj_pred[i] + j * 3 fields
     void step_ext_record(void* const* states, int nsteps, void* const* traj, void* const* traj_pred) override
     {
+        record(states, nsteps, traj, traj_pred, nullptr);
+    }
+    void step_ext_record_mixed(void* const* states, int nsteps, void* const* traj, void* const* traj_pred,
+                               void* const* mixed) override
+    {
+        record(states, nsteps, traj, traj_pred, mixed);
+    }
+    void record(void* const* states, int nsteps, void* const* traj, void* const* traj_pred, void* const* mixed)
+    {
         if (nsteps <= 0) return;
         refresh();
         std::vector<GBox<T>> h = h_gb;
@@ -670,9 +793,12 @@ class Group final : public GroupBase {
                 const long long off = (long long)j * 3 * h[i].fs;
                 h[i].pout = traj_pred ? static_cast<C*>(traj_pred[i]) + off : nullptr;
                 h[i].tout = static_cast<C*>(traj[i]) + off;
+                h[i].mix = mixed ? static_cast<C*>(mixed[i]) + 6 * off : nullptr;   // 18 fields per step
             }
             const GBox<T>* d = descriptors(h);
-            one_step(stream, d, pr);
+            if (!mixed) one_step(stream, d, pr);
+            else if (pr) one_step_rec<true>(stream, d);
+            else one_step_rec<false>(stream, d);
         }
     }
 
@@ -721,6 +847,52 @@ class Group final : public GroupBase {
         x6_valid = false;
         vjp_passes<VJP_S1>(stream, d);
         vjp_passes<VJP_S2>(stream, d);
+    }
+
+    void step_vjp_mixed(const void* const* u, const void* const* ustar, const void* const* lam, void* const* out,
+                        const double* dt, void* const* mixed, bool pruned_rec) override
+    {
+        bool pr = vjp_cube;
+        for (auto& b : boxes) pr = pr && b->may_prune();
+        if (pruned_rec && !pr)
+            throw std::invalid_argument("a group step recorded with pruned passes holds its x-pass inputs at the kept "
+                                        "kx only: its adjoint needs dnsbox_group_set_vjp_cube(g, 1)");
+        refresh();
+        std::vector<GBox<T>> h = h_gb;
+        for (size_t i = 0; i < h.size(); ++i) {
+            h[i].u = static_cast<const C*>(u[i]);
+            h[i].pred = const_cast<C*>(static_cast<const C*>(ustar[i]));   // read only by the VJP
+            h[i].lam = static_cast<const C*>(lam[i]);
+            h[i].out = static_cast<C*>(out[i]);
+            h[i].dt = dt[i];
+            h[i].mix = static_cast<C*>(mixed[i]);
+        }
+        const GBox<T>* d = descriptors(h);
+        x6_valid = false;
+        if (pr) {
+            vjp_rec_passes<VJP_S1, true>(stream, d);
+            vjp_rec_passes<VJP_S2, true>(stream, d);
+        } else {
+            vjp_rec_passes<VJP_S1, false>(stream, d);
+            vjp_rec_passes<VJP_S2, false>(stream, d);
+        }
+    }
+    // the VJP passes of the recorded-input step adjoint: the cotangent is transformed into fields
+    // 0..2 of the stage's record slot (slot 1 at u* for VJP_S1, slot 0 at u for VJP_S2), whose
+    // fields 3..8 hold the recorded x-pass inputs the x pass reads beside it
+    template <int MODE, bool PR>
+    void vjp_rec_passes(hipStream_t s, const GBox<T>* d)
+    {
+        const int nb = int(h_gb.size());
+        const int slot = MODE == VJP_S1 ? 1 : 0;
+        hipLaunchKernelGGL((k_g_zinv_vjp_rec<T, MODE, PR>), dim3(PR ? ztp + nb * kGroupCheck : zt),
+                           dim3(kGroupThreads), 0, s, d, nb, PR ? ztp : zt, boxes[0]->chk + 1);
+        hipLaunchKernelGGL((k_g_ycol_rec<T, +1, PR, 2>), dim3(PR ? ytp : yt, 3), dim3(kGroupThreads), 0, s, d, nb, slot);
+        hipLaunchKernelGGL((k_g_xpass_rec<T, OpVjp<T>, PR, false, 2>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0,
+                           slot);
+        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
+        hipLaunchKernelGGL((k_g_zfwd_vjp<T, MODE>), dim3(zt), dim3(kGroupThreads), 0, s, d, nb);
+        DNSBOX_LAUNCH_CHECK();
     }
 
     // the VJP passes; with vjp_cube (every box's linearisation state confined to its kept cube,

@@ -281,6 +281,17 @@ int dnsbox_group_step_dev_record(dnsbox_group* g, void* const* states, int nstep
    counterpart. */
 int dnsbox_group_step_vjp_pred_dev(dnsbox_group* g, const void* const* u, const void* const* ustar,
                                    const void* const* lam, void* const* lam_out, const double* dt);
+/* batched dnsbox_step_dev_record_mixed: box i's step j + 1 records the x-pass inputs of its two
+   right-hand sides in fields 3..8 of the two 9-field slots at mixed[i] + 18 j fields.  No
+   reference counterpart. */
+int dnsbox_group_step_dev_record_mixed(dnsbox_group* g, void* const* states, int nsteps, void* const* traj,
+                                       void* const* traj_pred, void* const* mixed);
+/* batched dnsbox_step_vjp_mixed_dev: mixed[i] is box i's slot pair of the step (18 fields, the RHS
+   at u then at u*), pruned_record as dnsbox_group_get_last_step_pruned after the recording (a
+   pruned recording needs dnsbox_group_set_vjp_cube).  No reference counterpart. */
+int dnsbox_group_step_vjp_mixed_dev(dnsbox_group* g, const void* const* u, const void* const* ustar,
+                                    const void* const* lam, void* const* lam_out, const double* dt,
+                                    void* const* mixed, int pruned_record);
 /* the group forms of dnsbox_set_vjp_cube / dnsbox_get_last_step_pruned (every box's states;
    the last dnsbox_group_step_dev(_record) call).  No reference counterpart. */
 int dnsbox_group_set_vjp_cube(dnsbox_group* g, int confined);

@@ -123,3 +123,74 @@ def test_step_map_gradient_with_recorded_inputs(gpu_lib, record, k):
     err = rel_l2(host(grads[True]), host(grads[False]))
     print(f"T-step gradient, recorded inputs vs recomputed ({record}): rel_l2 {err:.3e}", flush=True)
     assert err <= 1e-13
+
+
+GROUP = [(32, 0.02), (48, 0.03), (64, 0.015), (16, 0.05), (32, 0.04)]
+
+
+def group_kw(n, nu, prec):
+    return dict(n=n, precision=prec, nu=nu, kolmo_amp=1.0, kolmo_k=4, bandlim=0, adaptive_dt=0, dt=0.005)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_group_mixed_step_vjp_equals_recomputed(gpu_lib, prec):
+    """The group form: dnsbox_group_step_dev_record_mixed stores the same states as separate group
+    steps, and dnsbox_group_step_vjp_mixed_dev equals dnsbox_group_step_vjp_pred_dev per box."""
+    import torch
+    from dnsbox_amd import BoxGroup
+    dt, nsteps = 0.005, 2
+    tol = 1e-13 if prec == 64 else 2e-6
+    with BoxGroup([group_kw(n, nu, prec) for n, nu in GROUP]) as g:
+        for b in g.boxes:
+            b.rhs_int_fact(dt)
+        us = [dev(Oracle(n, nu=nu).init_solenoidal(20 + i, 0, 3.0, 0.5), prec) for i, (n, nu) in enumerate(GROUP)]
+        lams = [dev(rand_spec(Oracle(n).spec_shape, 30 + i), prec) for i, (n, _) in enumerate(GROUP)]
+        trajs = [torch.full((nsteps,) + tuple(u.shape), 7.0, dtype=u.dtype, device=u.device) for u in us]
+        preds = [torch.full_like(t, 7.0) for t in trajs]
+        mixeds = [torch.empty((2 * nsteps,) + b.mixed_slot_shape, dtype=u.dtype, device=u.device)
+                  for b, u in zip(g.boxes, us)]
+        st = [u.clone() for u in us]
+        assert g.step_dev_record_mixed(st, nsteps, trajs, preds, mixeds)
+        ref = [u.clone() for u in us]
+        states = [[u.clone() for u in us]]
+        for j in range(nsteps):
+            g.step_dev(ref, 1)
+            states.append([r.clone() for r in ref])
+            for i in range(len(us)):
+                assert rel_l2(host(trajs[i][j]), host(ref[i])) <= (1e-15 if prec == 64 else 1e-7)
+        g.set_vjp_cube(True)
+        for j in range(nsteps):
+            o1 = [torch.empty_like(u) for u in us]
+            o2 = [torch.empty_like(u) for u in us]
+            g.step_vjp_pred_dev(states[j], [p[j] for p in preds], lams, o1, [dt] * len(us))
+            g.step_vjp_mixed_dev(states[j], [p[j] for p in preds], lams, o2, [dt] * len(us),
+                                 [mx[2 * j:2 * j + 2] for mx in mixeds], True)
+            for i in range(len(us)):
+                err = rel_l2(host(o2[i]), host(o1[i]))
+                print(f"group mixed step-VJP prec={prec} step {j} box {i} (n={GROUP[i][0]}): rel_l2 {err:.3e}",
+                      flush=True)
+                assert err <= tol
+        g.set_vjp_cube(False)
+
+
+def test_group_gradients_with_recorded_inputs(gpu_lib):
+    """group_residual_grads with the recorded inputs equals the recomputing backward (FP32 group,
+    whole trajectory recorded and checkpointed)."""
+    import torch
+    from dnsbox_amd import BoxGroup, adjoint
+    dt, T = 0.005, 24
+    with BoxGroup([group_kw(n, nu, 32) for n, nu in GROUP]) as g:
+        us = [dev(Oracle(n, nu=nu).init_solenoidal(50 + i, 0, 3.0, 0.5), 32) for i, (n, nu) in enumerate(GROUP)]
+        saved = adjoint.RECORD_MIXED
+        for record, k in (("all", 8), ("checkpoint", 7)):
+            grads = {}
+            try:
+                for mixed in (True, False):
+                    adjoint.RECORD_MIXED = mixed
+                    _, grads[mixed] = adjoint.group_residual_grads(g, us, T, [dt] * len(us), k, record)
+            finally:
+                adjoint.RECORD_MIXED = saved
+            for i in range(len(us)):
+                err = rel_l2(host(grads[True][i]), host(grads[False][i]))
+                print(f"group gradient ({record}) box {i}: rel_l2 {err:.3e}", flush=True)
+                assert err <= 1e-5
