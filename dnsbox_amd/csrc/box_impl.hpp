@@ -74,11 +74,14 @@ inline bool use_plane()
 {
     return plane_used<Dm, T, Op>() && plane_enabled();
 }
-// the register x pass (xpass_reg.hpp) runs cubic boxes of its sizes
+// the register x pass (xpass_reg.hpp) runs the x-lines of its lengths NX: every block takes LPB whole
+// lines, and a line's byte offset within a field fits 32 bits
 template <class Dm, class T>
 constexpr bool xreg_used()
 {
-    if constexpr (Dm::CUBIC) return XReg<Dm::NX, T>::enabled && !kLegacyXPass;
+    if constexpr (XReg<Dm::NX, T>::enabled && !kLegacyXPass)
+        return (Dm::NY * Dm::NZ) % xreg::Shape<Dm::NX>::LPB == 0 &&
+               Dm::FS * (long long)sizeof(cpx<T>) < (1LL << 32);
     else return false;
 }
 

@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(kThreads)
                unsigned long long* red_bits)
 {
     using S = Shape<N>;
-    static_assert((long long)N * N * (N / 2 + 1) * sizeof(cpx<T>) < (1LL << 32), "32-bit offsets within a field");
+    // line offsets are 32-bit: the caller runs this kernel on fields under 4 GiB (xreg_used)
     constexpr int H = N / 2 + 1;
     __shared__ cpx<T> xbuf[S::LPB * S::ROW];
     __shared__ TwL<N, T> t;

@@ -35,7 +35,8 @@ def rand_spec(shape, seed):
     return [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(3)]
 
 
-CASES = [((32, 32, 32), 64), ((64, 64, 64), 64), ((48, 48, 48), 32), ((32, 48, 16), 64), ((256, 256, 256), 32)]
+CASES = [((32, 32, 32), 64), ((64, 64, 64), 64), ((48, 48, 48), 32), ((32, 48, 16), 64), ((256, 256, 256), 32),
+         ((256, 256, 128), 32)]
 
 
 @pytest.mark.parametrize("pruned", [True, False])
