@@ -244,3 +244,41 @@ def test_grid_tstep_gradient_and_descent(gpu_lib, record):
         _, hist = adjoint_descent(b, u0, T, dt, k=k, iters=3, record=record)
     print(f"grid descent ({record}): {hist}", flush=True)
     assert all(b_ <= a_ for a_, b_ in zip(hist, hist[1:])) and hist[-1] < hist[0]
+
+
+def test_grid_run_files_and_resume(gpu_lib, tmp_path):
+    """The run driver on the 32 x 48 x 16 grid: Ekin.dat rows against the oracle, the state file
+    holding the physical velocity of shape (Nz, Ny, Nx) (Fortran u(Nx, Ny, Nz)), and a run resumed
+    from it continuing the same trajectory."""
+    from dnsbox_amd.run import run
+    from dnsbox_amd.state_io import read_state, state_dims
+    shape, alpha, dt = (32, 48, 16), (1.0, 1.0, 1.0), 0.004
+    o = oracle(shape, alpha, nu=0.02, kolmo_amp=1.0, kolmo_k=3)
+    f0 = o.init_solenoidal(61, 0, 3.0, 0.5)
+    with box(shape, alpha, 64, nu=0.02, kolmo_amp=1.0, kolmo_k=3, dt=dt) as b:
+        b.set_state(*f0)
+        b.rhs_int_fact(dt)
+        steps, why = run(b, str(tmp_path), 20, isave1=10, isave2=10, tstep_fix=True, courant_max=10.0)
+    assert steps == 20
+    assert (tmp_path / "info.dat").read_text().splitlines()[0] == "Nx,y,z =   32  48  16"
+    ekin = np.loadtxt(tmp_path / "Ekin.dat")
+    f = f0
+    for j, nsteps in enumerate((0, 10, 10)):
+        f = o.heun(f, nsteps, dt) if nsteps else f
+        od, _ = o.diagnostics(f, dt)
+        assert abs(ekin[j][1] - od[0]) <= 1e-10 * od[0] and abs(ekin[j][2] - od[1]) <= 1e-10 * od[1]
+    path = tmp_path / "state0001.h5"
+    assert state_dims(path) == shape
+    st = read_state(path)
+    assert st[0].shape == o.phys_shape
+    m = o.nx * o.ny * o.nz
+    f10 = o.heun(f0, 10, dt)
+    ref = [o.c2r(a) / m for a in f10]
+    assert max(np.abs(a - r).max() for a, r in zip(st, ref)) < 1e-10 * max(np.abs(r).max() for r in ref)
+    with box(shape, alpha, 64, nu=0.02, kolmo_amp=1.0, kolmo_k=3, dt=dt) as b:
+        b.rhs_int_fact(dt)
+        run(b, str(tmp_path / "resume"), 10, isave1=10, isave2=10, tstep_fix=True, courant_max=10.0,
+            initial_state=str(path))
+        got = b.get_state()
+    want = o.heun(o.project(o.dealias([o.r2c(a) for a in st])), 10, dt)
+    assert rel_l2(got, want) < 1e-10
