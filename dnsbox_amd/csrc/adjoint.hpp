@@ -239,20 +239,17 @@ struct VjpArgs {
     BoxPhys ph;
 };
 
-template <class Dm, class T, int MODE>
-__device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long long xfs, int tile, cpx<T>* lds,
-                                              const cpx<T>* tw, const VjpArgs<T>& va)
+// the step-adjoint epilogue of zfwd_vjp_tile on the forward-transformed column of a valid lane;
+// NEXT: U receives Mu at the kept cube's modes of a kept column (zlive), zero elsewhere
+template <class Dm, class T, int MODE, bool NEXT, int UR>
+__device__ __forceinline__ void zfwd_vjp_epilogue(const cpx<T> (&O)[6][Geo<Dm, T>::ZRM], int col,
+                                                  const VjpArgs<T>& va, cpx<T> (&U)[3][UR], bool zlive)
 {
     using G = Geo<Dm, T>;
     using CF = typename G::CFZ_F;
     constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
     constexpr long long ZS = (long long)G::NY * H;
-    const int col = tile * W + threadIdx.x % W;
     const int g = threadIdx.x / W;
-    const bool valid = col < G::ZCOLS;
-    cpx<T> O[6][RM];
-    zfwd_regs<Dm, T>(X, xfs, col, valid, lds, tw, O);
-    if (!valid) return;
     const int ix = col % H, iy = col / H;
     const int kx = kval(ix, G::NX), ky = kval(iy, G::NY);
     const BoxPhys& ph = va.ph;
@@ -298,13 +295,55 @@ __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 const cpx<T> l = va.lam[f * va.fs + idx];
+                const cpx<T> mu = e * (hdt * l + tdt * gg[f]);
                 va.A[f * va.fs + idx] = e * (l + gg[f]);
-                va.Mu[f * va.fs + idx] = e * (hdt * l + tdt * gg[f]);
+                va.Mu[f * va.fs + idx] = mu;
+                if constexpr (NEXT) U[f][q] = zlive && Keep<G::NZ>::kept(iz) ? mu : cpx<T>{T(0), T(0)};
             }
         } else {
 #pragma unroll
             for (int f = 0; f < 3; ++f) va.out[f * va.fs + idx] = va.A[f * va.fs + idx] + gg[f];
         }
+    }
+}
+
+// NEXT (VJP_S1 only): Mu at this thread's modes also enters VJP_S2's cotangent z-inverse
+// (ProVjpLam over the kept columns, inputs at the kept kz: the pruned first pass of
+// vjp_rec_passes), whose outputs go to next[0..2] at every z, the way zfwd_zinv_tile hands the
+// stepper's result to the next right-hand side.  The symmetric z plan puts the inverse's input
+// positions at the forward's output positions, so Mu does not make the round trip through HBM
+// before that transform.
+template <class Dm, class T, int MODE, bool NEXT = false>
+__device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long long xfs, int tile, cpx<T>* lds,
+                                              const cpx<T>* tw, const VjpArgs<T>& va,
+                                              cpx<T>* const* next = nullptr, const ProVjpLam<T>& pro = {})
+{
+    static_assert(!NEXT || MODE == VJP_S1, "only the first VJP of a step has a next z-inverse");
+    using G = Geo<Dm, T>;
+    using CF = typename G::CFZ_F;
+    constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
+    constexpr long long ZS = (long long)G::NY * H;
+    const int col = tile * W + threadIdx.x % W;
+    const int g = threadIdx.x / W;
+    const bool valid = col < G::ZCOLS;
+    cpx<T> O[6][RM];
+    zfwd_regs<Dm, T>(X, xfs, col, valid, lds, tw, O);
+    if constexpr (NEXT) {
+        const bool zlive = valid && Dm::kept_col(col);
+        cpx<T> U[3][RM];
+        if (valid) zfwd_vjp_epilogue<Dm, T, MODE, NEXT>(O, col, va, U, zlive);
+        else {
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+#pragma unroll
+                for (int q = 0; q < RM; ++q) U[f][q] = cpx<T>{T(0), T(0)};
+        }
+        // a tile with no kept column (uniform over the block) skips the transform
+        if (__syncthreads_or(zlive)) zinv_from_regs<Dm, T, ProVjpLam<T>>(U, next, col, zlive, lds, tw, pro);
+    } else {
+        if (!valid) return;
+        cpx<T> U[3][1];
+        zfwd_vjp_epilogue<Dm, T, MODE, false>(O, col, va, U, false);
     }
 }
 
@@ -317,6 +356,31 @@ __global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     __shared__ cpx<T> stw_[G::NZ];
     const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
     zfwd_vjp_tile<Dm, T, MODE>(X, xfs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, va);
+}
+
+// VJP_S1's z-forward and epilogue fused with VJP_S2's cotangent z-inverse (zfwd_vjp_tile with
+// NEXT) in the first `tiles` blocks; the blocks after them check the three fields at chk
+// (VJP_S2's linearisation state) outside the kept cube, as k_zinv_p_chk does in the unfused
+// first pass of VJP_S2.
+template <class Dm, class T>
+__global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
+    k_zfwd_vjp_next(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, VjpArgs<T> va,
+                    cpx<T>* __restrict__ next0, long long next_fs, ProVjpLam<T> pro,
+                    const cpx<T>* __restrict__ chk, long long chk_fs, unsigned int* viol, int tiles)
+{
+    using G = Geo<Dm, T>;
+    __shared__ cpx<T> lds[G::ZLDS];
+    __shared__ cpx<T> stw_[G::NZ];
+    const int b = blockIdx.x;
+    if (b >= tiles) {   // a check block (the whole block returns here)
+        const long long nthr = (long long)(gridDim.x - tiles) * blockDim.x;
+        const bool nz = outside_cube_nonzero<Dm, T>(chk, chk_fs, (long long)(b - tiles) * blockDim.x + threadIdx.x, nthr);
+        if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(viol, 1u);
+        return;
+    }
+    const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
+    cpx<T>* next[3] = {next0, next0 + next_fs, next0 + 2 * next_fs};
+    zfwd_vjp_tile<Dm, T, VJP_S1, true>(X, xfs, xcd_contiguous(b, tiles), lds, tw, va, next, pro);
 }
 
 // [Eband(u), lam_scale * sum_{|k| <= kcut, unmasked} Re(conj(lamN) u)]  (one block)

@@ -874,12 +874,36 @@ class BoxT final : public BoxBase {
     template <int MODE, bool PR>
     void vjp_rec_passes(const C* u, const C* lam, double lam_scale, VjpArgs<T> va, C* slot)
     {
+        vjp_band(u, lam, lam_scale);
+        vjp_rec_zinv<PR>(u, lam, lam_scale, slot);
+        vjp_rec_middle<PR>(slot);
+        vjp_args(va, u, lam, lam_scale);
+        hipLaunchKernelGGL((k_zfwd_vjp<Dm, T, MODE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), va);
+        DNSBOX_LAUNCH_CHECK();
+    }
+    // the band-forcing scalars of a VJP of N at u with cotangent lam_scale * lam (into band2)
+    void vjp_band(const C* u, const C* lam, double lam_scale)
+    {
+        if (!cfg.ph.bandlim) return;
         const double sm = 1.0 / (double(NX) * NY * NZ);
-        if (cfg.ph.bandlim) {
-            hipLaunchKernelGGL((k_band_vjp_scalars<Dm, T>), dim3(1), dim3(256), 0, stream, u, lam, fs, cfg.ph,
-                               lam_scale, sm * sm, band2);
-            DNSBOX_LAUNCH_CHECK();
-        }
+        hipLaunchKernelGGL((k_band_vjp_scalars<Dm, T>), dim3(1), dim3(256), 0, stream, u, lam, fs, cfg.ph, lam_scale,
+                           sm * sm, band2);
+        DNSBOX_LAUNCH_CHECK();
+    }
+    void vjp_args(VjpArgs<T>& va, const C* u, const C* lam, double lam_scale) const
+    {
+        va.u = u;
+        va.lamN = lam;
+        va.lam_scale = lam_scale;
+        va.band = band2;
+        va.fs = fs;
+        va.ph = cfg.ph;
+    }
+    // first pass: the cotangent's z-inverse into fields 0..2 of the slot (PR: with the check of u
+    // outside the kept cube on otherwise idle blocks)
+    template <bool PR>
+    void vjp_rec_zinv(const C* u, const C* lam, double lam_scale, C* slot)
+    {
         Ptrs<T, 3> in{{lam, lam + fs, lam + 2 * fs}};
         ProVjpLam<T> pro{T(lam_scale), NX, cfg.ph};
         if constexpr (PR) {
@@ -890,20 +914,17 @@ class BoxT final : public BoxBase {
                                slot, fs, twz(), pro);
         }
         DNSBOX_LAUNCH_CHECK();
+    }
+    // y-inverse of the cotangent, x pass on it and the recorded inputs, y-forward of the products
+    template <bool PR>
+    void vjp_rec_middle(C* slot)
+    {
         hipLaunchKernelGGL((k_ycol<Dm, T, +1, PR>), dim3(ytiles<Dm, T, PR>(), 3), dim3(YNT), 0, stream, slot, fs, twy());
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_xpass<Dm, T, OpVjp<T>, PR, false>), dim3(G::XBLOCKS), dim3(G::NTX), 0, stream, slot, fs,
                            X(), fs, twx(), (unsigned long long*)nullptr, 0);
         DNSBOX_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_ycol<Dm, T, -1>), dim3(G::YTILES, 6), dim3(YNT), 0, stream, X(), fs, twy());
-        DNSBOX_LAUNCH_CHECK();
-        va.u = u;
-        va.lamN = lam;
-        va.lam_scale = lam_scale;
-        va.band = band2;
-        va.fs = fs;
-        va.ph = cfg.ph;
-        hipLaunchKernelGGL((k_zfwd_vjp<Dm, T, MODE>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), va);
         DNSBOX_LAUNCH_CHECK();
     }
 
@@ -931,8 +952,23 @@ class BoxT final : public BoxBase {
         v2.A = adjA();
         v2.dt = dt;
         if (pr) {
-            vjp_rec_passes<VJP_S1, true>(us, lam, 0.5 * dt, v1, static_cast<C*>(ms_));
-            vjp_rec_passes<VJP_S2, true>(u, adjMu(), 1.0, v2, static_cast<C*>(mu_));
+            // VJP_S1, whose z-forward also runs VJP_S2's first pass (the cotangent Mu's z-inverse
+            // into the slot at u, and the check of u outside the kept cube), then the rest of VJP_S2
+            C* ms = static_cast<C*>(ms_);
+            C* mu = static_cast<C*>(mu_);
+            vjp_band(us, lam, 0.5 * dt);
+            vjp_rec_zinv<true>(us, lam, 0.5 * dt, ms);
+            vjp_rec_middle<true>(ms);
+            vjp_args(v1, us, lam, 0.5 * dt);
+            ProVjpLam<T> pro{T(1.0), NX, cfg.ph};
+            hipLaunchKernelGGL((k_zfwd_vjp_next<Dm, T>), dim3(G::ZTILES + vjp_check_blocks<Dm>()), dim3(ZNT), 0, stream,
+                               X(), fs, twz(), v1, mu, fs, pro, u, fs, chk + 1, int(G::ZTILES));
+            DNSBOX_LAUNCH_CHECK();
+            vjp_band(u, adjMu(), 1.0);
+            vjp_rec_middle<true>(mu);
+            vjp_args(v2, u, adjMu(), 1.0);
+            hipLaunchKernelGGL((k_zfwd_vjp<Dm, T, VJP_S2>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), v2);
+            DNSBOX_LAUNCH_CHECK();
         } else {
             vjp_rec_passes<VJP_S1, false>(us, lam, 0.5 * dt, v1, static_cast<C*>(ms_));
             vjp_rec_passes<VJP_S2, false>(u, adjMu(), 1.0, v2, static_cast<C*>(mu_));
