@@ -470,11 +470,11 @@ def main():
         "err_steps": args.err_steps,
         "err_dt": err_dt(n),
         "fp64_steps_per_s": fp64_rate,
-        "speedup_vs_fp64": value / fp64_rate,
+        "speedup_vs_fp64": (value / fp64_rate) if fp64_rate else None,
         "fp64_unpruned_steps_per_s": fp64_full,
         "speedup_vs_fp64_unpruned": (value / fp64_full) if fp64_full else None,
         "fp32_pure": pure,
-        "speedup_fp32_pure_vs_fp64": (pure["steps_per_s"] / fp64_rate) if pure else None,
+        "speedup_fp32_pure_vs_fp64": (pure["steps_per_s"] / fp64_rate) if pure and fp64_rate else None,
         "roofline": {"bound": "hbm", "achieved": kd["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kd["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom, workload),
                      "kernel": dom, "alg_bytes_per_launch": kd["alg_bytes"], "ms_per_launch": kd["ms"],

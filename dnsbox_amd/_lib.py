@@ -70,6 +70,8 @@ SIGNATURES = {
     "dnsbox_step_vjp_pred_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double]),
     "dnsbox_step_dev_record_mixed": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp]),
     "dnsbox_step_vjp_mixed_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp, C.c_int]),
+    "dnsbox_step_vjp_mixed_seq_dev": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp, _vp, C.c_double,
+                                                C.POINTER(_vp), C.c_int, C.c_int]),
     "dnsbox_set_pruning": (C.c_int, [_vp, C.c_int]),
     "dnsbox_get_pruning": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "dnsbox_set_vjp_cube": (C.c_int, [_vp, C.c_int]),

@@ -118,11 +118,12 @@ class _StepMap(torch.autograd.Function):
         if ctx.last is not None:   # the last segment was recorded by the forward
             bufs.insert((nseg - 1) % 2, ctx.last[:3])
             rec_pruned[nseg - 1] = ctx.last[3]
-        rec = box.twin()
-        rec.rhs_int_fact(dt)
         main = torch.cuda.current_stream(lam.device)
-        side = torch.cuda.Stream(device=lam.device)
-        side.wait_stream(main)            # the checkpoints and buffers are ready
+        if nseg > 1 or ctx.last is None:  # segments to recompute
+            rec = box.twin()
+            rec.rhs_int_fact(dt)
+            side = _side_stream(box, lam.device)
+            side.wait_stream(main)        # the checkpoints and buffers are ready
         recorded = [None] * nseg          # segment s recomputed into bufs[s % 2]
         adjointed = [None] * nseg         # segment s's step adjoints done with bufs[s % 2]
 
@@ -158,16 +159,18 @@ class _StepMap(torch.autograd.Function):
                 states = [ctx.ckpts[seg]] + [traj[j] for j in range(stop - start - 1)]
                 # a pruned recording holds the kept kx only: its inputs serve the cube-confined adjoints
                 use_mixed = mixed is not None and (ctx.cube or not rec_pruned[seg])
-                # the recorded u* of every step but the segment's last, whose VJP recomputes its own
-                for j in reversed(range(stop - start)):
-                    if j < stop - start - 1 and use_mixed:
-                        box.step_vjp_mixed_dev(states[j], preds[j], lam, out, dt, mixed[2 * j], mixed[2 * j + 1],
-                                               rec_pruned[seg])
-                    elif j < stop - start - 1:
-                        box.step_vjp_pred_dev(states[j], preds[j], lam, out, dt)
-                    else:
-                        box.step_vjp_dev(states[j], lam, out, dt)
+                # the segment's last step recomputes its own u*; the steps before it read the
+                # recorded u* (and x-pass inputs, in one call for the whole run of steps)
+                last = stop - start - 1
+                box.step_vjp_dev(states[last], lam, out, dt)
+                lam, out = out, lam
+                if last > 0 and use_mixed:
+                    box.step_vjp_mixed_seq_dev(states, preds, lam, out, dt, mixed, last, rec_pruned[seg])
                     lam, out = out, lam
+                else:
+                    for j in reversed(range(last)):
+                        box.step_vjp_pred_dev(states[j], preds[j], lam, out, dt)
+                        lam, out = out, lam
                 adjointed[seg] = torch.cuda.Event()
                 adjointed[seg].record(main)
                 if seg > 0:
@@ -229,6 +232,17 @@ def _segment_length(T, k, state_bytes, device, record):
         return T
     free, _ = torch.cuda.mem_get_info(device)
     return T if 2 * (T - 1) * state_bytes <= RECORD_FRACTION * free else k
+
+
+def _side_stream(owner, device):
+    """The second stream of owner's checkpointed backwards, created once: creating a stream costs
+    milliseconds of host time, during which the device would idle between the forward and the
+    step adjoints."""
+    s = getattr(owner, "_adj_side_stream", None)
+    if s is None or s.device != device:
+        s = torch.cuda.Stream(device=device)
+        owner._adj_side_stream = s
+    return s
 
 
 def _copy_all(dst, src):
@@ -300,12 +314,13 @@ class _GroupStepMap(torch.autograd.Function):
         if ctx.last is not None:   # the last segment was recorded by the forward
             bufs.insert((nseg - 1) % 2, ctx.last[:2])
             rec_pruned[nseg - 1] = ctx.last[2]
-        rec = group.twin()
-        for b, dt in zip(rec.boxes, dts):
-            b.rhs_int_fact(dt)
         main = torch.cuda.current_stream(lams[0].device)
-        side = torch.cuda.Stream(device=lams[0].device)
-        side.wait_stream(main)
+        if nseg > 1 or ctx.last is None:  # segments to recompute
+            rec = group.twin()
+            for b, dt in zip(rec.boxes, dts):
+                b.rhs_int_fact(dt)
+            side = _side_stream(group, lams[0].device)
+            side.wait_stream(main)
         recorded, adjointed = [None] * nseg, [None] * nseg
 
         def span(seg):

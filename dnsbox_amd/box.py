@@ -382,6 +382,20 @@ class Box:
                                                  self._check_mixed(mixed_u, 1), self._check_mixed(mixed_ustar, 1),
                                                  int(bool(pruned_record))))
 
+    def step_vjp_mixed_seq_dev(self, states, ustars, lam, out, dt: float, mixed, nsteps: int, pruned_record: bool):
+        """step_vjp_mixed_dev over the recorded steps nsteps-1 .. 0 in one call: states[j], ustars[j]
+        and the slots mixed[2 j], mixed[2 j + 1] of step j + 1 (as step_dev_record_mixed wrote
+        them).  The result is left in out; lam is overwritten."""
+        self._on_torch_stream()
+        if nsteps < 1 or len(states) < nsteps or len(ustars) < nsteps:
+            raise ValueError(f"need {nsteps} states and predicted states")
+        us = (C.c_void_p * nsteps)(*[self._check_tensor(states[j]).value for j in range(nsteps)])
+        ps = (C.c_void_p * nsteps)(*[self._check_tensor(ustars[j]).value for j in range(nsteps)])
+        self._check_mixed(mixed, 2 * nsteps)
+        ms = (C.c_void_p * (2 * nsteps))(*[mixed[i].data_ptr() for i in range(2 * nsteps)])
+        check(self.lib.dnsbox_step_vjp_mixed_seq_dev(self._h, us, ps, self._check_tensor(lam), self._check_tensor(out),
+                                                     dt, ms, nsteps, int(bool(pruned_record))))
+
     def rhs_vjp_dev(self, u, lam, out):
         """out <- J_N(u)^T lam (Euclidean inner product of the stored coordinates)."""
         self._on_torch_stream()

@@ -302,23 +302,28 @@ __device__ __forceinline__ void zfwd_vjp_epilogue(const cpx<T> (&O)[6][Geo<Dm, T
             }
         } else {
 #pragma unroll
-            for (int f = 0; f < 3; ++f) va.out[f * va.fs + idx] = va.A[f * va.fs + idx] + gg[f];
+            for (int f = 0; f < 3; ++f) {
+                const cpx<T> o = va.A[f * va.fs + idx] + gg[f];
+                va.out[f * va.fs + idx] = o;
+                if constexpr (NEXT) U[f][q] = zlive && Keep<G::NZ>::kept(iz) ? o : cpx<T>{T(0), T(0)};
+            }
         }
     }
 }
 
-// NEXT (VJP_S1 only): Mu at this thread's modes also enters VJP_S2's cotangent z-inverse
-// (ProVjpLam over the kept columns, inputs at the kept kz: the pruned first pass of
-// vjp_rec_passes), whose outputs go to next[0..2] at every z, the way zfwd_zinv_tile hands the
-// stepper's result to the next right-hand side.  The symmetric z plan puts the inverse's input
-// positions at the forward's output positions, so Mu does not make the round trip through HBM
-// before that transform.
+// NEXT: the cotangent this epilogue writes also enters the first pass of the VJP that reads it
+// next, the cotangent z-inverse (ProVjpLam over the kept columns, inputs at the kept kz, as in
+// the pruned first pass of vjp_rec_passes), whose outputs go to next[0..2] at every z, the way
+// zfwd_zinv_tile hands the stepper's result to the next right-hand side: VJP_S1's Mu to VJP_S2
+// of the same step, VJP_S2's lambda_n to VJP_S1 of the step before.  The symmetric z plan puts
+// the inverse's input positions at the forward's output positions, so the cotangent does not
+// make the round trip through HBM before that transform.
 template <class Dm, class T, int MODE, bool NEXT = false>
 __device__ __forceinline__ void zfwd_vjp_tile(const cpx<T>* __restrict__ X, long long xfs, int tile, cpx<T>* lds,
                                               const cpx<T>* tw, const VjpArgs<T>& va,
                                               cpx<T>* const* next = nullptr, const ProVjpLam<T>& pro = {})
 {
-    static_assert(!NEXT || MODE == VJP_S1, "only the first VJP of a step has a next z-inverse");
+    static_assert(!NEXT || MODE != VJP_RHS, "a step VJP hands its cotangent on");
     using G = Geo<Dm, T>;
     using CF = typename G::CFZ_F;
     constexpr int W = G::ZW, RM = G::ZRM, H = G::H;
@@ -358,11 +363,11 @@ __global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     zfwd_vjp_tile<Dm, T, MODE>(X, xfs, xcd_contiguous(blockIdx.x, gridDim.x), lds, tw, va);
 }
 
-// VJP_S1's z-forward and epilogue fused with VJP_S2's cotangent z-inverse (zfwd_vjp_tile with
-// NEXT) in the first `tiles` blocks; the blocks after them check the three fields at chk
-// (VJP_S2's linearisation state) outside the kept cube, as k_zinv_p_chk does in the unfused
-// first pass of VJP_S2.
-template <class Dm, class T>
+// A step VJP's z-forward and epilogue fused with the next VJP's cotangent z-inverse
+// (zfwd_vjp_tile with NEXT) in the first `tiles` blocks; the blocks after them check the three
+// fields at chk (the next VJP's linearisation state) outside the kept cube, as k_zinv_p_chk does
+// in an unfused first pass.
+template <class Dm, class T, int MODE>
 __global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     k_zfwd_vjp_next(const cpx<T>* __restrict__ X, long long xfs, const cpx<T>* __restrict__ tw_g, VjpArgs<T> va,
                     cpx<T>* __restrict__ next0, long long next_fs, ProVjpLam<T> pro,
@@ -380,7 +385,7 @@ __global__ void __launch_bounds__((Geo<Dm, T>::ZNT))
     }
     const cpx<T>* tw = stage_twiddles<G::NZ, T>(tw_g, stw_);
     cpx<T>* next[3] = {next0, next0 + next_fs, next0 + 2 * next_fs};
-    zfwd_vjp_tile<Dm, T, VJP_S1, true>(X, xfs, xcd_contiguous(b, tiles), lds, tw, va, next, pro);
+    zfwd_vjp_tile<Dm, T, MODE, true>(X, xfs, xcd_contiguous(b, tiles), lds, tw, va, next, pro);
 }
 
 // [Eband(u), lam_scale * sum_{|k| <= kcut, unmasked} Re(conj(lamN) u)]  (one block)

@@ -253,6 +253,10 @@ class BoxBase {
     // recording step ran pruned, so the slots hold the kept kx only and vjp_cube must be set
     virtual void step_vjp_mixed(const void* u, const void* ustar, const void* lam, void* lam_out, double dt,
                                 void* mixed_u, void* mixed_us, bool pruned_rec) = 0;
+    // step_vjp_mixed over the recorded steps n-1 .. 0 of a segment (u[j], ustar[j], slots
+    // mixed[2j], mixed[2j+1]); the result lands in lam_out, lam is work space
+    virtual void step_vjp_mixed_seq(const void* const* u, const void* const* ustar, void* lam, void* lam_out,
+                                    double dt, void* const* mixed, int n, bool pruned_rec) = 0;
     // FP64 device array -> transform type / state type, and back
     virtual void convert_in(void* dst, const double* src, long long n, bool state_type) = 0;
     virtual void convert_out(double* dst, const void* src, long long n, bool state_type) = 0;
@@ -952,26 +956,80 @@ class BoxT final : public BoxBase {
         v2.A = adjA();
         v2.dt = dt;
         if (pr) {
-            // VJP_S1, whose z-forward also runs VJP_S2's first pass (the cotangent Mu's z-inverse
-            // into the slot at u, and the check of u outside the kept cube), then the rest of VJP_S2
-            C* ms = static_cast<C*>(ms_);
-            C* mu = static_cast<C*>(mu_);
-            vjp_band(us, lam, 0.5 * dt);
-            vjp_rec_zinv<true>(us, lam, 0.5 * dt, ms);
-            vjp_rec_middle<true>(ms);
-            vjp_args(v1, us, lam, 0.5 * dt);
-            ProVjpLam<T> pro{T(1.0), NX, cfg.ph};
-            hipLaunchKernelGGL((k_zfwd_vjp_next<Dm, T>), dim3(G::ZTILES + vjp_check_blocks<Dm>()), dim3(ZNT), 0, stream,
-                               X(), fs, twz(), v1, mu, fs, pro, u, fs, chk + 1, int(G::ZTILES));
-            DNSBOX_LAUNCH_CHECK();
-            vjp_band(u, adjMu(), 1.0);
-            vjp_rec_middle<true>(mu);
-            vjp_args(v2, u, adjMu(), 1.0);
-            hipLaunchKernelGGL((k_zfwd_vjp<Dm, T, VJP_S2>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), v2);
-            DNSBOX_LAUNCH_CHECK();
+            step_vjp_rec_pruned(u, us, lam, static_cast<C*>(lam_out), dt, static_cast<C*>(mu_), static_cast<C*>(ms_),
+                                false, nullptr, nullptr);
         } else {
             vjp_rec_passes<VJP_S1, false>(us, lam, 0.5 * dt, v1, static_cast<C*>(ms_));
             vjp_rec_passes<VJP_S2, false>(u, adjMu(), 1.0, v2, static_cast<C*>(mu_));
+        }
+    }
+
+    // One recorded step adjoint with pruned passes: lam -> lam_out at u (slot mu) with predicted
+    // state us (slot ms).  VJP_S1's z-forward also runs VJP_S2's first pass (the cotangent Mu's
+    // z-inverse into mu, and the check of u outside the kept cube).  first_done: VJP_S1's first
+    // pass (lam's z-inverse into ms, the check of us) was run by the step after's last z-forward.
+    // ms_next (with us_next): that z-forward here runs the first pass of the step before's VJP_S1
+    // on lam_out.  lam_out may be lam: VJP_S2 reads no lam.
+    void step_vjp_rec_pruned(const C* u, const C* us, const C* lam, C* lam_out, double dt, C* mu, C* ms,
+                             bool first_done, const C* us_next, C* ms_next)
+    {
+        VjpArgs<T> v1{};
+        v1.lam = lam;
+        v1.A = adjA();
+        v1.Mu = adjMu();
+        v1.dt = dt;
+        VjpArgs<T> v2{};
+        v2.out = lam_out;
+        v2.A = adjA();
+        v2.dt = dt;
+        vjp_band(us, lam, 0.5 * dt);
+        if (!first_done) vjp_rec_zinv<true>(us, lam, 0.5 * dt, ms);
+        vjp_rec_middle<true>(ms);
+        vjp_args(v1, us, lam, 0.5 * dt);
+        const int tiles = G::ZTILES, grid = G::ZTILES + vjp_check_blocks<Dm>();
+        hipLaunchKernelGGL((k_zfwd_vjp_next<Dm, T, VJP_S1>), dim3(grid), dim3(ZNT), 0, stream, X(), fs, twz(), v1, mu, fs,
+                           ProVjpLam<T>{T(1.0), NX, cfg.ph}, u, fs, chk + 1, tiles);
+        DNSBOX_LAUNCH_CHECK();
+        vjp_band(u, adjMu(), 1.0);
+        vjp_rec_middle<true>(mu);
+        vjp_args(v2, u, adjMu(), 1.0);
+        if (ms_next) {
+            hipLaunchKernelGGL((k_zfwd_vjp_next<Dm, T, VJP_S2>), dim3(grid), dim3(ZNT), 0, stream, X(), fs, twz(), v2,
+                               ms_next, fs, ProVjpLam<T>{T(0.5 * dt), NX, cfg.ph}, us_next, fs, chk + 1, tiles);
+        } else {
+            hipLaunchKernelGGL((k_zfwd_vjp<Dm, T, VJP_S2>), dim3(G::ZTILES), dim3(ZNT), 0, stream, X(), fs, twz(), v2);
+        }
+        DNSBOX_LAUNCH_CHECK();
+    }
+
+    // step_vjp_mixed over the recorded steps n-1 .. 0 of a segment (u[j], us[j], slots mixed[2j],
+    // mixed[2j+1]); the result lands in out, lam is work space.  With pruned passes each step's
+    // last z-forward runs the first pass of the step before.
+    void step_vjp_mixed_seq(const void* const* u, const void* const* us, void* lam_, void* out_, double dt,
+                            void* const* mixed, int n, bool pruned_rec) override
+    {
+        require_uniform("the step adjoint");
+        const bool pr = may_prune() && vjp_cube;
+        if (pruned_rec && !pr)
+            throw std::invalid_argument("a step recorded with pruned passes holds its x-pass inputs at the kept kx "
+                                        "only: its adjoint needs dnsbox_set_vjp_cube(ctx, 1) and pruning enabled");
+        x6_valid = false;
+        ensure_adj();
+        C* lam = static_cast<C*>(lam_);
+        C* out = static_cast<C*>(out_);
+        const C* src = lam;
+        for (int s = 0; s < n; ++s) {
+            const int j = n - 1 - s;
+            C* dst = j % 2 == 0 ? out : lam;   // step 0 writes out (the first step works in place when n is even)
+            if (pr) {
+                step_vjp_rec_pruned(static_cast<const C*>(u[j]), static_cast<const C*>(us[j]), src, dst, dt,
+                                    static_cast<C*>(mixed[2 * j]), static_cast<C*>(mixed[2 * j + 1]), s > 0,
+                                    j > 0 ? static_cast<const C*>(us[j - 1]) : nullptr,
+                                    j > 0 ? static_cast<C*>(mixed[2 * j - 1]) : nullptr);
+            } else {
+                step_vjp_mixed(u[j], us[j], src, dst, dt, mixed[2 * j], mixed[2 * j + 1], pruned_rec);
+            }
+            src = dst;
         }
     }
 

@@ -632,6 +632,27 @@ int dnsbox_step_vjp_mixed_dev(dnsbox_ctx* c, const void* u, const void* ustar, c
     });
 }
 
+int dnsbox_step_vjp_mixed_seq_dev(dnsbox_ctx* c, const void* const* u, const void* const* ustar, void* lam,
+                                  void* lam_out, double dt, void* const* mixed, int nsteps, int pruned_record)
+{
+    if (int e = check_ctx(c)) return e;
+    invalidate(c);
+    if (!u || !ustar || !lam || !lam_out || !mixed) return fail(DNSBOX_E_ARG, "null pointer");
+    if (nsteps < 1) return fail(DNSBOX_E_ARG, "nsteps must be >= 1");
+    if (!(dt > 0.0)) return fail(DNSBOX_E_ARG, "dt must be > 0");
+    if (lam_out == lam) return fail(DNSBOX_E_ARG, "lam_out must not alias lam");
+    for (int j = 0; j < nsteps; ++j) {
+        if (!u[j] || !ustar[j] || !mixed[2 * j] || !mixed[2 * j + 1]) return fail(DNSBOX_E_ARG, "null device pointer");
+        if (u[j] == lam || u[j] == lam_out || ustar[j] == lam || ustar[j] == lam_out)
+            return fail(DNSBOX_E_ARG, "lam and lam_out must not alias a state");
+        if (mixed[2 * j] == mixed[2 * j + 1]) return fail(DNSBOX_E_ARG, "the two record slots of a step must differ");
+    }
+    return guarded([&]() -> int {
+        c->box->step_vjp_mixed_seq(u, ustar, lam, lam_out, dt, mixed, nsteps, pruned_record != 0);
+        return 0;
+    });
+}
+
 int dnsbox_step_vjp_pred_dev(dnsbox_ctx* c, const void* u, const void* ustar, const void* lam, void* lam_out,
                              double dt)
 {

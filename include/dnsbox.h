@@ -223,6 +223,13 @@ int dnsbox_step_dev_record_mixed(dnsbox_ctx* ctx, void* state, int nsteps, void*
    needs dnsbox_set_vjp_cube(ctx, 1), else DNSBOX_E_ARG.  No reference counterpart. */
 int dnsbox_step_vjp_mixed_dev(dnsbox_ctx* ctx, const void* u, const void* ustar, const void* lam, void* lam_out,
                               double dt, void* mixed_u, void* mixed_ustar, int pruned_record);
+/* dnsbox_step_vjp_mixed_dev over the recorded steps of a segment, last step first: for
+   j = nsteps-1 .. 0, lam <- (d u_{j+1} / d u_j)^T lam at u[j] with predicted state ustar[j] and the
+   record slots mixed[2 j] (the RHS at u[j]) and mixed[2 j + 1] (at ustar[j]).  The result is left in
+   lam_out; lam is overwritten (work space).  With pruned passes, each step's last pass also runs
+   the first pass of the step before (one launch fewer per step).  No reference counterpart. */
+int dnsbox_step_vjp_mixed_seq_dev(dnsbox_ctx* ctx, const void* const* u, const void* const* ustar, void* lam,
+                                  void* lam_out, double dt, void* const* mixed, int nsteps, int pruned_record);
 /* rhsAdj (rhsAdj.f90:356-381 via rhsNonlinearAdj :261-354, rhsupptilde :177-213, convective
    rhsNonlinear :64-175), with the never-assigned phatadj (:343-348) read as phat (:335-340).
    lin_sign = +1 reproduces the reference's L u~ - P n_adj; lin_sign = -1 gives the descent

@@ -2,8 +2,9 @@
 dnsbox_step_vjp_mixed_dev) against the step adjoint that transforms u and u* again
 (dnsbox_step_vjp_pred_dev), on pruned and unpruned recordings, at sizes that run the plane
 pass, the LDS-staged and the register x pass, and on a non-cubic grid; the recorded states
-(stored by the corrector pass) against separately stepped ones; the T-step-map gradient with
-and without the recorded inputs; and the refusal of a pruned recording without the kept-cube
+(stored by the corrector pass) against separately stepped ones; the segment entry
+(dnsbox_step_vjp_mixed_seq_dev) against single step adjoints; the T-step-map gradient with and
+without the recorded inputs; and the refusal of a pruned recording without the kept-cube
 guarantee."""
 import numpy as np
 import pytest
@@ -78,6 +79,48 @@ def test_mixed_step_vjp_equals_recomputed(gpu_lib, grid, prec, pruned):
                 print(f"mixed step-VJP {grid} prec={prec} pruned={pruned} cube={cube} step {j}: rel_l2 {err:.3e}",
                       flush=True)
                 assert err <= tol
+            b.set_vjp_cube(False)
+
+
+SEQ_CASES = [((32, 32, 32), 64, 3), ((64, 64, 64), 64, 4), ((48, 48, 48), 32, 3), ((32, 48, 16), 64, 4),
+             ((256, 256, 256), 32, 2)]
+
+
+@pytest.mark.parametrize("pruned", [True, False])
+@pytest.mark.parametrize("grid,prec,nsteps", SEQ_CASES)
+def test_mixed_seq_equals_single_steps(gpu_lib, grid, prec, nsteps, pruned):
+    """dnsbox_step_vjp_mixed_seq_dev (a segment's step adjoints in one call; with the cube-confined
+    passes each step's last z-forward also runs the step before's first pass) against
+    dnsbox_step_vjp_mixed_dev step by step, odd and even step counts."""
+    import torch
+    from dnsbox_amd import Box
+    nx, ny, nz = grid
+    dt = 0.004
+    o = Oracle(nx, ny=ny, nz=nz, nu=0.02, kolmo_amp=1.0, kolmo_k=3)
+    with Box(n=nx, nx=nx, ny=ny, nz=nz, precision=prec, nu=0.02, kolmo_amp=1.0, kolmo_k=3, bandlim=0,
+             adaptive_dt=0, dt=dt) as b:
+        b.set_pruning(pruned)
+        b.rhs_int_fact(dt)
+        u0 = dev(o.init_solenoidal(5, 0, 3.0, 0.5), prec)
+        lam0 = dev(rand_spec(o.spec_shape, 6), prec)
+        traj = torch.empty((nsteps,) + tuple(u0.shape), dtype=u0.dtype, device=u0.device)
+        preds = torch.empty_like(traj)
+        mixed = torch.empty((2 * nsteps,) + b.mixed_slot_shape, dtype=u0.dtype, device=u0.device)
+        pr = b.step_dev_record_mixed(u0.clone(), nsteps, traj, preds, mixed)
+        states = [u0] + [traj[j] for j in range(nsteps - 1)]
+        tol = 1e-13 if prec == 64 else 2e-6
+        for cube in ([True, False] if not pruned else [True]):
+            b.set_vjp_cube(cube)
+            lam, out = lam0.clone(), torch.empty_like(lam0)
+            for j in reversed(range(nsteps)):
+                b.step_vjp_mixed_dev(states[j], preds[j], lam, out, dt, mixed[2 * j], mixed[2 * j + 1], pr)
+                lam, out = out, lam
+            l2, o2 = lam0.clone(), torch.full_like(lam0, 3.0)
+            b.step_vjp_mixed_seq_dev(states, preds, l2, o2, dt, mixed, nsteps, pr)
+            err = rel_l2(host(o2), host(lam))
+            print(f"mixed seq {grid} prec={prec} n={nsteps} pruned={pruned} cube={cube}: rel_l2 {err:.3e}",
+                  flush=True)
+            assert err <= tol
             b.set_vjp_cube(False)
 
 
