@@ -401,6 +401,50 @@ __global__ void __launch_bounds__(kGroupThreads) k_g_zfwd_vjp(const GBox<T>* __r
 #undef DNSBOX_G_ZF
 }
 
+// k_g_zfwd_vjp<T, VJP_S1> whose z-forward also runs VJP_S2's first pass (zfwd_vjp_tile with NEXT:
+// Mu's z-inverse from registers into fields 0..2 of the record slot at u, the kept columns), with
+// the kGroupCheck check blocks per box of that pass (the state u outside the kept cube) after
+// the `tiles` tile blocks
+template <class T>
+__global__ void __launch_bounds__(kGroupThreads)
+    k_g_zfwd_vjp_next(const GBox<T>* __restrict__ gb, int nbox, int tiles, unsigned int* viol)
+{
+    __shared__ cpx<T> lds[GroupLds<T>::Z];
+    __shared__ cpx<T> stw[128];
+    const int gt = blockIdx.x;
+    if (gt >= tiles) {   // a check block (the whole block returns here)
+        const int cb = gt - tiles, b = cb / kGroupCheck, part = cb % kGroupCheck;
+        const GBox<T>& bc = gb[b];
+        bool nz = false;
+#define DNSBOX_G_OC(NN) nz = outside_cube_nonzero<Cube<NN>, T>(bc.u, bc.fs, part * (long long)blockDim.x + threadIdx.x, \
+                                                         (long long)kGroupCheck * blockDim.x)
+        DNSBOX_GROUP_SWITCH(bc.n, DNSBOX_G_OC)
+#undef DNSBOX_G_OC
+        if (__any(nz) && threadIdx.x % 64 == 0) atomicOr(viol, 1u);
+        return;
+    }
+    const int b = find_box(gb, nbox, gt, 0);
+    const GBox<T> bx = gb[b];
+    const cpx<T>* tw = stage_tw_n(bx.tw, stw, bx.n);
+    const int tile = gt - bx.zt0;
+    VjpArgs<T> va{};
+    va.lam = bx.lam;
+    va.A = bx.A;
+    va.Mu = bx.Mu;
+    va.u = bx.u;
+    va.lamN = bx.lam;
+    va.lam_scale = 0.5 * bx.dt;
+    va.dt = bx.dt;
+    va.band = nullptr;  // group boxes have no band forcing
+    va.fs = bx.fs;
+    va.ph = bx.ph;
+    cpx<T>* next[3] = {bx.mix, bx.mix + bx.fs, bx.mix + 2 * bx.fs};
+    const ProVjpLam<T> pl{T(1.0), bx.n, bx.ph};
+#define DNSBOX_G_ZF(NN) zfwd_vjp_tile<Cube<NN>, T, VJP_S1, true>(bx.X, bx.fs, tile, lds, tw, va, next, pl)
+    DNSBOX_GROUP_SWITCH(bx.n, DNSBOX_G_ZF)
+#undef DNSBOX_G_ZF
+}
+
 // *flag |= 1 when the state of some box (box blockIdx.y per block row) has a nonzero mode outside
 // its kept cube: whether a step of caller buffers may run the pruned passes
 template <class T>
@@ -870,12 +914,32 @@ class Group final : public GroupBase {
         const GBox<T>* d = descriptors(h);
         x6_valid = false;
         if (pr) {
-            vjp_rec_passes<VJP_S1, true>(stream, d);
-            vjp_rec_passes<VJP_S2, true>(stream, d);
+            // VJP_S1, whose z-forward also runs VJP_S2's first pass, then the rest of VJP_S2
+            const int nb = int(h_gb.size());
+            hipLaunchKernelGGL((k_g_zinv_vjp_rec<T, VJP_S1, true>), dim3(ztp + nb * kGroupCheck), dim3(kGroupThreads), 0,
+                               stream, d, nb, ztp, boxes[0]->chk + 1);
+            vjp_rec_middle<true>(stream, d, 1);
+            hipLaunchKernelGGL((k_g_zfwd_vjp_next<T>), dim3(zt + nb * kGroupCheck), dim3(kGroupThreads), 0, stream, d,
+                               nb, zt, boxes[0]->chk + 1);
+            vjp_rec_middle<true>(stream, d, 0);
+            hipLaunchKernelGGL((k_g_zfwd_vjp<T, VJP_S2>), dim3(zt), dim3(kGroupThreads), 0, stream, d, nb);
+            DNSBOX_LAUNCH_CHECK();
         } else {
             vjp_rec_passes<VJP_S1, false>(stream, d);
             vjp_rec_passes<VJP_S2, false>(stream, d);
         }
+    }
+    // the middle passes of a recorded-input VJP on record slot `slot`: y-inverse of the cotangent,
+    // x pass on it and the recorded inputs, y-forward of the products
+    template <bool PR>
+    void vjp_rec_middle(hipStream_t s, const GBox<T>* d, int slot)
+    {
+        const int nb = int(h_gb.size());
+        hipLaunchKernelGGL((k_g_ycol_rec<T, +1, PR, 2>), dim3(PR ? ytp : yt, 3), dim3(kGroupThreads), 0, s, d, nb, slot);
+        hipLaunchKernelGGL((k_g_xpass_rec<T, OpVjp<T>, PR, false, 2>), dim3(xb), dim3(kGroupThreads), 0, s, d, nb, 0,
+                           slot);
+        hipLaunchKernelGGL((k_g_ycol<T, -1>), dim3(yt, 6), dim3(kGroupThreads), 0, s, d, nb);
+        DNSBOX_LAUNCH_CHECK();
     }
     // the VJP passes of the recorded-input step adjoint: the cotangent is transformed into fields
     // 0..2 of the stage's record slot (slot 1 at u* for VJP_S1, slot 0 at u for VJP_S2), whose
