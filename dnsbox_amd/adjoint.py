@@ -156,20 +156,20 @@ class _StepMap(torch.autograd.Function):
                 main.wait_event(recorded[seg])
                 start, stop = span(seg)
                 traj, preds, mixed = bufs[seg % 2]
-                states = [ctx.ckpts[seg]] + [traj[j] for j in range(stop - start - 1)]
+                u0 = ctx.ckpts[seg]   # step j of the segment starts at u0 (j = 0) or traj[j - 1]
                 # a pruned recording holds the kept kx only: its inputs serve the cube-confined adjoints
                 use_mixed = mixed is not None and (ctx.cube or not rec_pruned[seg])
                 # the segment's last step recomputes its own u*; the steps before it read the
                 # recorded u* (and x-pass inputs, in one call for the whole run of steps)
                 last = stop - start - 1
-                box.step_vjp_dev(states[last], lam, out, dt)
+                box.step_vjp_dev(traj[last - 1] if last > 0 else u0, lam, out, dt)
                 lam, out = out, lam
                 if last > 0 and use_mixed:
-                    box.step_vjp_mixed_seq_dev(states, preds, lam, out, dt, mixed, last, rec_pruned[seg])
+                    box.step_vjp_mixed_seq_dev(u0, traj, preds, lam, out, dt, mixed, last, rec_pruned[seg])
                     lam, out = out, lam
                 else:
                     for j in reversed(range(last)):
-                        box.step_vjp_pred_dev(states[j], preds[j], lam, out, dt)
+                        box.step_vjp_pred_dev(traj[j - 1] if j > 0 else u0, preds[j], lam, out, dt)
                         lam, out = out, lam
                 adjointed[seg] = torch.cuda.Event()
                 adjointed[seg].record(main)

@@ -382,17 +382,25 @@ class Box:
                                                  self._check_mixed(mixed_u, 1), self._check_mixed(mixed_ustar, 1),
                                                  int(bool(pruned_record))))
 
-    def step_vjp_mixed_seq_dev(self, states, ustars, lam, out, dt: float, mixed, nsteps: int, pruned_record: bool):
-        """step_vjp_mixed_dev over the recorded steps nsteps-1 .. 0 in one call: states[j], ustars[j]
-        and the slots mixed[2 j], mixed[2 j + 1] of step j + 1 (as step_dev_record_mixed wrote
-        them).  The result is left in out; lam is overwritten."""
+    def step_vjp_mixed_seq_dev(self, u0, traj, traj_pred, lam, out, dt: float, mixed, nsteps: int,
+                               pruned_record: bool):
+        """step_vjp_mixed_dev over the recorded steps nsteps-1 .. 0 in one call, on what
+        step_dev_record_mixed(u0, nsteps, traj, traj_pred, mixed) wrote: step j runs at u0 (j = 0)
+        or traj[j - 1], with predicted state traj_pred[j] and the slots mixed[2 j], mixed[2 j + 1].
+        The result is left in out; lam is overwritten."""
         self._on_torch_stream()
-        if nsteps < 1 or len(states) < nsteps or len(ustars) < nsteps:
-            raise ValueError(f"need {nsteps} states and predicted states")
-        us = (C.c_void_p * nsteps)(*[self._check_tensor(states[j]).value for j in range(nsteps)])
-        ps = (C.c_void_p * nsteps)(*[self._check_tensor(ustars[j]).value for j in range(nsteps)])
+        if nsteps < 1:
+            raise ValueError("nsteps must be >= 1")
+        self._check_traj(traj_pred, nsteps)
+        if nsteps > 1:
+            self._check_traj(traj, nsteps - 1)
         self._check_mixed(mixed, 2 * nsteps)
-        ms = (C.c_void_p * (2 * nsteps))(*[mixed[i].data_ptr() for i in range(2 * nsteps)])
+        step = int(np.prod(self.dev_shape)) * traj_pred.element_size()
+        slot = int(np.prod(self.mixed_slot_shape)) * mixed.element_size()
+        p0, pt, pp, pm = self._check_tensor(u0).value, traj.data_ptr(), traj_pred.data_ptr(), mixed.data_ptr()
+        us = (C.c_void_p * nsteps)(p0, *[pt + (j - 1) * step for j in range(1, nsteps)])
+        ps = (C.c_void_p * nsteps)(*[pp + j * step for j in range(nsteps)])
+        ms = (C.c_void_p * (2 * nsteps))(*[pm + i * slot for i in range(2 * nsteps)])
         check(self.lib.dnsbox_step_vjp_mixed_seq_dev(self._h, us, ps, self._check_tensor(lam), self._check_tensor(out),
                                                      dt, ms, nsteps, int(bool(pruned_record))))
 

@@ -116,7 +116,7 @@ def test_mixed_seq_equals_single_steps(gpu_lib, grid, prec, nsteps, pruned):
                 b.step_vjp_mixed_dev(states[j], preds[j], lam, out, dt, mixed[2 * j], mixed[2 * j + 1], pr)
                 lam, out = out, lam
             l2, o2 = lam0.clone(), torch.full_like(lam0, 3.0)
-            b.step_vjp_mixed_seq_dev(states, preds, l2, o2, dt, mixed, nsteps, pr)
+            b.step_vjp_mixed_seq_dev(u0, traj, preds, l2, o2, dt, mixed, nsteps, pr)
             err = rel_l2(host(o2), host(lam))
             print(f"mixed seq {grid} prec={prec} n={nsteps} pruned={pruned} cube={cube}: rel_l2 {err:.3e}",
                   flush=True)
