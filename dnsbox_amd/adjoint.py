@@ -39,11 +39,20 @@ def _k_tables(box, device):
     return kx, ky, kz
 
 
+_WEIGHTS = {}
+
+
 def hermitian_weight(n, device):
-    """1 on the kx = 0 and kx = NX/2 planes, 2 elsewhere (c2r counts those twice); n = NX."""
-    w = torch.full((1, 1, n // 2 + 1), 2.0, dtype=torch.float64, device=device)
-    w[..., 0] = 1.0
-    w[..., n // 2] = 1.0
+    """1 on the kx = 0 and kx = NX/2 planes, 2 elsewhere (c2r counts those twice); n = NX.  One
+    tensor per (n, device), made once (callers only read it): three fills per residual otherwise
+    stand between the forward and the step adjoints."""
+    key = (n, str(torch.device(device)))
+    w = _WEIGHTS.get(key)
+    if w is None:
+        w = torch.full((1, 1, n // 2 + 1), 2.0, dtype=torch.float64, device=device)
+        w[..., 0] = 1.0
+        w[..., n // 2] = 1.0
+        _WEIGHTS[key] = w
     return w
 
 
