@@ -125,3 +125,6 @@ def test_errors_without_device_do_not_abort():
     rc = lib.dnsbox_create(C.byref(p), C.byref(h))
     assert rc == -1 and b"unsupported grid 20 x 20 x 20" in lib.dnsbox_last_error()
     assert lib.dnsbox_step(None, 1, 1) == -1
+    # the segment step adjoint refuses a missing context before it reads its pointer arrays
+    arr = (C.c_void_p * 2)()
+    assert lib.dnsbox_step_vjp_mixed_seq_dev(None, arr, arr, None, None, 0.01, arr, 1, 1) == -1
